@@ -1,0 +1,634 @@
+// C ABI of the hoefem library: model setup, assembly entry points, Schwarz, PCG and the
+// Backward-Euler Newton driver (all device work is enqueued on the model's stream).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fem_device.hpp"
+#include "internal.hpp"
+
+namespace hoefem {
+
+// solver.hip
+void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu);
+void launch_residual(Model& m, const double* Mvals, const double* gforce, const uint8_t* dirmask, double dt,
+                     const double* w, double* F, CgWork& cw, double* norm2);
+void launch_axpy_neg(Model& m, double* w, double delta, const double* sol);
+void launch_be_begin(Model& m, const double* u, const double* v, double dt, double* up, double* vn);
+void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask);
+void launch_zero_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask);
+void launch_finite_check(Model& m, const double* a, const double* b);
+
+thread_local std::string g_last_error;
+
+static void upload_quadrature(Model& m, const hoefem_quadrature& q, DeviceQuadrature& dq) {
+    const int64_t nif = m.ne - m.ninterior;
+    if (m.ninterior > 0 && (q.interior_num_points <= 0 || !q.interior_weights || !q.interior_points))
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "missing interior quadrature");
+    if (nif > 0 && (!q.interface_offsets || !q.interface_weights || !q.interface_points))
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "missing interface quadrature");
+    std::vector<int32_t> off(m.ne + 1, 0);
+    for (int64_t e = 0; e < m.ne; ++e) {
+        const int64_t cnt = e < m.ninterior ? q.interior_num_points
+                                            : q.interface_offsets[e - m.ninterior + 1] - q.interface_offsets[e - m.ninterior];
+        if (cnt < 0) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad interface quadrature offsets");
+        if ((int64_t)off[e] + cnt > INT32_MAX) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "too many quadrature points");
+        off[e + 1] = off[e] + (int32_t)cnt;
+        dq.max_points = std::max<int>(dq.max_points, (int)cnt);
+    }
+    std::vector<double> w(off[m.ne]), xi(3 * (size_t)off[m.ne]);
+    for (int64_t e = 0; e < m.ne; ++e) {
+        const double *sw, *sx;
+        if (e < m.ninterior) {
+            sw = q.interior_weights;
+            sx = q.interior_points;
+        } else {
+            const int64_t k = q.interface_offsets[e - m.ninterior];
+            sw = q.interface_weights + k;
+            sx = q.interface_points + 3 * k;
+        }
+        const int cnt = off[e + 1] - off[e];
+        std::copy(sw, sw + cnt, w.begin() + off[e]);
+        std::copy(sx, sx + 3 * cnt, xi.begin() + 3 * (size_t)off[e]);
+    }
+    dq.off.upload(off, m.stream);
+    dq.w.upload(w, m.stream);
+    dq.xi.upload(xi, m.stream);
+}
+
+static void model_create(Model& mm, const hoefem_model_desc* d) {
+    if (!d) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null descriptor");
+    const int n = nodes_per_element(d->element_type);
+    if (n < 0) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unknown element type");
+    if (d->num_vertices < 0 || d->num_elements < 0 || d->num_interior < 0 || d->num_interior > d->num_elements)
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad mesh sizes");
+    if ((d->num_vertices && !d->vertices) || (d->num_elements && !d->connectivity))
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null mesh arrays");
+    Model* m = &mm;
+    m->type = d->element_type;
+    m->n = n;
+    m->gn = geometry_nodes(d->element_type);
+    m->nv = d->num_vertices;
+    m->ne = d->num_elements;
+    m->ninterior = d->num_interior;
+    m->mass_reg = d->mass_regularization_factor;
+    m->vertices.assign(d->vertices, d->vertices + 3 * m->nv);
+    m->conn.assign(d->connectivity, d->connectivity + (size_t)n * m->ne);
+    for (int32_t c : m->conn)
+        if (c < 0 || c >= m->nv) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "connectivity out of range");
+    // colors of the interior and interface passes (fenris/src/model.rs:90-102,
+    // fenris/src/embedding/mod.rs:262-270)
+    const int64_t bounds[3] = {0, m->ninterior, m->ne};
+    for (int p = 0; p < 2; ++p) {
+        const int64_t cnt = bounds[p + 1] - bounds[p];
+        std::vector<int64_t> off(cnt + 1);
+        for (int64_t i = 0; i <= cnt; ++i) off[i] = i * n;
+        auto cols = greedy_coloring(cnt, off.data(), m->conn.data() + bounds[p] * n);
+        for (auto& c : cols)
+            for (auto& l : c) l += (int32_t)bounds[p];
+        m->colors[p] = std::move(cols);
+    }
+    std::vector<int64_t> eoff(m->ne + 1);
+    for (int64_t i = 0; i <= m->ne; ++i) eoff[i] = i * n;
+    build_node_pattern(m->nv, m->ne, eoff.data(), m->conn.data(), {}, m->pattern);
+    m->scatter.build(m->pattern, m->ne, eoff.data(), m->conn.data(), m->colors, 2);
+    // device data
+    m->pattern.upload(m->stream);
+    m->scatter.upload(m->stream);
+    m->d_vertices.upload(m->vertices, m->stream);
+    std::vector<int32_t> soa((size_t)n * m->ne);
+    for (int64_t e = 0; e < m->ne; ++e)
+        for (int k = 0; k < n; ++k) soa[(size_t)k * m->ne + e] = m->conn[(size_t)e * n + k];
+    m->d_conn.upload(soa, m->stream);
+    upload_quadrature(*m, d->mass_quadrature, m->q_mass);
+    upload_quadrature(*m, d->stiffness_quadrature, m->q_stiff);
+    upload_quadrature(*m, d->elliptic_quadrature, m->q_ell);
+    if (m->mass_reg > 0.0) {
+        const auto& q = d->mass_quadrature;
+        if (q.interior_num_points <= 0) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "regularization needs an interior rule");
+        m->q_interior_mass.w.upload(q.interior_weights, q.interior_num_points, m->stream);
+        m->q_interior_mass.xi.upload(q.interior_points, 3 * (size_t)q.interior_num_points, m->stream);
+        m->q_interior_mass.max_points = q.interior_num_points;
+    }
+    m->d_err.alloc(1);
+    HIP_CHECK(hipMemsetAsync(m->d_err.get(), 0, sizeof(int), m->stream));
+    HIP_CHECK(hipStreamSynchronize(m->stream));
+}
+
+static void raise_kernel_errors(Model& m) {
+    const int e = read_and_clear_err(m);
+    if (e & KERR_SINGULAR) throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient or element Jacobian");
+    if (e & KERR_ZERO_DIAG) throw Error(HOEFEM_ERROR_SINGULAR, "zero diagonal entry");
+    if (e & KERR_NONFINITE) throw Error(HOEFEM_ERROR_NOT_FINITE, "non-finite values");
+    if (e) throw Error(HOEFEM_ERROR_DEVICE, "kernel error flag set");
+}
+
+// ------------------------------------------------------------------ simulation
+struct Sim {
+    Model* model = nullptr;
+    hoefem_sim_desc d{};
+    double mu_r = 0, lambda_r = 0;
+    std::vector<int32_t> dir;
+    std::unique_ptr<Schwarz> schwarz;
+    DevBuf<uint8_t> d_dirmask;
+    DevBuf<int32_t> d_bc_nodes;
+    int64_t n_bc = 0;
+    DevBuf<double> d_M, d_J, d_gforce, d_up, d_vn, d_w, d_F, d_sol, d_uu, d_us, d_vs, d_scale, d_norm2;
+    PinnedBuf<double> h_norm2{1};
+    CgWork cg;
+    double rep_force = 0.0;
+    int64_t cg_total = 0, evals = 0;
+    int last_err_flags = 0;
+};
+
+static void build_bc(Model& m, const std::vector<int32_t>& dir, DevBuf<uint8_t>& dmask, DevBuf<int32_t>& dbc,
+                     int64_t& nbc) {
+    std::vector<uint8_t> mask(m.nv, 0);
+    for (int32_t v : dir) {
+        if (v < 0 || v >= m.nv) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "Dirichlet node out of range");
+        mask[v] = 1;
+    }
+    // rows to touch: Dirichlet nodes and their pattern neighbours
+    std::vector<uint8_t> touch(m.nv, 0);
+    for (int32_t v : dir)
+        for (int32_t k = m.pattern.nptr[v]; k < m.pattern.nptr[v + 1]; ++k) touch[m.pattern.ncol[k]] = 1;
+    for (int32_t v : dir) touch[v] = 1;
+    std::vector<int32_t> bc;
+    for (int64_t v = 0; v < m.nv; ++v)
+        if (touch[v]) bc.push_back((int32_t)v);
+    nbc = (int64_t)bc.size();
+    dmask.upload(mask, m.stream);
+    dbc.upload(bc, m.stream);
+}
+
+static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
+    Sim* s = &sref;
+    s->model = &m;
+    s->d = *d;
+    dev::snh_reparam(d->mu, d->lambda, &s->mu_r, &s->lambda_r);
+    s->dir.assign(d->dirichlet_nodes, d->dirichlet_nodes + d->num_dirichlet);
+    build_bc(m, s->dir, s->d_dirmask, s->d_bc_nodes, s->n_bc);
+    const int64_t n = 3 * m.nv, nnz = m.pattern.nnz();
+    for (DevBuf<double>* b : {&s->d_gforce, &s->d_up, &s->d_vn, &s->d_w, &s->d_F, &s->d_sol, &s->d_uu, &s->d_us, &s->d_vs})
+        b->ensure(n);
+    s->d_M.ensure(nnz);
+    s->d_J.ensure(nnz);
+    s->d_scale.ensure(1);
+    s->d_norm2.ensure(1);
+    // mass matrix with Dirichlet conditions (fe_model.rs:490-515)
+    launch_element_mass(m, d->density);
+    launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s->d_M.get(), 0, 0.0, nullptr, true, m.stream);
+    launch_dirichlet(m.pattern, s->d_M.get(), s->d_dirmask.get(), s->d_bc_nodes.get(), s->n_bc, s->d_scale.get(), m.stream);
+    // representative force |M a_rep|, a_rep = sqrt(100/3) (fe_model.rs:423-431), and M g
+    std::vector<double> arep(n, std::sqrt((10.0 * 10.0) / 3.0)), gd(n, 0.0);
+    if (d->gravity_enabled)
+        for (int64_t i = 0; i < n; ++i)
+            if ((i % 3) + 1 == 3) gd[i] = d->gravity;
+    s->d_up.upload(arep, m.stream);
+    launch_spmv(m.pattern, s->d_M.get(), 0.0, s->d_F.get(), 1.0, s->d_up.get(), m.stream);
+    std::vector<double> Ma(n);
+    HIP_CHECK(hipMemcpyAsync(Ma.data(), s->d_F.get(), sizeof(double) * n, hipMemcpyDeviceToHost, m.stream));
+    s->d_vn.upload(gd, m.stream);
+    launch_spmv(m.pattern, s->d_M.get(), 0.0, s->d_gforce.get(), 1.0, s->d_vn.get(), m.stream);
+    HIP_CHECK(hipStreamSynchronize(m.stream));
+    double ss = 0.0;
+    for (double v : Ma) ss += v * v;
+    s->rep_force = std::sqrt(ss);
+    if (d->num_subdomains > 0) {
+        s->schwarz.reset(new Schwarz());
+        schwarz_setup(*s->schwarz, m, d->num_subdomains, d->subdomain_offsets, d->subdomain_nodes, d->num_untreated,
+                      d->untreated_nodes);
+    }
+    raise_kernel_errors(m);
+}
+
+// F(w) and |F|^2 for the current (u_p, v_n) held in the sim
+static double sim_eval(Sim& s, double dt, const double* w) {
+    Model& m = *s.model;
+    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
+    launch_element_forces(m, s.mu_r, s.lambda_r, s.d_uu.get());
+    launch_residual(m, s.d_M.get(), s.d_gforce.get(), s.d_dirmask.get(), dt, w, s.d_F.get(), s.cg, s.d_norm2.get());
+    HIP_CHECK(hipMemcpyAsync(m.h_err.p, m.d_err.get(), sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    HIP_CHECK(hipMemcpyAsync(s.h_norm2.p, s.d_norm2.get(), sizeof(double), hipMemcpyDeviceToHost, m.stream));
+    HIP_CHECK(hipStreamSynchronize(m.stream));
+    s.evals++;
+    if (*m.h_err.p) {
+        s.last_err_flags |= *m.h_err.p;
+        HIP_CHECK(hipMemsetAsync(m.d_err.get(), 0, sizeof(int), m.stream));
+        return NAN;  // a singular F makes the residual unusable; the line search then fails
+    }
+    return *s.h_norm2.p;
+}
+
+// NewtonCgDynamicSystem::solve_jacobian_combination (newton_cg.rs:680-802): K(u) -> J = M + dt^2 K
+// with Dirichlet conditions -> preconditioner update -> PCG from a zero guess. CG's
+// MaxIterationsReached and IndefiniteOperator are tolerated (newton_cg.rs:776-797).
+static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double* rhs, double* sol) {
+    Model& m = *s.model;
+    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
+    launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_uu.get());
+    const double alpha = -dt * dt;
+    launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream);
+    launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(), m.stream);
+    if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
+    const int e = read_and_clear_err(m);
+    if (e) {
+        s.last_err_flags |= e;
+        return HOEFEM_ERROR_SINGULAR;
+    }
+    HIP_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * 3 * m.nv, m.stream));
+    int64_t it = 0;
+    hoefem_status st = pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter, &it);
+    s.cg_total += it;
+    if (st == HOEFEM_ERROR_CG_MAX_ITERATIONS || st == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) st = HOEFEM_OK;
+    return st;
+}
+
+// backward_euler_step (hamilton2/src/integrators/euler.rs:155-199) with newton_line_search and
+// BacktrackingLineSearch (hamilton2/src/newton.rs:76-131, :171-246). u, v: device state.
+static hoefem_status sim_be_step(Sim& s, double dt, double* u, double* v, int64_t* newton_iters, double* final_res,
+                                 double* tol_out) {
+    Model& m = *s.model;
+    launch_be_begin(m, u, v, dt, s.d_up.get(), s.d_vn.get());
+    HIP_CHECK(hipMemsetAsync(s.d_w.get(), 0, sizeof(double) * 3 * m.nv, m.stream));
+    const double tol = dt * (s.d.newton_tolerance_factor * s.rep_force);
+    *tol_out = tol;
+    double n2 = sim_eval(s, dt, s.d_w.get());
+    int64_t it = 0;
+    *newton_iters = 0;
+    while (true) {
+        const double fnorm = std::sqrt(n2);
+        *final_res = fnorm;
+        if (!(fnorm > tol)) break;
+        if (s.d.max_newton_iter >= 0 && it == s.d.max_newton_iter) {
+            *newton_iters = it;
+            return HOEFEM_ERROR_NEWTON_MAX_ITERATIONS;
+        }
+        // solve J (-dx) = F
+        hoefem_status st = sim_solve(s, dt, s.d_w.get(), s.d_F.get(), s.d_sol.get());
+        if (st != HOEFEM_OK) {
+            *newton_iters = it;
+            return HOEFEM_ERROR_JACOBIAN;
+        }
+        // backtracking on g = 0.5 |F|^2 along p = -sol
+        const double c = 1e-4, alpha_min = 1e-6;
+        const double g_initial = 0.5 * n2;
+        double alpha_prev = 0.0, alpha = 1.0, tail = 0.25;
+        const double initial[3] = {1.0, 0.75, 0.5};
+        int idx = 0;
+        while (true) {
+            launch_axpy_neg(m, s.d_w.get(), alpha - alpha_prev, s.d_sol.get());
+            n2 = sim_eval(s, dt, s.d_w.get());
+            const double g = 0.5 * n2;
+            if (g <= (1.0 - c * alpha) * g_initial) break;
+            if (alpha < alpha_min) {
+                *newton_iters = it;
+                return HOEFEM_ERROR_LINE_SEARCH;
+            }
+            alpha_prev = alpha;
+            ++idx;
+            if (idx < 3) {
+                alpha = initial[idx];
+            } else {
+                alpha = tail;
+                tail *= 0.25;
+            }
+        }
+        ++it;
+    }
+    *newton_iters = it;
+    launch_be_end(m, u, v, s.d_w.get(), dt, s.d_dirmask.get());
+    return HOEFEM_OK;
+}
+
+// integrate_model_with_substeps (simulation_toolbox/src/fem/newton_cg.rs:146-228)
+static hoefem_status sim_step(Sim& s, double t, double dt, double* u, double* v, hoefem_step_stats* stats) {
+    Model& m = *s.model;
+    const int64_t n = 3 * m.nv;
+    hoefem_step_stats local{};
+    hoefem_step_stats& st = stats ? *stats : local;
+    st = hoefem_step_stats{};
+    s.cg_total = 0;
+    s.evals = 0;
+    s.last_err_flags = 0;
+    launch_zero_dirichlet(m, u, v, s.d_dirmask.get());
+    HIP_CHECK(hipMemcpyAsync(s.d_us.get(), u, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+    HIP_CHECK(hipMemcpyAsync(s.d_vs.get(), v, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+    const int max_substeps = 1000, factor = 5;
+    int num_substeps = 1;
+    hoefem_status last = HOEFEM_OK;
+    while (true) {
+        const double sdt = dt / (double)num_substeps;
+        bool failed = false;
+        for (int k = 0; k < num_substeps; ++k) {
+            (void)t;
+            int64_t it = 0;
+            last = sim_be_step(s, sdt, s.d_us.get(), s.d_vs.get(), &it, &st.final_residual, &st.tolerance);
+            st.newton_iterations += it;
+            if (last != HOEFEM_OK) {
+                failed = true;
+                break;
+            }
+        }
+        if (!failed) break;
+        num_substeps *= factor;
+        if (num_substeps >= max_substeps) {
+            st.cg_iterations = s.cg_total;
+            st.residual_evaluations = s.evals;
+            throw Error(HOEFEM_ERROR_SUBSTEPPING, "Substepping failed; last error " + std::to_string((int)last));
+        }
+        HIP_CHECK(hipMemcpyAsync(s.d_us.get(), u, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+        HIP_CHECK(hipMemcpyAsync(s.d_vs.get(), v, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+    }
+    st.substeps = num_substeps;
+    st.cg_iterations = s.cg_total;
+    st.residual_evaluations = s.evals;
+    launch_finite_check(m, s.d_us.get(), s.d_vs.get());
+    const int e = read_and_clear_err(m);
+    if (e & KERR_NONFINITE) throw Error(HOEFEM_ERROR_NOT_FINITE, "non-finite state after step");
+    HIP_CHECK(hipMemcpyAsync(u, s.d_us.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+    HIP_CHECK(hipMemcpyAsync(v, s.d_vs.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+    HIP_CHECK(hipStreamSynchronize(m.stream));
+    return HOEFEM_OK;
+}
+
+}  // namespace hoefem
+
+// ====================================================================== C ABI
+using namespace hoefem;
+
+struct hoefem_model {
+    Model m;
+};
+struct hoefem_schwarz {
+    Schwarz s;
+};
+struct hoefem_sim {
+    Sim s;
+};
+
+template <class F>
+static hoefem_status guarded(F&& f) {
+    try {
+        f();
+        return HOEFEM_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return HOEFEM_ERROR_DEVICE;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return HOEFEM_ERROR_DEVICE;
+    }
+}
+
+static Model& M_(hoefem_model* m) {
+    if (!m) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null model");
+    return m->m;
+}
+
+extern "C" {
+
+const char* hoefem_last_error(void) { return g_last_error.c_str(); }
+
+int32_t hoefem_nodes_per_element(int32_t t) { return nodes_per_element(t); }
+
+void hoefem_lame_from_young_poisson(double young, double poisson, double* mu, double* lambda) {
+    const double m = 0.5 * young / (1.0 + poisson);
+    *mu = m;
+    *lambda = 0.5 * m * poisson / (1.0 - 2.0 * poisson);
+}
+
+hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const double* F, double* P, double* dPdF) {
+    return guarded([&] {
+        if (nq < 0 || (nq > 0 && !F)) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
+        double mr, lr;
+        dev::snh_reparam(mu, lambda, &mr, &lr);
+        material_eval(mr, lr, nq, F, P, dPdF);
+    });
+}
+
+hoefem_status hoefem_model_create(const hoefem_model_desc* desc, hoefem_model** out) {
+    return guarded([&] {
+        if (!out) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null output");
+        std::unique_ptr<hoefem_model> h(new hoefem_model());
+        model_create(h->m, desc);
+        *out = h.release();
+    });
+}
+
+void hoefem_model_destroy(hoefem_model* model) { delete model; }
+
+hoefem_status hoefem_model_set_stream(hoefem_model* model, void* stream) {
+    return guarded([&] { M_(model).stream = (hipStream_t)stream; });
+}
+
+int64_t hoefem_model_ndof(const hoefem_model* m) { return m ? 3 * m->m.nv : 0; }
+int64_t hoefem_model_nnz(const hoefem_model* m) { return m ? m->m.pattern.nnz() : 0; }
+
+hoefem_status hoefem_model_pattern(const hoefem_model* h, int32_t* rowptr, int32_t* col) {
+    return guarded([&] {
+        if (!h || !rowptr || !col) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        const Model* m = &h->m;
+        std::copy(m->pattern.rowptr.begin(), m->pattern.rowptr.end(), rowptr);
+        std::copy(m->pattern.col.begin(), m->pattern.col.end(), col);
+    });
+}
+
+int32_t hoefem_model_num_colors(const hoefem_model* h, int32_t pass) {
+    if (!h || pass < 0 || pass > 1) return -1;
+    return (int32_t)h->m.colors[pass].size();
+}
+
+hoefem_status hoefem_model_colors(const hoefem_model* h, int32_t pass, int32_t* offsets, int32_t* labels) {
+    return guarded([&] {
+        if (!h || pass < 0 || pass > 1 || !offsets) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
+        const Model* m = &h->m;
+        int32_t k = 0;
+        offsets[0] = 0;
+        for (size_t c = 0; c < m->colors[pass].size(); ++c) {
+            for (int32_t l : m->colors[pass][c]) labels[k++] = l;
+            offsets[c + 1] = k;
+        }
+    });
+}
+
+hoefem_status hoefem_model_forces_add(hoefem_model* model, double mu, double lambda, const double* u, double* f) {
+    return guarded([&] {
+        Model& m = M_(model);
+        double mr, lr;
+        dev::snh_reparam(mu, lambda, &mr, &lr);
+        launch_element_forces(m, mr, lr, u);
+        launch_force_gather(m, f);
+        raise_kernel_errors(m);
+    });
+}
+
+hoefem_status hoefem_model_stiffness_add(hoefem_model* model, double mu, double lambda, const double* u,
+                                         double* values) {
+    return guarded([&] {
+        Model& m = M_(model);
+        double mr, lr;
+        dev::snh_reparam(mu, lambda, &mr, &lr);
+        launch_element_stiffness(m, mr, lr, u);
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), values, 0, 0.0, nullptr, false, m.stream);
+        raise_kernel_errors(m);
+    });
+}
+
+hoefem_status hoefem_model_mass(hoefem_model* model, double density, double* values) {
+    return guarded([&] {
+        Model& m = M_(model);
+        launch_element_mass(m, density);
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), values, 0, 0.0, nullptr, true, m.stream);
+        raise_kernel_errors(m);
+    });
+}
+
+hoefem_status hoefem_model_dirichlet_csr(hoefem_model* model, int64_t num_nodes, const int32_t* nodes, double* values) {
+    return guarded([&] {
+        Model& m = M_(model);
+        std::vector<int32_t> dir(nodes, nodes + num_nodes);
+        DevBuf<uint8_t> mask;
+        DevBuf<int32_t> bc;
+        DevBuf<double> scale;
+        scale.alloc(1);
+        int64_t nbc = 0;
+        build_bc(m, dir, mask, bc, nbc);
+        launch_dirichlet(m.pattern, values, mask.get(), bc.get(), nbc, scale.get(), m.stream);
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+    });
+}
+
+hoefem_status hoefem_model_spmv(hoefem_model* model, const double* values, double beta, double* y, double alpha,
+                                const double* x) {
+    return guarded([&] {
+        Model& m = M_(model);
+        launch_spmv(m.pattern, values, beta, y, alpha, x, m.stream);
+    });
+}
+
+hoefem_status hoefem_schwarz_create(hoefem_model* model, int64_t num_subdomains, const int64_t* offsets,
+                                    const int32_t* nodes, int64_t num_untreated, const int32_t* untreated,
+                                    hoefem_schwarz** out) {
+    return guarded([&] {
+        Model& m = M_(model);
+        if (!out || num_subdomains < 0 || num_untreated < 0 || (num_subdomains > 0 && (!offsets || !nodes)))
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
+        std::unique_ptr<hoefem_schwarz> s(new hoefem_schwarz());
+        static const int64_t zero_off = 0;
+        schwarz_setup(s->s, m, num_subdomains, num_subdomains > 0 ? offsets : &zero_off, nodes, num_untreated,
+                      untreated);
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+        *out = s.release();
+    });
+}
+
+void hoefem_schwarz_destroy(hoefem_schwarz* s) { delete s; }
+int64_t hoefem_schwarz_nnz(const hoefem_schwarz* s) { return s ? s->s.P.nnz() : 0; }
+
+hoefem_status hoefem_schwarz_pattern(const hoefem_schwarz* s, int32_t* rowptr, int32_t* col) {
+    return guarded([&] {
+        if (!s || !rowptr || !col) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        std::copy(s->s.P.rowptr.begin(), s->s.P.rowptr.end(), rowptr);
+        std::copy(s->s.P.col.begin(), s->s.P.col.end(), col);
+    });
+}
+
+hoefem_status hoefem_schwarz_update(hoefem_schwarz* s, const double* J) {
+    return guarded([&] {
+        if (!s) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null preconditioner");
+        schwarz_update(s->s, J);
+        raise_kernel_errors(*s->s.model);
+    });
+}
+
+hoefem_status hoefem_schwarz_values(hoefem_schwarz* s, double* values) {
+    return guarded([&] {
+        if (!s) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null preconditioner");
+        HIP_CHECK(hipMemcpyAsync(values, s->s.d_vals.get(), sizeof(double) * s->s.P.nnz(), hipMemcpyDeviceToDevice,
+                                 s->s.model->stream));
+        HIP_CHECK(hipStreamSynchronize(s->s.model->stream));
+    });
+}
+
+hoefem_status hoefem_schwarz_apply(hoefem_schwarz* s, const double* x, double* y) {
+    return guarded([&] {
+        if (!s) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null preconditioner");
+        schwarz_apply(s->s, x, y);
+    });
+}
+
+hoefem_status hoefem_pcg(hoefem_model* model, const double* A, hoefem_schwarz* schwarz, const double* b, double* x,
+                         double tolerance, int64_t max_iterations, int64_t* iterations) {
+    hoefem_status st = HOEFEM_OK;
+    int64_t it = 0;
+    const hoefem_status g = guarded([&] {
+        Model& m = M_(model);
+        static thread_local CgWork* work = nullptr;
+        if (!work) work = new CgWork();
+        st = pcg_solve(m, *work, A, schwarz ? &schwarz->s : nullptr, b, x, tolerance, max_iterations, &it);
+    });
+    if (iterations) *iterations = it;
+    if (g != HOEFEM_OK) return g;
+    if (st != HOEFEM_OK) g_last_error = "CG stopped with status " + std::to_string((int)st);
+    return st;
+}
+
+hoefem_status hoefem_sim_create(hoefem_model* model, const hoefem_sim_desc* desc, hoefem_sim** out) {
+    return guarded([&] {
+        Model& m = M_(model);
+        if (!desc || !out) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        if (desc->num_dirichlet > 0 && !desc->dirichlet_nodes) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null Dirichlet nodes");
+        std::unique_ptr<hoefem_sim> h(new hoefem_sim());
+        sim_create(h->s, m, desc);
+        *out = h.release();
+    });
+}
+
+void hoefem_sim_destroy(hoefem_sim* sim) { delete sim; }
+
+double hoefem_sim_representative_force(const hoefem_sim* sim) { return sim ? sim->s.rep_force : 0.0; }
+
+hoefem_status hoefem_sim_mass_values(hoefem_sim* sim, double* values) {
+    return guarded([&] {
+        if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+        Model& m = *sim->s.model;
+        HIP_CHECK(hipMemcpyAsync(values, sim->s.d_M.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice, m.stream));
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+    });
+}
+
+hoefem_status hoefem_sim_jacobian_values(hoefem_sim* sim, double* values) {
+    return guarded([&] {
+        if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+        Model& m = *sim->s.model;
+        HIP_CHECK(hipMemcpyAsync(values, sim->s.d_J.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice, m.stream));
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+    });
+}
+
+hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const double* u, const double* v,
+                                  const double* w, double* F) {
+    return guarded([&] {
+        if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+        (void)t;
+        Model& m = *sim->s.model;
+        launch_be_begin(m, u, v, dt, sim->s.d_up.get(), sim->s.d_vn.get());
+        const double n2 = sim_eval(sim->s, dt, w);
+        if (std::isnan(n2)) throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient");
+        HIP_CHECK(hipMemcpyAsync(F, sim->s.d_F.get(), sizeof(double) * 3 * m.nv, hipMemcpyDeviceToDevice, m.stream));
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+    });
+}
+
+hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, double* v, hoefem_step_stats* stats) {
+    return guarded([&] {
+        if (!sim || !u || !v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        sim_step(sim->s, t, dt, u, v, stats);
+    });
+}
+
+}  // extern "C"

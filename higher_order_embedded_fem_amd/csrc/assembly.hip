@@ -1,0 +1,596 @@
+// Assembly kernels for gfx950: element forces, element tangents and element mass into
+// staging buffers, then color-ordered, atomic-free gathers into node-owned CSR row blocks.
+//
+// Summation order matches the reference's colored assembly (fenris/src/assembly.rs:253-313,
+// :915-966): every CSR entry / force component receives its element contributions in color
+// order, because the incidence lists are built color by color and one work-group owns each
+// node's row block.
+#include <algorithm>
+
+#include "fem_device.hpp"
+#include "internal.hpp"
+
+namespace hoefem {
+
+using namespace dev;
+
+// ------------------------------------------------------------------ element forces
+// assemble_generalized_element_elliptic_term (fenris/src/assembly.rs:674-711) with
+// g = -P(F)^T (fenris/src/solid/assembly.rs:33-38). One 64-lane work-group per element:
+// lanes = quadrature points (phase 1), then lanes = element nodes (phase 2).
+template <int ET>
+__global__ __launch_bounds__(64) void k_element_forces(int64_t ne, const int32_t* __restrict__ conn,
+                                                       const double* __restrict__ verts,
+                                                       const int32_t* __restrict__ qoff,
+                                                       const double* __restrict__ qw,
+                                                       const double* __restrict__ qxi, double mu, double lambda,
+                                                       const double* __restrict__ u, double* __restrict__ fe,
+                                                       int* err) {
+    constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN;
+    const int64_t e = blockIdx.x;
+    const int t = threadIdx.x;
+    __shared__ int s_node[N];
+    __shared__ double s_u[3 * N];
+    __shared__ double s_X[3 * GN];
+    __shared__ double s_A[64][12];
+    if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
+    __syncthreads();
+    for (int k = t; k < 3 * N; k += 64) s_u[k] = u[3 * (int64_t)s_node[k / 3] + k % 3];
+    for (int k = t; k < 3 * GN; k += 64) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
+    __syncthreads();
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    const int q0 = qoff[e], q1 = qoff[e + 1];
+    for (int qb = q0; qb < q1; qb += 64) {
+        const int q = qb + t;
+        if (q < q1) {
+            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
+            double J[9], Jinv[9];
+            ref_jacobian<ET>(s_X, xi, J);
+            const double detJ = det3(J);
+            bool ok = inv3(J, Jinv);
+            double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            double psi[4];
+            tet4_psi(xi, psi);
+            if (ok) {
+                // mm(b, a) = sum_I u_I[b] * (G_ref^T J^-1)(I, a)
+                for (int I = 0; I < N; ++I) {
+                    double G[3];
+                    grad_node<ET>(I, xi, psi, G);
+                    double tmp[3];
+                    for (int a = 0; a < 3; ++a) tmp[a] = G[0] * Jinv[0 + 3 * a] + G[1] * Jinv[1 + 3 * a] + G[2] * Jinv[2 + 3 * a];
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) F[b + 3 * a] += s_u[3 * I + b] * tmp[a];
+                }
+                F[0] += 1.0;
+                F[4] += 1.0;
+                F[8] += 1.0;
+                double P[9];
+                ok = snh_stress(mu, lambda, F, P);
+                if (ok) {
+                    // A = (-P) * J^-T  : A(i, a) = sum_c -P(i, c) * Jinv(a, c)
+                    for (int a = 0; a < 3; ++a)
+                        for (int i = 0; i < 3; ++i)
+                            s_A[t][i + 3 * a] = (-P[i]) * Jinv[a] + (-P[i + 3]) * Jinv[a + 3] + (-P[i + 6]) * Jinv[a + 6];
+                    s_A[t][9] = qw[q];
+                    s_A[t][10] = fabs(detJ);
+                }
+            }
+            if (!ok) {
+                atomicOr(err, KERR_SINGULAR);
+                for (int k = 0; k < 11; ++k) s_A[t][k] = 0.0;
+            }
+        }
+        __syncthreads();
+        if (t < N) {
+            const int cnt = min(64, q1 - qb);
+            for (int k = 0; k < cnt; ++k) {
+                const int q2 = qb + k;
+                const double xi[3] = {qxi[3 * q2], qxi[3 * q2 + 1], qxi[3 * q2 + 2]};
+                double psi[4], G[3];
+                tet4_psi(xi, psi);
+                grad_node<ET>(t, xi, psi, G);
+                const double* A = s_A[k];
+                const double w = A[9], dj = A[10];
+                f0 += (A[0] * G[0] + A[3] * G[1] + A[6] * G[2]) * w * dj;
+                f1 += (A[1] * G[0] + A[4] * G[1] + A[7] * G[2]) * w * dj;
+                f2 += (A[2] * G[0] + A[5] * G[1] + A[8] * G[2]) * w * dj;
+            }
+        }
+        __syncthreads();
+    }
+    if (t < N) {
+        double* out = fe + e * 3 * N + 3 * t;
+        out[0] = f0;
+        out[1] = f1;
+        out[2] = f2;
+    }
+}
+
+// ------------------------------------------------------------------ element tangents
+// assemble_generalized_element_stiffness (fenris/src/assembly.rs:747-797) with the SNH
+// contraction (fenris/src/solid/materials.rs:349-376, :690-743). One 256-lane work-group per
+// element; lanes = quadrature points build dP/dF and the scaled gradients in LDS, then every
+// lane accumulates a fixed set of K_e entries (lower block triangle, mirrored on write).
+// Output: row-major (3N)^2 block per element in the staging buffer.
+constexpr int STIFF_THREADS = 256;
+constexpr int STIFF_QC = 16;
+
+template <int ET>
+__global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
+    int64_t ne, const int32_t* __restrict__ conn, const double* __restrict__ verts, const int32_t* __restrict__ qoff,
+    const double* __restrict__ qw, const double* __restrict__ qxi, double mu, double lambda,
+    const double* __restrict__ u, double* __restrict__ ke, int* err) {
+    constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN, M3 = 3 * N;
+    constexpr int PAIRS = N * (N + 1) / 2;
+    constexpr int ENTRIES = 9 * PAIRS;
+    constexpr int PER = (ENTRIES + STIFF_THREADS - 1) / STIFF_THREADS;
+    const int64_t e = blockIdx.x;
+    const int t = threadIdx.x;
+    __shared__ int s_node[N];
+    __shared__ double s_u[3 * N];
+    __shared__ double s_X[3 * GN];
+    __shared__ double s_C[STIFF_QC][81];
+    __shared__ double s_a[STIFF_QC][M3];
+    __shared__ unsigned char s_pI[PAIRS], s_pJ[PAIRS];
+    if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
+    if (t == 0) {
+        int p = 0;
+        for (int J = 0; J < N; ++J)
+            for (int I = J; I < N; ++I) {
+                s_pI[p] = (unsigned char)I;
+                s_pJ[p] = (unsigned char)J;
+                ++p;
+            }
+    }
+    __syncthreads();
+    for (int k = t; k < 3 * N; k += STIFF_THREADS) s_u[k] = u[3 * (int64_t)s_node[k / 3] + k % 3];
+    for (int k = t; k < 3 * GN; k += STIFF_THREADS) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
+    __syncthreads();
+    double acc[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) acc[j] = 0.0;
+    const int q0 = qoff[e], q1 = qoff[e + 1];
+    for (int qb = q0; qb < q1; qb += STIFF_QC) {
+        const int cnt = min(STIFF_QC, q1 - qb);
+        if (t < cnt) {
+            const int q = qb + t;
+            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
+            double J[9], Jinv[9];
+            ref_jacobian<ET>(s_X, xi, J);
+            const double detJ = det3(J);
+            bool ok = inv3(J, Jinv);
+            double psi[4];
+            tet4_psi(xi, psi);
+            double ug[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // u_grad(a, b) = sum_I G(a, I) u(b, I)
+            if (ok) {
+                for (int I = 0; I < N; ++I) {
+                    double Gr[3];
+                    grad_node<ET>(I, xi, psi, Gr);
+                    double G[3];
+                    // G = J^-T G_ref : G(a) = sum_c Jinv(c, a) G_ref(c)
+                    for (int a = 0; a < 3; ++a) G[a] = Jinv[0 + 3 * a] * Gr[0] + Jinv[1 + 3 * a] * Gr[1] + Jinv[2 + 3 * a] * Gr[2];
+                    for (int b = 0; b < 3; ++b)
+                        for (int a = 0; a < 3; ++a) ug[a + 3 * b] += G[a] * s_u[3 * I + b];
+                    for (int a = 0; a < 3; ++a) s_a[t][3 * I + a] = G[a];
+                }
+                const double sq = sqrt(qw[q] * fabs(detJ));
+                for (int k = 0; k < M3; ++k) s_a[t][k] *= sq;
+                double F[9];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) F[b + 3 * a] = ug[a + 3 * b];
+                F[0] += 1.0;
+                F[4] += 1.0;
+                F[8] += 1.0;
+                ok = snh_dpdf(mu, lambda, F, s_C[t]);
+            }
+            if (!ok) {
+                atomicOr(err, KERR_SINGULAR);
+                for (int k = 0; k < 81; ++k) s_C[t][k] = 0.0;
+                for (int k = 0; k < M3; ++k) s_a[t][k] = 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int ent = t + STIFF_THREADS * j;
+            if (ent < ENTRIES) {
+                const int p = ent / 9, ij = ent % 9;
+                const int i = ij % 3, jj = ij / 3;
+                const int I = s_pI[p], Jn = s_pJ[p];
+                for (int k2 = 0; k2 < cnt; ++k2) {
+                    const double* C = s_C[k2];
+                    const double* aI = s_a[k2] + 3 * I;
+                    const double* aJ = s_a[k2] + 3 * Jn;
+                    double res = 0.0;
+                    for (int l = 0; l < 3; ++l)
+                        for (int k = 0; k < 3; ++k) res += C[(3 * k + i) + 9 * (3 * l + jj)] * aI[k] * aJ[l];
+                    acc[j] += res;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* out = ke + e * (int64_t)M3 * M3;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int ent = t + STIFF_THREADS * j;
+        if (ent < ENTRIES) {
+            const int p = ent / 9, ij = ent % 9;
+            const int i = ij % 3, jj = ij / 3;
+            const int I = s_pI[p], Jn = s_pJ[p];
+            out[(3 * I + i) * M3 + 3 * Jn + jj] = acc[j];
+            if (I != Jn) out[(3 * Jn + jj) * M3 + 3 * I + i] = acc[j];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ element mass
+// assemble_generalized_element_mass (fenris/src/assembly.rs:624-671): the scalar matrix
+// rho * sum_q |det J| w phi_i phi_j duplicated on the diagonal of each 3x3 block. With a
+// regularization factor > 0 the interior rule is added with density * factor
+// (fenris/src/embedding/mod.rs:438-451).
+template <int ET>
+__global__ __launch_bounds__(256) void k_element_mass(int64_t ne, const int32_t* __restrict__ conn,
+                                                      const double* __restrict__ verts,
+                                                      const int32_t* __restrict__ qoff, const double* __restrict__ qw,
+                                                      const double* __restrict__ qxi, double density,
+                                                      int nreg, const double* __restrict__ rw,
+                                                      const double* __restrict__ rxi, double reg_density,
+                                                      double* __restrict__ ke) {
+    constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN, M3 = 3 * N;
+    const int64_t e = blockIdx.x;
+    const int t = threadIdx.x;
+    __shared__ int s_node[N];
+    __shared__ double s_X[3 * GN];
+    if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
+    __syncthreads();
+    for (int k = t; k < 3 * GN; k += 256) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
+    __syncthreads();
+    double* out = ke + e * (int64_t)M3 * M3;
+    for (int ent = t; ent < N * N; ent += 256) {
+        const int i = ent / N, j = ent % N;
+        double m = 0.0;
+        for (int q = qoff[e]; q < qoff[e + 1]; ++q) {
+            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
+            double J[9], psi[4];
+            ref_jacobian<ET>(s_X, xi, J);
+            tet4_psi(xi, psi);
+            m += density * fabs(det3(J)) * qw[q] * basis_node<ET>(i, xi, psi) * basis_node<ET>(j, xi, psi);
+        }
+        for (int q = 0; q < nreg; ++q) {
+            const double xi[3] = {rxi[3 * q], rxi[3 * q + 1], rxi[3 * q + 2]};
+            double J[9], psi[4];
+            ref_jacobian<ET>(s_X, xi, J);
+            tet4_psi(xi, psi);
+            m += reg_density * fabs(det3(J)) * rw[q] * basis_node<ET>(i, xi, psi) * basis_node<ET>(j, xi, psi);
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) out[(3 * i + a) * M3 + 3 * j + b] = (a == b) ? m : 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ block gather
+// One 64-lane work-group owns the row block (3 rows) of pattern node v and accumulates the
+// incident element blocks in color order in LDS (add_element_row_to_csr_row semantics,
+// fenris/src/assembly.rs:1042-1075, without the linear search: slots are precomputed).
+// mode 0: values = (values or 0) + sum; mode 1: values = (0 + c * sum) + 1 * M (J combination,
+// simulation_toolbox/src/fem/system_assembly.rs:196-235).
+__global__ __launch_bounds__(64) void k_block_gather(
+    const int32_t* __restrict__ nptr, const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ diag_only,
+    const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk, const int32_t* __restrict__ inc_loc,
+    const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off, const int64_t* __restrict__ slot_off,
+    const int32_t* __restrict__ slot, const double* __restrict__ blocks, double* __restrict__ values, int mode,
+    double c, const double* __restrict__ M, int zero_first) {
+    extern __shared__ double s_rb[];
+    const int64_t v = blockIdx.x;
+    const int t = threadIdx.x;
+    if (diag_only[v]) return;
+    const int k = nptr[v + 1] - nptr[v];
+    if (k == 0) return;
+    const int rw = 3 * k;
+    const int R = 9 * k;
+    const int64_t r0 = rowptr[3 * v];
+    // rows 3v, 3v+1, 3v+2 are contiguous: row i starts at r0 + i * rw
+    for (int idx = t; idx < R; idx += 64) s_rb[idx] = (zero_first || mode == 1) ? 0.0 : values[r0 + idx];
+    __syncthreads();
+    for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
+        const int b = inc_blk[it], a = inc_loc[it];
+        const int mb = msz[b];
+        const int bw = 3 * mb;
+        const double* base = blocks + val_off[b] + (int64_t)(3 * a) * bw;
+        const int32_t* sl = slot + slot_off[b] + (int64_t)a * mb;
+        for (int idx = t; idx < 3 * bw; idx += 64) {
+            const int i = idx / bw, cc = idx % bw;
+            const int bb = cc / 3, jj = cc % 3;
+            s_rb[i * rw + 3 * sl[bb] + jj] += base[idx];
+        }
+        __syncthreads();
+    }
+    for (int idx = t; idx < R; idx += 64) {
+        const double s = s_rb[idx];
+        if (mode == 0)
+            values[r0 + idx] = s;
+        else {
+            double v2 = 0.0;
+            v2 += c * s;
+            v2 += 1.0 * M[r0 + idx];
+            values[r0 + idx] = v2;
+        }
+    }
+}
+
+// f += sum of element force blocks in color order (one thread per node)
+__global__ void k_force_gather(int64_t nn, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk,
+                               const int32_t* __restrict__ inc_loc, int n, const double* __restrict__ fe,
+                               double* __restrict__ f) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    double a0 = f[3 * v], a1 = f[3 * v + 1], a2 = f[3 * v + 2];
+    for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
+        const double* src = fe + ((int64_t)inc_blk[it] * n + inc_loc[it]) * 3;
+        a0 += src[0];
+        a1 += src[1];
+        a2 += src[2];
+    }
+    f[3 * v] = a0;
+    f[3 * v + 1] = a1;
+    f[3 * v + 2] = a2;
+}
+
+// ------------------------------------------------------------------ SpMV
+// y = beta*y + alpha*A x on a node-blocked CSR matrix: G lanes per node share the node's
+// column list; each lane reads 3 contiguous values of each of the node's 3 rows.
+template <int G>
+__global__ __launch_bounds__(256) void k_spmv(int64_t nn, const int32_t* __restrict__ nptr,
+                                              const int32_t* __restrict__ ncol, const int32_t* __restrict__ rowptr,
+                                              const uint8_t* __restrict__ diag_only, const double* __restrict__ A,
+                                              double beta, double* __restrict__ y, double alpha,
+                                              const double* __restrict__ x) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const bool valid = v < nn;
+    if (valid) {
+        if (diag_only[v]) {
+            if (lane == 0) {
+                a0 = A[rowptr[3 * v]] * x[3 * v];
+                a1 = A[rowptr[3 * v + 1]] * x[3 * v + 1];
+                a2 = A[rowptr[3 * v + 2]] * x[3 * v + 2];
+            }
+        } else {
+            const int kb = nptr[v], k = nptr[v + 1] - kb;
+            const double* R0 = A + rowptr[3 * v];
+            const double* R1 = R0 + 3 * k;
+            const double* R2 = R1 + 3 * k;
+            for (int j = lane; j < k; j += G) {
+                const int64_t b = ncol[kb + j];
+                const double x0 = x[3 * b], x1 = x[3 * b + 1], x2 = x[3 * b + 2];
+                a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
+                a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
+                a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, G);
+        a1 += __shfl_xor(a1, off, G);
+        a2 += __shfl_xor(a2, off, G);
+    }
+    if (valid && lane == 0) {
+        if (beta == 0.0) {
+            y[3 * v] = alpha * a0;
+            y[3 * v + 1] = alpha * a1;
+            y[3 * v + 2] = alpha * a2;
+        } else {
+            double y0 = beta * y[3 * v], y1 = beta * y[3 * v + 1], y2 = beta * y[3 * v + 2];
+            y0 += alpha * a0;
+            y1 += alpha * a1;
+            y2 += alpha * a2;
+            y[3 * v] = y0;
+            y[3 * v + 1] = y1;
+            y[3 * v + 2] = y2;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Dirichlet
+// apply_homogeneous_dirichlet_bc_csr (fenris/src/assembly.rs:799-869): scale = |first non-zero
+// diagonal entry|; Dirichlet rows and (by symmetry of the pattern) Dirichlet columns zeroed,
+// Dirichlet diagonal := scale.
+__global__ __launch_bounds__(256) void k_first_nonzero_diag(int64_t ndof, const int32_t* __restrict__ diag_pos,
+                                                            const double* __restrict__ vals, double* scale) {
+    __shared__ int64_t s_min[256];
+    for (int64_t base = 0; base < ndof; base += 256) {
+        const int64_t r = base + threadIdx.x;
+        int64_t found = INT64_MAX;
+        if (r < ndof && diag_pos[r] >= 0 && vals[diag_pos[r]] != 0.0) found = r;
+        s_min[threadIdx.x] = found;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) s_min[threadIdx.x] = min(s_min[threadIdx.x], s_min[threadIdx.x + s]);
+            __syncthreads();
+        }
+        const int64_t m = s_min[0];
+        __syncthreads();
+        if (m != INT64_MAX) {
+            if (threadIdx.x == 0) *scale = fabs(vals[diag_pos[m]]);
+            return;
+        }
+    }
+    if (threadIdx.x == 0) *scale = 1.0;
+}
+
+__global__ void k_dirichlet_rows(const int32_t* __restrict__ bc_nodes, int64_t nbc, const int32_t* __restrict__ nptr,
+                                 const int32_t* __restrict__ ncol, const int32_t* __restrict__ rowptr,
+                                 const uint8_t* __restrict__ diag_only, const uint8_t* __restrict__ dirmask,
+                                 double* __restrict__ vals, const double* __restrict__ scale) {
+    const int64_t idx = blockIdx.x;
+    if (idx >= nbc) return;
+    const int64_t v = bc_nodes[idx];
+    const bool vdir = dirmask[v] != 0;
+    const double sc = *scale;
+    if (diag_only[v]) {
+        if (vdir && threadIdx.x < 3) vals[rowptr[3 * v + threadIdx.x]] = sc;
+        return;
+    }
+    const int kb = nptr[v], k = nptr[v + 1] - kb;
+    for (int ent = threadIdx.x; ent < 9 * k; ent += blockDim.x) {
+        const int i = ent / (3 * k), cc = ent % (3 * k);
+        const int j = cc / 3, jj = cc % 3;
+        const int64_t b = ncol[kb + j];
+        const int64_t pos = rowptr[3 * v + i] + cc;
+        if (vdir)
+            vals[pos] = (b == v && jj == i) ? sc : 0.0;
+        else if (dirmask[b])
+            vals[pos] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ material (batched)
+__global__ void k_material(double mu, double lambda, int64_t nq, const double* __restrict__ F, double* __restrict__ P,
+                           double* __restrict__ dP, int* err) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    double f[9];
+    for (int k = 0; k < 9; ++k) f[k] = F[9 * q + k];
+    if (P) {
+        double p[9];
+        if (!snh_stress(mu, lambda, f, p)) {
+            atomicOr(err, KERR_SINGULAR);
+            for (int k = 0; k < 9; ++k) p[k] = 0.0;
+        }
+        for (int k = 0; k < 9; ++k) P[9 * q + k] = p[k];
+    }
+    if (dP) {
+        double d[81];
+        if (!snh_dpdf(mu, lambda, f, d)) {
+            atomicOr(err, KERR_SINGULAR);
+            for (int k = 0; k < 81; ++k) d[k] = 0.0;
+        }
+        for (int k = 0; k < 81; ++k) dP[81 * q + k] = d[k];
+    }
+}
+
+// ================================================================== launchers
+#define HOEFEM_DISPATCH(type, ...)                                                 \
+    switch (type) {                                                                \
+        case TET4: { constexpr int ET = 0; __VA_ARGS__; } break;                   \
+        case TET10: { constexpr int ET = 1; __VA_ARGS__; } break;                  \
+        case TET20: { constexpr int ET = 2; __VA_ARGS__; } break;                  \
+        case HEX8: { constexpr int ET = 3; __VA_ARGS__; } break;                   \
+        case HEX20: { constexpr int ET = 4; __VA_ARGS__; } break;                  \
+        case HEX27: { constexpr int ET = 5; __VA_ARGS__; } break;                  \
+        default: throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad element type");  \
+    }
+
+void launch_element_forces(Model& m, double mu_r, double lambda_r, const double* u) {
+    m.d_fe.ensure((size_t)m.ne * 3 * m.n);
+    if (m.ne == 0) return;
+    HOEFEM_DISPATCH(m.type, k_element_forces<ET><<<dim3((unsigned)m.ne), dim3(64), 0, m.stream>>>(
+                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_ell.off.get(), m.q_ell.w.get(),
+                                m.q_ell.xi.get(), mu_r, lambda_r, u, m.d_fe.get(), m.d_err.get()));
+    check_launch("k_element_forces");
+}
+
+void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const double* u) {
+    m.d_ke.ensure((size_t)m.ne * 9 * m.n * m.n);
+    if (m.ne == 0) return;
+    HOEFEM_DISPATCH(m.type, k_element_stiffness<ET><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
+                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
+                                m.q_stiff.xi.get(), mu_r, lambda_r, u, m.d_ke.get(), m.d_err.get()));
+    check_launch("k_element_stiffness");
+}
+
+void launch_element_mass(Model& m, double density) {
+    m.d_ke.ensure((size_t)m.ne * 9 * m.n * m.n);
+    if (m.ne == 0) return;
+    const int nreg = m.mass_reg > 0.0 ? (int)m.q_interior_mass.max_points : 0;
+    HOEFEM_DISPATCH(m.type, k_element_mass<ET><<<dim3((unsigned)m.ne), dim3(256), 0, m.stream>>>(
+                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_mass.off.get(), m.q_mass.w.get(),
+                                m.q_mass.xi.get(), density, nreg, m.q_interior_mass.w.get(),
+                                m.q_interior_mass.xi.get(), m.mass_reg * density, m.d_ke.get()));
+    check_launch("k_element_mass");
+}
+
+void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const double* blocks, double* values,
+                         int mode, double c, const double* M, bool zero_first, hipStream_t s) {
+    if (pat.num_nodes == 0) return;
+    const size_t lds = (size_t)9 * std::max(pat.max_nbr, 1) * sizeof(double);
+    if (lds > 64 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "node row block exceeds 64 KiB of LDS");
+    hipLaunchKernelGGL(k_block_gather, dim3((unsigned)pat.num_nodes), dim3(64), lds, s, pat.d_nptr.get(),
+                       pat.d_rowptr.get(), pat.d_diag_only.get(), sc.d_inc_ptr.get(), sc.d_inc_blk.get(),
+                       sc.d_inc_loc.get(), sc.d_m.get(), sc.d_val_off.get(), sc.d_slot_off.get(), sc.d_slot.get(),
+                       blocks, values, mode, c, M, zero_first ? 1 : 0);
+    check_launch("k_block_gather");
+}
+
+void launch_force_gather(Model& m, double* f) {
+    if (m.nv == 0) return;
+    hipLaunchKernelGGL(k_force_gather, dim3((unsigned)((m.nv + 255) / 256)), dim3(256), 0, m.stream, m.nv,
+                       m.scatter.d_inc_ptr.get(), m.scatter.d_inc_blk.get(), m.scatter.d_inc_loc.get(), m.n,
+                       m.d_fe.get(), f);
+    check_launch("k_force_gather");
+}
+
+int spmv_group_size(const NodePattern& pat) {
+    const double avg = pat.num_nodes ? (double)pat.ncol.size() / (double)pat.num_nodes : 1.0;
+    if (avg >= 40.0) return 32;
+    if (avg >= 12.0) return 16;
+    return 8;
+}
+
+void launch_spmv(const NodePattern& pat, const double* vals, double beta, double* y, double alpha, const double* x,
+                 hipStream_t s) {
+    if (pat.num_nodes == 0) return;
+    const int G = spmv_group_size(pat);
+    const int64_t threads = pat.num_nodes * G;
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+#define SPMV_CASE(GG)                                                                                          \
+    hipLaunchKernelGGL(k_spmv<GG>, dim3(blocks), dim3(256), 0, s, pat.num_nodes, pat.d_nptr.get(),            \
+                       pat.d_ncol.get(), pat.d_rowptr.get(), pat.d_diag_only.get(), vals, beta, y, alpha, x)
+    if (G == 32)
+        SPMV_CASE(32);
+    else if (G == 16)
+        SPMV_CASE(16);
+    else
+        SPMV_CASE(8);
+#undef SPMV_CASE
+    check_launch("k_spmv");
+}
+
+void launch_dirichlet(const NodePattern& pat, double* vals, const uint8_t* d_dirmask, const int32_t* d_bc_nodes,
+                      int64_t n_bc_nodes, double* d_scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_first_nonzero_diag, dim3(1), dim3(256), 0, s, pat.ndof(), pat.d_diag_pos.get(), vals, d_scale);
+    check_launch("k_first_nonzero_diag");
+    if (n_bc_nodes == 0) return;
+    hipLaunchKernelGGL(k_dirichlet_rows, dim3((unsigned)n_bc_nodes), dim3(64), 0, s, d_bc_nodes, n_bc_nodes,
+                       pat.d_nptr.get(), pat.d_ncol.get(), pat.d_rowptr.get(), pat.d_diag_only.get(), d_dirmask, vals,
+                       d_scale);
+    check_launch("k_dirichlet_rows");
+}
+
+void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF) {
+    DevBuf<int> err;
+    err.alloc(1);
+    HIP_CHECK(hipMemset(err.get(), 0, sizeof(int)));
+    if (nq > 0) {
+        hipLaunchKernelGGL(k_material, dim3((unsigned)((nq + 127) / 128)), dim3(128), 0, nullptr, mu_r, lambda_r, nq, F,
+                           P, dPdF, err.get());
+        check_launch("k_material");
+    }
+    int h = 0;
+    HIP_CHECK(hipMemcpy(&h, err.get(), sizeof(int), hipMemcpyDeviceToHost));
+    if (h & KERR_SINGULAR) throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient");
+}
+
+int read_and_clear_err(Model& m) {
+    HIP_CHECK(hipMemcpyAsync(m.h_err.p, m.d_err.get(), sizeof(int), hipMemcpyDeviceToHost, m.stream));
+    HIP_CHECK(hipStreamSynchronize(m.stream));
+    const int e = *m.h_err.p;
+    if (e) HIP_CHECK(hipMemsetAsync(m.d_err.get(), 0, sizeof(int), m.stream));
+    return e;
+}
+
+}  // namespace hoefem

@@ -1,0 +1,216 @@
+// Internal declarations of the hoefem HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hoefem.h"
+
+namespace hoefem {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+    hoefem_status code;
+    Error(hoefem_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                                    \
+    do {                                                                                                   \
+        hipError_t _e = (expr);                                                                            \
+        if (_e != hipSuccess)                                                                              \
+            throw ::hoefem::Error(HOEFEM_ERROR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+void check_launch(const char* what);
+
+// ------------------------------------------------------------------ buffers
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void ensure(size_t count) {
+        if (count > n || p == nullptr) alloc(count ? count : 1);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        ensure(count);
+        if (count) HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    T* get() const { return p; }
+};
+
+template <class T>
+struct PinnedBuf {
+    T* p = nullptr;
+    explicit PinnedBuf(size_t count = 1) {
+        HIP_CHECK(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+    }
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+};
+
+// ------------------------------------------------------------------ element types
+enum : int { TET4 = 0, TET10 = 1, TET20 = 2, HEX8 = 3, HEX20 = 4, HEX27 = 5 };
+int nodes_per_element(int t);
+int geometry_nodes(int t);
+
+// error flags raised by kernels (bitwise OR into a device int)
+enum : int { KERR_SINGULAR = 1, KERR_PATTERN = 2, KERR_NONFINITE = 4, KERR_ZERO_DIAG = 8 };
+
+// ------------------------------------------------------------------ node-blocked CSR
+// A CSR pattern whose rows come in xyz triples sharing one node-level column set (element
+// node pairs x 3x3 blocks, fenris/src/assembly.rs:177-249). "Diagonal-only" nodes carry a
+// single diagonal entry per row (untreated Schwarz nodes, schwarz_precond.rs:150-158).
+struct NodePattern {
+    int64_t num_nodes = 0;
+    std::vector<int32_t> nptr, ncol;  // node-level neighbour lists, sorted
+    std::vector<int32_t> rowptr, col; // DOF-level CSR
+    std::vector<int32_t> diag_pos;    // position of each row's diagonal entry
+    std::vector<uint8_t> diag_only;   // per node
+    int max_nbr = 0;
+    DevBuf<int32_t> d_nptr, d_ncol, d_rowptr, d_col, d_diag_pos;
+    DevBuf<uint8_t> d_diag_only;
+    int64_t nnz() const { return (int64_t)col.size(); }
+    int64_t ndof() const { return 3 * num_nodes; }
+    int32_t find(int32_t a, int32_t b) const;  // local index of b in nbr(a), or -1
+    void upload(hipStream_t s);
+};
+
+// Build the pattern of the union of node sets (each set couples all its node pairs) plus
+// diagonal-only nodes.
+void build_node_pattern(int64_t num_nodes, int64_t nsets, const int64_t* off, const int32_t* nodes,
+                        const std::vector<int32_t>& diag_only_nodes, NodePattern& out);
+
+// First-fit greedy coloring of node sets (paradis/src/coloring.rs:48-72), labels in order.
+std::vector<std::vector<int32_t>> greedy_coloring(int64_t nsets, const int64_t* off, const int32_t* nodes);
+
+// Color-ordered incidence of dense blocks ("element matrices") into a node pattern; shared
+// by the stiffness / mass assembly and the Schwarz preconditioner assembly.
+struct BlockScatter {
+    int64_t nblocks = 0;
+    int max_m = 0;
+    std::vector<int32_t> m;         // nodes per block
+    std::vector<int64_t> val_off;   // offset of block b's (3m)^2 dense values in the staging
+    std::vector<int64_t> slot_off;  // offset of block b's m*m slot table
+    std::vector<int32_t> slot;      // slot[slot_off[b] + a*m + c] = local index of node c in nbr(node a)
+    std::vector<int32_t> inc_ptr;   // per pattern node
+    std::vector<int32_t> inc_blk, inc_loc;  // incident (block, local node) in color order
+    DevBuf<int32_t> d_m, d_slot, d_inc_ptr, d_inc_blk, d_inc_loc;
+    DevBuf<int64_t> d_val_off, d_slot_off;
+    void build(const NodePattern& pat, int64_t nblocks, const int64_t* off, const int32_t* nodes,
+               const std::vector<std::vector<int32_t>>* colors, int ncolor_lists);
+    void upload(hipStream_t s);
+};
+
+// ------------------------------------------------------------------ model
+struct DeviceQuadrature {
+    DevBuf<int32_t> off;  // [E + 1]
+    DevBuf<double> w;     // [Q]
+    DevBuf<double> xi;    // [Q * 3]
+    int max_points = 0;
+};
+
+struct Model {
+    int type = 0, n = 0, gn = 0;
+    int64_t nv = 0, ne = 0, ninterior = 0;
+    std::vector<double> vertices;
+    std::vector<int32_t> conn;                    // element-major [E * n]
+    std::vector<std::vector<int32_t>> colors[2];  // pass -> color -> element ids
+    NodePattern pattern;
+    BlockScatter scatter;                         // element blocks -> pattern, color order
+    double mass_reg = 0.0;
+    hipStream_t stream = nullptr;
+    DevBuf<double> d_vertices;
+    DevBuf<int32_t> d_conn;  // SoA: [k * E + e]
+    DeviceQuadrature q_mass, q_stiff, q_ell, q_interior_mass;
+    DevBuf<double> d_fe;  // element force staging [E * 3n]
+    DevBuf<double> d_ke;  // element matrix staging [E * (3n)^2]
+    DevBuf<int> d_err;
+    PinnedBuf<int> h_err{1};
+};
+
+// ------------------------------------------------------------------ Schwarz
+struct Schwarz {
+    Model* model = nullptr;
+    int64_t nsub = 0;
+    std::vector<int64_t> off;
+    std::vector<int32_t> nodes, untreated;
+    std::vector<std::vector<int32_t>> colors;
+    NodePattern P;
+    BlockScatter scatter;          // blocks -> P pattern
+    std::vector<int32_t> jslot;    // per block entry (a, c): local index of node c in J's nbr(node a)
+    DevBuf<int32_t> d_jslot, d_untreated, d_nodes;
+    DevBuf<int64_t> d_off;
+    DevBuf<double> d_vals;         // P values
+    DevBuf<double> d_blocks;       // gathered / inverted blocks
+};
+
+// ------------------------------------------------------------------ CG workspace
+struct CgScalars {
+    double zTr, pAp, alpha, beta, rr, bnorm, tol;
+    int64_t iters, max_iter;
+    int status;  // 0 running, 1 converged, hoefem_status codes for errors
+    int pad;
+};
+
+struct CgWork {
+    int64_t n = 0;
+    DevBuf<double> r[2], p[2], z, Ap, diag_inv;
+    DevBuf<double> partials;
+    DevBuf<unsigned> counter;
+    DevBuf<CgScalars> scalars;
+    PinnedBuf<CgScalars> h_scalars{3};  // two polling slots + final readback
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void ensure(int64_t n, int nblocks);
+    ~CgWork();
+};
+
+// ------------------------------------------------------------------ kernels (assembly.hip)
+void launch_element_forces(Model& m, double mu_r, double lambda_r, const double* u);  // -> m.d_fe
+void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const double* u);  // -> m.d_ke
+void launch_element_mass(Model& m, double density);  // -> m.d_ke (includes regularization)
+// values (+)= gathered blocks; mode 0: values += sum; mode 1: values = (0 + c*sum) + 1*M
+void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const double* blocks, double* values,
+                         int mode, double c, const double* M, bool zero_first, hipStream_t s);
+void launch_force_gather(Model& m, double* f);
+void launch_spmv(const NodePattern& pat, const double* vals, double beta, double* y, double alpha,
+                 const double* x, hipStream_t s);
+void launch_dirichlet(const NodePattern& pat, double* vals, const uint8_t* d_dirmask, const int32_t* d_bc_nodes,
+                      int64_t n_bc_nodes, double* d_scale, hipStream_t s);
+void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF);
+int read_and_clear_err(Model& m);  // synchronizes the model stream
+
+// ------------------------------------------------------------------ Schwarz (schwarz.hip)
+void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const int32_t* nodes, int64_t nunt,
+                   const int32_t* unt);
+void schwarz_update(Schwarz& s, const double* J);  // enqueues on the model stream; errors flagged in m.d_err
+void schwarz_apply(Schwarz& s, const double* x, double* y);
+
+// ------------------------------------------------------------------ CG (solver.hip)
+// Solves A x = b. A has the model pattern; preconditioner = Schwarz or Jacobi(diag(A)).
+hoefem_status pcg_solve(Model& m, CgWork& w, const double* A, Schwarz* P, const double* b, double* x, double tol,
+                        int64_t max_iter, int64_t* iterations);
+
+}  // namespace hoefem

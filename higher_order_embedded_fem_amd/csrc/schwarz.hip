@@ -1,0 +1,280 @@
+// Additive Schwarz preconditioner (simulation_toolbox/src/fem/schwarz_precond.rs:134-342):
+// P = sum_s R_s^T B_s^+ R_s + sum_untreated 1 / J_ii, with B_s the restriction of J to the
+// subdomain's DOFs and B_s^+ = V diag(1 / max(|lambda|, 1e-12 max|lambda|)) V^T.
+//
+// Per update: (1) gather every B_s from J (precomputed slots), (2) batched FP64 symmetric
+// eigen-decomposition by parallel (round-robin) Jacobi in LDS, one work-group per block,
+// and recomposition of B_s^+, (3) color-ordered gather of the blocks into P's row blocks,
+// (4) reciprocal diagonal for untreated nodes.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace hoefem {
+
+void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const int32_t* nodes, int64_t nunt,
+                   const int32_t* unt) {
+    s.model = &m;
+    s.nsub = nsub;
+    s.off.assign(off, off + nsub + 1);
+    if (s.off[0] != 0) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "subdomain offsets must start at 0");
+    for (int64_t k = 0; k < nsub; ++k)
+        if (s.off[k + 1] < s.off[k]) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "subdomain offsets must be sorted");
+    s.nodes.assign(nodes, nodes + s.off[nsub]);
+    s.untreated.assign(unt, unt + nunt);
+    for (int32_t v : s.nodes)
+        if (v < 0 || v >= m.nv) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "subdomain node out of range");
+    s.colors = greedy_coloring(nsub, s.off.data(), s.nodes.data());
+    build_node_pattern(m.nv, nsub, s.off.data(), s.nodes.data(), s.untreated, s.P);
+    s.scatter.build(s.P, nsub, s.off.data(), s.nodes.data(), &s.colors, 1);
+    if (3 * s.scatter.max_m > 96)
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "Schwarz subdomain too large for the batched eigensolver (max 32 nodes)");
+    // slots of every block entry in J's (model) pattern: SchwarzElementAssembler gathers with
+    // csr_row.get(col).expect(..) (schwarz_precond.rs:304-320)
+    s.jslot.resize(s.scatter.slot.size());
+    for (int64_t b = 0; b < nsub; ++b) {
+        const int32_t* g = s.nodes.data() + s.off[b];
+        const int mb = (int)(s.off[b + 1] - s.off[b]);
+        for (int a = 0; a < mb; ++a)
+            for (int c = 0; c < mb; ++c) {
+                const int32_t k = m.pattern.find(g[a], g[c]);
+                if (k < 0) throw Error(HOEFEM_ERROR_PATTERN, "Schwarz block entry missing from the system matrix pattern");
+                s.jslot[s.scatter.slot_off[b] + (int64_t)a * mb + c] = k;
+            }
+    }
+    for (int32_t v : s.untreated)
+        if (v < 0 || v >= m.nv || !s.P.diag_only[v])
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "untreated node must not belong to a subdomain");
+    s.P.upload(m.stream);
+    s.scatter.upload(m.stream);
+    s.d_jslot.upload(s.jslot, m.stream);
+    s.d_untreated.upload(s.untreated, m.stream);
+    s.d_off.upload(s.off, m.stream);
+    s.d_nodes.upload(s.nodes, m.stream);
+    s.d_vals.ensure(std::max<int64_t>(s.P.nnz(), 1));
+    HIP_CHECK(hipMemsetAsync(s.d_vals.get(), 0, sizeof(double) * s.d_vals.n, m.stream));
+    s.d_blocks.ensure(std::max<int64_t>(s.scatter.val_off[nsub], 1));
+}
+
+// B_s(r, c) = J[3 g_a + i, 3 g_c + j], row-major (3m)^2
+__global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off,
+                                 const int64_t* __restrict__ slot_off, const int64_t* __restrict__ noff,
+                                 const int32_t* __restrict__ nodes, const int32_t* __restrict__ jslot,
+                                 const int32_t* __restrict__ jrowptr, const double* __restrict__ J,
+                                 double* __restrict__ blocks) {
+    const int64_t b = blockIdx.x;
+    const int mb = msz[b], dim = 3 * mb;
+    const int32_t* g = nodes + noff[b];
+    double* out = blocks + val_off[b];
+    for (int idx = threadIdx.x; idx < dim * dim; idx += blockDim.x) {
+        const int r = idx / dim, c = idx % dim;
+        const int a = r / 3, i = r % 3, cb = c / 3, j = c % 3;
+        const int k = jslot[slot_off[b] + (int64_t)a * mb + cb];
+        out[idx] = J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
+    }
+}
+
+// Parallel cyclic Jacobi (Brent-Luk round-robin ordering) on a symmetric dim x dim block in
+// LDS, then B^+ = V diag(f(lambda)) V^T with f = 1/max(|lambda|, 1e-12 max|lambda|)
+// (invert_element_block, schwarz_precond.rs:329-342). Rotations follow Golub & Van Loan,
+// "Matrix Computations", sym.schur2.
+constexpr int EIG_THREADS = 256;
+
+__global__ __launch_bounds__(EIG_THREADS) void k_jacobi_inverse(const int32_t* __restrict__ msz,
+                                                                const int64_t* __restrict__ val_off,
+                                                                double* __restrict__ blocks, int ldpad) {
+    extern __shared__ double lds[];
+    const int64_t b = blockIdx.x;
+    const int dim = 3 * msz[b];
+    const int nn = dim + (dim & 1);  // even number of players (one idle index when dim is odd)
+    const int ld = ldpad;            // row stride (>= nn + 1)
+    double* A = lds;
+    double* V = lds + (size_t)nn * ld;
+    double* cs = V + (size_t)nn * ld;  // [nn/2][2]
+    int* perm = (int*)(cs + nn);
+    __shared__ double s_red[EIG_THREADS];
+    __shared__ int s_done;
+    double* blk = blocks + val_off[b];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < nn * nn; idx += EIG_THREADS) {
+        const int r = idx / nn, c = idx % nn;
+        A[r * ld + c] = (r < dim && c < dim) ? blk[r * dim + c] : 0.0;
+        V[r * ld + c] = (r == c) ? 1.0 : 0.0;
+    }
+    for (int k = t; k < nn; k += EIG_THREADS) perm[k] = k;
+    __syncthreads();
+    // total Frobenius norm^2 for the stopping rule
+    double loc = 0.0;
+    for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+        const double v = A[(idx / dim) * ld + idx % dim];
+        loc += v * v;
+    }
+    s_red[t] = loc;
+    __syncthreads();
+    for (int s = EIG_THREADS / 2; s > 0; s >>= 1) {
+        if (t < s) s_red[t] += s_red[t + s];
+        __syncthreads();
+    }
+    const double total = s_red[0];
+    __syncthreads();
+    const int half = nn / 2;
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        // off-diagonal norm^2
+        loc = 0.0;
+        for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+            const int r = idx / dim, c = idx % dim;
+            if (r != c) {
+                const double v = A[r * ld + c];
+                loc += v * v;
+            }
+        }
+        s_red[t] = loc;
+        __syncthreads();
+        for (int s = EIG_THREADS / 2; s > 0; s >>= 1) {
+            if (t < s) s_red[t] += s_red[t + s];
+            __syncthreads();
+        }
+        if (t == 0) s_done = (s_red[0] <= 1e-32 * total || s_red[0] == 0.0) ? 1 : 0;
+        __syncthreads();
+        if (s_done) break;
+        for (int round = 0; round < nn - 1; ++round) {
+            // rotation parameters for each disjoint pair
+            for (int k = t; k < half; k += EIG_THREADS) {
+                int p = perm[k], q = perm[nn - 1 - k];
+                if (p > q) { const int tmp = p; p = q; q = tmp; }
+                double c = 1.0, s = 0.0;
+                if (q < dim) {
+                    const double apq = A[p * ld + q];
+                    if (apq != 0.0) {
+                        const double app = A[p * ld + p], aqq = A[q * ld + q];
+                        const double theta = (aqq - app) / (2.0 * apq);
+                        const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(tt * tt + 1.0);
+                        s = tt * c;
+                    }
+                }
+                cs[2 * k] = c;
+                cs[2 * k + 1] = s;
+            }
+            __syncthreads();
+            // columns: A <- A R, V <- V R
+            for (int idx = t; idx < half * nn; idx += EIG_THREADS) {
+                const int k = idx / nn, r = idx % nn;
+                int p = perm[k], q = perm[nn - 1 - k];
+                if (p > q) { const int tmp = p; p = q; q = tmp; }
+                const double c = cs[2 * k], s = cs[2 * k + 1];
+                if (s == 0.0) continue;
+                const double arp = A[r * ld + p], arq = A[r * ld + q];
+                A[r * ld + p] = c * arp - s * arq;
+                A[r * ld + q] = s * arp + c * arq;
+                const double vrp = V[r * ld + p], vrq = V[r * ld + q];
+                V[r * ld + p] = c * vrp - s * vrq;
+                V[r * ld + q] = s * vrp + c * vrq;
+            }
+            __syncthreads();
+            // rows: A <- R^T A
+            for (int idx = t; idx < half * nn; idx += EIG_THREADS) {
+                const int k = idx / nn, r = idx % nn;
+                int p = perm[k], q = perm[nn - 1 - k];
+                if (p > q) { const int tmp = p; p = q; q = tmp; }
+                const double c = cs[2 * k], s = cs[2 * k + 1];
+                if (s == 0.0) continue;
+                const double apr = A[p * ld + r], aqr = A[q * ld + r];
+                A[p * ld + r] = c * apr - s * aqr;
+                A[q * ld + r] = s * apr + c * aqr;
+            }
+            __syncthreads();
+            for (int k = t; k < half; k += EIG_THREADS) {
+                int p = perm[k], q = perm[nn - 1 - k];
+                if (cs[2 * k + 1] != 0.0) {
+                    A[p * ld + q] = 0.0;
+                    A[q * ld + p] = 0.0;
+                }
+            }
+            __syncthreads();
+            // round-robin: keep perm[0], rotate perm[1..nn-1]
+            if (t == 0) {
+                const int last = perm[nn - 1];
+                for (int k = nn - 1; k > 1; --k) perm[k] = perm[k - 1];
+                perm[1] = last;
+            }
+            __syncthreads();
+        }
+    }
+    // eigenvalues on the diagonal; f(lambda) into cs[0..dim)
+    loc = 0.0;
+    for (int k = t; k < dim; k += EIG_THREADS) loc = fmax(loc, fabs(A[k * ld + k]));
+    s_red[t] = loc;
+    __syncthreads();
+    for (int s = EIG_THREADS / 2; s > 0; s >>= 1) {
+        if (t < s) s_red[t] = fmax(s_red[t], s_red[t + s]);
+        __syncthreads();
+    }
+    const double lower = 1e-12 * s_red[0];
+    __syncthreads();
+    for (int k = t; k < dim; k += EIG_THREADS) {
+        const double ev = fabs(A[k * ld + k]);
+        cs[k] = ev >= lower ? 1.0 / ev : 1.0 / lower;
+    }
+    __syncthreads();
+    for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+        const int r = idx / dim, c = idx % dim;
+        double s = 0.0;
+        for (int k = 0; k < dim; ++k) s += V[r * ld + k] * cs[k] * V[c * ld + k];
+        blk[idx] = s;
+    }
+}
+
+// untreated nodes: P_ii = 1 / J_ii (schwarz_precond.rs:195-213)
+__global__ void k_schwarz_untreated(int64_t nunt, const int32_t* __restrict__ unt, const int32_t* __restrict__ prowptr,
+                                    const int32_t* __restrict__ jdiag, const double* __restrict__ J,
+                                    double* __restrict__ P, int* err) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 3 * nunt) return;
+    const int64_t row = 3 * (int64_t)unt[idx / 3] + idx % 3;
+    const int32_t dp = jdiag[row];
+    const double entry = dp >= 0 ? J[dp] : 0.0;
+    if (entry != 0.0)
+        P[prowptr[row]] = 1.0 / entry;
+    else
+        atomicOr(err, KERR_ZERO_DIAG);
+}
+
+void schwarz_update(Schwarz& s, const double* J) {
+    Model& m = *s.model;
+    const hipStream_t st = m.stream;
+    if (s.nsub > 0) {
+        hipLaunchKernelGGL(k_schwarz_gather, dim3((unsigned)s.nsub), dim3(256), 0, st, s.scatter.d_m.get(),
+                           s.scatter.d_val_off.get(), s.scatter.d_slot_off.get(), s.d_off.get(), s.d_nodes.get(),
+                           s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get());
+        check_launch("k_schwarz_gather");
+        const int dim = 3 * s.scatter.max_m;
+        const int nn = dim + (dim & 1);
+        const int ld = nn + 1;
+        const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
+        static bool attr_set = false;
+        if (!attr_set) {
+            HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          120 * 1024));
+            attr_set = true;
+        }
+        hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st, s.scatter.d_m.get(),
+                           s.scatter.d_val_off.get(), s.d_blocks.get(), ld);
+        check_launch("k_jacobi_inverse");
+        launch_block_gather(s.P, s.scatter, s.d_blocks.get(), s.d_vals.get(), 0, 0.0, nullptr, true, st);
+    }
+    if (!s.untreated.empty()) {
+        const int64_t n = 3 * (int64_t)s.untreated.size();
+        hipLaunchKernelGGL(k_schwarz_untreated, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           (int64_t)s.untreated.size(), s.d_untreated.get(), s.P.d_rowptr.get(),
+                           m.pattern.d_diag_pos.get(), J, s.d_vals.get(), m.d_err.get());
+        check_launch("k_schwarz_untreated");
+    }
+}
+
+void schwarz_apply(Schwarz& s, const double* x, double* y) {
+    launch_spmv(s.P, s.d_vals.get(), 0.0, y, 1.0, x, s.model->stream);
+}
+
+}  // namespace hoefem

@@ -1,0 +1,601 @@
+// Preconditioned conjugate gradient (fenris/src/cg.rs:377-482) as two fused kernels per
+// iteration, with the iteration state (zTr, alpha, beta, status) kept on the device:
+//   A-kernel: p <- z + beta p (on the fly, double-buffered), Ap = A p, pAp  -> alpha
+//   B-kernel: x += alpha p, r <- r - alpha Ap (on the fly, double-buffered), z = P r,
+//             zTr, |r|^2 -> beta, convergence / max-iteration status
+// Reductions use a deterministic last-block pass over per-block partials. The host enqueues
+// batches of iterations and polls the status of the batch before the one just enqueued.
+// Also: the residual / state kernels of the Backward-Euler Newton driver.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace hoefem {
+
+enum : int { CG_RUNNING = 0, CG_CONVERGED = 100, CG_ZERO_RHS = 101 };
+constexpr int CG_THREADS = 256;
+constexpr int CG_BATCH = 8;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum NV values over the whole grid. Returns true in the (single) last block, where tot
+// holds the grid sums in every thread. Deterministic: fixed grid, fixed summation tree.
+template <int NV>
+__device__ inline bool grid_reduce_last(const double (&v)[NV], double* partials, unsigned* counter, double (&tot)[NV]) {
+    __shared__ double s_w[CG_THREADS / 64][NV];
+    __shared__ int s_last;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(v[k]);
+        if (lane == 0) s_w[wid][k] = s;
+    }
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
+            partials[(int64_t)blockIdx.x * NV + k] = s;
+        }
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        s_last = (prev == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = t; b < (int)gridDim.x; b += CG_THREADS)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            acc[k] += __hip_atomic_load(partials + (int64_t)b * NV + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (lane == 0) s_w[wid][k] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
+        tot[k] = s;
+    }
+    if (t == 0) *counter = 0u;
+    return true;
+}
+
+struct Csr3 {  // node-blocked CSR view
+    int64_t nn;
+    const int32_t* nptr;
+    const int32_t* ncol;
+    const int32_t* rowptr;
+    const uint8_t* diag_only;
+    const double* vals;
+};
+
+// acc_i = sum_j A(3v+i, j) * xfun(j) over the node row block of v (G lanes, reduced into all lanes)
+template <int G, class XF>
+__device__ inline void row_block_dot(const Csr3& A, int64_t v, int lane, bool valid, XF xfun, double& a0, double& a1,
+                                     double& a2) {
+    a0 = a1 = a2 = 0.0;
+    if (valid) {
+        if (A.diag_only[v]) {
+            if (lane == 0) {
+                a0 = A.vals[A.rowptr[3 * v]] * xfun(3 * v);
+                a1 = A.vals[A.rowptr[3 * v + 1]] * xfun(3 * v + 1);
+                a2 = A.vals[A.rowptr[3 * v + 2]] * xfun(3 * v + 2);
+            }
+        } else {
+            const int kb = A.nptr[v], k = A.nptr[v + 1] - kb;
+            const double* R0 = A.vals + A.rowptr[3 * v];
+            const double* R1 = R0 + 3 * k;
+            const double* R2 = R1 + 3 * k;
+            for (int j = lane; j < k; j += G) {
+                const int64_t b = A.ncol[kb + j];
+                const double x0 = xfun(3 * b), x1 = xfun(3 * b + 1), x2 = xfun(3 * b + 2);
+                a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
+                a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
+                a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, G);
+        a1 += __shfl_xor(a1, off, G);
+        a2 += __shfl_xor(a2, off, G);
+    }
+}
+
+struct CgPtrs {
+    double* x;
+    double* z;
+    double* Ap;
+    const double* r_old;
+    double* r_new;
+    const double* p_old;
+    double* p_new;
+    const double* b;
+    const double* dinv;
+    double* partials;
+    unsigned* counter;
+    CgScalars* sc;
+};
+
+// preamble 1: r = b - A x ; |b|^2
+template <int G>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3 A, CgPtrs P) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    const bool valid = v < A.nn;
+    const double* x = P.x;
+    double a0, a1, a2;
+    row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return x[j]; }, a0, a1, a2);
+    double part[1] = {0.0};
+    if (valid && lane == 0) {
+        const double b0 = P.b[3 * v], b1 = P.b[3 * v + 1], b2 = P.b[3 * v + 2];
+        P.r_new[3 * v] = b0 - a0;
+        P.r_new[3 * v + 1] = b1 - a1;
+        P.r_new[3 * v + 2] = b2 - a2;
+        part[0] = b0 * b0 + b1 * b1 + b2 * b2;
+    }
+    double tot[1];
+    if (grid_reduce_last<1>(part, P.partials, P.counter, tot) && threadIdx.x == 0) P.sc->bnorm = sqrt(tot[0]);
+}
+
+// preamble 2: z = P r ; zTr, |r|^2 ; initial status
+template <int G, bool JAC>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3 Pm, CgPtrs P) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    const bool valid = v < Pm.nn;
+    const double* r = P.r_new;
+    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+    if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return r[j]; }, z0, z1, z2);
+    double part[2] = {0.0, 0.0};
+    if (valid && lane == 0) {
+        const double r0 = r[3 * v], r1 = r[3 * v + 1], r2 = r[3 * v + 2];
+        if (JAC) {
+            z0 = P.dinv[3 * v] * r0;
+            z1 = P.dinv[3 * v + 1] * r1;
+            z2 = P.dinv[3 * v + 2] * r2;
+        }
+        P.z[3 * v] = z0;
+        P.z[3 * v + 1] = z1;
+        P.z[3 * v + 2] = z2;
+        part[0] = z0 * r0 + z1 * r1 + z2 * r2;
+        part[1] = r0 * r0 + r1 * r1 + r2 * r2;
+    }
+    double tot[2];
+    if (grid_reduce_last<2>(part, P.partials, P.counter, tot) && threadIdx.x == 0) {
+        CgScalars* sc = P.sc;
+        sc->zTr = tot[0];
+        sc->rr = tot[1];
+        sc->beta = 0.0;
+        sc->iters = 0;
+        if (sc->bnorm == 0.0)
+            sc->status = CG_ZERO_RHS;
+        else if (sqrt(tot[1]) <= sc->tol * sc->bnorm)
+            sc->status = CG_CONVERGED;
+        else if (sc->max_iter >= 0 && sc->iters >= sc->max_iter)
+            sc->status = HOEFEM_ERROR_CG_MAX_ITERATIONS;
+        else
+            sc->status = CG_RUNNING;
+    }
+}
+
+// A-kernel: p_new = z + beta * p_old ; Ap = A p_new ; pAp -> alpha
+template <int G>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3 A, CgPtrs P) {
+    CgScalars* sc = P.sc;
+    if (sc->status != CG_RUNNING) return;
+    const double beta = sc->beta;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    const bool valid = v < A.nn;
+    const double* z = P.z;
+    const double* po = P.p_old;
+    double a0, a1, a2;
+    row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return z[j] + beta * po[j]; }, a0, a1, a2);
+    double part[1] = {0.0};
+    if (valid && lane == 0) {
+        const double p0 = z[3 * v] + beta * po[3 * v];
+        const double p1 = z[3 * v + 1] + beta * po[3 * v + 1];
+        const double p2 = z[3 * v + 2] + beta * po[3 * v + 2];
+        P.p_new[3 * v] = p0;
+        P.p_new[3 * v + 1] = p1;
+        P.p_new[3 * v + 2] = p2;
+        P.Ap[3 * v] = a0;
+        P.Ap[3 * v + 1] = a1;
+        P.Ap[3 * v + 2] = a2;
+        part[0] = p0 * a0 + p1 * a1 + p2 * a2;
+    }
+    double tot[1];
+    if (grid_reduce_last<1>(part, P.partials, P.counter, tot) && threadIdx.x == 0) {
+        sc->pAp = tot[0];
+        if (tot[0] <= 0.0)
+            sc->status = HOEFEM_ERROR_CG_INDEFINITE_OPERATOR;
+        else if (sc->zTr <= 0.0)
+            sc->status = HOEFEM_ERROR_CG_INDEFINITE_PRECONDITIONER;
+        else
+            sc->alpha = sc->zTr / tot[0];
+    }
+}
+
+// B-kernel: x += alpha p ; r_new = r_old - alpha Ap ; z = P r_new ; zTr, rr -> beta, status
+template <int G, bool JAC>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3 Pm, CgPtrs P) {
+    CgScalars* sc = P.sc;
+    if (sc->status != CG_RUNNING) return;
+    const double alpha = sc->alpha;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    const bool valid = v < Pm.nn;
+    const double* ro = P.r_old;
+    const double* Ap = P.Ap;
+    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+    if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return ro[j] - alpha * Ap[j]; }, z0, z1, z2);
+    double part[2] = {0.0, 0.0};
+    if (valid && lane == 0) {
+        const double r0 = ro[3 * v] - alpha * Ap[3 * v];
+        const double r1 = ro[3 * v + 1] - alpha * Ap[3 * v + 1];
+        const double r2 = ro[3 * v + 2] - alpha * Ap[3 * v + 2];
+        if (JAC) {
+            z0 = P.dinv[3 * v] * r0;
+            z1 = P.dinv[3 * v + 1] * r1;
+            z2 = P.dinv[3 * v + 2] * r2;
+        }
+        P.x[3 * v] = P.x[3 * v] + alpha * P.p_new[3 * v];
+        P.x[3 * v + 1] = P.x[3 * v + 1] + alpha * P.p_new[3 * v + 1];
+        P.x[3 * v + 2] = P.x[3 * v + 2] + alpha * P.p_new[3 * v + 2];
+        P.r_new[3 * v] = r0;
+        P.r_new[3 * v + 1] = r1;
+        P.r_new[3 * v + 2] = r2;
+        P.z[3 * v] = z0;
+        P.z[3 * v + 1] = z1;
+        P.z[3 * v + 2] = z2;
+        part[0] = z0 * r0 + z1 * r1 + z2 * r2;
+        part[1] = r0 * r0 + r1 * r1 + r2 * r2;
+    }
+    double tot[2];
+    if (grid_reduce_last<2>(part, P.partials, P.counter, tot) && threadIdx.x == 0) {
+        sc->iters += 1;
+        const double zTr_next = tot[0];
+        sc->beta = zTr_next / sc->zTr;
+        sc->zTr = zTr_next;
+        sc->rr = tot[1];
+        if (sqrt(tot[1]) <= sc->tol * sc->bnorm)
+            sc->status = CG_CONVERGED;
+        else if (sc->max_iter >= 0 && sc->iters >= sc->max_iter)
+            sc->status = HOEFEM_ERROR_CG_MAX_ITERATIONS;
+    }
+}
+
+__global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
+    sc->tol = tol;
+    sc->max_iter = max_iter;
+    sc->status = CG_RUNNING;
+    sc->iters = 0;
+    sc->beta = 0.0;
+}
+
+// DiagonalPreconditioner::update (simulation_toolbox/src/fem/newton_cg.rs:558-575)
+__global__ void k_jacobi_diag(int64_t ndof, const int32_t* __restrict__ diag_pos, const double* __restrict__ A,
+                              double* __restrict__ dinv, int* err) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ndof) return;
+    const int32_t dp = diag_pos[r];
+    const double a = dp >= 0 ? A[dp] : 0.0;
+    if (a != 0.0)
+        dinv[r] = 1.0 / a;
+    else {
+        dinv[r] = 0.0;
+        atomicOr(err, KERR_ZERO_DIAG);
+    }
+}
+
+int spmv_group_size(const NodePattern& pat);
+
+void CgWork::ensure(int64_t nn_dof, int nblocks) {
+    if (n != nn_dof) {
+        n = nn_dof;
+        for (int k = 0; k < 2; ++k) {
+            r[k].alloc(n);
+            p[k].alloc(n);
+        }
+        z.alloc(n);
+        Ap.alloc(n);
+        diag_inv.alloc(n);
+    }
+    if ((int64_t)partials.n < 2 * (int64_t)nblocks) partials.alloc(2 * (size_t)nblocks);
+    if (!counter.p) {
+        counter.alloc(1);
+        HIP_CHECK(hipMemset(counter.get(), 0, sizeof(unsigned)));
+        scalars.alloc(1);
+        HIP_CHECK(hipMemset(scalars.get(), 0, sizeof(CgScalars)));
+    }
+    if (!ev[0]) {
+        HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    }
+}
+
+CgWork::~CgWork() {
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+static Csr3 view(const NodePattern& p, const double* vals) {
+    return Csr3{p.num_nodes, p.d_nptr.get(), p.d_ncol.get(), p.d_rowptr.get(), p.d_diag_only.get(), vals};
+}
+
+template <int GA, int GB, bool JAC>
+static void cg_launch_all(Model& m, CgWork& w, const Csr3& A, const Csr3& Pm, const double* b, double* x,
+                          double tol, int64_t max_iter, int64_t* iterations, hoefem_status* out_status) {
+    const hipStream_t s = m.stream;
+    const int64_t nn = A.nn;
+    const unsigned blkA = (unsigned)((nn * GA + CG_THREADS - 1) / CG_THREADS);
+    const unsigned blkB = (unsigned)((nn * GB + CG_THREADS - 1) / CG_THREADS);
+    CgPtrs P{};
+    P.x = x;
+    P.z = w.z.get();
+    P.Ap = w.Ap.get();
+    P.b = b;
+    P.dinv = w.diag_inv.get();
+    P.partials = w.partials.get();
+    P.counter = w.counter.get();
+    P.sc = w.scalars.get();
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
+    HIP_CHECK(hipMemsetAsync(w.p[0].get(), 0, sizeof(double) * w.n, s));
+    P.r_new = w.r[0].get();
+    hipLaunchKernelGGL(k_cg_pre1<GA>, dim3(blkA), dim3(CG_THREADS), 0, s, A, P);
+    hipLaunchKernelGGL((k_cg_pre2<GB, JAC>), dim3(blkB), dim3(CG_THREADS), 0, s, Pm, P);
+    check_launch("cg preamble");
+    int cur = 0;
+    auto enqueue_batch = [&]() {
+        for (int k = 0; k < CG_BATCH; ++k) {
+            P.p_old = w.p[cur].get();
+            P.p_new = w.p[1 - cur].get();
+            P.r_old = w.r[cur].get();
+            P.r_new = w.r[1 - cur].get();
+            hipLaunchKernelGGL(k_cg_a<GA>, dim3(blkA), dim3(CG_THREADS), 0, s, A, P);
+            hipLaunchKernelGGL((k_cg_b<GB, JAC>), dim3(blkB), dim3(CG_THREADS), 0, s, Pm, P);
+            cur ^= 1;
+        }
+        check_launch("cg iteration");
+    };
+    CgScalars* slots = w.h_scalars.p;  // [0], [1]: polling slots
+    CgScalars* hs = w.h_scalars.p + 2;
+    // two batches in flight; poll the older one
+    int64_t nbatches = 0;
+    const int64_t max_batches = (max_iter >= 0 ? max_iter / CG_BATCH + 2 : INT64_MAX);
+    auto submit = [&](int slot) {
+        enqueue_batch();
+        HIP_CHECK(hipMemcpyAsync(slots + slot, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipEventRecord(w.ev[slot], s));
+        ++nbatches;
+    };
+    submit(0);
+    if (nbatches < max_batches) submit(1);
+    int poll = 0;
+    while (true) {
+        HIP_CHECK(hipEventSynchronize(w.ev[poll]));
+        if (slots[poll].status != CG_RUNNING || nbatches >= max_batches) break;
+        submit(poll);
+        poll ^= 1;
+    }
+    HIP_CHECK(hipMemcpyAsync(hs, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    *iterations = hs->iters;
+    switch (hs->status) {
+        case CG_CONVERGED: *out_status = HOEFEM_OK; break;
+        case CG_ZERO_RHS:
+            HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * w.n, s));
+            *out_status = HOEFEM_OK;
+            break;
+        case CG_RUNNING: *out_status = HOEFEM_ERROR_CG_MAX_ITERATIONS; break;
+        default: *out_status = (hoefem_status)hs->status; break;
+    }
+}
+
+hoefem_status pcg_solve(Model& m, CgWork& w, const double* Avals, Schwarz* Pre, const double* b, double* x,
+                        double tol, int64_t max_iter, int64_t* iterations) {
+    const NodePattern& pat = m.pattern;
+    const int64_t nn = pat.num_nodes;
+    const int GA = spmv_group_size(pat);
+    const int GB = Pre ? spmv_group_size(Pre->P) : 8;
+    const int64_t blocks = (nn * 32 + CG_THREADS - 1) / CG_THREADS + 1;
+    w.ensure(pat.ndof(), (int)blocks);
+    const Csr3 A = view(pat, Avals);
+    Csr3 Pm = Pre ? view(Pre->P, Pre->d_vals.get()) : A;
+    if (!Pre) {
+        hipLaunchKernelGGL(k_jacobi_diag, dim3((unsigned)((pat.ndof() + 255) / 256)), dim3(256), 0, m.stream,
+                           pat.ndof(), pat.d_diag_pos.get(), Avals, w.diag_inv.get(), m.d_err.get());
+        check_launch("k_jacobi_diag");
+        if (read_and_clear_err(m) & KERR_ZERO_DIAG) throw Error(HOEFEM_ERROR_SINGULAR, "Diagonal matrix is not invertible.");
+    }
+    hoefem_status st = HOEFEM_OK;
+#define CG_CASE(a, b, j) \
+    if (GA == a && GB == b && JAC == j) { cg_launch_all<a, b, j>(m, w, A, Pm, b_, x, tol, max_iter, iterations, &st); return st; }
+    const bool JAC = (Pre == nullptr);
+    const double* b_ = b;
+    CG_CASE(8, 8, true) CG_CASE(16, 8, true) CG_CASE(32, 8, true)
+    CG_CASE(8, 8, false) CG_CASE(8, 16, false) CG_CASE(8, 32, false)
+    CG_CASE(16, 8, false) CG_CASE(16, 16, false) CG_CASE(16, 32, false)
+    CG_CASE(32, 8, false) CG_CASE(32, 16, false) CG_CASE(32, 32, false)
+#undef CG_CASE
+    throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unsupported CG configuration");
+}
+
+// ================================================================== Newton / BE kernels
+// uu = u_p + w * dt with Dirichlet DOFs zeroed (BackwardEulerImplicitFunction::eval_into,
+// hamilton2/src/integrators/euler.rs:97-114; apply_dirichlet_bcs_unknowns,
+// simulation_toolbox/src/fem/system_assembly.rs:78-104)
+__global__ void k_state(int64_t ndof, const double* __restrict__ up, const double* __restrict__ w, double dt,
+                        const uint8_t* __restrict__ dirmask, double* __restrict__ uu) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndof) return;
+    const double val = up[i] + w[i] * dt;
+    uu[i] = dirmask[i / 3] ? 0.0 : val;
+}
+
+// F = -dt * [ (M g + f_int(uu)) with Dirichlet rows zeroed ] + M w  and |F|^2
+// (NewtonCgDynamicSystem::eval_f, newton_cg.rs:441-478, then euler.rs:107-113)
+template <int G>
+__global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3 M, const int32_t* __restrict__ inc_ptr,
+                                                         const int32_t* __restrict__ inc_blk,
+                                                         const int32_t* __restrict__ inc_loc, int n,
+                                                         const double* __restrict__ fe,
+                                                         const double* __restrict__ gforce,
+                                                         const uint8_t* __restrict__ dirmask, double dt,
+                                                         const double* __restrict__ w, double* __restrict__ F,
+                                                         double* partials, unsigned* counter, double* norm2) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t v = gid / G;
+    const int lane = (int)(gid % G);
+    const bool valid = v < M.nn;
+    double m0, m1, m2;
+    row_block_dot<G>(M, v, lane, valid, [&](int64_t j) { return w[j]; }, m0, m1, m2);
+    double part[1] = {0.0};
+    if (valid && lane == 0) {
+        double f0 = 1.0 * gforce[3 * v], f1 = 1.0 * gforce[3 * v + 1], f2 = 1.0 * gforce[3 * v + 2];
+        for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
+            const double* src = fe + ((int64_t)inc_blk[it] * n + inc_loc[it]) * 3;
+            f0 += src[0];
+            f1 += src[1];
+            f2 += src[2];
+        }
+        if (dirmask[v]) f0 = f1 = f2 = 0.0;
+        const double sc = dt * -1.0;
+        double F0 = f0 * sc, F1 = f1 * sc, F2 = f2 * sc;
+        F0 = 1.0 * F0;
+        F1 = 1.0 * F1;
+        F2 = 1.0 * F2;
+        F0 += 1.0 * m0;
+        F1 += 1.0 * m1;
+        F2 += 1.0 * m2;
+        F[3 * v] = F0;
+        F[3 * v + 1] = F1;
+        F[3 * v + 2] = F2;
+        part[0] = F0 * F0 + F1 * F1 + F2 * F2;
+    }
+    double tot[1];
+    if (grid_reduce_last<1>(part, partials, counter, tot) && threadIdx.x == 0) *norm2 = tot[0];
+}
+
+// w <- w + delta * (-sol)  (line-search step along p = -sol, hamilton2/src/newton.rs:113-116, :219)
+__global__ void k_axpy_neg(int64_t n, double* __restrict__ w, double delta, const double* __restrict__ sol) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    w[i] = w[i] + delta * (-sol[i]);
+}
+
+// v_n = v ; u_p = u + v * dt
+__global__ void k_be_begin(int64_t n, const double* __restrict__ u, const double* __restrict__ v, double dt,
+                           double* __restrict__ up, double* __restrict__ vn) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    vn[i] = v[i];
+    up[i] = u[i] + v[i] * dt;
+}
+
+// v += w ; u += v * dt ; then Dirichlet unknowns zeroed (euler.rs:195-196, newton_cg.rs:293)
+__global__ void k_be_end(int64_t n, double* __restrict__ u, double* __restrict__ v, const double* __restrict__ w,
+                         double dt, const uint8_t* __restrict__ dirmask) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vv = v[i] + w[i];
+    const double uu = u[i] + vv * dt;
+    const bool d = dirmask[i / 3] != 0;
+    v[i] = d ? 0.0 : vv;
+    u[i] = d ? 0.0 : uu;
+}
+
+__global__ void k_zero_dirichlet(int64_t n, double* __restrict__ a, double* __restrict__ b,
+                                 const uint8_t* __restrict__ dirmask) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (dirmask[i / 3]) {
+        a[i] = 0.0;
+        if (b) b[i] = 0.0;
+    }
+}
+
+__global__ void k_finite(int64_t n, const double* __restrict__ a, const double* __restrict__ b, int* err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!isfinite(a[i]) || !isfinite(b[i])) atomicOr(err, KERR_NONFINITE);
+}
+
+void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu) {
+    const int64_t n = 3 * m.nv;
+    hipLaunchKernelGGL(k_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, up, w, dt, dirmask, uu);
+    check_launch("k_state");
+}
+
+void launch_residual(Model& m, const double* Mvals, const double* gforce, const uint8_t* dirmask, double dt,
+                     const double* w, double* F, CgWork& cw, double* norm2) {
+    const NodePattern& pat = m.pattern;
+    const int G = spmv_group_size(pat);
+    const Csr3 Mv = view(pat, Mvals);
+    const unsigned blocks = (unsigned)((pat.num_nodes * G + CG_THREADS - 1) / CG_THREADS);
+    cw.ensure(pat.ndof(), (int)blocks + 1);
+#define RES_CASE(GG)                                                                                              \
+    hipLaunchKernelGGL(k_residual<GG>, dim3(blocks), dim3(CG_THREADS), 0, m.stream, Mv, m.scatter.d_inc_ptr.get(), \
+                       m.scatter.d_inc_blk.get(), m.scatter.d_inc_loc.get(), m.n, m.d_fe.get(), gforce, dirmask, dt,  \
+                       w, F, cw.partials.get(), cw.counter.get(), norm2)
+    if (G == 32)
+        RES_CASE(32);
+    else if (G == 16)
+        RES_CASE(16);
+    else
+        RES_CASE(8);
+#undef RES_CASE
+    check_launch("k_residual");
+}
+
+void launch_axpy_neg(Model& m, double* w, double delta, const double* sol) {
+    const int64_t n = 3 * m.nv;
+    hipLaunchKernelGGL(k_axpy_neg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, w, delta, sol);
+    check_launch("k_axpy_neg");
+}
+
+void launch_be_begin(Model& m, const double* u, const double* v, double dt, double* up, double* vn) {
+    const int64_t n = 3 * m.nv;
+    hipLaunchKernelGGL(k_be_begin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, u, v, dt, up, vn);
+    check_launch("k_be_begin");
+}
+
+void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask) {
+    const int64_t n = 3 * m.nv;
+    hipLaunchKernelGGL(k_be_end, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, u, v, w, dt, dirmask);
+    check_launch("k_be_end");
+}
+
+void launch_zero_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask) {
+    const int64_t n = 3 * m.nv;
+    hipLaunchKernelGGL(k_zero_dirichlet, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, a, b, dirmask);
+    check_launch("k_zero_dirichlet");
+}
+
+void launch_finite_check(Model& m, const double* a, const double* b) {
+    const int64_t n = 3 * m.nv;
+    hipLaunchKernelGGL(k_finite, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, a, b, m.d_err.get());
+    check_launch("k_finite");
+}
+
+}  // namespace hoefem

@@ -1,0 +1,264 @@
+"""Python mirror of the reference's hot-path interface over libhoefem.so.
+
+Names follow the reference: ``StableNeoHookeanMaterial`` (fenris/src/solid/materials.rs),
+``ElasticityModel`` (the NodalModel / EmbeddedModel trait surface of fenris/src/solid/mod.rs),
+``SchwarzPreconditioner`` (simulation_toolbox/src/fem/schwarz_precond.rs),
+``conjugate_gradient`` (fenris/src/cg.rs) and ``NewtonCgIntegrator`` (the Backward-Euler
+Newton-CG step of simulation_toolbox/src/fem/newton_cg.rs). Hot-path arrays are torch CUDA
+(HIP) tensors; every call runs the HIP library, there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+from .scenes import Scene, lame_from_young_poisson
+
+
+def _dev(t: torch.Tensor) -> int:
+    if not t.is_cuda:
+        raise ValueError("expected a device (cuda/hip) tensor")
+    if t.dtype != torch.float64:
+        raise ValueError("expected float64")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def _np(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class StableNeoHookeanMaterial:
+    """StableNeoHookeanMaterial with Lame parameters from YoungPoisson (reference convention)."""
+
+    def __init__(self, young: float = None, poisson: float = None, mu: float = None, lam: float = None):
+        if mu is None:
+            mu, lam = lame_from_young_poisson(young, poisson)
+        self.mu, self.lam = float(mu), float(lam)
+
+    def evaluate(self, F: torch.Tensor, stress: bool = True, tangent: bool = True):
+        """F: (nq, 3, 3) device float64 (row-major torch layout). Returns (P, dP/dF[9x9 vec])."""
+        L = _lib.load()
+        nq = F.shape[0]
+        Fc = F.transpose(1, 2).contiguous()  # column-major storage
+        P = torch.empty_like(Fc) if stress else None
+        D = torch.empty((nq, 9, 9), dtype=torch.float64, device=F.device) if tangent else None
+        check(L.hoefem_material_eval(self.mu, self.lam, nq, Fc.data_ptr(), P.data_ptr() if P is not None else None,
+                                     D.data_ptr() if D is not None else None))
+        torch.cuda.synchronize()
+        Pout = P.transpose(1, 2) if P is not None else None
+        Dout = D.transpose(1, 2) if D is not None else None  # [q, row, col]
+        return Pout, Dout
+
+
+def _quad_struct(q, keep):
+    s = _lib.Quadrature()
+    iw, ixi = _np(q.interior_w, np.float64), _np(q.interior_xi, np.float64)
+    keep += [iw, ixi]
+    s.interior_num_points = iw.size
+    s.interior_weights, s.interior_points = iw.ctypes.data, ixi.ctypes.data
+    if q.interface_offsets is not None:
+        off, fw, fxi = _np(q.interface_offsets, np.int64), _np(q.interface_w, np.float64), _np(q.interface_xi, np.float64)
+        keep += [off, fw, fxi]
+        s.interface_offsets, s.interface_weights, s.interface_points = off.ctypes.data, fw.ctypes.data, fxi.ctypes.data
+    return s
+
+
+class ElasticityModel:
+    """NodalModel / EmbeddedModel on the GPU (ElasticityModelParallel, fenris/src/solid/mod.rs:124-172)."""
+
+    def __init__(self, scene: Scene):
+        L = _lib.load()
+        self.scene = scene
+        keep = []
+        d = _lib.ModelDesc()
+        v, c = _np(scene.vertices, np.float64), _np(scene.connectivity, np.int32)
+        keep += [v, c]
+        d.element_type = scene.element_type
+        d.num_vertices, d.vertices = v.shape[0], v.ctypes.data
+        d.num_elements, d.connectivity = c.shape[0], c.ctypes.data
+        d.num_interior = scene.num_interior
+        d.mass_quadrature = _quad_struct(scene.mass_quadrature, keep)
+        d.stiffness_quadrature = _quad_struct(scene.stiffness_quadrature, keep)
+        d.elliptic_quadrature = _quad_struct(scene.elliptic_quadrature, keep)
+        d.mass_regularization_factor = scene.mass_regularization
+        h = C.c_void_p()
+        check(L.hoefem_model_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.ndof = L.hoefem_model_ndof(h)
+        self.nnz = L.hoefem_model_nnz(h)
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib._lib.hoefem_model_destroy(self.h)
+            self.h = None
+
+    def pattern(self):
+        rp = np.zeros(self.ndof + 1, np.int32)
+        col = np.zeros(self.nnz, np.int32)
+        check(_lib.load().hoefem_model_pattern(self.h, rp.ctypes.data, col.ctypes.data))
+        return rp, col
+
+    def colors(self, p: int):
+        L = _lib.load()
+        nc = L.hoefem_model_num_colors(self.h, p)
+        cnt = self.scene.num_interior if p == 0 else self.scene.num_elements - self.scene.num_interior
+        off = np.zeros(nc + 1, np.int32)
+        labels = np.zeros(max(cnt, 1), np.int32)
+        check(L.hoefem_model_colors(self.h, p, off.ctypes.data, labels.ctypes.data))
+        return off, labels[:cnt]
+
+    def zeros_vector(self):
+        return torch.zeros(self.ndof, dtype=torch.float64, device=self.device)
+
+    def zeros_matrix(self):
+        return torch.zeros(self.nnz, dtype=torch.float64, device=self.device)
+
+    def assemble_elastic_pseudo_forces_into(self, f: torch.Tensor, u: torch.Tensor, material: StableNeoHookeanMaterial):
+        """f += f_int(u) (assemble_elastic_pseudo_forces_into_par)."""
+        check(_lib.load().hoefem_model_forces_add(self.h, material.mu, material.lam, _dev(u), _dev(f)))
+        return f
+
+    def assemble_stiffness_into(self, values: torch.Tensor, u: torch.Tensor, material: StableNeoHookeanMaterial):
+        """values += K(u) on the model pattern (assemble_transformed_stiffness_into_par)."""
+        check(_lib.load().hoefem_model_stiffness_add(self.h, material.mu, material.lam, _dev(u), _dev(values)))
+        return values
+
+    def assemble_mass(self, density: float) -> torch.Tensor:
+        vals = self.zeros_matrix()
+        check(_lib.load().hoefem_model_mass(self.h, density, _dev(vals)))
+        return vals
+
+    def apply_homogeneous_dirichlet_bc_csr(self, values: torch.Tensor, nodes) -> torch.Tensor:
+        nodes = _np(nodes, np.int32)
+        check(_lib.load().hoefem_model_dirichlet_csr(self.h, nodes.size, nodes.ctypes.data if nodes.size else None,
+                                                     _dev(values)))
+        return values
+
+    def spmv(self, values: torch.Tensor, x: torch.Tensor, y: Optional[torch.Tensor] = None, alpha=1.0, beta=0.0):
+        y = self.zeros_vector() if y is None else y
+        check(_lib.load().hoefem_model_spmv(self.h, _dev(values), beta, _dev(y), alpha, _dev(x)))
+        torch.cuda.synchronize()
+        return y
+
+
+class SchwarzPreconditioner:
+    """SchwarzPreconditioner (simulation_toolbox/src/fem/newton_cg.rs:297-370) over subdomain node sets."""
+
+    def __init__(self, model: ElasticityModel, offsets, nodes, untreated):
+        L = _lib.load()
+        self.model = model
+        off, nd, un = _np(offsets, np.int64), _np(nodes, np.int32), _np(untreated, np.int32)
+        self._keep = [off, nd, un]
+        h = C.c_void_p()
+        check(L.hoefem_schwarz_create(model.h, off.size - 1, off.ctypes.data, nd.ctypes.data if nd.size else None,
+                                      un.size, un.ctypes.data if un.size else None, C.byref(h)))
+        self.h = h
+        self.nnz = L.hoefem_schwarz_nnz(h)
+
+    @classmethod
+    def from_scene(cls, model: ElasticityModel, scene: Scene):
+        return cls(model, scene.subdomain_offsets, scene.subdomain_nodes, scene.untreated_nodes)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib._lib.hoefem_schwarz_destroy(self.h)
+            self.h = None
+
+    def pattern(self):
+        rp = np.zeros(self.model.ndof + 1, np.int32)
+        col = np.zeros(self.nnz, np.int32)
+        check(_lib.load().hoefem_schwarz_pattern(self.h, rp.ctypes.data, col.ctypes.data))
+        return rp, col
+
+    def update(self, J: torch.Tensor):
+        check(_lib.load().hoefem_schwarz_update(self.h, _dev(J)))
+
+    def values(self) -> torch.Tensor:
+        out = torch.zeros(self.nnz, dtype=torch.float64, device=self.model.device)
+        check(_lib.load().hoefem_schwarz_values(self.h, _dev(out)))
+        return out
+
+    def apply(self, x: torch.Tensor) -> torch.Tensor:
+        y = self.model.zeros_vector()
+        check(_lib.load().hoefem_schwarz_apply(self.h, _dev(x), _dev(y)))
+        torch.cuda.synchronize()
+        return y
+
+
+def conjugate_gradient(model: ElasticityModel, A: torch.Tensor, b: torch.Tensor, x: torch.Tensor,
+                       preconditioner: Optional[SchwarzPreconditioner] = None, tol: float = 1e-5,
+                       max_iter: int = 500):
+    """ConjugateGradient::solve_with_guess with RelativeResidualCriterion. Returns (iterations, status)."""
+    it = C.c_int64()
+    st = _lib.load().hoefem_pcg(model.h, _dev(A), preconditioner.h if preconditioner is not None else None, _dev(b),
+                                _dev(x), tol, max_iter, C.byref(it))
+    if st not in (0, 5, 6, 7):
+        check(st)
+    return it.value, st
+
+
+class NewtonCgIntegrator:
+    """One Backward-Euler Newton-CG step per call (NewtonCgIntegrator3d::run)."""
+
+    def __init__(self, model: ElasticityModel, scene: Scene, newton_tolerance_factor: float = 1e-5,
+                 cg_tolerance: float = 1e-5, cg_max_iter: int = 500, max_newton_iter: int = 100,
+                 schwarz: bool = True, gravity_enabled: bool = True):
+        L = _lib.load()
+        self.model, self.scene = model, scene
+        mu, lam = lame_from_young_poisson(scene.young, scene.poisson)
+        d = _lib.SimDesc()
+        d.mu, d.lam, d.density = mu, lam, scene.density
+        d.gravity, d.gravity_enabled = scene.gravity, 1 if gravity_enabled else 0
+        dn = _np(scene.dirichlet_nodes, np.int32)
+        so, sn, su = (_np(scene.subdomain_offsets, np.int64), _np(scene.subdomain_nodes, np.int32),
+                      _np(scene.untreated_nodes, np.int32))
+        self._keep = [dn, so, sn, su]
+        d.num_dirichlet, d.dirichlet_nodes = dn.size, dn.ctypes.data if dn.size else None
+        if schwarz and so.size > 1:
+            d.num_subdomains, d.subdomain_offsets, d.subdomain_nodes = so.size - 1, so.ctypes.data, sn.ctypes.data
+            d.num_untreated, d.untreated_nodes = su.size, su.ctypes.data if su.size else None
+        d.newton_tolerance_factor = newton_tolerance_factor
+        d.max_newton_iter = max_newton_iter
+        d.cg_tolerance = cg_tolerance
+        d.cg_max_iter = cg_max_iter
+        h = C.c_void_p()
+        check(L.hoefem_sim_create(model.h, C.byref(d), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib._lib.hoefem_sim_destroy(self.h)
+            self.h = None
+
+    @property
+    def representative_force(self) -> float:
+        return _lib.load().hoefem_sim_representative_force(self.h)
+
+    def mass(self) -> torch.Tensor:
+        out = self.model.zeros_matrix()
+        check(_lib.load().hoefem_sim_mass_values(self.h, _dev(out)))
+        return out
+
+    def jacobian(self) -> torch.Tensor:
+        out = self.model.zeros_matrix()
+        check(_lib.load().hoefem_sim_jacobian_values(self.h, _dev(out)))
+        return out
+
+    def residual(self, t, dt, u, v, w) -> torch.Tensor:
+        F = self.model.zeros_vector()
+        check(_lib.load().hoefem_sim_residual(self.h, t, dt, _dev(u), _dev(v), _dev(w), _dev(F)))
+        return F
+
+    def step(self, t: float, dt: float, u: torch.Tensor, v: torch.Tensor) -> Dict:
+        st = _lib.StepStats()
+        check(_lib.load().hoefem_sim_step(self.h, t, dt, _dev(u), _dev(v), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _lib.StepStats._fields_}

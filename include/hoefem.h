@@ -1,0 +1,194 @@
+/*
+ * hoefem: MI355X (gfx950) implementation of the implicit elastodynamics step of the
+ * higher-order embedded FEM reference (InteractiveComputerGraphics/higher_order_embedded_fem).
+ *
+ * C ABI. Setup calls take HOST arrays (mesh, quadrature, boundary conditions); hot-path
+ * calls take DEVICE pointers and are ordered on the handle's HIP stream (the null stream
+ * unless hoefem_*_set_stream is called). Handles are not re-entrant; different handles
+ * may be used from different threads. Every function returns a hoefem_status; the
+ * message of the last failure on the calling thread is hoefem_last_error().
+ *
+ * Conventions shared with the reference:
+ *  - DOF vectors are node-major xyz (DVector of 3 * num_vertices entries).
+ *  - CSR matrices use int32 row offsets / column indices (the reference's MKL_INT boundary,
+ *    simulation_toolbox/src/fem/newton_cg.rs:621-658) and the pattern of
+ *    CsrParAssembler::assemble_pattern (fenris/src/assembly.rs:177-249).
+ *  - 3x3 matrices (F, P) are column-major; dP/dF is the 9x9 column-major matrix in the
+ *    vec() ordering of fenris/src/solid/materials.rs:745-761.
+ */
+#ifndef HOEFEM_H
+#define HOEFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hoefem_status {
+    HOEFEM_OK = 0,
+    HOEFEM_ERROR_INVALID_ARGUMENT = 1,
+    HOEFEM_ERROR_DEVICE = 2,
+    HOEFEM_ERROR_SINGULAR = 3,                     /* singular F or J (reference panics) */
+    HOEFEM_ERROR_PATTERN = 4,                      /* missing CSR entry (reference panics) */
+    HOEFEM_ERROR_CG_MAX_ITERATIONS = 5,            /* SolveErrorKind::MaxIterationsReached */
+    HOEFEM_ERROR_CG_INDEFINITE_OPERATOR = 6,       /* SolveErrorKind::IndefiniteOperator */
+    HOEFEM_ERROR_CG_INDEFINITE_PRECONDITIONER = 7, /* SolveErrorKind::IndefinitePreconditioner */
+    HOEFEM_ERROR_NEWTON_MAX_ITERATIONS = 8,        /* NewtonError::MaximumIterationsReached */
+    HOEFEM_ERROR_LINE_SEARCH = 9,                  /* NewtonError::LineSearchError */
+    HOEFEM_ERROR_SUBSTEPPING = 10,                 /* integrate_model_with_substeps gave up */
+    HOEFEM_ERROR_NOT_FINITE = 11,                  /* non-finite state (newton_cg.rs:224-225) */
+    HOEFEM_ERROR_JACOBIAN = 12                     /* NewtonError::JacobianError */
+} hoefem_status;
+
+typedef enum hoefem_element_type {
+    HOEFEM_TET4 = 0,
+    HOEFEM_TET10 = 1,
+    HOEFEM_TET20 = 2,
+    HOEFEM_HEX8 = 3,
+    HOEFEM_HEX20 = 4,
+    HOEFEM_HEX27 = 5
+} hoefem_element_type;
+
+/* EmbeddedQuadrature (fenris/src/embedding/mod.rs:67-99): one rule shared by the interior
+ * elements [0, num_interior) and one rule per interface element. A NodalModel
+ * (fenris/src/model.rs:32-46) is the case num_interior == num_elements. */
+typedef struct hoefem_quadrature {
+    int32_t interior_num_points;
+    const double* interior_weights;        /* [interior_num_points] */
+    const double* interior_points;         /* [interior_num_points * 3], reference coordinates */
+    const int64_t* interface_offsets;      /* [num_interface + 1], or NULL if no interface elements */
+    const double* interface_weights;
+    const double* interface_points;
+} hoefem_quadrature;
+
+typedef struct hoefem_model_desc {
+    int32_t element_type;                  /* hoefem_element_type */
+    int64_t num_vertices;
+    const double* vertices;                /* [num_vertices * 3] */
+    int64_t num_elements;
+    const int32_t* connectivity;           /* [num_elements * nodes_per_element], reference node order */
+    int64_t num_interior;                  /* interior elements first (separately colored) */
+    hoefem_quadrature mass_quadrature;
+    hoefem_quadrature stiffness_quadrature;
+    hoefem_quadrature elliptic_quadrature;
+    double mass_regularization_factor;     /* EmbeddedModel::mass_regularization_factor, 0 = off */
+} hoefem_model_desc;
+
+typedef struct hoefem_model hoefem_model;
+typedef struct hoefem_schwarz hoefem_schwarz;
+typedef struct hoefem_sim hoefem_sim;
+
+const char* hoefem_last_error(void);
+int32_t hoefem_nodes_per_element(int32_t element_type);
+
+/* From<YoungPoisson> for LameParameters (fenris/src/solid/materials.rs:27-38), including the
+ * reference's lambda convention documented in errata/tex/errata.tex:24-38. */
+void hoefem_lame_from_young_poisson(double young, double poisson, double* mu, double* lambda);
+
+/* StableNeoHookeanMaterial::compute_stress_tensor / build_stable_neohookean_dp_df
+ * (fenris/src/solid/materials.rs:299-322, :506-557) on nq deformation gradients.
+ * F, P: [nq * 9] column-major; dPdF: [nq * 81]. Either output may be NULL. Device pointers. */
+hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const double* F, double* P, double* dPdF);
+
+/* ElasticityModelParallel for NodalModel / EmbeddedModel (fenris/src/solid/mod.rs:124-172,
+ * fenris/src/solid/impl_model.rs:111-164, fenris/src/embedding/mod.rs:506-639).
+ * Creation builds the greedy element coloring (paradis/src/coloring.rs:48-72), the CSR
+ * pattern (fenris/src/assembly.rs:177-249) and the device-side element data. */
+hoefem_status hoefem_model_create(const hoefem_model_desc* desc, hoefem_model** out);
+void hoefem_model_destroy(hoefem_model* model);
+hoefem_status hoefem_model_set_stream(hoefem_model* model, void* hip_stream);
+int64_t hoefem_model_ndof(const hoefem_model* model);
+int64_t hoefem_model_nnz(const hoefem_model* model);
+/* host outputs: rowptr [ndof + 1], col [nnz] */
+hoefem_status hoefem_model_pattern(const hoefem_model* model, int32_t* rowptr, int32_t* col);
+/* pass 0 = interior elements, pass 1 = interface elements; offsets [ncolors + 1], labels = element ids */
+int32_t hoefem_model_num_colors(const hoefem_model* model, int32_t pass);
+hoefem_status hoefem_model_colors(const hoefem_model* model, int32_t pass, int32_t* offsets, int32_t* labels);
+
+/* assemble_elastic_pseudo_forces_into_par: f += f_int(u) (device vectors of ndof). */
+hoefem_status hoefem_model_forces_add(hoefem_model* model, double mu, double lambda, const double* u, double* f);
+/* assemble_transformed_stiffness_into_par with NoTransformation: values += K(u) (device, nnz). */
+hoefem_status hoefem_model_stiffness_add(hoefem_model* model, double mu, double lambda, const double* u,
+                                         double* values);
+/* ElasticityModel::assemble_mass + to_csr: values = M (device, nnz). */
+hoefem_status hoefem_model_mass(hoefem_model* model, double density, double* values);
+/* apply_homogeneous_dirichlet_bc_csr (fenris/src/assembly.rs:799-869) on a matrix with the
+ * model pattern; nodes is a HOST array. */
+hoefem_status hoefem_model_dirichlet_csr(hoefem_model* model, int64_t num_nodes, const int32_t* nodes,
+                                         double* values);
+/* y = beta * y + alpha * A x for a matrix with the model pattern (spmv_csr,
+ * fenris/src/sparse.rs:1168-1199; MklCsrLinearOperator::apply, newton_cg.rs:595-619). */
+hoefem_status hoefem_model_spmv(hoefem_model* model, const double* values, double beta, double* y, double alpha,
+                                const double* x);
+
+/* Schwarz preconditioner (simulation_toolbox/src/fem/schwarz_precond.rs:26-216, :250-342;
+ * SchwarzPreconditioner::update / apply, newton_cg.rs:297-370). Subdomains are node sets
+ * given as a NestedVec (offsets [num_subdomains + 1], nodes); untreated nodes get a
+ * Jacobi diagonal. All inputs are HOST arrays. */
+hoefem_status hoefem_schwarz_create(hoefem_model* model, int64_t num_subdomains, const int64_t* offsets,
+                                    const int32_t* nodes, int64_t num_untreated, const int32_t* untreated,
+                                    hoefem_schwarz** out);
+void hoefem_schwarz_destroy(hoefem_schwarz* s);
+int64_t hoefem_schwarz_nnz(const hoefem_schwarz* s);
+hoefem_status hoefem_schwarz_pattern(const hoefem_schwarz* s, int32_t* rowptr, int32_t* col);
+/* P <- Schwarz(J); J_values: device, model pattern. */
+hoefem_status hoefem_schwarz_update(hoefem_schwarz* s, const double* J_values);
+/* copies P's values (device, schwarz nnz) */
+hoefem_status hoefem_schwarz_values(hoefem_schwarz* s, double* values);
+/* y = P x (device) */
+hoefem_status hoefem_schwarz_apply(hoefem_schwarz* s, const double* x, double* y);
+
+/* ConjugateGradient::solve_with_guess (fenris/src/cg.rs:377-482) with the relative residual
+ * criterion (:85-124). A has the model pattern (device values); the preconditioner is the
+ * Schwarz operator, or the Jacobi diagonal of A when schwarz == NULL. b, x: device.
+ * Returns HOEFEM_ERROR_CG_* for the SolveErrorKind cases; *iterations is always written. */
+hoefem_status hoefem_pcg(hoefem_model* model, const double* A_values, hoefem_schwarz* schwarz, const double* b,
+                         double* x, double tolerance, int64_t max_iterations, int64_t* iterations);
+
+/* NewtonCgIntegrator3d (simulation_toolbox/src/fem/newton_cg.rs:43-295) over
+ * backward_euler_step (hamilton2/src/integrators/euler.rs:155-199) and
+ * newton_line_search + BacktrackingLineSearch (hamilton2/src/newton.rs:76-246). */
+typedef struct hoefem_sim_desc {
+    double mu, lambda, density;
+    double gravity;                        /* Gravity::Scalar on the last axis */
+    int32_t gravity_enabled;
+    int64_t num_dirichlet;
+    const int32_t* dirichlet_nodes;        /* Homogeneous BC nodes (host) */
+    int64_t num_subdomains;                /* 0: DiagonalPreconditioner */
+    const int64_t* subdomain_offsets;
+    const int32_t* subdomain_nodes;
+    int64_t num_untreated;
+    const int32_t* untreated_nodes;
+    double newton_tolerance_factor;        /* reference: 1e-5 (newton_cg.rs:282-286) */
+    int64_t max_newton_iter;               /* reference: 100 */
+    double cg_tolerance;                   /* reference: 1e-5 (newton_cg.rs:761) */
+    int64_t cg_max_iter;                   /* reference: 500 (newton_cg.rs:772) */
+} hoefem_sim_desc;
+
+typedef struct hoefem_step_stats {
+    int64_t newton_iterations;
+    int64_t cg_iterations;
+    int64_t substeps;
+    int64_t residual_evaluations;
+    double final_residual;
+    double tolerance;
+} hoefem_step_stats;
+
+hoefem_status hoefem_sim_create(hoefem_model* model, const hoefem_sim_desc* desc, hoefem_sim** out);
+void hoefem_sim_destroy(hoefem_sim* sim);
+double hoefem_sim_representative_force(const hoefem_sim* sim);
+/* mass matrix with Dirichlet conditions applied (device, nnz) */
+hoefem_status hoefem_sim_mass_values(hoefem_sim* sim, double* values);
+/* J = M + dt^2 K with Dirichlet conditions of the last Newton iteration (device, nnz) */
+hoefem_status hoefem_sim_jacobian_values(hoefem_sim* sim, double* values);
+/* F(w) of BackwardEulerImplicitFunction::eval_into at (u, v) (device vectors) */
+hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const double* u, const double* v,
+                                  const double* w, double* F);
+/* one time step: u, v device vectors updated in place */
+hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, double* v, hoefem_step_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
