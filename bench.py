@@ -1,0 +1,170 @@
+#!/usr/bin/env python
+"""Throughput of the implicit elastodynamics step on one MI355X.
+
+Workload (default): the embedded higher-order scene - a 24^3 Hex20 background grid over
+[-1, 1]^3 cut by a hollow cylinder (r in [0.6, 0.8], |z| <= 0.8), 8-point reduced quadrature
+per cell, Stable Neo-Hookean E = 5e5, nu = 0.45, rho = 1000, bottom cap fixed, gravity,
+dt = 1/200, 20 Backward-Euler steps, Newton with backtracking line search to a 1e-6 relative
+residual (|F| <= dt * 1e-6 * |M a_rep|), Schwarz-preconditioned CG (rel. 1e-5, <= 500
+iterations), all in FP64. Synthetic, seeded inputs.
+
+A "step" is one time step. The timed region runs K steps from the rest state after W
+untimed warm-up steps (also from rest), bracketed by device synchronizations.
+
+Printed JSON line: the metric, the roofline of the dominant kernel (the fused CG
+system-matrix kernel: direction update + SpMV + p.Ap) measured with HIP events inside the
+library on the model stream, and the CPU baseline: the FP64 oracle (the C++ restatement of
+the reference's CPU path, OpenMP over element colors) on a bounded sample of the same
+workload, with the displacement error of the GPU run against it.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
+    p.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
+    p.add_argument("--no-profile", action="store_true", help="do not bracket the CG kernel with events")
+    return p.parse_args()
+
+
+def cg_kernel_bytes(model) -> int:
+    """Algorithmic bytes of one launch of the CG system-matrix kernel: every matrix value
+    once (8 B), one node-level column index per 3x3 block (4 B per 9 values), row offsets,
+    node offsets, and four DOF vectors (read z, read p_old, write p_new, write Ap)."""
+    nnz, ndof = model.nnz, model.ndof
+    nv = ndof // 3
+    return nnz * 8 + (nnz // 9) * 4 + (ndof + 1) * 4 + (nv + 1) * 4 + 4 * ndof * 8
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from higher_order_embedded_fem_amd import _lib, fem, scenes
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    t_setup = time.time()
+    sc = scenes.config_scene(args.config)
+    model = fem.ElasticityModel(sc)
+    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6)
+    L = _lib.load()
+    print(f"[bench] {sc.name}: {sc.num_elements} elements, {sc.ndof} dofs, nnz {model.nnz}, "
+          f"setup {time.time() - t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    def run(steps, record=None):
+        u = torch.zeros(sc.ndof, dtype=torch.float64, device="cuda")
+        v = torch.zeros_like(u)
+        tot = {"newton_iterations": 0, "cg_iterations": 0, "substeps": 0}
+        for k in range(steps):
+            st = sim.step(k * sc.dt, sc.dt, u, v)
+            for key in tot:
+                tot[key] += st[key]
+            if record is not None and k + 1 == record[0]:
+                record[1] = u.cpu().numpy().copy()
+        return u, tot
+
+    run(args.warmup)
+    torch.cuda.synchronize()
+    L.hoefem_profile_enable(model.h, 0 if args.no_profile else 1)
+    rec = [max(args.cpu_steps, 1), None]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, tot = run(args.steps, record=rec)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    full_n, full_ms, all_n, all_ms = C.c_int64(), C.c_double(), C.c_int64(), C.c_double()
+    L.hoefem_profile_read(model.h, C.byref(full_n), C.byref(full_ms), C.byref(all_n), C.byref(all_ms))
+    L.hoefem_profile_enable(model.h, 0)
+    value = args.steps / elapsed
+    print(f"[bench] {args.steps} steps in {elapsed:.3f}s: {value:.2f} steps/s; newton {tot['newton_iterations']}, "
+          f"cg {tot['cg_iterations']}", file=sys.stderr, flush=True)
+
+    roofline = None
+    if full_n.value > 0:
+        avg_ms = full_ms.value / full_n.value
+        nbytes = cg_kernel_bytes(model)
+        achieved = nbytes / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_cg_matrix_kernel_{sc.name}.json")
+        if os.path.exists(tpath):
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": "k_cg_a (CG system-matrix kernel: p = z + beta p, Ap = J p, p.Ap)",
+                    "bytes_per_launch": nbytes, "avg_launch_us": round(avg_ms * 1e3, 3),
+                    "launches": full_n.value, "all_launches_avg_us": round(all_ms.value / max(all_n.value, 1) * 1e3, 3),
+                    "all_launches": all_n.value}
+
+    cpu = None
+    err = None
+    if args.cpu_steps > 0:
+        import oracle as O
+
+        threads = os.cpu_count() or 1
+        om = O.Model(sc)
+        osim = O.Sim(om, newton_tolerance_factor=1e-6, num_threads=threads)
+        uo = np.zeros(sc.ndof)
+        vo = np.zeros(sc.ndof)
+        c0 = time.perf_counter()
+        for k in range(args.cpu_steps):
+            osim.step(k * sc.dt, sc.dt, uo, vo)
+        c1 = time.perf_counter()
+        cpu = {"value": round(args.cpu_steps / (c1 - c0), 4), "unit": "timesteps/s", "cores": threads, "kind": "port",
+               "sample": f"first {args.cpu_steps} of the {sc.steps} steps of {sc.name} (same settings), "
+                         f"C++ FP64 oracle with OpenMP over element colors"}
+        if rec[1] is not None and args.cpu_steps <= args.steps:
+            err = float(np.linalg.norm(rec[1] - uo) / max(np.linalg.norm(uo), 1e-300))
+        print(f"[bench] cpu baseline {cpu['value']} steps/s on {threads} threads; gpu-vs-oracle u rel-L2 after "
+              f"{args.cpu_steps} steps: {err}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "timesteps/s at 1e-6 rel Newton residual",
+        "value": round(value, 3),
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": sc.name, "element": scenes.ELEMENT_NAMES[sc.element_type], "cells": sc.num_elements,
+                   "dofs": sc.ndof, "nnz": model.nnz, "dt": sc.dt, "newton_rel_tol": 1e-6, "cg_rel_tol": 1e-5,
+                   "preconditioner": "schwarz", "parallelism": "single GPU"},
+        "newton_iterations_per_step": tot["newton_iterations"] / args.steps,
+        "cg_iterations_per_step": tot["cg_iterations"] / args.steps,
+        "error_vs_oracle": None if err is None else {"u_rel_l2": err, "steps": args.cpu_steps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -39,6 +39,8 @@ EXPORTED = [
     "hoefem_schwarz_values",
     "hoefem_schwarz_apply",
     "hoefem_pcg",
+    "hoefem_profile_enable",
+    "hoefem_profile_read",
     "hoefem_sim_create",
     "hoefem_sim_destroy",
     "hoefem_sim_representative_force",
@@ -120,6 +122,7 @@ class SimDesc(C.Structure):
         ("max_newton_iter", C.c_int64),
         ("cg_tolerance", C.c_double),
         ("cg_max_iter", C.c_int64),
+        ("mixed_precision", C.c_int32),
     ]
 
 
@@ -181,6 +184,8 @@ def load(path: str = LIB_PATH):
     L.hoefem_schwarz_values.argtypes = [vp, vp]
     L.hoefem_schwarz_apply.argtypes = [vp, vp, vp]
     L.hoefem_pcg.argtypes = [vp, vp, vp, vp, vp, f64, i64, C.POINTER(i64)]
+    L.hoefem_profile_enable.argtypes = [vp, i32]
+    L.hoefem_profile_read.argtypes = [vp, C.POINTER(i64), C.POINTER(f64), C.POINTER(i64), C.POINTER(f64)]
     L.hoefem_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]
     L.hoefem_sim_destroy.argtypes = [vp]
     L.hoefem_sim_destroy.restype = None

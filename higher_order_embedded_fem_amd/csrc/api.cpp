@@ -18,6 +18,7 @@ void launch_be_begin(Model& m, const double* u, const double* v, double dt, doub
 void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask);
 void launch_zero_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask);
 void launch_finite_check(Model& m, const double* a, const double* b);
+void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n);
 
 thread_local std::string g_last_error;
 
@@ -134,6 +135,8 @@ struct Sim {
     DevBuf<int32_t> d_bc_nodes;
     int64_t n_bc = 0;
     DevBuf<double> d_M, d_J, d_gforce, d_up, d_vn, d_w, d_F, d_sol, d_uu, d_us, d_vs, d_scale, d_norm2;
+    bool mixed = false;   // FP32 element tangents, J and Schwarz operator
+    DevBuf<float> d_J32;
     PinnedBuf<double> h_norm2{1};
     CgWork cg;
     double rep_force = 0.0;
@@ -171,8 +174,12 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
     const int64_t n = 3 * m.nv, nnz = m.pattern.nnz();
     for (DevBuf<double>* b : {&s->d_gforce, &s->d_up, &s->d_vn, &s->d_w, &s->d_F, &s->d_sol, &s->d_uu, &s->d_us, &s->d_vs})
         b->ensure(n);
+    s->mixed = d->mixed_precision != 0;
     s->d_M.ensure(nnz);
-    s->d_J.ensure(nnz);
+    if (s->mixed)
+        s->d_J32.ensure(nnz);
+    else
+        s->d_J.ensure(nnz);
     s->d_scale.ensure(1);
     s->d_norm2.ensure(1);
     // mass matrix with Dirichlet conditions (fe_model.rs:490-515)
@@ -196,6 +203,7 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
     s->rep_force = std::sqrt(ss);
     if (d->num_subdomains > 0) {
         s->schwarz.reset(new Schwarz());
+        s->schwarz->fp32 = s->mixed;
         schwarz_setup(*s->schwarz, m, d->num_subdomains, d->subdomain_offsets, d->subdomain_nodes, d->num_untreated,
                       d->untreated_nodes);
     }
@@ -226,11 +234,19 @@ static double sim_eval(Sim& s, double dt, const double* w) {
 static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double* rhs, double* sol) {
     Model& m = *s.model;
     launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
-    launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_uu.get());
+    launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_uu.get(), s.mixed);
     const double alpha = -dt * dt;
-    launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream);
-    launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(), m.stream);
-    if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
+    if (s.mixed) {
+        launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -alpha, s.d_M.get(), true, m.stream);
+        launch_dirichlet(m.pattern, s.d_J32.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
+                         m.stream);
+        if (s.schwarz) schwarz_update(*s.schwarz, s.d_J32.get());
+    } else {
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream);
+        launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
+                         m.stream);
+        if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
+    }
     const int e = read_and_clear_err(m);
     if (e) {
         s.last_err_flags |= e;
@@ -238,7 +254,10 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
     }
     HIP_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * 3 * m.nv, m.stream));
     int64_t it = 0;
-    hoefem_status st = pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter, &it);
+    hoefem_status st = s.mixed ? pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
+                                           s.d.cg_max_iter, &it)
+                               : pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
+                                           s.d.cg_max_iter, &it);
     s.cg_total += it;
     if (st == HOEFEM_ERROR_CG_MAX_ITERATIONS || st == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) st = HOEFEM_OK;
     return st;
@@ -413,6 +432,7 @@ hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const d
 hoefem_status hoefem_model_create(const hoefem_model_desc* desc, hoefem_model** out) {
     return guarded([&] {
         if (!out) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null output");
+        if (!desc) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null descriptor");
         std::unique_ptr<hoefem_model> h(new hoefem_model());
         model_create(h->m, desc);
         *out = h.release();
@@ -472,7 +492,7 @@ hoefem_status hoefem_model_stiffness_add(hoefem_model* model, double mu, double 
         Model& m = M_(model);
         double mr, lr;
         dev::snh_reparam(mu, lambda, &mr, &lr);
-        launch_element_stiffness(m, mr, lr, u);
+        launch_element_stiffness(m, mr, lr, u, false);
         launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), values, 0, 0.0, nullptr, false, m.stream);
         raise_kernel_errors(m);
     });
@@ -577,6 +597,26 @@ hoefem_status hoefem_pcg(hoefem_model* model, const double* A, hoefem_schwarz* s
     return st;
 }
 
+hoefem_status hoefem_profile_enable(hoefem_model* model, int32_t enable) {
+    return guarded([&] {
+        Model& m = M_(model);
+        m.prof = enable != 0;
+        m.prof_full_launches = m.prof_all_launches = 0;
+        m.prof_full_ms = m.prof_all_ms = 0.0;
+    });
+}
+
+hoefem_status hoefem_profile_read(hoefem_model* model, int64_t* full_launches, double* full_ms, int64_t* all_launches,
+                                  double* all_ms) {
+    return guarded([&] {
+        Model& m = M_(model);
+        if (full_launches) *full_launches = m.prof_full_launches;
+        if (full_ms) *full_ms = m.prof_full_ms;
+        if (all_launches) *all_launches = m.prof_all_launches;
+        if (all_ms) *all_ms = m.prof_all_ms;
+    });
+}
+
 hoefem_status hoefem_sim_create(hoefem_model* model, const hoefem_sim_desc* desc, hoefem_sim** out) {
     return guarded([&] {
         Model& m = M_(model);
@@ -605,7 +645,11 @@ hoefem_status hoefem_sim_jacobian_values(hoefem_sim* sim, double* values) {
     return guarded([&] {
         if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
         Model& m = *sim->s.model;
-        HIP_CHECK(hipMemcpyAsync(values, sim->s.d_J.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice, m.stream));
+        if (sim->s.mixed)
+            launch_f32_to_f64(m, sim->s.d_J32.get(), values, m.pattern.nnz());
+        else
+            HIP_CHECK(hipMemcpyAsync(values, sim->s.d_J.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice,
+                                     m.stream));
         HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }

@@ -115,11 +115,11 @@ __global__ __launch_bounds__(64) void k_element_forces(int64_t ne, const int32_t
 constexpr int STIFF_THREADS = 256;
 constexpr int STIFF_QC = 16;
 
-template <int ET>
+template <int ET, class T>
 __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
     int64_t ne, const int32_t* __restrict__ conn, const double* __restrict__ verts, const int32_t* __restrict__ qoff,
-    const double* __restrict__ qw, const double* __restrict__ qxi, double mu, double lambda,
-    const double* __restrict__ u, double* __restrict__ ke, int* err) {
+    const double* __restrict__ qw, const double* __restrict__ qxi, T mu, T lambda,
+    const double* __restrict__ u, T* __restrict__ ke, int* err) {
     constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN, M3 = 3 * N;
     constexpr int PAIRS = N * (N + 1) / 2;
     constexpr int ENTRIES = 9 * PAIRS;
@@ -127,10 +127,10 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
     const int64_t e = blockIdx.x;
     const int t = threadIdx.x;
     __shared__ int s_node[N];
-    __shared__ double s_u[3 * N];
-    __shared__ double s_X[3 * GN];
-    __shared__ double s_C[STIFF_QC][81];
-    __shared__ double s_a[STIFF_QC][M3];
+    __shared__ T s_u[3 * N];
+    __shared__ T s_X[3 * GN];
+    __shared__ T s_C[STIFF_QC][81];
+    __shared__ T s_a[STIFF_QC][M3];
     __shared__ unsigned char s_pI[PAIRS], s_pJ[PAIRS];
     if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
     if (t == 0) {
@@ -143,50 +143,50 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
             }
     }
     __syncthreads();
-    for (int k = t; k < 3 * N; k += STIFF_THREADS) s_u[k] = u[3 * (int64_t)s_node[k / 3] + k % 3];
-    for (int k = t; k < 3 * GN; k += STIFF_THREADS) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
+    for (int k = t; k < 3 * N; k += STIFF_THREADS) s_u[k] = T(u[3 * (int64_t)s_node[k / 3] + k % 3]);
+    for (int k = t; k < 3 * GN; k += STIFF_THREADS) s_X[k] = T(verts[3 * (int64_t)s_node[k / 3] + k % 3]);
     __syncthreads();
-    double acc[PER];
+    T acc[PER];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) acc[j] = 0.0;
+    for (int j = 0; j < PER; ++j) acc[j] = T(0);
     const int q0 = qoff[e], q1 = qoff[e + 1];
     for (int qb = q0; qb < q1; qb += STIFF_QC) {
         const int cnt = min(STIFF_QC, q1 - qb);
         if (t < cnt) {
             const int q = qb + t;
-            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
-            double J[9], Jinv[9];
+            const T xi[3] = {T(qxi[3 * q]), T(qxi[3 * q + 1]), T(qxi[3 * q + 2])};
+            T J[9], Jinv[9];
             ref_jacobian<ET>(s_X, xi, J);
-            const double detJ = det3(J);
+            const T detJ = det3(J);
             bool ok = inv3(J, Jinv);
-            double psi[4];
+            T psi[4];
             tet4_psi(xi, psi);
-            double ug[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // u_grad(a, b) = sum_I G(a, I) u(b, I)
+            T ug[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};  // u_grad(a, b) = sum_I G(a, I) u(b, I)
             if (ok) {
                 for (int I = 0; I < N; ++I) {
-                    double Gr[3];
+                    T Gr[3];
                     grad_node<ET>(I, xi, psi, Gr);
-                    double G[3];
+                    T G[3];
                     // G = J^-T G_ref : G(a) = sum_c Jinv(c, a) G_ref(c)
                     for (int a = 0; a < 3; ++a) G[a] = Jinv[0 + 3 * a] * Gr[0] + Jinv[1 + 3 * a] * Gr[1] + Jinv[2 + 3 * a] * Gr[2];
                     for (int b = 0; b < 3; ++b)
                         for (int a = 0; a < 3; ++a) ug[a + 3 * b] += G[a] * s_u[3 * I + b];
                     for (int a = 0; a < 3; ++a) s_a[t][3 * I + a] = G[a];
                 }
-                const double sq = sqrt(qw[q] * fabs(detJ));
+                const T sq = sqrt(T(qw[q]) * fabs(detJ));
                 for (int k = 0; k < M3; ++k) s_a[t][k] *= sq;
-                double F[9];
+                T F[9];
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b) F[b + 3 * a] = ug[a + 3 * b];
-                F[0] += 1.0;
-                F[4] += 1.0;
-                F[8] += 1.0;
+                F[0] += T(1);
+                F[4] += T(1);
+                F[8] += T(1);
                 ok = snh_dpdf(mu, lambda, F, s_C[t]);
             }
             if (!ok) {
                 atomicOr(err, KERR_SINGULAR);
-                for (int k = 0; k < 81; ++k) s_C[t][k] = 0.0;
-                for (int k = 0; k < M3; ++k) s_a[t][k] = 0.0;
+                for (int k = 0; k < 81; ++k) s_C[t][k] = T(0);
+                for (int k = 0; k < M3; ++k) s_a[t][k] = T(0);
             }
         }
         __syncthreads();
@@ -198,10 +198,10 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
                 const int i = ij % 3, jj = ij / 3;
                 const int I = s_pI[p], Jn = s_pJ[p];
                 for (int k2 = 0; k2 < cnt; ++k2) {
-                    const double* C = s_C[k2];
-                    const double* aI = s_a[k2] + 3 * I;
-                    const double* aJ = s_a[k2] + 3 * Jn;
-                    double res = 0.0;
+                    const T* C = s_C[k2];
+                    const T* aI = s_a[k2] + 3 * I;
+                    const T* aJ = s_a[k2] + 3 * Jn;
+                    T res = T(0);
                     for (int l = 0; l < 3; ++l)
                         for (int k = 0; k < 3; ++k) res += C[(3 * k + i) + 9 * (3 * l + jj)] * aI[k] * aJ[l];
                     acc[j] += res;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
         }
         __syncthreads();
     }
-    double* out = ke + e * (int64_t)M3 * M3;
+    T* out = ke + e * (int64_t)M3 * M3;
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         const int ent = t + STIFF_THREADS * j;
@@ -275,11 +275,12 @@ __global__ __launch_bounds__(256) void k_element_mass(int64_t ne, const int32_t*
 // fenris/src/assembly.rs:1042-1075, without the linear search: slots are precomputed).
 // mode 0: values = (values or 0) + sum; mode 1: values = (0 + c * sum) + 1 * M (J combination,
 // simulation_toolbox/src/fem/system_assembly.rs:196-235).
+template <class TB, class TV>
 __global__ __launch_bounds__(64) void k_block_gather(
     const int32_t* __restrict__ nptr, const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ diag_only,
     const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk, const int32_t* __restrict__ inc_loc,
     const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off, const int64_t* __restrict__ slot_off,
-    const int32_t* __restrict__ slot, const double* __restrict__ blocks, double* __restrict__ values, int mode,
+    const int32_t* __restrict__ slot, const TB* __restrict__ blocks, TV* __restrict__ values, int mode,
     double c, const double* __restrict__ M, int zero_first) {
     extern __shared__ double s_rb[];
     const int64_t v = blockIdx.x;
@@ -291,30 +292,30 @@ __global__ __launch_bounds__(64) void k_block_gather(
     const int R = 9 * k;
     const int64_t r0 = rowptr[3 * v];
     // rows 3v, 3v+1, 3v+2 are contiguous: row i starts at r0 + i * rw
-    for (int idx = t; idx < R; idx += 64) s_rb[idx] = (zero_first || mode == 1) ? 0.0 : values[r0 + idx];
+    for (int idx = t; idx < R; idx += 64) s_rb[idx] = (zero_first || mode == 1) ? 0.0 : (double)values[r0 + idx];
     __syncthreads();
     for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
         const int b = inc_blk[it], a = inc_loc[it];
         const int mb = msz[b];
         const int bw = 3 * mb;
-        const double* base = blocks + val_off[b] + (int64_t)(3 * a) * bw;
+        const TB* base = blocks + val_off[b] + (int64_t)(3 * a) * bw;
         const int32_t* sl = slot + slot_off[b] + (int64_t)a * mb;
         for (int idx = t; idx < 3 * bw; idx += 64) {
             const int i = idx / bw, cc = idx % bw;
             const int bb = cc / 3, jj = cc % 3;
-            s_rb[i * rw + 3 * sl[bb] + jj] += base[idx];
+            s_rb[i * rw + 3 * sl[bb] + jj] += (double)base[idx];
         }
         __syncthreads();
     }
     for (int idx = t; idx < R; idx += 64) {
         const double s = s_rb[idx];
         if (mode == 0)
-            values[r0 + idx] = s;
+            values[r0 + idx] = (TV)s;
         else {
             double v2 = 0.0;
             v2 += c * s;
             v2 += 1.0 * M[r0 + idx];
-            values[r0 + idx] = v2;
+            values[r0 + idx] = (TV)v2;
         }
     }
 }
@@ -340,32 +341,32 @@ __global__ void k_force_gather(int64_t nn, const int32_t* __restrict__ inc_ptr, 
 // ------------------------------------------------------------------ SpMV
 // y = beta*y + alpha*A x on a node-blocked CSR matrix: G lanes per node share the node's
 // column list; each lane reads 3 contiguous values of each of the node's 3 rows.
-template <int G>
+template <int G, class VT>
 __global__ __launch_bounds__(256) void k_spmv(int64_t nn, const int32_t* __restrict__ nptr,
                                               const int32_t* __restrict__ ncol, const int32_t* __restrict__ rowptr,
-                                              const uint8_t* __restrict__ diag_only, const double* __restrict__ A,
+                                              const uint8_t* __restrict__ diag_only, const VT* __restrict__ A,
                                               double beta, double* __restrict__ y, double alpha,
                                               const double* __restrict__ x) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
     const int lane = (int)(gid % G);
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
     const bool valid = v < nn;
     if (valid) {
         if (diag_only[v]) {
             if (lane == 0) {
-                a0 = A[rowptr[3 * v]] * x[3 * v];
-                a1 = A[rowptr[3 * v + 1]] * x[3 * v + 1];
-                a2 = A[rowptr[3 * v + 2]] * x[3 * v + 2];
+                a0 = A[rowptr[3 * v]] * VT(x[3 * v]);
+                a1 = A[rowptr[3 * v + 1]] * VT(x[3 * v + 1]);
+                a2 = A[rowptr[3 * v + 2]] * VT(x[3 * v + 2]);
             }
         } else {
             const int kb = nptr[v], k = nptr[v + 1] - kb;
-            const double* R0 = A + rowptr[3 * v];
-            const double* R1 = R0 + 3 * k;
-            const double* R2 = R1 + 3 * k;
+            const VT* R0 = A + rowptr[3 * v];
+            const VT* R1 = R0 + 3 * k;
+            const VT* R2 = R1 + 3 * k;
             for (int j = lane; j < k; j += G) {
                 const int64_t b = ncol[kb + j];
-                const double x0 = x[3 * b], x1 = x[3 * b + 1], x2 = x[3 * b + 2];
+                const VT x0 = VT(x[3 * b]), x1 = VT(x[3 * b + 1]), x2 = VT(x[3 * b + 2]);
                 a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
                 a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
                 a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
@@ -380,14 +381,14 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t nn, const int32_t* __restr
     }
     if (valid && lane == 0) {
         if (beta == 0.0) {
-            y[3 * v] = alpha * a0;
-            y[3 * v + 1] = alpha * a1;
-            y[3 * v + 2] = alpha * a2;
+            y[3 * v] = alpha * double(a0);
+            y[3 * v + 1] = alpha * double(a1);
+            y[3 * v + 2] = alpha * double(a2);
         } else {
             double y0 = beta * y[3 * v], y1 = beta * y[3 * v + 1], y2 = beta * y[3 * v + 2];
-            y0 += alpha * a0;
-            y1 += alpha * a1;
-            y2 += alpha * a2;
+            y0 += alpha * double(a0);
+            y1 += alpha * double(a1);
+            y2 += alpha * double(a2);
             y[3 * v] = y0;
             y[3 * v + 1] = y1;
             y[3 * v + 2] = y2;
@@ -399,13 +400,14 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t nn, const int32_t* __restr
 // apply_homogeneous_dirichlet_bc_csr (fenris/src/assembly.rs:799-869): scale = |first non-zero
 // diagonal entry|; Dirichlet rows and (by symmetry of the pattern) Dirichlet columns zeroed,
 // Dirichlet diagonal := scale.
+template <class VT>
 __global__ __launch_bounds__(256) void k_first_nonzero_diag(int64_t ndof, const int32_t* __restrict__ diag_pos,
-                                                            const double* __restrict__ vals, double* scale) {
+                                                            const VT* __restrict__ vals, double* scale) {
     __shared__ int64_t s_min[256];
     for (int64_t base = 0; base < ndof; base += 256) {
         const int64_t r = base + threadIdx.x;
         int64_t found = INT64_MAX;
-        if (r < ndof && diag_pos[r] >= 0 && vals[diag_pos[r]] != 0.0) found = r;
+        if (r < ndof && diag_pos[r] >= 0 && vals[diag_pos[r]] != VT(0)) found = r;
         s_min[threadIdx.x] = found;
         __syncthreads();
         for (int s = 128; s > 0; s >>= 1) {
@@ -415,22 +417,23 @@ __global__ __launch_bounds__(256) void k_first_nonzero_diag(int64_t ndof, const 
         const int64_t m = s_min[0];
         __syncthreads();
         if (m != INT64_MAX) {
-            if (threadIdx.x == 0) *scale = fabs(vals[diag_pos[m]]);
+            if (threadIdx.x == 0) *scale = fabs((double)vals[diag_pos[m]]);
             return;
         }
     }
     if (threadIdx.x == 0) *scale = 1.0;
 }
 
+template <class VT>
 __global__ void k_dirichlet_rows(const int32_t* __restrict__ bc_nodes, int64_t nbc, const int32_t* __restrict__ nptr,
                                  const int32_t* __restrict__ ncol, const int32_t* __restrict__ rowptr,
                                  const uint8_t* __restrict__ diag_only, const uint8_t* __restrict__ dirmask,
-                                 double* __restrict__ vals, const double* __restrict__ scale) {
+                                 VT* __restrict__ vals, const double* __restrict__ scale) {
     const int64_t idx = blockIdx.x;
     if (idx >= nbc) return;
     const int64_t v = bc_nodes[idx];
     const bool vdir = dirmask[v] != 0;
-    const double sc = *scale;
+    const VT sc = (VT)*scale;
     if (diag_only[v]) {
         if (vdir && threadIdx.x < 3) vals[rowptr[3 * v + threadIdx.x]] = sc;
         return;
@@ -442,9 +445,9 @@ __global__ void k_dirichlet_rows(const int32_t* __restrict__ bc_nodes, int64_t n
         const int64_t b = ncol[kb + j];
         const int64_t pos = rowptr[3 * v + i] + cc;
         if (vdir)
-            vals[pos] = (b == v && jj == i) ? sc : 0.0;
+            vals[pos] = (b == v && jj == i) ? sc : VT(0);
         else if (dirmask[b])
-            vals[pos] = 0.0;
+            vals[pos] = VT(0);
     }
 }
 
@@ -494,12 +497,19 @@ void launch_element_forces(Model& m, double mu_r, double lambda_r, const double*
     check_launch("k_element_forces");
 }
 
-void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const double* u) {
-    m.d_ke.ensure((size_t)m.ne * 9 * m.n * m.n);
+void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const double* u, bool fp32) {
     if (m.ne == 0) return;
-    HOEFEM_DISPATCH(m.type, k_element_stiffness<ET><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
-                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
-                                m.q_stiff.xi.get(), mu_r, lambda_r, u, m.d_ke.get(), m.d_err.get()));
+    if (fp32) {
+        m.d_ke32.ensure((size_t)m.ne * 9 * m.n * m.n);
+        HOEFEM_DISPATCH(m.type, k_element_stiffness<ET, float><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
+                                    m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
+                                    m.q_stiff.xi.get(), (float)mu_r, (float)lambda_r, u, m.d_ke32.get(), m.d_err.get()));
+    } else {
+        m.d_ke.ensure((size_t)m.ne * 9 * m.n * m.n);
+        HOEFEM_DISPATCH(m.type, k_element_stiffness<ET, double><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
+                                    m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
+                                    m.q_stiff.xi.get(), mu_r, lambda_r, u, m.d_ke.get(), m.d_err.get()));
+    }
     check_launch("k_element_stiffness");
 }
 
@@ -514,17 +524,24 @@ void launch_element_mass(Model& m, double density) {
     check_launch("k_element_mass");
 }
 
-void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const double* blocks, double* values,
-                         int mode, double c, const double* M, bool zero_first, hipStream_t s) {
+template <class TB, class TV>
+void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const TB* blocks, TV* values, int mode,
+                         double c, const double* M, bool zero_first, hipStream_t s) {
     if (pat.num_nodes == 0) return;
     const size_t lds = (size_t)9 * std::max(pat.max_nbr, 1) * sizeof(double);
     if (lds > 64 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "node row block exceeds 64 KiB of LDS");
-    hipLaunchKernelGGL(k_block_gather, dim3((unsigned)pat.num_nodes), dim3(64), lds, s, pat.d_nptr.get(),
-                       pat.d_rowptr.get(), pat.d_diag_only.get(), sc.d_inc_ptr.get(), sc.d_inc_blk.get(),
-                       sc.d_inc_loc.get(), sc.d_m.get(), sc.d_val_off.get(), sc.d_slot_off.get(), sc.d_slot.get(),
-                       blocks, values, mode, c, M, zero_first ? 1 : 0);
+    k_block_gather<TB, TV><<<dim3((unsigned)pat.num_nodes), dim3(64), lds, s>>>(
+        pat.d_nptr.get(), pat.d_rowptr.get(), pat.d_diag_only.get(), sc.d_inc_ptr.get(), sc.d_inc_blk.get(),
+        sc.d_inc_loc.get(), sc.d_m.get(), sc.d_val_off.get(), sc.d_slot_off.get(), sc.d_slot.get(), blocks, values, mode,
+        c, M, zero_first ? 1 : 0);
     check_launch("k_block_gather");
 }
+template void launch_block_gather<double, double>(const NodePattern&, const BlockScatter&, const double*, double*, int,
+                                                  double, const double*, bool, hipStream_t);
+template void launch_block_gather<float, float>(const NodePattern&, const BlockScatter&, const float*, float*, int,
+                                                double, const double*, bool, hipStream_t);
+template void launch_block_gather<double, float>(const NodePattern&, const BlockScatter&, const double*, float*, int,
+                                                 double, const double*, bool, hipStream_t);
 
 void launch_force_gather(Model& m, double* f) {
     if (m.nv == 0) return;
@@ -541,15 +558,16 @@ int spmv_group_size(const NodePattern& pat) {
     return 8;
 }
 
-void launch_spmv(const NodePattern& pat, const double* vals, double beta, double* y, double alpha, const double* x,
+template <class VT>
+void launch_spmv(const NodePattern& pat, const VT* vals, double beta, double* y, double alpha, const double* x,
                  hipStream_t s) {
     if (pat.num_nodes == 0) return;
     const int G = spmv_group_size(pat);
     const int64_t threads = pat.num_nodes * G;
     const unsigned blocks = (unsigned)((threads + 255) / 256);
-#define SPMV_CASE(GG)                                                                                          \
-    hipLaunchKernelGGL(k_spmv<GG>, dim3(blocks), dim3(256), 0, s, pat.num_nodes, pat.d_nptr.get(),            \
-                       pat.d_ncol.get(), pat.d_rowptr.get(), pat.d_diag_only.get(), vals, beta, y, alpha, x)
+#define SPMV_CASE(GG)                                                                                             \
+    k_spmv<GG, VT><<<dim3(blocks), dim3(256), 0, s>>>(pat.num_nodes, pat.d_nptr.get(), pat.d_ncol.get(),           \
+                                                      pat.d_rowptr.get(), pat.d_diag_only.get(), vals, beta, y, alpha, x)
     if (G == 32)
         SPMV_CASE(32);
     else if (G == 16)
@@ -559,17 +577,24 @@ void launch_spmv(const NodePattern& pat, const double* vals, double beta, double
 #undef SPMV_CASE
     check_launch("k_spmv");
 }
+template void launch_spmv<double>(const NodePattern&, const double*, double, double*, double, const double*, hipStream_t);
+template void launch_spmv<float>(const NodePattern&, const float*, double, double*, double, const double*, hipStream_t);
 
-void launch_dirichlet(const NodePattern& pat, double* vals, const uint8_t* d_dirmask, const int32_t* d_bc_nodes,
+template <class VT>
+void launch_dirichlet(const NodePattern& pat, VT* vals, const uint8_t* d_dirmask, const int32_t* d_bc_nodes,
                       int64_t n_bc_nodes, double* d_scale, hipStream_t s) {
-    hipLaunchKernelGGL(k_first_nonzero_diag, dim3(1), dim3(256), 0, s, pat.ndof(), pat.d_diag_pos.get(), vals, d_scale);
+    k_first_nonzero_diag<VT><<<dim3(1), dim3(256), 0, s>>>(pat.ndof(), pat.d_diag_pos.get(), vals, d_scale);
     check_launch("k_first_nonzero_diag");
     if (n_bc_nodes == 0) return;
-    hipLaunchKernelGGL(k_dirichlet_rows, dim3((unsigned)n_bc_nodes), dim3(64), 0, s, d_bc_nodes, n_bc_nodes,
-                       pat.d_nptr.get(), pat.d_ncol.get(), pat.d_rowptr.get(), pat.d_diag_only.get(), d_dirmask, vals,
-                       d_scale);
+    k_dirichlet_rows<VT><<<dim3((unsigned)n_bc_nodes), dim3(64), 0, s>>>(d_bc_nodes, n_bc_nodes, pat.d_nptr.get(),
+                                                                        pat.d_ncol.get(), pat.d_rowptr.get(),
+                                                                        pat.d_diag_only.get(), d_dirmask, vals, d_scale);
     check_launch("k_dirichlet_rows");
 }
+template void launch_dirichlet<double>(const NodePattern&, double*, const uint8_t*, const int32_t*, int64_t, double*,
+                                       hipStream_t);
+template void launch_dirichlet<float>(const NodePattern&, float*, const uint8_t*, const int32_t*, int64_t, double*,
+                                      hipStream_t);
 
 void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF) {
     DevBuf<int> err;

@@ -50,9 +50,13 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // setup-time upload; blocking, so the host array may be released right after
     void upload(const T* h, size_t count, hipStream_t s) {
         ensure(count);
-        if (count) HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+        if (count) {
+            HIP_CHECK(hipStreamSynchronize(s));
+            HIP_CHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+        }
     }
     void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
     T* get() const { return p; }
@@ -147,8 +151,17 @@ struct Model {
     DeviceQuadrature q_mass, q_stiff, q_ell, q_interior_mass;
     DevBuf<double> d_fe;  // element force staging [E * 3n]
     DevBuf<double> d_ke;  // element matrix staging [E * (3n)^2]
+    DevBuf<float> d_ke32;  // FP32 element matrix staging (mixed precision)
     DevBuf<int> d_err;
     PinnedBuf<int> h_err{1};
+    // kernel timing of the CG system-matrix kernel (HIP events on the model stream)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    int64_t prof_full_launches = 0, prof_all_launches = 0;
+    double prof_full_ms = 0.0, prof_all_ms = 0.0;
+    ~Model() {
+        for (auto e : prof_ev) (void)hipEventDestroy(e);
+    }
 };
 
 // ------------------------------------------------------------------ Schwarz
@@ -163,8 +176,10 @@ struct Schwarz {
     std::vector<int32_t> jslot;    // per block entry (a, c): local index of node c in J's nbr(node a)
     DevBuf<int32_t> d_jslot, d_untreated, d_nodes;
     DevBuf<int64_t> d_off;
-    DevBuf<double> d_vals;         // P values
-    DevBuf<double> d_blocks;       // gathered / inverted blocks
+    bool fp32 = false;             // P stored in FP32 (mixed precision)
+    DevBuf<double> d_vals;         // P values (FP64)
+    DevBuf<float> d_vals32;        // P values (FP32)
+    DevBuf<double> d_blocks;       // gathered / inverted blocks (always FP64)
 };
 
 // ------------------------------------------------------------------ CG workspace
@@ -189,28 +204,36 @@ struct CgWork {
 
 // ------------------------------------------------------------------ kernels (assembly.hip)
 void launch_element_forces(Model& m, double mu_r, double lambda_r, const double* u);  // -> m.d_fe
-void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const double* u);  // -> m.d_ke
+void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const double* u, bool fp32);  // -> d_ke / d_ke32
 void launch_element_mass(Model& m, double density);  // -> m.d_ke (includes regularization)
 // values (+)= gathered blocks; mode 0: values += sum; mode 1: values = (0 + c*sum) + 1*M
-void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const double* blocks, double* values,
-                         int mode, double c, const double* M, bool zero_first, hipStream_t s);
+template <class TB, class TV>
+void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const TB* blocks, TV* values, int mode,
+                         double c, const double* M, bool zero_first, hipStream_t s);
 void launch_force_gather(Model& m, double* f);
-void launch_spmv(const NodePattern& pat, const double* vals, double beta, double* y, double alpha,
-                 const double* x, hipStream_t s);
-void launch_dirichlet(const NodePattern& pat, double* vals, const uint8_t* d_dirmask, const int32_t* d_bc_nodes,
+template <class VT>
+void launch_spmv(const NodePattern& pat, const VT* vals, double beta, double* y, double alpha, const double* x,
+                 hipStream_t s);
+template <class VT>
+void launch_dirichlet(const NodePattern& pat, VT* vals, const uint8_t* d_dirmask, const int32_t* d_bc_nodes,
                       int64_t n_bc_nodes, double* d_scale, hipStream_t s);
+int spmv_group_size(const NodePattern& pat);
 void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF);
 int read_and_clear_err(Model& m);  // synchronizes the model stream
 
 // ------------------------------------------------------------------ Schwarz (schwarz.hip)
 void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const int32_t* nodes, int64_t nunt,
-                   const int32_t* unt);
+                   const int32_t* unt);  // s.fp32 selects the value type of P
 void schwarz_update(Schwarz& s, const double* J);  // enqueues on the model stream; errors flagged in m.d_err
+void schwarz_update(Schwarz& s, const float* J);
 void schwarz_apply(Schwarz& s, const double* x, double* y);
 
 // ------------------------------------------------------------------ CG (solver.hip)
 // Solves A x = b. A has the model pattern; preconditioner = Schwarz or Jacobi(diag(A)).
 hoefem_status pcg_solve(Model& m, CgWork& w, const double* A, Schwarz* P, const double* b, double* x, double tol,
+                        int64_t max_iter, int64_t* iterations);
+// mixed precision: FP32 system matrix and FP32 Schwarz operator (P->fp32 must be set), FP64 vectors / dots
+hoefem_status pcg_solve(Model& m, CgWork& w, const float* A, Schwarz* P, const double* b, double* x, double tol,
                         int64_t max_iter, int64_t* iterations);
 
 }  // namespace hoefem

@@ -52,16 +52,22 @@ void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const
     s.d_untreated.upload(s.untreated, m.stream);
     s.d_off.upload(s.off, m.stream);
     s.d_nodes.upload(s.nodes, m.stream);
-    s.d_vals.ensure(std::max<int64_t>(s.P.nnz(), 1));
-    HIP_CHECK(hipMemsetAsync(s.d_vals.get(), 0, sizeof(double) * s.d_vals.n, m.stream));
+    if (s.fp32) {
+        s.d_vals32.ensure(std::max<int64_t>(s.P.nnz(), 1));
+        HIP_CHECK(hipMemsetAsync(s.d_vals32.get(), 0, sizeof(float) * s.d_vals32.n, m.stream));
+    } else {
+        s.d_vals.ensure(std::max<int64_t>(s.P.nnz(), 1));
+        HIP_CHECK(hipMemsetAsync(s.d_vals.get(), 0, sizeof(double) * s.d_vals.n, m.stream));
+    }
     s.d_blocks.ensure(std::max<int64_t>(s.scatter.val_off[nsub], 1));
 }
 
-// B_s(r, c) = J[3 g_a + i, 3 g_c + j], row-major (3m)^2
+// B_s(r, c) = J[3 g_a + i, 3 g_c + j], row-major (3m)^2, always FP64
+template <class VJ>
 __global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off,
                                  const int64_t* __restrict__ slot_off, const int64_t* __restrict__ noff,
                                  const int32_t* __restrict__ nodes, const int32_t* __restrict__ jslot,
-                                 const int32_t* __restrict__ jrowptr, const double* __restrict__ J,
+                                 const int32_t* __restrict__ jrowptr, const VJ* __restrict__ J,
                                  double* __restrict__ blocks) {
     const int64_t b = blockIdx.x;
     const int mb = msz[b], dim = 3 * mb;
@@ -71,7 +77,7 @@ __global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t*
         const int r = idx / dim, c = idx % dim;
         const int a = r / 3, i = r % 3, cb = c / 3, j = c % 3;
         const int k = jslot[slot_off[b] + (int64_t)a * mb + cb];
-        out[idx] = J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
+        out[idx] = (double)J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
     }
 }
 
@@ -227,27 +233,29 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_inverse(const int32_t* _
 }
 
 // untreated nodes: P_ii = 1 / J_ii (schwarz_precond.rs:195-213)
+template <class VJ, class VP>
 __global__ void k_schwarz_untreated(int64_t nunt, const int32_t* __restrict__ unt, const int32_t* __restrict__ prowptr,
-                                    const int32_t* __restrict__ jdiag, const double* __restrict__ J,
-                                    double* __restrict__ P, int* err) {
+                                    const int32_t* __restrict__ jdiag, const VJ* __restrict__ J,
+                                    VP* __restrict__ P, int* err) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= 3 * nunt) return;
     const int64_t row = 3 * (int64_t)unt[idx / 3] + idx % 3;
     const int32_t dp = jdiag[row];
-    const double entry = dp >= 0 ? J[dp] : 0.0;
+    const double entry = dp >= 0 ? (double)J[dp] : 0.0;
     if (entry != 0.0)
-        P[prowptr[row]] = 1.0 / entry;
+        P[prowptr[row]] = (VP)(1.0 / entry);
     else
         atomicOr(err, KERR_ZERO_DIAG);
 }
 
-void schwarz_update(Schwarz& s, const double* J) {
+template <class VJ>
+static void schwarz_update_t(Schwarz& s, const VJ* J) {
     Model& m = *s.model;
     const hipStream_t st = m.stream;
     if (s.nsub > 0) {
-        hipLaunchKernelGGL(k_schwarz_gather, dim3((unsigned)s.nsub), dim3(256), 0, st, s.scatter.d_m.get(),
-                           s.scatter.d_val_off.get(), s.scatter.d_slot_off.get(), s.d_off.get(), s.d_nodes.get(),
-                           s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get());
+        k_schwarz_gather<VJ><<<dim3((unsigned)s.nsub), dim3(256), 0, st>>>(
+            s.scatter.d_m.get(), s.scatter.d_val_off.get(), s.scatter.d_slot_off.get(), s.d_off.get(), s.d_nodes.get(),
+            s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get());
         check_launch("k_schwarz_gather");
         const int dim = 3 * s.scatter.max_m;
         const int nn = dim + (dim & 1);
@@ -262,19 +270,34 @@ void schwarz_update(Schwarz& s, const double* J) {
         hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st, s.scatter.d_m.get(),
                            s.scatter.d_val_off.get(), s.d_blocks.get(), ld);
         check_launch("k_jacobi_inverse");
-        launch_block_gather(s.P, s.scatter, s.d_blocks.get(), s.d_vals.get(), 0, 0.0, nullptr, true, st);
+        if (s.fp32)
+            launch_block_gather(s.P, s.scatter, s.d_blocks.get(), s.d_vals32.get(), 0, 0.0, nullptr, true, st);
+        else
+            launch_block_gather(s.P, s.scatter, s.d_blocks.get(), s.d_vals.get(), 0, 0.0, nullptr, true, st);
     }
     if (!s.untreated.empty()) {
         const int64_t n = 3 * (int64_t)s.untreated.size();
-        hipLaunchKernelGGL(k_schwarz_untreated, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                           (int64_t)s.untreated.size(), s.d_untreated.get(), s.P.d_rowptr.get(),
-                           m.pattern.d_diag_pos.get(), J, s.d_vals.get(), m.d_err.get());
+        const unsigned g = (unsigned)((n + 255) / 256);
+        if (s.fp32)
+            k_schwarz_untreated<VJ, float><<<dim3(g), dim3(256), 0, st>>>(
+                (int64_t)s.untreated.size(), s.d_untreated.get(), s.P.d_rowptr.get(), m.pattern.d_diag_pos.get(), J,
+                s.d_vals32.get(), m.d_err.get());
+        else
+            k_schwarz_untreated<VJ, double><<<dim3(g), dim3(256), 0, st>>>(
+                (int64_t)s.untreated.size(), s.d_untreated.get(), s.P.d_rowptr.get(), m.pattern.d_diag_pos.get(), J,
+                s.d_vals.get(), m.d_err.get());
         check_launch("k_schwarz_untreated");
     }
 }
 
+void schwarz_update(Schwarz& s, const double* J) { schwarz_update_t<double>(s, J); }
+void schwarz_update(Schwarz& s, const float* J) { schwarz_update_t<float>(s, J); }
+
 void schwarz_apply(Schwarz& s, const double* x, double* y) {
-    launch_spmv(s.P, s.d_vals.get(), 0.0, y, 1.0, x, s.model->stream);
+    if (s.fp32)
+        launch_spmv(s.P, s.d_vals32.get(), 0.0, y, 1.0, x, s.model->stream);
+    else
+        launch_spmv(s.P, s.d_vals.get(), 0.0, y, 1.0, x, s.model->stream);
 }
 
 }  // namespace hoefem

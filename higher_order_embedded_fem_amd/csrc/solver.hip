@@ -74,35 +74,38 @@ __device__ inline bool grid_reduce_last(const double (&v)[NV], double* partials,
     return true;
 }
 
+template <class VT>
 struct Csr3 {  // node-blocked CSR view
     int64_t nn;
     const int32_t* nptr;
     const int32_t* ncol;
     const int32_t* rowptr;
     const uint8_t* diag_only;
-    const double* vals;
+    const VT* vals;
 };
 
-// acc_i = sum_j A(3v+i, j) * xfun(j) over the node row block of v (G lanes, reduced into all lanes)
-template <int G, class XF>
-__device__ inline void row_block_dot(const Csr3& A, int64_t v, int lane, bool valid, XF xfun, double& a0, double& a1,
-                                     double& a2) {
-    a0 = a1 = a2 = 0.0;
+// acc_i = sum_j A(3v+i, j) * xfun(j) over the node row block of v (G lanes, reduced into all
+// lanes). Products and accumulation in the matrix value type VT (FP64, or FP32 for the
+// mixed-precision operator); the result is returned in FP64.
+template <int G, class VT, class XF>
+__device__ inline void row_block_dot(const Csr3<VT>& A, int64_t v, int lane, bool valid, XF xfun, double& r0,
+                                     double& r1, double& r2) {
+    VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
     if (valid) {
         if (A.diag_only[v]) {
             if (lane == 0) {
-                a0 = A.vals[A.rowptr[3 * v]] * xfun(3 * v);
-                a1 = A.vals[A.rowptr[3 * v + 1]] * xfun(3 * v + 1);
-                a2 = A.vals[A.rowptr[3 * v + 2]] * xfun(3 * v + 2);
+                a0 = A.vals[A.rowptr[3 * v]] * VT(xfun(3 * v));
+                a1 = A.vals[A.rowptr[3 * v + 1]] * VT(xfun(3 * v + 1));
+                a2 = A.vals[A.rowptr[3 * v + 2]] * VT(xfun(3 * v + 2));
             }
         } else {
             const int kb = A.nptr[v], k = A.nptr[v + 1] - kb;
-            const double* R0 = A.vals + A.rowptr[3 * v];
-            const double* R1 = R0 + 3 * k;
-            const double* R2 = R1 + 3 * k;
+            const VT* R0 = A.vals + A.rowptr[3 * v];
+            const VT* R1 = R0 + 3 * k;
+            const VT* R2 = R1 + 3 * k;
             for (int j = lane; j < k; j += G) {
                 const int64_t b = A.ncol[kb + j];
-                const double x0 = xfun(3 * b), x1 = xfun(3 * b + 1), x2 = xfun(3 * b + 2);
+                const VT x0 = VT(xfun(3 * b)), x1 = VT(xfun(3 * b + 1)), x2 = VT(xfun(3 * b + 2));
                 a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
                 a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
                 a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
@@ -115,6 +118,9 @@ __device__ inline void row_block_dot(const Csr3& A, int64_t v, int lane, bool va
         a1 += __shfl_xor(a1, off, G);
         a2 += __shfl_xor(a2, off, G);
     }
+    r0 = double(a0);
+    r1 = double(a1);
+    r2 = double(a2);
 }
 
 struct CgPtrs {
@@ -133,8 +139,8 @@ struct CgPtrs {
 };
 
 // preamble 1: r = b - A x ; |b|^2
-template <int G>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3 A, CgPtrs P) {
+template <int G, class VT>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3<VT> A, CgPtrs P) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
     const int lane = (int)(gid % G);
@@ -155,8 +161,8 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3 A, CgPtrs P) {
 }
 
 // preamble 2: z = P r ; zTr, |r|^2 ; initial status
-template <int G, bool JAC>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3 Pm, CgPtrs P) {
+template <int G, bool JAC, class VT>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
     const int lane = (int)(gid % G);
@@ -197,8 +203,8 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3 Pm, CgPtrs P) {
 }
 
 // A-kernel: p_new = z + beta * p_old ; Ap = A p_new ; pAp -> alpha
-template <int G>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3 A, CgPtrs P) {
+template <int G, class VT>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
     const double beta = sc->beta;
@@ -236,8 +242,8 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3 A, CgPtrs P) {
 }
 
 // B-kernel: x += alpha p ; r_new = r_old - alpha Ap ; z = P r_new ; zTr, rr -> beta, status
-template <int G, bool JAC>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3 Pm, CgPtrs P) {
+template <int G, bool JAC, class VT>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
     const double alpha = sc->alpha;
@@ -294,12 +300,13 @@ __global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
 }
 
 // DiagonalPreconditioner::update (simulation_toolbox/src/fem/newton_cg.rs:558-575)
-__global__ void k_jacobi_diag(int64_t ndof, const int32_t* __restrict__ diag_pos, const double* __restrict__ A,
+template <class VT>
+__global__ void k_jacobi_diag(int64_t ndof, const int32_t* __restrict__ diag_pos, const VT* __restrict__ A,
                               double* __restrict__ dinv, int* err) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= ndof) return;
     const int32_t dp = diag_pos[r];
-    const double a = dp >= 0 ? A[dp] : 0.0;
+    const double a = dp >= 0 ? (double)A[dp] : 0.0;
     if (a != 0.0)
         dinv[r] = 1.0 / a;
     else {
@@ -307,8 +314,6 @@ __global__ void k_jacobi_diag(int64_t ndof, const int32_t* __restrict__ diag_pos
         atomicOr(err, KERR_ZERO_DIAG);
     }
 }
-
-int spmv_group_size(const NodePattern& pat);
 
 void CgWork::ensure(int64_t nn_dof, int nblocks) {
     if (n != nn_dof) {
@@ -339,12 +344,13 @@ CgWork::~CgWork() {
         if (e) (void)hipEventDestroy(e);
 }
 
-static Csr3 view(const NodePattern& p, const double* vals) {
-    return Csr3{p.num_nodes, p.d_nptr.get(), p.d_ncol.get(), p.d_rowptr.get(), p.d_diag_only.get(), vals};
+template <class VT>
+static Csr3<VT> view(const NodePattern& p, const VT* vals) {
+    return Csr3<VT>{p.num_nodes, p.d_nptr.get(), p.d_ncol.get(), p.d_rowptr.get(), p.d_diag_only.get(), vals};
 }
 
-template <int GA, int GB, bool JAC>
-static void cg_launch_all(Model& m, CgWork& w, const Csr3& A, const Csr3& Pm, const double* b, double* x,
+template <int GA, int GB, bool JAC, class VT>
+static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>& Pm, const double* b, double* x,
                           double tol, int64_t max_iter, int64_t* iterations, hoefem_status* out_status) {
     const hipStream_t s = m.stream;
     const int64_t nn = A.nn;
@@ -362,18 +368,31 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3& A, const Csr3& Pm, co
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
     HIP_CHECK(hipMemsetAsync(w.p[0].get(), 0, sizeof(double) * w.n, s));
     P.r_new = w.r[0].get();
-    hipLaunchKernelGGL(k_cg_pre1<GA>, dim3(blkA), dim3(CG_THREADS), 0, s, A, P);
-    hipLaunchKernelGGL((k_cg_pre2<GB, JAC>), dim3(blkB), dim3(CG_THREADS), 0, s, Pm, P);
+    k_cg_pre1<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
+    k_cg_pre2<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
     check_launch("cg preamble");
     int cur = 0;
+    size_t nev = 0;  // events used by this solve (profiling only)
     auto enqueue_batch = [&]() {
         for (int k = 0; k < CG_BATCH; ++k) {
             P.p_old = w.p[cur].get();
             P.p_new = w.p[1 - cur].get();
             P.r_old = w.r[cur].get();
             P.r_new = w.r[1 - cur].get();
-            hipLaunchKernelGGL(k_cg_a<GA>, dim3(blkA), dim3(CG_THREADS), 0, s, A, P);
-            hipLaunchKernelGGL((k_cg_b<GB, JAC>), dim3(blkB), dim3(CG_THREADS), 0, s, Pm, P);
+            if (m.prof) {
+                while (m.prof_ev.size() < nev + 2) {
+                    hipEvent_t e;
+                    HIP_CHECK(hipEventCreate(&e));
+                    m.prof_ev.push_back(e);
+                }
+                HIP_CHECK(hipEventRecord(m.prof_ev[nev], s));
+            }
+            k_cg_a<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
+            if (m.prof) {
+                HIP_CHECK(hipEventRecord(m.prof_ev[nev + 1], s));
+                nev += 2;
+            }
+            k_cg_b<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
             cur ^= 1;
         }
         check_launch("cg iteration");
@@ -401,6 +420,19 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3& A, const Csr3& Pm, co
     HIP_CHECK(hipMemcpyAsync(hs, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     *iterations = hs->iters;
+    if (m.prof) {
+        // launches [0, iters) did a full iteration; the rest found the solve finished and returned
+        for (size_t k = 0; k + 1 < nev; k += 2) {
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, m.prof_ev[k], m.prof_ev[k + 1]));
+            m.prof_all_ms += ms;
+            m.prof_all_launches += 1;
+            if ((int64_t)(k / 2) < hs->iters) {
+                m.prof_full_ms += ms;
+                m.prof_full_launches += 1;
+            }
+        }
+    }
     switch (hs->status) {
         case CG_CONVERGED: *out_status = HOEFEM_OK; break;
         case CG_ZERO_RHS:
@@ -412,33 +444,48 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3& A, const Csr3& Pm, co
     }
 }
 
-hoefem_status pcg_solve(Model& m, CgWork& w, const double* Avals, Schwarz* Pre, const double* b, double* x,
-                        double tol, int64_t max_iter, int64_t* iterations) {
+template <class VT>
+static hoefem_status pcg_solve_t(Model& m, CgWork& w, const VT* Avals, const VT* Pvals, Schwarz* Pre, const double* b,
+                                 double* x, double tol, int64_t max_iter, int64_t* iterations) {
     const NodePattern& pat = m.pattern;
     const int64_t nn = pat.num_nodes;
     const int GA = spmv_group_size(pat);
     const int GB = Pre ? spmv_group_size(Pre->P) : 8;
     const int64_t blocks = (nn * 32 + CG_THREADS - 1) / CG_THREADS + 1;
     w.ensure(pat.ndof(), (int)blocks);
-    const Csr3 A = view(pat, Avals);
-    Csr3 Pm = Pre ? view(Pre->P, Pre->d_vals.get()) : A;
+    const Csr3<VT> A = view(pat, Avals);
+    const Csr3<VT> Pm = Pre ? view(Pre->P, Pvals) : A;
     if (!Pre) {
-        hipLaunchKernelGGL(k_jacobi_diag, dim3((unsigned)((pat.ndof() + 255) / 256)), dim3(256), 0, m.stream,
-                           pat.ndof(), pat.d_diag_pos.get(), Avals, w.diag_inv.get(), m.d_err.get());
+        k_jacobi_diag<VT><<<dim3((unsigned)((pat.ndof() + 255) / 256)), dim3(256), 0, m.stream>>>(
+            pat.ndof(), pat.d_diag_pos.get(), Avals, w.diag_inv.get(), m.d_err.get());
         check_launch("k_jacobi_diag");
         if (read_and_clear_err(m) & KERR_ZERO_DIAG) throw Error(HOEFEM_ERROR_SINGULAR, "Diagonal matrix is not invertible.");
     }
     hoefem_status st = HOEFEM_OK;
-#define CG_CASE(a, b, j) \
-    if (GA == a && GB == b && JAC == j) { cg_launch_all<a, b, j>(m, w, A, Pm, b_, x, tol, max_iter, iterations, &st); return st; }
     const bool JAC = (Pre == nullptr);
-    const double* b_ = b;
+#define CG_CASE(ga, gb, j)                                                                               \
+    if (GA == ga && GB == gb && JAC == j) {                                                              \
+        cg_launch_all<ga, gb, j, VT>(m, w, A, Pm, b, x, tol, max_iter, iterations, &st);                   \
+        return st;                                                                                       \
+    }
     CG_CASE(8, 8, true) CG_CASE(16, 8, true) CG_CASE(32, 8, true)
     CG_CASE(8, 8, false) CG_CASE(8, 16, false) CG_CASE(8, 32, false)
     CG_CASE(16, 8, false) CG_CASE(16, 16, false) CG_CASE(16, 32, false)
     CG_CASE(32, 8, false) CG_CASE(32, 16, false) CG_CASE(32, 32, false)
 #undef CG_CASE
     throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unsupported CG configuration");
+}
+
+hoefem_status pcg_solve(Model& m, CgWork& w, const double* A, Schwarz* P, const double* b, double* x, double tol,
+                        int64_t max_iter, int64_t* iterations) {
+    if (P && P->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "FP32 preconditioner with an FP64 operator");
+    return pcg_solve_t<double>(m, w, A, P ? P->d_vals.get() : nullptr, P, b, x, tol, max_iter, iterations);
+}
+
+hoefem_status pcg_solve(Model& m, CgWork& w, const float* A, Schwarz* P, const double* b, double* x, double tol,
+                        int64_t max_iter, int64_t* iterations) {
+    if (P && !P->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "FP64 preconditioner with an FP32 operator");
+    return pcg_solve_t<float>(m, w, A, P ? P->d_vals32.get() : nullptr, P, b, x, tol, max_iter, iterations);
 }
 
 // ================================================================== Newton / BE kernels
@@ -456,7 +503,7 @@ __global__ void k_state(int64_t ndof, const double* __restrict__ up, const doubl
 // F = -dt * [ (M g + f_int(uu)) with Dirichlet rows zeroed ] + M w  and |F|^2
 // (NewtonCgDynamicSystem::eval_f, newton_cg.rs:441-478, then euler.rs:107-113)
 template <int G>
-__global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3 M, const int32_t* __restrict__ inc_ptr,
+__global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3<double> M, const int32_t* __restrict__ inc_ptr,
                                                          const int32_t* __restrict__ inc_blk,
                                                          const int32_t* __restrict__ inc_loc, int n,
                                                          const double* __restrict__ fe,
@@ -541,6 +588,17 @@ __global__ void k_finite(int64_t n, const double* __restrict__ a, const double* 
     if (!isfinite(a[i]) || !isfinite(b[i])) atomicOr(err, KERR_NONFINITE);
 }
 
+__global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n) {
+    if (n <= 0) return;
+    k_f32_to_f64<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream>>>(n, a, b);
+    check_launch("k_f32_to_f64");
+}
+
 void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu) {
     const int64_t n = 3 * m.nv;
     hipLaunchKernelGGL(k_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, up, w, dt, dirmask, uu);
@@ -551,7 +609,7 @@ void launch_residual(Model& m, const double* Mvals, const double* gforce, const 
                      const double* w, double* F, CgWork& cw, double* norm2) {
     const NodePattern& pat = m.pattern;
     const int G = spmv_group_size(pat);
-    const Csr3 Mv = view(pat, Mvals);
+    const Csr3<double> Mv = view(pat, Mvals);
     const unsigned blocks = (unsigned)((pat.num_nodes * G + CG_THREADS - 1) / CG_THREADS);
     cw.ensure(pat.ndof(), (int)blocks + 1);
 #define RES_CASE(GG)                                                                                              \

@@ -211,7 +211,7 @@ class NewtonCgIntegrator:
 
     def __init__(self, model: ElasticityModel, scene: Scene, newton_tolerance_factor: float = 1e-5,
                  cg_tolerance: float = 1e-5, cg_max_iter: int = 500, max_newton_iter: int = 100,
-                 schwarz: bool = True, gravity_enabled: bool = True):
+                 schwarz: bool = True, gravity_enabled: bool = True, mixed_precision: bool = False):
         L = _lib.load()
         self.model, self.scene = model, scene
         mu, lam = lame_from_young_poisson(scene.young, scene.poisson)
@@ -230,6 +230,7 @@ class NewtonCgIntegrator:
         d.max_newton_iter = max_newton_iter
         d.cg_tolerance = cg_tolerance
         d.cg_max_iter = cg_max_iter
+        d.mixed_precision = 1 if mixed_precision else 0
         h = C.c_void_p()
         check(L.hoefem_sim_create(model.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -146,6 +146,15 @@ hoefem_status hoefem_schwarz_apply(hoefem_schwarz* s, const double* x, double* y
 hoefem_status hoefem_pcg(hoefem_model* model, const double* A_values, hoefem_schwarz* schwarz, const double* b,
                          double* x, double tolerance, int64_t max_iterations, int64_t* iterations);
 
+/* Kernel timing (no counterpart in the reference, which times phases with coarse_prof scopes,
+ * simulation_toolbox/src/fem/newton_cg.rs:685-764): when enabled, every launch of the CG
+ * system-matrix kernel (fused direction update + SpMV + p.Ap) is bracketed by HIP events on
+ * the model stream. "full" launches performed a CG iteration; the remaining launches found
+ * the solve already finished and returned immediately. */
+hoefem_status hoefem_profile_enable(hoefem_model* model, int32_t enable);
+hoefem_status hoefem_profile_read(hoefem_model* model, int64_t* full_launches, double* full_ms,
+                                  int64_t* all_launches, double* all_ms);
+
 /* NewtonCgIntegrator3d (simulation_toolbox/src/fem/newton_cg.rs:43-295) over
  * backward_euler_step (hamilton2/src/integrators/euler.rs:155-199) and
  * newton_line_search + BacktrackingLineSearch (hamilton2/src/newton.rs:76-246). */
@@ -164,6 +173,11 @@ typedef struct hoefem_sim_desc {
     int64_t max_newton_iter;               /* reference: 100 */
     double cg_tolerance;                   /* reference: 1e-5 (newton_cg.rs:761) */
     int64_t cg_max_iter;                   /* reference: 500 (newton_cg.rs:772) */
+    int32_t mixed_precision;               /* 0: FP64 step (the reference's arithmetic). 1: element tangents,
+                                              the Jacobian values, its SpMV and the Schwarz operator in FP32
+                                              (FP32 accumulation); residual, forces, line search, CG dots and
+                                              norms and the Schwarz eigensolves stay FP64 (Newton is the FP64
+                                              refinement loop around the FP32 linear operator). */
 } hoefem_sim_desc;
 
 typedef struct hoefem_step_stats {
