@@ -39,6 +39,9 @@ EXPORTED = [
     "hoefem_schwarz_values",
     "hoefem_schwarz_apply",
     "hoefem_pcg",
+    "hoefem_model_autotune",
+    "hoefem_schwarz_autotune",
+    "hoefem_sim_autotune",
     "hoefem_profile_enable",
     "hoefem_profile_read",
     "hoefem_sim_create",
@@ -184,6 +187,9 @@ def load(path: str = LIB_PATH):
     L.hoefem_schwarz_values.argtypes = [vp, vp]
     L.hoefem_schwarz_apply.argtypes = [vp, vp, vp]
     L.hoefem_pcg.argtypes = [vp, vp, vp, vp, vp, f64, i64, C.POINTER(i64)]
+    L.hoefem_model_autotune.argtypes = [vp, C.POINTER(i32), vp]
+    L.hoefem_schwarz_autotune.argtypes = [vp, C.POINTER(i32), vp]
+    L.hoefem_sim_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     L.hoefem_profile_enable.argtypes = [vp, i32]
     L.hoefem_profile_read.argtypes = [vp, C.POINTER(i64), C.POINTER(f64), C.POINTER(i64), C.POINTER(f64)]
     L.hoefem_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]

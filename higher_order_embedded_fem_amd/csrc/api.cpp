@@ -597,6 +597,33 @@ hoefem_status hoefem_pcg(hoefem_model* model, const double* A, hoefem_schwarz* s
     return st;
 }
 
+hoefem_status hoefem_model_autotune(hoefem_model* model, int32_t* lanes_per_node, double* candidate_us) {
+    return guarded([&] {
+        Model& m = M_(model);
+        const int g = autotune_spmv(m.pattern, m.stream, candidate_us);
+        if (lanes_per_node) *lanes_per_node = g;
+    });
+}
+
+hoefem_status hoefem_schwarz_autotune(hoefem_schwarz* s, int32_t* lanes_per_node, double* candidate_us) {
+    return guarded([&] {
+        if (!s) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null preconditioner");
+        const int g = autotune_spmv(s->s.P, s->s.model->stream, candidate_us);
+        if (lanes_per_node) *lanes_per_node = g;
+    });
+}
+
+hoefem_status hoefem_sim_autotune(hoefem_sim* sim, int32_t* lanes_matrix, int32_t* lanes_preconditioner) {
+    return guarded([&] {
+        if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+        Model& m = *sim->s.model;
+        const int ga = autotune_spmv(m.pattern, m.stream, nullptr);
+        const int gb = sim->s.schwarz ? autotune_spmv(sim->s.schwarz->P, m.stream, nullptr) : 0;
+        if (lanes_matrix) *lanes_matrix = ga;
+        if (lanes_preconditioner) *lanes_preconditioner = gb;
+    });
+}
+
 hoefem_status hoefem_profile_enable(hoefem_model* model, int32_t enable) {
     return guarded([&] {
         Model& m = M_(model);

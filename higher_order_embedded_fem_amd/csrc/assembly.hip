@@ -552,6 +552,7 @@ void launch_force_gather(Model& m, double* f) {
 }
 
 int spmv_group_size(const NodePattern& pat) {
+    if (pat.tuned_group) return pat.tuned_group;
     const double avg = pat.num_nodes ? (double)pat.ncol.size() / (double)pat.num_nodes : 1.0;
     if (avg >= 40.0) return 32;
     if (avg >= 12.0) return 16;
@@ -595,6 +596,52 @@ template void launch_dirichlet<double>(const NodePattern&, double*, const uint8_
                                        hipStream_t);
 template void launch_dirichlet<float>(const NodePattern&, float*, const uint8_t*, const int32_t*, int64_t, double*,
                                       hipStream_t);
+
+__global__ void k_fill_pattern_values(int64_t n, double* __restrict__ v, double a, double b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = a + b * (double)(i % 7);
+}
+
+int autotune_spmv(NodePattern& pat, hipStream_t s, double* times_us) {
+    if (pat.num_nodes == 0) return 8;
+    const int64_t nnz = pat.nnz(), ndof = pat.ndof();
+    DevBuf<double> vals, x, y;
+    vals.alloc(std::max<int64_t>(nnz, 1));
+    x.alloc(ndof);
+    y.alloc(ndof);
+    k_fill_pattern_values<<<dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s>>>(nnz, vals.get(), 1.0, 0.125);
+    k_fill_pattern_values<<<dim3((unsigned)((ndof + 255) / 256)), dim3(256), 0, s>>>(ndof, x.get(), -1.0, 0.5);
+    check_launch("k_fill_pattern_values");
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const int cand[3] = {8, 16, 32};
+    int best = 8;
+    double best_t = 1e300;
+    const int saved = pat.tuned_group;
+    for (int c = 0; c < 3; ++c) {
+        pat.tuned_group = cand[c];
+        for (int w = 0; w < 2; ++w) launch_spmv(pat, vals.get(), 0.0, y.get(), 1.0, x.get(), s);
+        HIP_CHECK(hipEventRecord(e0, s));
+        const int reps = 10;
+        for (int r = 0; r < reps; ++r) launch_spmv(pat, vals.get(), 0.0, y.get(), 1.0, x.get(), s);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * ms / reps;
+        if (times_us) times_us[c] = us;
+        if (us < best_t) {
+            best_t = us;
+            best = cand[c];
+        }
+    }
+    (void)saved;
+    pat.tuned_group = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
 
 void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF) {
     DevBuf<int> err;

@@ -94,6 +94,7 @@ struct NodePattern {
     std::vector<int32_t> diag_pos;    // position of each row's diagonal entry
     std::vector<uint8_t> diag_only;   // per node
     int max_nbr = 0;
+    int tuned_group = 0;  // lanes per node row block chosen by autotune_spmv (0: heuristic)
     DevBuf<int32_t> d_nptr, d_ncol, d_rowptr, d_col, d_diag_pos;
     DevBuf<uint8_t> d_diag_only;
     int64_t nnz() const { return (int64_t)col.size(); }
@@ -218,6 +219,9 @@ template <class VT>
 void launch_dirichlet(const NodePattern& pat, VT* vals, const uint8_t* d_dirmask, const int32_t* d_bc_nodes,
                       int64_t n_bc_nodes, double* d_scale, hipStream_t s);
 int spmv_group_size(const NodePattern& pat);
+// Times the node-blocked SpMV with 8, 16 and 32 lanes per node on this pattern and keeps the
+// fastest in pat.tuned_group; returns it. Per-candidate averages (us) go to times_us[3] if given.
+int autotune_spmv(NodePattern& pat, hipStream_t s, double* times_us);
 void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF);
 int read_and_clear_err(Model& m);  // synchronizes the model stream
 

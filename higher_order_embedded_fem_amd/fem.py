@@ -240,6 +240,14 @@ class NewtonCgIntegrator:
             _lib._lib.hoefem_sim_destroy(self.h)
             self.h = None
 
+    def autotune(self):
+        """Pick the SpMV lanes-per-node of the Jacobian and Schwarz kernels by timing (opt-in:
+        a different choice changes the reduction order, so runs stay bitwise reproducible only
+        with a fixed choice)."""
+        a, b = C.c_int32(), C.c_int32()
+        check(_lib.load().hoefem_sim_autotune(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     @property
     def representative_force(self) -> float:
         return _lib.load().hoefem_sim_representative_force(self.h)

@@ -146,6 +146,15 @@ hoefem_status hoefem_schwarz_apply(hoefem_schwarz* s, const double* x, double* y
 hoefem_status hoefem_pcg(hoefem_model* model, const double* A_values, hoefem_schwarz* schwarz, const double* b,
                          double* x, double tolerance, int64_t max_iterations, int64_t* iterations);
 
+/* SpMV tile autotuner (no counterpart in the reference, whose SpMV is MKL's inspector-executor
+ * with set_mv_hint/optimize, newton_cg.rs:731-743): times the node-row-block kernel with 8, 16
+ * and 32 lanes per node on this pattern and keeps the fastest for every later SpMV and CG
+ * launch. candidate_us (optional, [3]) receives the measured average launch times. */
+hoefem_status hoefem_model_autotune(hoefem_model* model, int32_t* lanes_per_node, double* candidate_us);
+hoefem_status hoefem_schwarz_autotune(hoefem_schwarz* s, int32_t* lanes_per_node, double* candidate_us);
+/* the same for the Jacobian and Schwarz patterns of a simulation (lanes_preconditioner = 0 without Schwarz) */
+hoefem_status hoefem_sim_autotune(hoefem_sim* sim, int32_t* lanes_matrix, int32_t* lanes_preconditioner);
+
 /* Kernel timing (no counterpart in the reference, which times phases with coarse_prof scopes,
  * simulation_toolbox/src/fem/newton_cg.rs:685-764): when enabled, every launch of the CG
  * system-matrix kernel (fused direction update + SpMV + p.Ap) is bracketed by HIP events on
