@@ -181,21 +181,27 @@ struct Schwarz {
     DevBuf<double> d_vals;         // P values (FP64)
     DevBuf<float> d_vals32;        // P values (FP32)
     DevBuf<double> d_blocks;       // gathered / inverted blocks (always FP64)
+    DevBuf<int> d_fallback;        // per block: 1 if the Gauss-Jordan path rejected it (eigen path)
 };
 
 // ------------------------------------------------------------------ CG workspace
 struct CgScalars {
     double zTr, pAp, alpha, beta, rr, bnorm, tol;
+    double zrec[2];  // zTr of iteration k in zrec[k & 1] (read by the next iteration's beta)
     int64_t iters, max_iter;
     int status;  // 0 running, 1 converged, hoefem_status codes for errors
     int pad;
 };
 
+// Per-block partial sums of one launch, summed again by every block of the next launch
+// (no atomics or device-scope fences in the solver loop). Sized for the largest grid.
+constexpr int CG_MAX_BLOCKS = 1024;
+
 struct CgWork {
     int64_t n = 0;
     DevBuf<double> r[2], p[2], z, Ap, diag_inv;
-    DevBuf<double> partials;
-    DevBuf<unsigned> counter;
+    DevBuf<double> part_pre, part_a, part_b;  // b.b (preamble 1), pAp (A-kernel), zTr and rr (B-kernel)
+    DevBuf<double> partials;                  // residual kernel partials
     DevBuf<CgScalars> scalars;
     PinnedBuf<CgScalars> h_scalars{3};  // two polling slots + final readback
     hipEvent_t ev[2] = {nullptr, nullptr};

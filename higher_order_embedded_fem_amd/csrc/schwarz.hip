@@ -2,10 +2,12 @@
 // P = sum_s R_s^T B_s^+ R_s + sum_untreated 1 / J_ii, with B_s the restriction of J to the
 // subdomain's DOFs and B_s^+ = V diag(1 / max(|lambda|, 1e-12 max|lambda|)) V^T.
 //
-// Per update: (1) gather every B_s from J (precomputed slots), (2) batched FP64 symmetric
-// eigen-decomposition by parallel (round-robin) Jacobi in LDS, one work-group per block,
-// and recomposition of B_s^+, (3) color-ordered gather of the blocks into P's row blocks,
-// (4) reciprocal diagonal for untreated nodes.
+// Per update: (1) gather every B_s from J (precomputed slots), (2) batched FP64 Gauss-Jordan
+// inversion in LDS, one work-group per block, accepted where it certifies that B_s is positive
+// definite with kappa_2 < 1e11 (then the eigenvalue clamp is inactive and B_s^+ = B_s^-1),
+// (3) for the remaining blocks, symmetric eigen-decomposition by parallel (round-robin) Jacobi
+// in LDS and recomposition of B_s^+, (4) color-ordered gather of the blocks into P's row
+// blocks, (5) reciprocal diagonal for untreated nodes.
 #include <algorithm>
 #include <cmath>
 
@@ -60,6 +62,7 @@ void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const
         HIP_CHECK(hipMemsetAsync(s.d_vals.get(), 0, sizeof(double) * s.d_vals.n, m.stream));
     }
     s.d_blocks.ensure(std::max<int64_t>(s.scatter.val_off[nsub], 1));
+    s.d_fallback.ensure(std::max<int64_t>(nsub, 1));
 }
 
 // B_s(r, c) = J[3 g_a + i, 3 g_c + j], row-major (3m)^2, always FP64
@@ -86,10 +89,97 @@ __global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t*
 // (invert_element_block, schwarz_precond.rs:329-342). Rotations follow Golub & Van Loan,
 // "Matrix Computations", sym.schur2.
 constexpr int EIG_THREADS = 256;
+constexpr double SPD_COND_LIMIT = 1e11;
+
+__device__ inline double block_sum(double v, double* s_red) {
+    const int t = threadIdx.x;
+    s_red[t] = v;
+    __syncthreads();
+    for (int s = EIG_THREADS / 2; s > 0; s >>= 1) {
+        if (t < s) s_red[t] += s_red[t + s];
+        __syncthreads();
+    }
+    const double r = s_red[0];
+    __syncthreads();
+    return r;
+}
+
+// Gauss-Jordan inversion of a symmetric block in LDS without pivoting. Each pivot is a
+// diagonal entry of a Schur complement, so all pivots are positive iff B is positive
+// definite. The block is accepted when they are and |B|_F |B^-1|_F < SPD_COND_LIMIT, which
+// bounds kappa_2(B) below the 1e12 of the eigenvalue clamp: then B^+ = B^-1 and the
+// symmetrized inverse replaces the block. Otherwise fallback[b] = 1 and the block is kept
+// for the eigen path.
+__global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __restrict__ msz,
+                                                             const int64_t* __restrict__ val_off,
+                                                             double* __restrict__ blocks, int* __restrict__ fallback,
+                                                             int ld) {
+    extern __shared__ double lds[];
+    const int64_t b = blockIdx.x;
+    const int dim = 3 * msz[b];
+    double* A = lds;
+    double* rowk = lds + (size_t)dim * ld;
+    double* colk = rowk + dim;
+    __shared__ double s_red[EIG_THREADS];
+    double* blk = blocks + val_off[b];
+    const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    constexpr int NW = EIG_THREADS / 64;
+    double loc = 0.0;
+    for (int i = wid; i < dim; i += NW)
+        for (int j = lane; j < dim; j += 64) {
+            const double v = blk[(int64_t)i * dim + j];
+            A[i * ld + j] = v;
+            loc += v * v;
+        }
+    const double nb2 = block_sum(loc, s_red);
+    bool bad = false;
+    for (int k = 0; k < dim; ++k) {
+        for (int i = t; i < dim; i += EIG_THREADS) {
+            rowk[i] = A[k * ld + i];
+            colk[i] = A[i * ld + k];
+        }
+        __syncthreads();
+        const double piv = rowk[k];
+        if (!(piv > 0.0) || !isfinite(piv)) {  // same value in every thread: uniform exit
+            bad = true;
+            break;
+        }
+        const double ip = 1.0 / piv;
+        for (int i = wid; i < dim; i += NW)
+            for (int j = lane; j < dim; j += 64) {
+                double v;
+                if (i == k)
+                    v = j == k ? ip : rowk[j] * ip;
+                else if (j == k)
+                    v = -colk[i] * ip;
+                else
+                    v = A[i * ld + j] - colk[i] * rowk[j] * ip;
+                A[i * ld + j] = v;
+            }
+        __syncthreads();
+    }
+    if (bad) {
+        if (t == 0) fallback[b] = 1;
+        return;
+    }
+    loc = 0.0;
+    for (int i = wid; i < dim; i += NW)
+        for (int j = lane; j < dim; j += 64) loc += A[i * ld + j] * A[i * ld + j];
+    const double ni2 = block_sum(loc, s_red);
+    if (!(sqrt(nb2) * sqrt(ni2) < SPD_COND_LIMIT)) {
+        if (t == 0) fallback[b] = 1;
+        return;
+    }
+    for (int i = wid; i < dim; i += NW)
+        for (int j = lane; j < dim; j += 64) blk[(int64_t)i * dim + j] = 0.5 * (A[i * ld + j] + A[j * ld + i]);
+    if (t == 0) fallback[b] = 0;
+}
 
 __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_inverse(const int32_t* __restrict__ msz,
                                                                 const int64_t* __restrict__ val_off,
-                                                                double* __restrict__ blocks, int ldpad) {
+                                                                double* __restrict__ blocks,
+                                                                const int* __restrict__ fallback, int ldpad) {
+    if (fallback && fallback[blockIdx.x] == 0) return;
     extern __shared__ double lds[];
     const int64_t b = blockIdx.x;
     const int dim = 3 * msz[b];
@@ -258,17 +348,25 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
             s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get());
         check_launch("k_schwarz_gather");
         const int dim = 3 * s.scatter.max_m;
-        const int nn = dim + (dim & 1);
-        const int ld = nn + 1;
-        const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
         static bool attr_set = false;
         if (!attr_set) {
             HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           120 * 1024));
+            HIP_CHECK(hipFuncSetAttribute((const void*)k_spd_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          120 * 1024));
             attr_set = true;
         }
+        // Gauss-Jordan fast path, then the eigen path for the blocks it rejected
+        const int ld_gj = dim + 1;
+        const size_t lds_gj = ((size_t)dim * ld_gj + 2 * (size_t)dim) * sizeof(double);
+        hipLaunchKernelGGL(k_spd_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds_gj, st, s.scatter.d_m.get(),
+                           s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get(), ld_gj);
+        check_launch("k_spd_inverse");
+        const int nn = dim + (dim & 1);
+        const int ld = nn + 1;
+        const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
         hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st, s.scatter.d_m.get(),
-                           s.scatter.d_val_off.get(), s.d_blocks.get(), ld);
+                           s.scatter.d_val_off.get(), s.d_blocks.get(), (const int*)s.d_fallback.get(), ld);
         check_launch("k_jacobi_inverse");
         if (s.fp32)
             launch_block_gather(s.P, s.scatter, s.d_blocks.get(), s.d_vals32.get(), 0, 0.0, nullptr, true, st);

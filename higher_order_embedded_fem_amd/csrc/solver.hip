@@ -1,13 +1,19 @@
 // Preconditioned conjugate gradient (fenris/src/cg.rs:377-482) as two fused kernels per
 // iteration, with the iteration state (zTr, alpha, beta, status) kept on the device:
-//   A-kernel: p <- z + beta p (on the fly, double-buffered), Ap = A p, pAp  -> alpha
-//   B-kernel: x += alpha p, r <- r - alpha Ap (on the fly, double-buffered), z = P r,
-//             zTr, |r|^2 -> beta, convergence / max-iteration status
-// Reductions use a deterministic last-block pass over per-block partials. The host enqueues
-// batches of iterations and polls the status of the batch before the one just enqueued.
-// Also: the residual / state kernels of the Backward-Euler Newton driver.
+//   A-kernel: [sum zTr, |r|^2 of the previous B-kernel; convergence / max-iteration status;
+//             beta] p <- z + beta p (on the fly, double-buffered), Ap = A p, partial pAp
+//   B-kernel: [sum pAp; indefiniteness status; alpha] x += alpha p, r <- r - alpha Ap
+//             (on the fly, double-buffered), z = P r, partial zTr and |r|^2
+// Every kernel runs a persistent grid of at most CG_MAX_BLOCKS blocks that writes one partial
+// sum per block; the next kernel's blocks each add those partials up again in the same fixed
+// order, so all blocks agree on the scalars bit for bit and the result is deterministic. No
+// atomics or device-scope fences are needed: the kernel boundary orders the partials.
+// The host enqueues batches of iterations and polls the status of the batch before the one
+// just enqueued. Also: the residual / state kernels of the Backward-Euler Newton driver.
 #include <algorithm>
 #include <cmath>
+
+#include <hip/hip_ext.h>
 
 #include "internal.hpp"
 
@@ -23,12 +29,36 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// Sum NV values over the whole grid. Returns true in the (single) last block, where tot
-// holds the grid sums in every thread. Deterministic: fixed grid, fixed summation tree.
+// tot[k] = sum over b < nb of partials[b * NV + k], the same value in every thread of every
+// block (fixed per-thread strides, fixed wave and cross-wave summation order).
 template <int NV>
-__device__ inline bool grid_reduce_last(const double (&v)[NV], double* partials, unsigned* counter, double (&tot)[NV]) {
+__device__ inline void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[NV]) {
     __shared__ double s_w[CG_THREADS / 64][NV];
-    __shared__ int s_last;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = t; b < nb; b += CG_THREADS)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (lane == 0) s_w[wid][k] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
+        tot[k] = s;
+    }
+}
+
+// partials[blockIdx.x * NV + k] = sum of v[k] over the block's threads.
+template <int NV>
+__device__ inline void block_partials(const double (&v)[NV], double* __restrict__ partials) {
+    __shared__ double s_w[CG_THREADS / 64][NV];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -43,35 +73,7 @@ __device__ inline bool grid_reduce_last(const double (&v)[NV], double* partials,
             for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
             partials[(int64_t)blockIdx.x * NV + k] = s;
         }
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        s_last = (prev == gridDim.x - 1) ? 1 : 0;
     }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int b = t; b < (int)gridDim.x; b += CG_THREADS)
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            acc[k] += __hip_atomic_load(partials + (int64_t)b * NV + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double s = wave_sum(acc[k]);
-        if (lane == 0) s_w[wid][k] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
-        tot[k] = s;
-    }
-    if (t == 0) *counter = 0u;
-    return true;
 }
 
 template <class VT>
@@ -123,6 +125,24 @@ __device__ inline void row_block_dot(const Csr3<VT>& A, int64_t v, int lane, boo
     r2 = double(a2);
 }
 
+// Grid-stride loop over groups of CG_THREADS / G nodes; body(v, lane, valid) is called the same
+// number of times by every thread of a block (the G-lane shuffles need whole groups).
+template <int G, class F>
+__device__ inline void for_node_groups(int64_t nn, F body) {
+    constexpr int per = CG_THREADS / G;
+    const int64_t groups = (nn + per - 1) / per;
+    const int lane = (int)(threadIdx.x % G);
+    for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int64_t v = g * per + threadIdx.x / G;
+        body(v, lane, v < nn);
+    }
+}
+
+inline unsigned cg_grid(int64_t nn, int G) {
+    const int64_t groups = (nn + CG_THREADS / G - 1) / (CG_THREADS / G);
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(groups, CG_MAX_BLOCKS));
+}
+
 struct CgPtrs {
     double* x;
     double* z;
@@ -133,162 +153,173 @@ struct CgPtrs {
     double* p_new;
     const double* b;
     const double* dinv;
-    double* partials;
-    unsigned* counter;
+    double* part_pre;
+    double* part_a;
+    double* part_b;
+    int nb_a;  // grid of the preamble-1 / A-kernels
+    int nb_b;  // grid of the preamble-2 / B-kernels
     CgScalars* sc;
 };
 
-// preamble 1: r = b - A x ; |b|^2
+// preamble 1: r = b - A x ; partial |b|^2
 template <int G, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3<VT> A, CgPtrs P) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t v = gid / G;
-    const int lane = (int)(gid % G);
-    const bool valid = v < A.nn;
     const double* x = P.x;
-    double a0, a1, a2;
-    row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return x[j]; }, a0, a1, a2);
     double part[1] = {0.0};
-    if (valid && lane == 0) {
-        const double b0 = P.b[3 * v], b1 = P.b[3 * v + 1], b2 = P.b[3 * v + 2];
-        P.r_new[3 * v] = b0 - a0;
-        P.r_new[3 * v + 1] = b1 - a1;
-        P.r_new[3 * v + 2] = b2 - a2;
-        part[0] = b0 * b0 + b1 * b1 + b2 * b2;
-    }
-    double tot[1];
-    if (grid_reduce_last<1>(part, P.partials, P.counter, tot) && threadIdx.x == 0) P.sc->bnorm = sqrt(tot[0]);
+    for_node_groups<G>(A.nn, [&](int64_t v, int lane, bool valid) {
+        double a0, a1, a2;
+        row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return x[j]; }, a0, a1, a2);
+        if (valid && lane == 0) {
+            const double b0 = P.b[3 * v], b1 = P.b[3 * v + 1], b2 = P.b[3 * v + 2];
+            P.r_new[3 * v] = b0 - a0;
+            P.r_new[3 * v + 1] = b1 - a1;
+            P.r_new[3 * v + 2] = b2 - a2;
+            part[0] += b0 * b0 + b1 * b1 + b2 * b2;
+        }
+    });
+    block_partials<1>(part, P.part_pre);
 }
 
-// preamble 2: z = P r ; zTr, |r|^2 ; initial status
+// preamble 2: |b| ; z = P r ; partial zTr, |r|^2
 template <int G, bool JAC, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t v = gid / G;
-    const int lane = (int)(gid % G);
-    const bool valid = v < Pm.nn;
+    double bb[1];
+    sum_partials<1>(P.part_pre, P.nb_a, bb);
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.sc->bnorm = sqrt(bb[0]);
     const double* r = P.r_new;
-    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-    if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return r[j]; }, z0, z1, z2);
     double part[2] = {0.0, 0.0};
-    if (valid && lane == 0) {
-        const double r0 = r[3 * v], r1 = r[3 * v + 1], r2 = r[3 * v + 2];
-        if (JAC) {
-            z0 = P.dinv[3 * v] * r0;
-            z1 = P.dinv[3 * v + 1] * r1;
-            z2 = P.dinv[3 * v + 2] * r2;
+    for_node_groups<G>(Pm.nn, [&](int64_t v, int lane, bool valid) {
+        double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+        if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return r[j]; }, z0, z1, z2);
+        if (valid && lane == 0) {
+            const double r0 = r[3 * v], r1 = r[3 * v + 1], r2 = r[3 * v + 2];
+            if (JAC) {
+                z0 = P.dinv[3 * v] * r0;
+                z1 = P.dinv[3 * v + 1] * r1;
+                z2 = P.dinv[3 * v + 2] * r2;
+            }
+            P.z[3 * v] = z0;
+            P.z[3 * v + 1] = z1;
+            P.z[3 * v + 2] = z2;
+            part[0] += z0 * r0 + z1 * r1 + z2 * r2;
+            part[1] += r0 * r0 + r1 * r1 + r2 * r2;
         }
-        P.z[3 * v] = z0;
-        P.z[3 * v + 1] = z1;
-        P.z[3 * v + 2] = z2;
-        part[0] = z0 * r0 + z1 * r1 + z2 * r2;
-        part[1] = r0 * r0 + r1 * r1 + r2 * r2;
-    }
-    double tot[2];
-    if (grid_reduce_last<2>(part, P.partials, P.counter, tot) && threadIdx.x == 0) {
-        CgScalars* sc = P.sc;
-        sc->zTr = tot[0];
-        sc->rr = tot[1];
-        sc->beta = 0.0;
-        sc->iters = 0;
-        if (sc->bnorm == 0.0)
-            sc->status = CG_ZERO_RHS;
-        else if (sqrt(tot[1]) <= sc->tol * sc->bnorm)
-            sc->status = CG_CONVERGED;
-        else if (sc->max_iter >= 0 && sc->iters >= sc->max_iter)
-            sc->status = HOEFEM_ERROR_CG_MAX_ITERATIONS;
-        else
-            sc->status = CG_RUNNING;
-    }
+    });
+    block_partials<2>(part, P.part_b);
 }
 
-// A-kernel: p_new = z + beta * p_old ; Ap = A p_new ; pAp -> alpha
+// A-kernel of iteration k: stopping tests on r_k, beta_k = zTr_k / zTr_{k-1},
+// p_new = z + beta p_old ; Ap = A p_new ; partial pAp
 template <int G, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
+__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64_t k) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
-    const double beta = sc->beta;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t v = gid / G;
-    const int lane = (int)(gid % G);
-    const bool valid = v < A.nn;
+    double tot[2];
+    sum_partials<2>(P.part_b, P.nb_b, tot);
+    const double zTr = tot[0], rr = tot[1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    int decided = CG_RUNNING;
+    if (sc->bnorm == 0.0)
+        decided = CG_ZERO_RHS;
+    else if (sqrt(rr) <= sc->tol * sc->bnorm)
+        decided = CG_CONVERGED;
+    else if (sc->max_iter >= 0 && k >= sc->max_iter)
+        decided = HOEFEM_ERROR_CG_MAX_ITERATIONS;
+    if (decided != CG_RUNNING) {
+        if (lead) {
+            sc->zTr = zTr;
+            sc->rr = rr;
+            sc->status = decided;
+        }
+        return;
+    }
+    const double beta = k == 0 ? 0.0 : zTr / sc->zrec[(k - 1) & 1];
+    if (lead) {
+        sc->zrec[k & 1] = zTr;
+        sc->zTr = zTr;
+        sc->rr = rr;
+        sc->beta = beta;
+    }
     const double* z = P.z;
     const double* po = P.p_old;
-    double a0, a1, a2;
-    row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return z[j] + beta * po[j]; }, a0, a1, a2);
     double part[1] = {0.0};
-    if (valid && lane == 0) {
-        const double p0 = z[3 * v] + beta * po[3 * v];
-        const double p1 = z[3 * v + 1] + beta * po[3 * v + 1];
-        const double p2 = z[3 * v + 2] + beta * po[3 * v + 2];
-        P.p_new[3 * v] = p0;
-        P.p_new[3 * v + 1] = p1;
-        P.p_new[3 * v + 2] = p2;
-        P.Ap[3 * v] = a0;
-        P.Ap[3 * v + 1] = a1;
-        P.Ap[3 * v + 2] = a2;
-        part[0] = p0 * a0 + p1 * a1 + p2 * a2;
-    }
-    double tot[1];
-    if (grid_reduce_last<1>(part, P.partials, P.counter, tot) && threadIdx.x == 0) {
-        sc->pAp = tot[0];
-        if (tot[0] <= 0.0)
-            sc->status = HOEFEM_ERROR_CG_INDEFINITE_OPERATOR;
-        else if (sc->zTr <= 0.0)
-            sc->status = HOEFEM_ERROR_CG_INDEFINITE_PRECONDITIONER;
-        else
-            sc->alpha = sc->zTr / tot[0];
-    }
+    for_node_groups<G>(A.nn, [&](int64_t v, int lane, bool valid) {
+        double a0, a1, a2;
+        row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return z[j] + beta * po[j]; }, a0, a1, a2);
+        if (valid && lane == 0) {
+            const double p0 = z[3 * v] + beta * po[3 * v];
+            const double p1 = z[3 * v + 1] + beta * po[3 * v + 1];
+            const double p2 = z[3 * v + 2] + beta * po[3 * v + 2];
+            P.p_new[3 * v] = p0;
+            P.p_new[3 * v + 1] = p1;
+            P.p_new[3 * v + 2] = p2;
+            P.Ap[3 * v] = a0;
+            P.Ap[3 * v + 1] = a1;
+            P.Ap[3 * v + 2] = a2;
+            part[0] += p0 * a0 + p1 * a1 + p2 * a2;
+        }
+    });
+    block_partials<1>(part, P.part_a);
 }
 
-// B-kernel: x += alpha p ; r_new = r_old - alpha Ap ; z = P r_new ; zTr, rr -> beta, status
+// B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p ;
+// r_new = r_old - alpha Ap ; z = P r_new ; partial zTr, |r|^2
 template <int G, bool JAC, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
+__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P, int64_t k) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
-    const double alpha = sc->alpha;
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t v = gid / G;
-    const int lane = (int)(gid % G);
-    const bool valid = v < Pm.nn;
+    double tot[1];
+    sum_partials<1>(P.part_a, P.nb_a, tot);
+    const double pAp = tot[0];
+    const double zTr = sc->zrec[k & 1];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    int decided = CG_RUNNING;
+    if (pAp <= 0.0)
+        decided = HOEFEM_ERROR_CG_INDEFINITE_OPERATOR;
+    else if (zTr <= 0.0)
+        decided = HOEFEM_ERROR_CG_INDEFINITE_PRECONDITIONER;
+    if (decided != CG_RUNNING) {
+        if (lead) {
+            sc->pAp = pAp;
+            sc->status = decided;
+        }
+        return;
+    }
+    const double alpha = zTr / pAp;
+    if (lead) {
+        sc->pAp = pAp;
+        sc->alpha = alpha;
+        sc->iters = k + 1;
+    }
     const double* ro = P.r_old;
     const double* Ap = P.Ap;
-    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-    if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return ro[j] - alpha * Ap[j]; }, z0, z1, z2);
     double part[2] = {0.0, 0.0};
-    if (valid && lane == 0) {
-        const double r0 = ro[3 * v] - alpha * Ap[3 * v];
-        const double r1 = ro[3 * v + 1] - alpha * Ap[3 * v + 1];
-        const double r2 = ro[3 * v + 2] - alpha * Ap[3 * v + 2];
-        if (JAC) {
-            z0 = P.dinv[3 * v] * r0;
-            z1 = P.dinv[3 * v + 1] * r1;
-            z2 = P.dinv[3 * v + 2] * r2;
+    for_node_groups<G>(Pm.nn, [&](int64_t v, int lane, bool valid) {
+        double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+        if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return ro[j] - alpha * Ap[j]; }, z0, z1, z2);
+        if (valid && lane == 0) {
+            const double r0 = ro[3 * v] - alpha * Ap[3 * v];
+            const double r1 = ro[3 * v + 1] - alpha * Ap[3 * v + 1];
+            const double r2 = ro[3 * v + 2] - alpha * Ap[3 * v + 2];
+            if (JAC) {
+                z0 = P.dinv[3 * v] * r0;
+                z1 = P.dinv[3 * v + 1] * r1;
+                z2 = P.dinv[3 * v + 2] * r2;
+            }
+            P.x[3 * v] = P.x[3 * v] + alpha * P.p_new[3 * v];
+            P.x[3 * v + 1] = P.x[3 * v + 1] + alpha * P.p_new[3 * v + 1];
+            P.x[3 * v + 2] = P.x[3 * v + 2] + alpha * P.p_new[3 * v + 2];
+            P.r_new[3 * v] = r0;
+            P.r_new[3 * v + 1] = r1;
+            P.r_new[3 * v + 2] = r2;
+            P.z[3 * v] = z0;
+            P.z[3 * v + 1] = z1;
+            P.z[3 * v + 2] = z2;
+            part[0] += z0 * r0 + z1 * r1 + z2 * r2;
+            part[1] += r0 * r0 + r1 * r1 + r2 * r2;
         }
-        P.x[3 * v] = P.x[3 * v] + alpha * P.p_new[3 * v];
-        P.x[3 * v + 1] = P.x[3 * v + 1] + alpha * P.p_new[3 * v + 1];
-        P.x[3 * v + 2] = P.x[3 * v + 2] + alpha * P.p_new[3 * v + 2];
-        P.r_new[3 * v] = r0;
-        P.r_new[3 * v + 1] = r1;
-        P.r_new[3 * v + 2] = r2;
-        P.z[3 * v] = z0;
-        P.z[3 * v + 1] = z1;
-        P.z[3 * v + 2] = z2;
-        part[0] = z0 * r0 + z1 * r1 + z2 * r2;
-        part[1] = r0 * r0 + r1 * r1 + r2 * r2;
-    }
-    double tot[2];
-    if (grid_reduce_last<2>(part, P.partials, P.counter, tot) && threadIdx.x == 0) {
-        sc->iters += 1;
-        const double zTr_next = tot[0];
-        sc->beta = zTr_next / sc->zTr;
-        sc->zTr = zTr_next;
-        sc->rr = tot[1];
-        if (sqrt(tot[1]) <= sc->tol * sc->bnorm)
-            sc->status = CG_CONVERGED;
-        else if (sc->max_iter >= 0 && sc->iters >= sc->max_iter)
-            sc->status = HOEFEM_ERROR_CG_MAX_ITERATIONS;
-    }
+    });
+    block_partials<2>(part, P.part_b);
 }
 
 __global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
@@ -297,6 +328,7 @@ __global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
     sc->status = CG_RUNNING;
     sc->iters = 0;
     sc->beta = 0.0;
+    sc->zrec[0] = sc->zrec[1] = 0.0;
 }
 
 // DiagonalPreconditioner::update (simulation_toolbox/src/fem/newton_cg.rs:558-575)
@@ -315,6 +347,14 @@ __global__ void k_jacobi_diag(int64_t ndof, const int32_t* __restrict__ diag_pos
     }
 }
 
+// single block: *out = sum of nb partials (the residual norm)
+__global__ __launch_bounds__(CG_THREADS) void k_sum_partials(const double* __restrict__ partials, int nb,
+                                                             double* __restrict__ out) {
+    double tot[1];
+    sum_partials<1>(partials, nb, tot);
+    if (threadIdx.x == 0) *out = tot[0];
+}
+
 void CgWork::ensure(int64_t nn_dof, int nblocks) {
     if (n != nn_dof) {
         n = nn_dof;
@@ -326,10 +366,13 @@ void CgWork::ensure(int64_t nn_dof, int nblocks) {
         Ap.alloc(n);
         diag_inv.alloc(n);
     }
-    if ((int64_t)partials.n < 2 * (int64_t)nblocks) partials.alloc(2 * (size_t)nblocks);
-    if (!counter.p) {
-        counter.alloc(1);
-        HIP_CHECK(hipMemset(counter.get(), 0, sizeof(unsigned)));
+    if (!part_pre.p) {
+        part_pre.alloc(CG_MAX_BLOCKS);
+        part_a.alloc(CG_MAX_BLOCKS);
+        part_b.alloc(2 * CG_MAX_BLOCKS);
+    }
+    if ((int64_t)partials.n < (int64_t)nblocks) partials.alloc((size_t)nblocks);
+    if (!scalars.p) {
         scalars.alloc(1);
         HIP_CHECK(hipMemset(scalars.get(), 0, sizeof(CgScalars)));
     }
@@ -353,17 +396,19 @@ template <int GA, int GB, bool JAC, class VT>
 static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>& Pm, const double* b, double* x,
                           double tol, int64_t max_iter, int64_t* iterations, hoefem_status* out_status) {
     const hipStream_t s = m.stream;
-    const int64_t nn = A.nn;
-    const unsigned blkA = (unsigned)((nn * GA + CG_THREADS - 1) / CG_THREADS);
-    const unsigned blkB = (unsigned)((nn * GB + CG_THREADS - 1) / CG_THREADS);
+    const unsigned blkA = cg_grid(A.nn, GA);
+    const unsigned blkB = cg_grid(Pm.nn, GB);
     CgPtrs P{};
     P.x = x;
     P.z = w.z.get();
     P.Ap = w.Ap.get();
     P.b = b;
     P.dinv = w.diag_inv.get();
-    P.partials = w.partials.get();
-    P.counter = w.counter.get();
+    P.part_pre = w.part_pre.get();
+    P.part_a = w.part_a.get();
+    P.part_b = w.part_b.get();
+    P.nb_a = (int)blkA;
+    P.nb_b = (int)blkB;
     P.sc = w.scalars.get();
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
     HIP_CHECK(hipMemsetAsync(w.p[0].get(), 0, sizeof(double) * w.n, s));
@@ -372,28 +417,31 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     k_cg_pre2<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
     check_launch("cg preamble");
     int cur = 0;
-    size_t nev = 0;  // events used by this solve (profiling only)
+    int64_t k_next = 0;  // iteration index of the next A/B pair
+    size_t nev = 0;      // events used by this solve (profiling only)
     auto enqueue_batch = [&]() {
-        for (int k = 0; k < CG_BATCH; ++k) {
+        for (int q = 0; q < CG_BATCH; ++q) {
             P.p_old = w.p[cur].get();
             P.p_new = w.p[1 - cur].get();
             P.r_old = w.r[cur].get();
             P.r_new = w.r[1 - cur].get();
             if (m.prof) {
+                // the events take the kernel's own start / end timestamps (not a marker
+                // before and after the launch), so dispatch gaps are not counted
                 while (m.prof_ev.size() < nev + 2) {
                     hipEvent_t e;
                     HIP_CHECK(hipEventCreate(&e));
                     m.prof_ev.push_back(e);
                 }
-                HIP_CHECK(hipEventRecord(m.prof_ev[nev], s));
-            }
-            k_cg_a<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
-            if (m.prof) {
-                HIP_CHECK(hipEventRecord(m.prof_ev[nev + 1], s));
+                hipExtLaunchKernelGGL(k_cg_a<GA, VT>, dim3(blkA), dim3(CG_THREADS), 0, s, m.prof_ev[nev],
+                                      m.prof_ev[nev + 1], 0, A, P, k_next);
                 nev += 2;
+            } else {
+                k_cg_a<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P, k_next);
             }
-            k_cg_b<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+            k_cg_b<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P, k_next);
             cur ^= 1;
+            ++k_next;
         }
         check_launch("cg iteration");
     };
@@ -510,7 +558,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3<double> M, const i
                                                          const double* __restrict__ gforce,
                                                          const uint8_t* __restrict__ dirmask, double dt,
                                                          const double* __restrict__ w, double* __restrict__ F,
-                                                         double* partials, unsigned* counter, double* norm2) {
+                                                         double* partials) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
     const int lane = (int)(gid % G);
@@ -540,8 +588,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3<double> M, const i
         F[3 * v + 2] = F2;
         part[0] = F0 * F0 + F1 * F1 + F2 * F2;
     }
-    double tot[1];
-    if (grid_reduce_last<1>(part, partials, counter, tot) && threadIdx.x == 0) *norm2 = tot[0];
+    block_partials<1>(part, partials);
 }
 
 // w <- w + delta * (-sol)  (line-search step along p = -sol, hamilton2/src/newton.rs:113-116, :219)
@@ -615,7 +662,7 @@ void launch_residual(Model& m, const double* Mvals, const double* gforce, const 
 #define RES_CASE(GG)                                                                                              \
     hipLaunchKernelGGL(k_residual<GG>, dim3(blocks), dim3(CG_THREADS), 0, m.stream, Mv, m.scatter.d_inc_ptr.get(), \
                        m.scatter.d_inc_blk.get(), m.scatter.d_inc_loc.get(), m.n, m.d_fe.get(), gforce, dirmask, dt,  \
-                       w, F, cw.partials.get(), cw.counter.get(), norm2)
+                       w, F, cw.partials.get())
     if (G == 32)
         RES_CASE(32);
     else if (G == 16)
@@ -623,6 +670,7 @@ void launch_residual(Model& m, const double* Mvals, const double* gforce, const 
     else
         RES_CASE(8);
 #undef RES_CASE
+    k_sum_partials<<<dim3(1), dim3(CG_THREADS), 0, m.stream>>>(cw.partials.get(), (int)blocks, norm2);
     check_launch("k_residual");
 }
 

@@ -54,6 +54,42 @@ def random_displacement(scene, scale, seed):
     return u
 
 
+def schwarz_lapack(rp, col, J, prp, pcol, sc):
+    """P = sum_s R_s^T B_s^+ R_s + 1/J_ii on untreated nodes, with B_s^+ from numpy's eigh
+    (LAPACK ?syevd) and the clamp max(|lambda|, 1e-12 max|lambda|)
+    (simulation_toolbox/src/fem/schwarz_precond.rs:171-216, 329-342). A third computation of
+    the Schwarz operator, independent of both Jacobi eigen-solvers."""
+    n = len(rp) - 1
+    keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(prp)) * n + pcol
+    P = np.zeros(len(pcol))
+
+    def entries(rows, cols):
+        k = rows[:, None].astype(np.int64) * n + cols[None, :]
+        pos = np.searchsorted(keys, k)
+        assert np.all(keys[pos] == k)
+        return pos
+
+    for s in range(len(sc.subdomain_offsets) - 1):
+        nodes = sc.subdomain_nodes[sc.subdomain_offsets[s]:sc.subdomain_offsets[s + 1]]
+        dofs = (3 * nodes[:, None] + np.arange(3)[None, :]).ravel()
+        B = np.zeros((len(dofs), len(dofs)))
+        for a, r in enumerate(dofs):
+            cols = col[rp[r]:rp[r + 1]]
+            pos = np.searchsorted(cols, dofs)
+            assert np.all(cols[pos] == dofs)
+            B[a] = J[rp[r] + pos]
+        w, V = np.linalg.eigh(B)
+        lower = 1e-12 * np.abs(w).max()
+        f = np.where(np.abs(w) >= lower, 1.0 / np.abs(w), 1.0 / lower)
+        np.add.at(P, entries(dofs, dofs), (V * f) @ V.T)
+    for v_ in sc.untreated_nodes:
+        for i in range(3):
+            r = 3 * v_ + i
+            k = rp[r] + np.searchsorted(col[rp[r]:rp[r + 1]], r)
+            P[prp[r]] = 1.0 / J[k]
+    return P
+
+
 def test_material_eval_matches_oracle(fem):
     rng = np.random.default_rng(1)
     F = np.eye(3) + 0.2 * rng.standard_normal((64, 3, 3))
@@ -125,9 +161,19 @@ def test_dirichlet_and_schwarz_match_oracle(fem, name):
     assert np.array_equal(prp_o, prp_g) and np.array_equal(pcol_o, pcol_g)
     gsw.update(dev(J_o))
     P_g = host(gsw.values())
-    assert rel(P_g, P_o) < 1e-9
+    # The block pseudo-inverses are as accurate as the blocks' conditioning allows (eps * kappa);
+    # Hex27 blocks of nearly empty cut cells reach kappa ~ 1e8. The bar: within 1e-9 of the
+    # oracle, or no further from a LAPACK-based P than the oracle itself is (times 4).
+    err_go = rel(P_g, P_o)
+    if err_go >= 1e-9:
+        P_l = schwarz_lapack(rp, col, J_o, prp_o, pcol_o, sc)
+        err_gl, err_ol = rel(P_g, P_l), rel(P_o, P_l)
+        print(f"{name}: |P_gpu - P_oracle| {err_go:.2e}, |P_gpu - P_lapack| {err_gl:.2e}, "
+              f"|P_oracle - P_lapack| {err_ol:.2e}")
+        assert err_gl <= 4.0 * err_ol and err_go < 1e-7
+    # the apply kernel against a CPU SpMV with the same (GPU) operator values
     x = np.random.default_rng(5).standard_normal(sc.ndof)
-    assert rel(host(gsw.apply(dev(x))), O.spmv(prp_o, pcol_o, P_o, x)) < 1e-9
+    assert rel(host(gsw.apply(dev(x))), O.spmv(prp_o, pcol_o, P_g, x)) < 1e-13
     # PCG with the Schwarz preconditioner, tight tolerance, on the SPD rest-state Jacobian
     # (a deformed state can make M + c K indefinite where background nodes carry tiny mass)
     K0 = om.stiffness_add(mu, lam, np.zeros(sc.ndof), np.zeros(om.nnz))
