@@ -42,16 +42,20 @@ def parse():
     p.add_argument("--no-profile", action="store_true", help="do not bracket the CG kernel with events")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run a plain 256 MiB device copy 3x first (reference kernel for HBM counters)")
+    p.add_argument("--mixed", action="store_true",
+                   help="FP32 element tangents / Jacobian SpMV / Schwarz operator inside the FP64 Newton refinement")
+    p.add_argument("--no-autotune", action="store_true", help="keep the heuristic SpMV lanes per node")
     return p.parse_args()
 
 
-def cg_kernel_bytes(model) -> int:
+def cg_kernel_bytes(model, value_bytes: int = 8) -> int:
     """Algorithmic bytes of one launch of the CG system-matrix kernel: every matrix value
-    once (8 B), one node-level column index per 3x3 block (4 B per 9 values), row offsets,
-    node offsets, and four DOF vectors (read z, read p_old, write p_new, write Ap)."""
+    once (8 B, or 4 B in mixed precision), one node-level column index per 3x3 block (4 B per
+    9 values), row offsets, node offsets, and four FP64 DOF vectors (read z, read p_old, write
+    p_new, write Ap)."""
     nnz, ndof = model.nnz, model.ndof
     nv = ndof // 3
-    return nnz * 8 + (nnz // 9) * 4 + (ndof + 1) * 4 + (nv + 1) * 4 + 4 * ndof * 8
+    return nnz * value_bytes + (nnz // 9) * 4 + (ndof + 1) * 4 + (nv + 1) * 4 + 4 * ndof * 8
 
 
 def main():
@@ -79,19 +83,22 @@ def main():
     t_setup = time.time()
     sc = scenes.config_scene(args.config)
     model = fem.ElasticityModel(sc)
-    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6)
+    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mixed)
     L = _lib.load()
+    lanes = None if args.no_autotune else sim.autotune()
     print(f"[bench] {sc.name}: {sc.num_elements} elements, {sc.ndof} dofs, nnz {model.nnz}, "
-          f"setup {time.time() - t_setup:.1f}s", file=sys.stderr, flush=True)
+          f"setup {time.time() - t_setup:.1f}s, SpMV lanes per node (J, P) {lanes}", file=sys.stderr, flush=True)
+    precision = "mixed" if args.mixed else "fp64"
 
     def run(steps, record=None):
         u = torch.zeros(sc.ndof, dtype=torch.float64, device="cuda")
         v = torch.zeros_like(u)
-        tot = {"newton_iterations": 0, "cg_iterations": 0, "substeps": 0}
+        tot = {"newton_iterations": 0, "cg_iterations": 0, "substeps": 0, "res_over_tol": 0.0}
         for k in range(steps):
             st = sim.step(k * sc.dt, sc.dt, u, v)
-            for key in tot:
+            for key in ("newton_iterations", "cg_iterations", "substeps"):
                 tot[key] += st[key]
+            tot["res_over_tol"] = max(tot["res_over_tol"], st["final_residual"] / st["tolerance"])
             if record is not None and k + 1 == record[0]:
                 record[1] = u.cpu().numpy().copy()
         return u, tot
@@ -109,16 +116,17 @@ def main():
     L.hoefem_profile_read(model.h, C.byref(full_n), C.byref(full_ms), C.byref(all_n), C.byref(all_ms))
     L.hoefem_profile_enable(model.h, 0)
     value = args.steps / elapsed
+    last_res = round(tot["res_over_tol"], 4)
     print(f"[bench] {args.steps} steps in {elapsed:.3f}s: {value:.2f} steps/s; newton {tot['newton_iterations']}, "
           f"cg {tot['cg_iterations']}", file=sys.stderr, flush=True)
 
     roofline = None
     if full_n.value > 0:
         avg_ms = full_ms.value / full_n.value
-        nbytes = cg_kernel_bytes(model)
+        nbytes = cg_kernel_bytes(model, 4 if args.mixed else 8)
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
         traffic = None
-        tpath = os.path.join(ROOT, "profiles", f"traffic_cg_matrix_kernel_{sc.name}.json")
+        tpath = os.path.join(ROOT, "profiles", f"traffic_cg_matrix_kernel_{sc.name}_{precision}.json")
         if os.path.exists(tpath):
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -147,8 +155,14 @@ def main():
         cpu = {"value": round(args.cpu_steps / (c1 - c0), 4), "unit": "timesteps/s", "cores": threads, "kind": "port",
                "sample": f"first {args.cpu_steps} of the {sc.steps} steps of {sc.name} (same settings), "
                          f"C++ FP64 oracle with OpenMP over element colors"}
-        err = float(np.linalg.norm(rec[1] - uo) / max(np.linalg.norm(uo), 1e-300))
-        print(f"[bench] cpu baseline {cpu['value']} steps/s on {threads} threads; gpu-vs-oracle u rel-L2 after "
+        rp, col = om.pattern()
+        Mv = om.mass(sc.density)
+        d = rec[1] - uo
+        err = {"u_rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(uo), 1e-300)),
+               "u_rel_mass_norm": float(np.sqrt(max(d @ O.spmv(rp, col, Mv, d), 0.0))
+                                        / max(np.sqrt(uo @ O.spmv(rp, col, Mv, uo)), 1e-300)),
+               "steps": args.cpu_steps}
+        print(f"[bench] cpu baseline {cpu['value']} steps/s on {threads} threads; gpu-vs-oracle after "
               f"{args.cpu_steps} steps: {err}", file=sys.stderr, flush=True)
 
     line = {
@@ -162,14 +176,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if args.mixed else "f64",
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": sc.name, "element": scenes.ELEMENT_NAMES[sc.element_type], "cells": sc.num_elements,
                    "dofs": sc.ndof, "nnz": model.nnz, "dt": sc.dt, "newton_rel_tol": 1e-6, "cg_rel_tol": 1e-5,
-                   "preconditioner": "schwarz", "parallelism": "single GPU"},
+                   "preconditioner": "schwarz", "precision": precision, "spmv_lanes_per_node": lanes,
+                   "parallelism": "single GPU"},
         "newton_iterations_per_step": tot["newton_iterations"] / args.steps,
         "cg_iterations_per_step": tot["cg_iterations"] / args.steps,
-        "error_vs_oracle": None if err is None else {"u_rel_l2": err, "steps": args.cpu_steps},
+        "error_vs_oracle": err,
+        "final_residual_over_tolerance": last_res,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -19,6 +19,8 @@ void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, c
 void launch_zero_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask);
 void launch_finite_check(Model& m, const double* a, const double* b);
 void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n);
+void launch_f64_to_f32(Model& m, const double* a, float* b, int64_t n);
+void launch_fill(Model& m, double* a, int64_t n, double value);
 
 thread_local std::string g_last_error;
 
@@ -613,12 +615,66 @@ hoefem_status hoefem_schwarz_autotune(hoefem_schwarz* s, int32_t* lanes_per_node
     });
 }
 
+// Times a fixed number of preconditioned CG iterations for every pair (lanes per node of the
+// Jacobian kernel, lanes per node of the preconditioner kernel) and keeps the fastest. The
+// trial system is J = M (symmetric positive definite) with the Schwarz operator of M and
+// tolerance 0, so every trial runs exactly the same number of full iterations. J and P are
+// overwritten; the next step assembles them again.
+static void sim_autotune_cg(Sim& s, int* ga_out, int* gb_out) {
+    Model& m = *s.model;
+    const int64_t n = 3 * m.nv, nnz = m.pattern.nnz();
+    if (s.mixed) {
+        launch_f64_to_f32(m, s.d_M.get(), s.d_J32.get(), nnz);
+        if (s.schwarz) schwarz_update(*s.schwarz, s.d_J32.get());
+    } else {
+        HIP_CHECK(hipMemcpyAsync(s.d_J.get(), s.d_M.get(), sizeof(double) * nnz, hipMemcpyDeviceToDevice, m.stream));
+        if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
+    }
+    launch_fill(m, s.d_F.get(), n, 1.0);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    const int cand[3] = {8, 16, 32};
+    int best_a = 8, best_b = 8;
+    double best_t = 1e300;
+    for (int ia = 0; ia < 3; ++ia)
+        for (int ib = 0; ib < (s.schwarz ? 3 : 1); ++ib) {
+            m.pattern.tuned_group = cand[ia];
+            if (s.schwarz) s.schwarz->P.tuned_group = cand[ib];
+            int64_t it = 0;
+            auto run = [&](int64_t iters) {
+                launch_fill(m, s.d_sol.get(), n, 0.0);
+                if (s.mixed)
+                    (void)pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), s.d_F.get(), s.d_sol.get(), 0.0, iters, &it);
+                else
+                    (void)pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), s.d_F.get(), s.d_sol.get(), 0.0, iters, &it);
+            };
+            run(4);
+            HIP_CHECK(hipEventRecord(e0, m.stream));
+            run(48);
+            HIP_CHECK(hipEventRecord(e1, m.stream));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_t) {
+                best_t = ms;
+                best_a = cand[ia];
+                best_b = cand[ib];
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    m.pattern.tuned_group = best_a;
+    if (s.schwarz) s.schwarz->P.tuned_group = best_b;
+    *ga_out = best_a;
+    *gb_out = s.schwarz ? best_b : 0;
+}
+
 hoefem_status hoefem_sim_autotune(hoefem_sim* sim, int32_t* lanes_matrix, int32_t* lanes_preconditioner) {
     return guarded([&] {
         if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
-        Model& m = *sim->s.model;
-        const int ga = autotune_spmv(m.pattern, m.stream, nullptr);
-        const int gb = sim->s.schwarz ? autotune_spmv(sim->s.schwarz->P, m.stream, nullptr) : 0;
+        int ga = 0, gb = 0;
+        sim_autotune_cg(sim->s, &ga, &gb);
         if (lanes_matrix) *lanes_matrix = ga;
         if (lanes_preconditioner) *lanes_preconditioner = gb;
     });

@@ -87,6 +87,13 @@ enum : int { KERR_SINGULAR = 1, KERR_PATTERN = 2, KERR_NONFINITE = 4, KERR_ZERO_
 // A CSR pattern whose rows come in xyz triples sharing one node-level column set (element
 // node pairs x 3x3 blocks, fenris/src/assembly.rs:177-249). "Diagonal-only" nodes carry a
 // single diagonal entry per row (untreated Schwarz nodes, schwarz_precond.rs:150-158).
+// One node row block as the SpMV kernels consume it: node id, first neighbour (into ncol),
+// neighbour count, offset of the block's first value (rowptr[3 node]). Nodes with neighbours
+// come first, diagonal-only nodes after them.
+struct __align__(16) NodeEntry {
+    int32_t node, first, count, rowoff;
+};
+
 struct NodePattern {
     int64_t num_nodes = 0;
     std::vector<int32_t> nptr, ncol;  // node-level neighbour lists, sorted
@@ -97,6 +104,9 @@ struct NodePattern {
     int tuned_group = 0;  // lanes per node row block chosen by autotune_spmv (0: heuristic)
     DevBuf<int32_t> d_nptr, d_ncol, d_rowptr, d_col, d_diag_pos;
     DevBuf<uint8_t> d_diag_only;
+    std::vector<NodeEntry> entries;   // block nodes, then diagonal-only nodes
+    int64_t n_block = 0, n_diag = 0;
+    DevBuf<NodeEntry> d_entries;
     int64_t nnz() const { return (int64_t)col.size(); }
     int64_t ndof() const { return 3 * num_nodes; }
     int32_t find(int32_t a, int32_t b) const;  // local index of b in nbr(a), or -1
@@ -199,7 +209,8 @@ constexpr int CG_MAX_BLOCKS = 1024;
 
 struct CgWork {
     int64_t n = 0;
-    DevBuf<double> r[2], p[2], z, Ap, diag_inv;
+    DevBuf<double> zp[2], ra[2];  // per node [z | p] and [r | Ap], six doubles, double-buffered
+    DevBuf<double> diag_inv;
     DevBuf<double> part_pre, part_a, part_b;  // b.b (preamble 1), pAp (A-kernel), zTr and rr (B-kernel)
     DevBuf<double> partials;                  // residual kernel partials
     DevBuf<CgScalars> scalars;

@@ -47,6 +47,15 @@ void NodePattern::upload(hipStream_t s) {
     d_col.upload(col, s);
     d_diag_pos.upload(diag_pos, s);
     d_diag_only.upload(diag_only, s);
+    entries.clear();
+    for (int64_t v = 0; v < num_nodes; ++v)
+        if (!diag_only[v])
+            entries.push_back(NodeEntry{(int32_t)v, nptr[v], nptr[v + 1] - nptr[v], rowptr[3 * v]});
+    n_block = (int64_t)entries.size();
+    for (int64_t v = 0; v < num_nodes; ++v)
+        if (diag_only[v]) entries.push_back(NodeEntry{(int32_t)v, 0, 0, rowptr[3 * v]});
+    n_diag = (int64_t)entries.size() - n_block;
+    d_entries.upload(entries, s);
 }
 
 // The set of node pairs of every node set, expanded to 3x3 blocks, sorted by row and column:

@@ -84,21 +84,26 @@ struct Csr3 {  // node-blocked CSR view
     const int32_t* rowptr;
     const uint8_t* diag_only;
     const VT* vals;
+    const NodeEntry* ent;  // block nodes [0, n_block), diagonal-only nodes after them
+    int64_t n_block, n_diag;
 };
 
-// acc_i = sum_j A(3v+i, j) * xfun(j) over the node row block of v (G lanes, reduced into all
-// lanes). Products and accumulation in the matrix value type VT (FP64, or FP32 for the
-// mixed-precision operator); the result is returned in FP64.
-template <int G, class VT, class XF>
-__device__ inline void row_block_dot(const Csr3<VT>& A, int64_t v, int lane, bool valid, XF xfun, double& r0,
+// acc_i = sum_j A(3v+i, 3b+c) * x_c(b) over the node row block of v (G lanes, reduced into
+// all lanes); xnode(b, x0, x1, x2) yields the three components of node b. Products and
+// accumulation in the matrix value type VT (FP64, or FP32 for the mixed-precision operator);
+// the result is returned in FP64.
+template <int G, class VT, class XN>
+__device__ inline void row_block_dot(const Csr3<VT>& A, int64_t v, int lane, bool valid, XN xnode, double& r0,
                                      double& r1, double& r2) {
     VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
     if (valid) {
         if (A.diag_only[v]) {
             if (lane == 0) {
-                a0 = A.vals[A.rowptr[3 * v]] * VT(xfun(3 * v));
-                a1 = A.vals[A.rowptr[3 * v + 1]] * VT(xfun(3 * v + 1));
-                a2 = A.vals[A.rowptr[3 * v + 2]] * VT(xfun(3 * v + 2));
+                double x0, x1, x2;
+                xnode(v, x0, x1, x2);
+                a0 = A.vals[A.rowptr[3 * v]] * VT(x0);
+                a1 = A.vals[A.rowptr[3 * v + 1]] * VT(x1);
+                a2 = A.vals[A.rowptr[3 * v + 2]] * VT(x2);
             }
         } else {
             const int kb = A.nptr[v], k = A.nptr[v + 1] - kb;
@@ -107,7 +112,9 @@ __device__ inline void row_block_dot(const Csr3<VT>& A, int64_t v, int lane, boo
             const VT* R2 = R1 + 3 * k;
             for (int j = lane; j < k; j += G) {
                 const int64_t b = A.ncol[kb + j];
-                const VT x0 = VT(xfun(3 * b)), x1 = VT(xfun(3 * b + 1)), x2 = VT(xfun(3 * b + 2));
+                double y0, y1, y2;
+                xnode(b, y0, y1, y2);
+                const VT x0 = VT(y0), x1 = VT(y1), x2 = VT(y2);
                 a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
                 a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
                 a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
@@ -125,32 +132,90 @@ __device__ inline void row_block_dot(const Csr3<VT>& A, int64_t v, int lane, boo
     r2 = double(a2);
 }
 
-// Grid-stride loop over groups of CG_THREADS / G nodes; body(v, lane, valid) is called the same
-// number of times by every thread of a block (the G-lane shuffles need whole groups).
-template <int G, class F>
-__device__ inline void for_node_groups(int64_t nn, F body) {
-    constexpr int per = CG_THREADS / G;
-    const int64_t groups = (nn + per - 1) / per;
-    const int lane = (int)(threadIdx.x % G);
-    for (int64_t g = blockIdx.x; g < groups; g += gridDim.x) {
-        const int64_t v = g * per + threadIdx.x / G;
-        body(v, lane, v < nn);
+// acc_i = sum over the neighbours b of entry e of A(3v+i, 3b+c) x_c(b) (G lanes, reduced into
+// all lanes of the group). Products and accumulation in VT; the result is returned in FP64.
+template <int G, class VT, class XN>
+__device__ inline void entry_row_dot(const Csr3<VT>& A, const NodeEntry& e, int lane, bool valid, XN xnode,
+                                     double& r0, double& r1, double& r2) {
+    VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
+    if (valid) {
+        const int k = e.count;
+        const VT* R0 = A.vals + e.rowoff;
+        const VT* R1 = R0 + 3 * k;
+        const VT* R2 = R1 + 3 * k;
+        const int32_t* nc = A.ncol + e.first;
+        for (int j = lane; j < k; j += G) {
+            double y0, y1, y2;
+            xnode((int64_t)nc[j], y0, y1, y2);
+            const VT x0 = VT(y0), x1 = VT(y1), x2 = VT(y2);
+            a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
+            a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
+            a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
+        }
     }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, G);
+        a1 += __shfl_xor(a1, off, G);
+        a2 += __shfl_xor(a2, off, G);
+    }
+    r0 = double(a0);
+    r1 = double(a1);
+    r2 = double(a2);
 }
 
-inline unsigned cg_grid(int64_t nn, int G) {
-    const int64_t groups = (nn + CG_THREADS / G - 1) / (CG_THREADS / G);
+// Grid-stride loops over the entries of a pattern: block nodes in groups of CG_THREADS / G
+// (G lanes each; block(e, lane, valid) is called the same number of times by every thread of a
+// work-group, as the G-lane shuffles need whole groups), the next pass's entry prefetched;
+// then diagonal-only nodes one per thread (diag(e)).
+template <int G, class FB, class FD>
+__device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb, int64_t nd, FB block, FD diag) {
+    constexpr int per = CG_THREADS / G;
+    const int lane = (int)(threadIdx.x % G);
+    const int64_t groups = (nb + per - 1) / per;
+    const int64_t sub = threadIdx.x / G;
+    int64_t g = blockIdx.x;
+    NodeEntry e{0, 0, 0, 0};
+    if (g < groups && g * per + sub < nb) e = ent[g * per + sub];
+    for (; g < groups; g += gridDim.x) {
+        const NodeEntry cur = e;
+        const bool valid = g * per + sub < nb;
+        const int64_t gn = g + gridDim.x;
+        if (gn < groups && gn * per + sub < nb) e = ent[gn * per + sub];
+        block(cur, lane, valid);
+    }
+    for (int64_t i = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; i < nd; i += (int64_t)gridDim.x * CG_THREADS)
+        diag(ent[nb + i]);
+}
+
+inline unsigned cg_grid(int64_t n_block, int64_t n_diag, int G) {
+    const int64_t groups = (n_block + CG_THREADS / G - 1) / (CG_THREADS / G) + (n_diag + CG_THREADS - 1) / CG_THREADS;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(groups, CG_MAX_BLOCKS));
+}
+
+// The two vectors each SpMV gathers are stored interleaved per node, six doubles (48 B, three
+// 16-byte loads): zp = [z | p] and ra = [r | Ap], each double-buffered. Iteration k (c = k & 1):
+//   A_k gathers zp[c] = (z_k, p_{k-1}), writes p_k -> zp[1-c].p and Ap_k -> ra[c].a
+//   B_k gathers ra[c] = (r_k, Ap_k), reads p_k from zp[1-c], writes r_{k+1} -> ra[1-c].r and
+//       z_{k+1} -> zp[1-c].z
+__device__ inline void load6(const double* __restrict__ base, int64_t b, double (&q)[6]) {
+    const double2* s = reinterpret_cast<const double2*>(base + 6 * b);
+    const double2 u0 = s[0], u1 = s[1], u2 = s[2];
+    q[0] = u0.x;
+    q[1] = u0.y;
+    q[2] = u1.x;
+    q[3] = u1.y;
+    q[4] = u2.x;
+    q[5] = u2.y;
 }
 
 struct CgPtrs {
     double* x;
-    double* z;
-    double* Ap;
-    const double* r_old;
-    double* r_new;
-    const double* p_old;
-    double* p_new;
+    const double* zp_old;  // A_k: zp[c]                       (pre2: unused)
+    double* zp_new;        // A_k and B_k: zp[1-c]              (pre2: zp[0])
+    double* ap_out;        // A_k: ra[c] (its Ap half is written)
+    const double* ra_old;  // B_k: ra[c]                        (pre2: ra[0])
+    double* ra_new;        // B_k: ra[1-c]                      (pre1: ra[0])
     const double* b;
     const double* dinv;
     double* part_pre;
@@ -161,55 +226,85 @@ struct CgPtrs {
     CgScalars* sc;
 };
 
-// preamble 1: r = b - A x ; partial |b|^2
+// preamble 1: r = b - A x -> ra_new.r ; partial |b|^2
 template <int G, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3<VT> A, CgPtrs P) {
     const double* x = P.x;
     double part[1] = {0.0};
-    for_node_groups<G>(A.nn, [&](int64_t v, int lane, bool valid) {
-        double a0, a1, a2;
-        row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return x[j]; }, a0, a1, a2);
-        if (valid && lane == 0) {
-            const double b0 = P.b[3 * v], b1 = P.b[3 * v + 1], b2 = P.b[3 * v + 2];
-            P.r_new[3 * v] = b0 - a0;
-            P.r_new[3 * v + 1] = b1 - a1;
-            P.r_new[3 * v + 2] = b2 - a2;
-            part[0] += b0 * b0 + b1 * b1 + b2 * b2;
-        }
-    });
+    auto finish = [&](int64_t v, double a0, double a1, double a2) {
+        const double b0 = P.b[3 * v], b1 = P.b[3 * v + 1], b2 = P.b[3 * v + 2];
+        P.ra_new[6 * v] = b0 - a0;
+        P.ra_new[6 * v + 1] = b1 - a1;
+        P.ra_new[6 * v + 2] = b2 - a2;
+        part[0] += b0 * b0 + b1 * b1 + b2 * b2;
+    };
+    for_entries<G>(
+        A.ent, A.n_block, A.n_diag,
+        [&](const NodeEntry& e, int lane, bool valid) {
+            double a0, a1, a2;
+            entry_row_dot<G>(A, e, lane, valid, [&](int64_t b, double& x0, double& x1, double& x2) {
+                x0 = x[3 * b];
+                x1 = x[3 * b + 1];
+                x2 = x[3 * b + 2];
+            }, a0, a1, a2);
+            if (valid && lane == 0) finish(e.node, a0, a1, a2);
+        },
+        [&](const NodeEntry& e) {
+            const int64_t v = e.node;
+            const VT* d = A.vals + e.rowoff;
+            finish(v, double(d[0] * VT(x[3 * v])), double(d[1] * VT(x[3 * v + 1])), double(d[2] * VT(x[3 * v + 2])));
+        });
     block_partials<1>(part, P.part_pre);
 }
 
-// preamble 2: |b| ; z = P r ; partial zTr, |r|^2
+// preamble 2: |b| ; z = P r (r from ra_old) -> zp_new.z ; partial zTr, |r|^2
 template <int G, bool JAC, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
     double bb[1];
     sum_partials<1>(P.part_pre, P.nb_a, bb);
     if (blockIdx.x == 0 && threadIdx.x == 0) P.sc->bnorm = sqrt(bb[0]);
-    const double* r = P.r_new;
+    const double* ra = P.ra_old;
     double part[2] = {0.0, 0.0};
-    for_node_groups<G>(Pm.nn, [&](int64_t v, int lane, bool valid) {
-        double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-        if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return r[j]; }, z0, z1, z2);
-        if (valid && lane == 0) {
-            const double r0 = r[3 * v], r1 = r[3 * v + 1], r2 = r[3 * v + 2];
-            if (JAC) {
-                z0 = P.dinv[3 * v] * r0;
-                z1 = P.dinv[3 * v + 1] * r1;
-                z2 = P.dinv[3 * v + 2] * r2;
-            }
-            P.z[3 * v] = z0;
-            P.z[3 * v + 1] = z1;
-            P.z[3 * v + 2] = z2;
-            part[0] += z0 * r0 + z1 * r1 + z2 * r2;
-            part[1] += r0 * r0 + r1 * r1 + r2 * r2;
+    auto finish = [&](int64_t v, double z0, double z1, double z2) {
+        const double r0 = ra[6 * v], r1 = ra[6 * v + 1], r2 = ra[6 * v + 2];
+        if (JAC) {
+            z0 = P.dinv[3 * v] * r0;
+            z1 = P.dinv[3 * v + 1] * r1;
+            z2 = P.dinv[3 * v + 2] * r2;
         }
-    });
+        P.zp_new[6 * v] = z0;
+        P.zp_new[6 * v + 1] = z1;
+        P.zp_new[6 * v + 2] = z2;
+        part[0] += z0 * r0 + z1 * r1 + z2 * r2;
+        part[1] += r0 * r0 + r1 * r1 + r2 * r2;
+    };
+    if (JAC) {
+        for (int64_t v = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; v < Pm.nn; v += (int64_t)gridDim.x * CG_THREADS)
+            finish(v, 0.0, 0.0, 0.0);
+    } else {
+        for_entries<G>(
+            Pm.ent, Pm.n_block, Pm.n_diag,
+            [&](const NodeEntry& e, int lane, bool valid) {
+                double z0, z1, z2;
+                entry_row_dot<G>(Pm, e, lane, valid, [&](int64_t b, double& x0, double& x1, double& x2) {
+                    x0 = ra[6 * b];
+                    x1 = ra[6 * b + 1];
+                    x2 = ra[6 * b + 2];
+                }, z0, z1, z2);
+                if (valid && lane == 0) finish(e.node, z0, z1, z2);
+            },
+            [&](const NodeEntry& e) {
+                const int64_t v = e.node;
+                const VT* d = Pm.vals + e.rowoff;
+                finish(v, double(d[0] * VT(ra[6 * v])), double(d[1] * VT(ra[6 * v + 1])),
+                       double(d[2] * VT(ra[6 * v + 2])));
+            });
+    }
     block_partials<2>(part, P.part_b);
 }
 
 // A-kernel of iteration k: stopping tests on r_k, beta_k = zTr_k / zTr_{k-1},
-// p_new = z + beta p_old ; Ap = A p_new ; partial pAp
+// p_k = z_k + beta p_{k-1} ; Ap_k = A p_k ; partial pAp
 template <int G, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64_t k) {
     CgScalars* sc = P.sc;
@@ -240,30 +335,47 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64
         sc->rr = rr;
         sc->beta = beta;
     }
-    const double* z = P.z;
-    const double* po = P.p_old;
+    const double* zp = P.zp_old;
     double part[1] = {0.0};
-    for_node_groups<G>(A.nn, [&](int64_t v, int lane, bool valid) {
-        double a0, a1, a2;
-        row_block_dot<G>(A, v, lane, valid, [&](int64_t j) { return z[j] + beta * po[j]; }, a0, a1, a2);
-        if (valid && lane == 0) {
-            const double p0 = z[3 * v] + beta * po[3 * v];
-            const double p1 = z[3 * v + 1] + beta * po[3 * v + 1];
-            const double p2 = z[3 * v + 2] + beta * po[3 * v + 2];
-            P.p_new[3 * v] = p0;
-            P.p_new[3 * v + 1] = p1;
-            P.p_new[3 * v + 2] = p2;
-            P.Ap[3 * v] = a0;
-            P.Ap[3 * v + 1] = a1;
-            P.Ap[3 * v + 2] = a2;
-            part[0] += p0 * a0 + p1 * a1 + p2 * a2;
-        }
-    });
+    auto pvec = [&](int64_t b, double& x0, double& x1, double& x2) {
+        double q[6];
+        load6(zp, b, q);
+        x0 = q[0] + beta * q[3];
+        x1 = q[1] + beta * q[4];
+        x2 = q[2] + beta * q[5];
+    };
+    auto finish = [&](int64_t v, double p0, double p1, double p2, double a0, double a1, double a2) {
+        P.zp_new[6 * v + 3] = p0;
+        P.zp_new[6 * v + 4] = p1;
+        P.zp_new[6 * v + 5] = p2;
+        P.ap_out[6 * v + 3] = a0;
+        P.ap_out[6 * v + 4] = a1;
+        P.ap_out[6 * v + 5] = a2;
+        part[0] += p0 * a0 + p1 * a1 + p2 * a2;
+    };
+    for_entries<G>(
+        A.ent, A.n_block, A.n_diag,
+        [&](const NodeEntry& e, int lane, bool valid) {
+            double a0, a1, a2;
+            entry_row_dot<G>(A, e, lane, valid, pvec, a0, a1, a2);
+            if (valid && lane == 0) {
+                double p0, p1, p2;
+                pvec(e.node, p0, p1, p2);
+                finish(e.node, p0, p1, p2, a0, a1, a2);
+            }
+        },
+        [&](const NodeEntry& e) {
+            const int64_t v = e.node;
+            const VT* d = A.vals + e.rowoff;
+            double p0, p1, p2;
+            pvec(v, p0, p1, p2);
+            finish(v, p0, p1, p2, double(d[0] * VT(p0)), double(d[1] * VT(p1)), double(d[2] * VT(p2)));
+        });
     block_partials<1>(part, P.part_a);
 }
 
-// B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p ;
-// r_new = r_old - alpha Ap ; z = P r_new ; partial zTr, |r|^2
+// B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p_k ;
+// r_{k+1} = r_k - alpha Ap_k ; z_{k+1} = P r_{k+1} ; partial zTr, |r|^2
 template <int G, bool JAC, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P, int64_t k) {
     CgScalars* sc = P.sc;
@@ -291,34 +403,60 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P, int6
         sc->alpha = alpha;
         sc->iters = k + 1;
     }
-    const double* ro = P.r_old;
-    const double* Ap = P.Ap;
+    const double* ra = P.ra_old;
     double part[2] = {0.0, 0.0};
-    for_node_groups<G>(Pm.nn, [&](int64_t v, int lane, bool valid) {
-        double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-        if (!JAC) row_block_dot<G>(Pm, v, lane, valid, [&](int64_t j) { return ro[j] - alpha * Ap[j]; }, z0, z1, z2);
-        if (valid && lane == 0) {
-            const double r0 = ro[3 * v] - alpha * Ap[3 * v];
-            const double r1 = ro[3 * v + 1] - alpha * Ap[3 * v + 1];
-            const double r2 = ro[3 * v + 2] - alpha * Ap[3 * v + 2];
-            if (JAC) {
-                z0 = P.dinv[3 * v] * r0;
-                z1 = P.dinv[3 * v + 1] * r1;
-                z2 = P.dinv[3 * v + 2] * r2;
-            }
-            P.x[3 * v] = P.x[3 * v] + alpha * P.p_new[3 * v];
-            P.x[3 * v + 1] = P.x[3 * v + 1] + alpha * P.p_new[3 * v + 1];
-            P.x[3 * v + 2] = P.x[3 * v + 2] + alpha * P.p_new[3 * v + 2];
-            P.r_new[3 * v] = r0;
-            P.r_new[3 * v + 1] = r1;
-            P.r_new[3 * v + 2] = r2;
-            P.z[3 * v] = z0;
-            P.z[3 * v + 1] = z1;
-            P.z[3 * v + 2] = z2;
-            part[0] += z0 * r0 + z1 * r1 + z2 * r2;
-            part[1] += r0 * r0 + r1 * r1 + r2 * r2;
+    auto rvec = [&](int64_t b, double& x0, double& x1, double& x2) {
+        double q[6];
+        load6(ra, b, q);
+        x0 = q[0] - alpha * q[3];
+        x1 = q[1] - alpha * q[4];
+        x2 = q[2] - alpha * q[5];
+    };
+    auto finish = [&](int64_t v, double r0, double r1, double r2, double z0, double z1, double z2) {
+        if (JAC) {
+            z0 = P.dinv[3 * v] * r0;
+            z1 = P.dinv[3 * v + 1] * r1;
+            z2 = P.dinv[3 * v + 2] * r2;
         }
-    });
+        const double* pk = P.zp_new + 6 * v + 3;
+        P.x[3 * v] = P.x[3 * v] + alpha * pk[0];
+        P.x[3 * v + 1] = P.x[3 * v + 1] + alpha * pk[1];
+        P.x[3 * v + 2] = P.x[3 * v + 2] + alpha * pk[2];
+        P.ra_new[6 * v] = r0;
+        P.ra_new[6 * v + 1] = r1;
+        P.ra_new[6 * v + 2] = r2;
+        P.zp_new[6 * v] = z0;
+        P.zp_new[6 * v + 1] = z1;
+        P.zp_new[6 * v + 2] = z2;
+        part[0] += z0 * r0 + z1 * r1 + z2 * r2;
+        part[1] += r0 * r0 + r1 * r1 + r2 * r2;
+    };
+    if (JAC) {
+        for (int64_t v = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; v < Pm.nn; v += (int64_t)gridDim.x * CG_THREADS) {
+            double r0, r1, r2;
+            rvec(v, r0, r1, r2);
+            finish(v, r0, r1, r2, 0.0, 0.0, 0.0);
+        }
+    } else {
+        for_entries<G>(
+            Pm.ent, Pm.n_block, Pm.n_diag,
+            [&](const NodeEntry& e, int lane, bool valid) {
+                double z0, z1, z2;
+                entry_row_dot<G>(Pm, e, lane, valid, rvec, z0, z1, z2);
+                if (valid && lane == 0) {
+                    double r0, r1, r2;
+                    rvec(e.node, r0, r1, r2);
+                    finish(e.node, r0, r1, r2, z0, z1, z2);
+                }
+            },
+            [&](const NodeEntry& e) {
+                const int64_t v = e.node;
+                const VT* d = Pm.vals + e.rowoff;
+                double r0, r1, r2;
+                rvec(v, r0, r1, r2);
+                finish(v, r0, r1, r2, double(d[0] * VT(r0)), double(d[1] * VT(r1)), double(d[2] * VT(r2)));
+            });
+    }
     block_partials<2>(part, P.part_b);
 }
 
@@ -359,11 +497,9 @@ void CgWork::ensure(int64_t nn_dof, int nblocks) {
     if (n != nn_dof) {
         n = nn_dof;
         for (int k = 0; k < 2; ++k) {
-            r[k].alloc(n);
-            p[k].alloc(n);
+            zp[k].alloc(2 * n);
+            ra[k].alloc(2 * n);
         }
-        z.alloc(n);
-        Ap.alloc(n);
         diag_inv.alloc(n);
     }
     if (!part_pre.p) {
@@ -389,19 +525,18 @@ CgWork::~CgWork() {
 
 template <class VT>
 static Csr3<VT> view(const NodePattern& p, const VT* vals) {
-    return Csr3<VT>{p.num_nodes, p.d_nptr.get(), p.d_ncol.get(), p.d_rowptr.get(), p.d_diag_only.get(), vals};
+    return Csr3<VT>{p.num_nodes, p.d_nptr.get(), p.d_ncol.get(), p.d_rowptr.get(), p.d_diag_only.get(), vals,
+                    p.d_entries.get(), p.n_block, p.n_diag};
 }
 
 template <int GA, int GB, bool JAC, class VT>
 static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>& Pm, const double* b, double* x,
                           double tol, int64_t max_iter, int64_t* iterations, hoefem_status* out_status) {
     const hipStream_t s = m.stream;
-    const unsigned blkA = cg_grid(A.nn, GA);
-    const unsigned blkB = cg_grid(Pm.nn, GB);
+    const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA);
+    const unsigned blkB = JAC ? cg_grid(0, Pm.nn, GB) : cg_grid(Pm.n_block, Pm.n_diag, GB);
     CgPtrs P{};
     P.x = x;
-    P.z = w.z.get();
-    P.Ap = w.Ap.get();
     P.b = b;
     P.dinv = w.diag_inv.get();
     P.part_pre = w.part_pre.get();
@@ -411,9 +546,11 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     P.nb_b = (int)blkB;
     P.sc = w.scalars.get();
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
-    HIP_CHECK(hipMemsetAsync(w.p[0].get(), 0, sizeof(double) * w.n, s));
-    P.r_new = w.r[0].get();
+    HIP_CHECK(hipMemsetAsync(w.zp[0].get(), 0, sizeof(double) * 2 * w.n, s));  // p_{-1} = 0
+    P.ra_new = w.ra[0].get();
     k_cg_pre1<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
+    P.ra_old = w.ra[0].get();
+    P.zp_new = w.zp[0].get();
     k_cg_pre2<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
     check_launch("cg preamble");
     int cur = 0;
@@ -421,10 +558,11 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     size_t nev = 0;      // events used by this solve (profiling only)
     auto enqueue_batch = [&]() {
         for (int q = 0; q < CG_BATCH; ++q) {
-            P.p_old = w.p[cur].get();
-            P.p_new = w.p[1 - cur].get();
-            P.r_old = w.r[cur].get();
-            P.r_new = w.r[1 - cur].get();
+            P.zp_old = w.zp[cur].get();
+            P.zp_new = w.zp[1 - cur].get();
+            P.ap_out = w.ra[cur].get();
+            P.ra_old = w.ra[cur].get();
+            P.ra_new = w.ra[1 - cur].get();
             if (m.prof) {
                 // the events take the kernel's own start / end timestamps (not a marker
                 // before and after the launch), so dispatch gaps are not counted
@@ -564,7 +702,11 @@ __global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3<double> M, const i
     const int lane = (int)(gid % G);
     const bool valid = v < M.nn;
     double m0, m1, m2;
-    row_block_dot<G>(M, v, lane, valid, [&](int64_t j) { return w[j]; }, m0, m1, m2);
+    row_block_dot<G>(M, v, lane, valid, [&](int64_t b, double& x0, double& x1, double& x2) {
+        x0 = w[3 * b];
+        x1 = w[3 * b + 1];
+        x2 = w[3 * b + 2];
+    }, m0, m1, m2);
     double part[1] = {0.0};
     if (valid && lane == 0) {
         double f0 = 1.0 * gforce[3 * v], f1 = 1.0 * gforce[3 * v + 1], f2 = 1.0 * gforce[3 * v + 2];
@@ -638,6 +780,28 @@ __global__ void k_finite(int64_t n, const double* __restrict__ a, const double* 
 __global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (double)a[i];
+}
+
+__global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_fill(int64_t n, double* __restrict__ a, double value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = value;
+}
+
+void launch_f64_to_f32(Model& m, const double* a, float* b, int64_t n) {
+    if (n <= 0) return;
+    k_f64_to_f32<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream>>>(n, a, b);
+    check_launch("k_f64_to_f32");
+}
+
+void launch_fill(Model& m, double* a, int64_t n, double value) {
+    if (n <= 0) return;
+    k_fill<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream>>>(n, a, value);
+    check_launch("k_fill");
 }
 
 void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n) {
