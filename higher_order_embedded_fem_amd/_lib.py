@@ -51,6 +51,17 @@ EXPORTED = [
     "hoefem_sim_jacobian_values",
     "hoefem_sim_residual",
     "hoefem_sim_step",
+    "hoefem_learned_param_count",
+    "hoefem_learned_create",
+    "hoefem_learned_destroy",
+    "hoefem_learned_harvest",
+    "hoefem_learned_info",
+    "hoefem_learned_train",
+    "hoefem_learned_enable",
+    "hoefem_learned_get_params",
+    "hoefem_learned_set_params",
+    "hoefem_learned_mlp_forward",
+    "hoefem_learned_mlp_grad",
 ]
 
 
@@ -104,6 +115,25 @@ class ModelDesc(C.Structure):
         ("stiffness_quadrature", Quadrature),
         ("elliptic_quadrature", Quadrature),
         ("mass_regularization_factor", C.c_double),
+    ]
+
+
+class LearnedDesc(C.Structure):
+    _fields_ = [("num_aggregates", C.c_int32), ("seed", C.c_uint64)]
+
+
+class LearnedStats(C.Structure):
+    _fields_ = [
+        ("num_aggregates", C.c_int32),
+        ("max_aggregate_dofs", C.c_int32),
+        ("harvested_pairs", C.c_int64),
+        ("stored_pairs", C.c_int64),
+        ("trained", C.c_int32),
+        ("enabled", C.c_int32),
+        ("learned_solves", C.c_int64),
+        ("fallbacks", C.c_int64),
+        ("learned_iterations", C.c_int64),
+        ("last_status", C.c_int32),
     ]
 
 
@@ -201,11 +231,23 @@ def load(path: str = LIB_PATH):
     L.hoefem_sim_jacobian_values.argtypes = [vp, vp]
     L.hoefem_sim_residual.argtypes = [vp, f64, f64, vp, vp, vp, vp]
     L.hoefem_sim_step.argtypes = [vp, f64, f64, vp, vp, C.POINTER(StepStats)]
+    L.hoefem_learned_param_count.argtypes = []
+    L.hoefem_learned_param_count.restype = i64
+    L.hoefem_learned_create.argtypes = [vp, C.POINTER(LearnedDesc), C.POINTER(vp)]
+    L.hoefem_learned_destroy.argtypes = [vp]
+    L.hoefem_learned_harvest.argtypes = [vp, i32, i32, i32]
+    L.hoefem_learned_info.argtypes = [vp, C.POINTER(LearnedStats)]
+    L.hoefem_learned_train.argtypes = [vp, i32, f64, i32, C.POINTER(f64), C.POINTER(f64)]
+    L.hoefem_learned_enable.argtypes = [vp, i32]
+    L.hoefem_learned_get_params.argtypes = [vp, vp]
+    L.hoefem_learned_set_params.argtypes = [vp, vp]
+    L.hoefem_learned_mlp_forward.argtypes = [vp, i64, vp, vp]
+    L.hoefem_learned_mlp_grad.argtypes = [vp, i64, vp, vp, vp]
     for name in EXPORTED:
         if name not in ("hoefem_last_error", "hoefem_nodes_per_element", "hoefem_lame_from_young_poisson",
                         "hoefem_model_destroy", "hoefem_model_ndof", "hoefem_model_nnz", "hoefem_model_num_colors",
                         "hoefem_schwarz_destroy", "hoefem_schwarz_nnz", "hoefem_sim_destroy",
-                        "hoefem_sim_representative_force"):
+                        "hoefem_sim_representative_force", "hoefem_learned_param_count"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L

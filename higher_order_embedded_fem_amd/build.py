@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhoefem.so")
-SOURCES = ["setup.cpp", "assembly.hip", "schwarz.hip", "solver.hip", "api.cpp"]
+SOURCES = ["setup.cpp", "assembly.hip", "schwarz.hip", "solver.hip", "learned.hip", "api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",

@@ -141,6 +141,8 @@ struct Sim {
     DevBuf<float> d_J32;
     PinnedBuf<double> h_norm2{1};
     CgWork cg;
+    Learned* learned = nullptr;                 // optional learned coarse correction (not owned)
+    struct hoefem_learned* learned_handle = nullptr;
     double rep_force = 0.0;
     int64_t cg_total = 0, evals = 0;
     int last_err_flags = 0;
@@ -256,10 +258,46 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
     }
     HIP_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * 3 * m.nv, m.stream));
     int64_t it = 0;
-    hoefem_status st = s.mixed ? pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
-                                           s.d.cg_max_iter, &it)
-                               : pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
-                                           s.d.cg_max_iter, &it);
+    Learned* L = s.learned;
+    if (L && L->enabled && L->trained && s.schwarz) {
+        // learned flexible PCG. Its result is used under the plain path's rules (reaching the
+        // iteration limit is tolerated, NewtonCgDynamicSystem::solve_jacobian_combination,
+        // simulation_toolbox/src/fem/newton_cg.rs:680-802), and the Newton loop still has to
+        // reach the same residual tolerance; on a breakdown (an indefinite or non-finite
+        // preconditioned residual) the plain Schwarz-CG solves the system from zero.
+        const LearnedView lv = learned_view(*L);
+        const hoefem_status lst =
+            s.mixed ? pcg_solve_learned(m, s.cg, s.d_J32.get(), s.schwarz.get(), lv, rhs, sol, s.d.cg_tolerance,
+                                        s.d.cg_max_iter, &it)
+                    : pcg_solve_learned(m, s.cg, s.d_J.get(), s.schwarz.get(), lv, rhs, sol, s.d.cg_tolerance,
+                                        s.d.cg_max_iter, &it);
+        L->solves += 1;
+        L->iterations += it;
+        L->last_status = (int)lst;
+        s.cg_total += it;
+        if (lst == HOEFEM_OK || lst == HOEFEM_ERROR_CG_MAX_ITERATIONS) return HOEFEM_OK;
+        L->fallbacks += 1;
+        HIP_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * 3 * m.nv, m.stream));
+        it = 0;
+    }
+    hoefem_status st;
+    if (L && L->harvesting) {
+        CgHarvest h;
+        h.xs = L->d_snap_x.get();
+        h.rs = L->d_snap_r.get();
+        h.max = L->snap_max;
+        h.every = L->every;
+        st = s.mixed ? pcg_solve_harvest(m, s.cg, s.d_J32.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
+                                         s.d.cg_max_iter, &it, &h)
+                     : pcg_solve_harvest(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
+                                         s.d.cg_max_iter, &it, &h);
+        learned_harvest_solve(*L, sol, h.taken);
+    } else {
+        st = s.mixed ? pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
+                                 s.d.cg_max_iter, &it)
+                     : pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter,
+                                 &it);
+    }
     s.cg_total += it;
     if (st == HOEFEM_ERROR_CG_MAX_ITERATIONS || st == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) st = HOEFEM_OK;
     return st;
@@ -386,6 +424,10 @@ struct hoefem_schwarz {
 };
 struct hoefem_sim {
     Sim s;
+};
+struct hoefem_learned {
+    Learned L;
+    Sim* sim = nullptr;
 };
 
 template <class F>
@@ -711,7 +753,10 @@ hoefem_status hoefem_sim_create(hoefem_model* model, const hoefem_sim_desc* desc
     });
 }
 
-void hoefem_sim_destroy(hoefem_sim* sim) { delete sim; }
+void hoefem_sim_destroy(hoefem_sim* sim) {
+    if (sim && sim->s.learned_handle) sim->s.learned_handle->sim = nullptr;
+    delete sim;
+}
 
 double hoefem_sim_representative_force(const hoefem_sim* sim) { return sim ? sim->s.rep_force : 0.0; }
 
@@ -755,6 +800,124 @@ hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, d
     return guarded([&] {
         if (!sim || !u || !v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
         sim_step(sim->s, t, dt, u, v, stats);
+    });
+}
+
+
+// ------------------------------------------------------------------ learned coarse correction
+static Learned& LC_(hoefem_learned* l) {
+    if (!l) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null learned correction");
+    return l->L;
+}
+
+int64_t hoefem_learned_param_count(void) { return LC_PARAMS; }
+
+hoefem_status hoefem_learned_create(hoefem_sim* sim, const hoefem_learned_desc* desc, hoefem_learned** out) {
+    return guarded([&] {
+        if (!sim || !desc || !out) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        if (!sim->s.schwarz) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "the learned correction needs Schwarz subdomains");
+        Model& m = *sim->s.model;
+        std::unique_ptr<hoefem_learned> h(new hoefem_learned());
+        std::vector<uint8_t> mask(m.nv, 0);
+        for (int32_t v : sim->s.dir) mask[v] = 1;
+        learned_setup(h->L, m, desc->num_aggregates, mask, desc->seed);
+        if (h->L.na > CG_MAX_BLOCKS) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "too many aggregates");
+        if (sim->s.learned_handle) sim->s.learned_handle->sim = nullptr;  // replaces an earlier one
+        h->sim = &sim->s;
+        sim->s.learned = &h->L;
+        sim->s.learned_handle = h.get();
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+        *out = h.release();
+    });
+}
+
+hoefem_status hoefem_learned_destroy(hoefem_learned* l) {
+    return guarded([&] {
+        if (!l) return;
+        if (l->sim && l->sim->learned == &l->L) {
+            l->sim->learned = nullptr;
+            l->sim->learned_handle = nullptr;
+        }
+        delete l;
+    });
+}
+
+hoefem_status hoefem_learned_harvest(hoefem_learned* l, int32_t enable, int32_t capacity, int32_t every) {
+    return guarded([&] { learned_harvest_enable(LC_(l), enable != 0, capacity, every); });
+}
+
+hoefem_status hoefem_learned_info(hoefem_learned* l, hoefem_learned_stats* out) {
+    return guarded([&] {
+        Learned& L = LC_(l);
+        if (!out) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null output");
+        out->num_aggregates = L.na;
+        out->max_aggregate_dofs = L.max_dofs;
+        out->harvested_pairs = L.pairs;
+        out->stored_pairs = L.count;
+        out->trained = L.trained ? 1 : 0;
+        out->enabled = L.enabled ? 1 : 0;
+        out->learned_solves = L.solves;
+        out->fallbacks = L.fallbacks;
+        out->learned_iterations = L.iterations;
+        out->last_status = L.last_status;
+    });
+}
+
+hoefem_status hoefem_learned_train(hoefem_learned* l, int32_t steps, double learning_rate, int32_t batch,
+                                   double* initial_loss, double* final_loss) {
+    return guarded([&] {
+        Learned& L = LC_(l);
+        if (!l->sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "the simulation of this correction was destroyed");
+        Sim& s = *l->sim;
+        if (steps < 0 || batch < 1) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad training settings");
+        const double fl = s.mixed ? learned_train(L, s.d_J32.get(), s.schwarz.get(), steps, learning_rate, batch, initial_loss)
+                                  : learned_train(L, s.d_J.get(), s.schwarz.get(), steps, learning_rate, batch, initial_loss);
+        if (final_loss) *final_loss = fl;
+    });
+}
+
+hoefem_status hoefem_learned_enable(hoefem_learned* l, int32_t enable) {
+    return guarded([&] {
+        Learned& L = LC_(l);
+        if (enable && !L.trained) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "train the correction before enabling it");
+        L.enabled = enable != 0;
+    });
+}
+
+hoefem_status hoefem_learned_get_params(hoefem_learned* l, float* params) {
+    return guarded([&] {
+        Learned& L = LC_(l);
+        if (!params) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null output");
+        HIP_CHECK(hipStreamSynchronize(L.model->stream));
+        HIP_CHECK(hipMemcpy(params, L.d_params.get(), sizeof(float) * LC_PARAMS, hipMemcpyDeviceToHost));
+    });
+}
+
+hoefem_status hoefem_learned_set_params(hoefem_learned* l, const float* params) {
+    return guarded([&] {
+        Learned& L = LC_(l);
+        if (!params) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null input");
+        learned_set_params(L, params, false);
+        HIP_CHECK(hipStreamSynchronize(L.model->stream));
+    });
+}
+
+hoefem_status hoefem_learned_mlp_forward(hoefem_learned* l, int64_t n, const float* x, float* y) {
+    return guarded([&] {
+        Learned& L = LC_(l);
+        if (n < 0 || (n > 0 && (!x || !y))) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad batch");
+        if (n == 0) return;
+        learned_mlp_forward(L, n, x, y);
+        HIP_CHECK(hipStreamSynchronize(L.model->stream));
+    });
+}
+
+hoefem_status hoefem_learned_mlp_grad(hoefem_learned* l, int64_t n, const float* x, const float* dy, float* grad) {
+    return guarded([&] {
+        Learned& L = LC_(l);
+        if (n < 1 || !x || !dy || !grad) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad batch");
+        learned_mlp_grad(L, n, x, dy, grad, 1.0f);
+        HIP_CHECK(hipStreamSynchronize(L.model->stream));
     });
 }
 

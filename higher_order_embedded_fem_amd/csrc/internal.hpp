@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/hoefem.h"
+#include "mlp_device.hpp"
 
 namespace hoefem {
 
@@ -256,5 +257,74 @@ hoefem_status pcg_solve(Model& m, CgWork& w, const double* A, Schwarz* P, const 
 // mixed precision: FP32 system matrix and FP32 Schwarz operator (P->fp32 must be set), FP64 vectors / dots
 hoefem_status pcg_solve(Model& m, CgWork& w, const float* A, Schwarz* P, const double* b, double* x, double tol,
                         int64_t max_iter, int64_t* iterations);
+
+// ------------------------------------------------------------------ learned coarse correction (learned.hip)
+constexpr int LC_MAX_AGG_NODES = 64;   // nodes per aggregate (LDS staging of the apply kernels)
+constexpr int LC_SNAP_PER_SOLVE = 64;  // iterate snapshots kept per harvested solve
+
+struct LearnedView {  // what the CG apply kernel reads
+    int32_t na;
+    const int32_t* agg_ptr;   // [na + 1] offsets into agg_dofs (3 per node)
+    const int32_t* agg_dofs;  // DOFs grouped by aggregate
+    const float* basis;       // aggregate a: 64 modes x n_a DOFs at 64 * agg_ptr[a]
+    const uint16_t* wbf;      // BF16 weights
+    const float* params;      // FP32 master parameters (biases)
+    float kappa;              // output scale: c_a = kappa |g_a| MLP(g_a / |g_a|)
+};
+
+struct Learned {
+    Model* model = nullptr;
+    int32_t na = 0, max_dofs = 0;
+    std::vector<int32_t> agg_ptr, agg_dofs;
+    DevBuf<int32_t> d_agg_ptr, d_agg_dofs;
+    DevBuf<uint8_t> d_dofmask;
+    DevBuf<float> d_basis;
+    bool has_basis = false, trained = false, enabled = false, harvesting = false;
+    float kappa = 0.0f;  // mean |P r| / |r| over the first training batch
+    // network
+    DevBuf<float> d_params, d_m, d_v, d_grad, d_gpart;
+    DevBuf<uint16_t> d_wbf;
+    int adam_t = 0;
+    int64_t batch_cap = 0;
+    DevBuf<float> d_z1, d_h1, d_z2, d_h2, d_y, d_dy, d_x, d_gn;
+    void ensure_batch(int64_t N);
+    // harvest rings: error snapshots x* - x_k and residuals r_k
+    int cap = 0, every = 32, snap_max = 0;
+    int64_t head = 0, count = 0, pairs = 0;
+    DevBuf<double> d_err, d_res, d_snap_x, d_snap_r;
+    // statistics of the learned solves
+    int64_t solves = 0, fallbacks = 0, iterations = 0;
+    int last_status = 0;  // status of the last learned solve (0: converged)
+};
+
+void learned_setup(Learned& L, Model& m, int target_aggregates, const std::vector<uint8_t>& dir_node_mask,
+                   uint64_t seed);
+void learned_set_params(Learned& L, const float* host_or_dev, bool on_device);
+void learned_harvest_enable(Learned& L, bool on, int capacity, int every);
+void learned_harvest_solve(Learned& L, const double* xstar, int nsnap);
+void learned_build_basis(Learned& L);
+double learned_train(Learned& L, const double* J, Schwarz* P, int steps, double lr, int B, double* initial_loss);
+double learned_train(Learned& L, const float* J, Schwarz* P, int steps, double lr, int B, double* initial_loss);
+void learned_mlp_forward(Learned& L, int64_t N, const float* X, float* Y);
+void learned_mlp_grad(Learned& L, int64_t N, const float* X, const float* dY, float* grad, float scale);
+LearnedView learned_view(const Learned& L);
+
+// Snapshot hook of the plain CG: every `every` iterations x_k and r_k are copied into
+// xs / rs (at most max snapshots); *taken receives the count.
+struct CgHarvest {
+    double* xs = nullptr;
+    double* rs = nullptr;
+    int max = 0, every = 32;
+    int taken = 0;
+};
+hoefem_status pcg_solve_harvest(Model& m, CgWork& w, const double* A, Schwarz* P, const double* b, double* x,
+                                double tol, int64_t max_iter, int64_t* iterations, CgHarvest* h);
+hoefem_status pcg_solve_harvest(Model& m, CgWork& w, const float* A, Schwarz* P, const double* b, double* x,
+                                double tol, int64_t max_iter, int64_t* iterations, CgHarvest* h);
+// flexible PCG (Polak-Ribiere beta) with z = P r + learned coarse correction
+hoefem_status pcg_solve_learned(Model& m, CgWork& w, const double* A, Schwarz* P, const LearnedView& lv,
+                                const double* b, double* x, double tol, int64_t max_iter, int64_t* iterations);
+hoefem_status pcg_solve_learned(Model& m, CgWork& w, const float* A, Schwarz* P, const LearnedView& lv,
+                                const double* b, double* x, double tol, int64_t max_iter, int64_t* iterations);
 
 }  // namespace hoefem

@@ -258,7 +258,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3<VT> A, CgPtrs P) {
 }
 
 // preamble 2: |b| ; z = P r (r from ra_old) -> zp_new.z ; partial zTr, |r|^2
-template <int G, bool JAC, class VT>
+template <int G, bool JAC, bool FLEX, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
     double bb[1];
     sum_partials<1>(P.part_pre, P.nb_a, bb);
@@ -300,18 +300,28 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
                        double(d[2] * VT(ra[6 * v + 2])));
             });
     }
-    block_partials<2>(part, P.part_b);
+    if (!FLEX) block_partials<2>(part, P.part_b);
 }
 
 // A-kernel of iteration k: stopping tests on r_k, beta_k = zTr_k / zTr_{k-1},
 // p_k = z_k + beta p_{k-1} ; Ap_k = A p_k ; partial pAp
-template <int G, class VT>
+template <int G, bool FLEX, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64_t k) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
-    double tot[2];
-    sum_partials<2>(P.part_b, P.nb_b, tot);
-    const double zTr = tot[0], rr = tot[1];
+    double zTr, rr, zTr_cross = 0.0;  // z_k.r_k, |r_k|^2, z_k.r_{k-1} (flexible variant)
+    if (FLEX) {
+        double tot[3];
+        sum_partials<3>(P.part_b, P.nb_b, tot);
+        zTr = tot[0];
+        zTr_cross = tot[1];
+        rr = tot[2];
+    } else {
+        double tot[2];
+        sum_partials<2>(P.part_b, P.nb_b, tot);
+        zTr = tot[0];
+        rr = tot[1];
+    }
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     int decided = CG_RUNNING;
     if (sc->bnorm == 0.0)
@@ -328,7 +338,9 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64
         }
         return;
     }
-    const double beta = k == 0 ? 0.0 : zTr / sc->zrec[(k - 1) & 1];
+    // Fletcher-Reeves form for a fixed preconditioner; Polak-Ribiere z_k.(r_k - r_{k-1}) when
+    // the preconditioner depends on its input (Notay, SIAM J. Sci. Comput. 22 (2000) 1444)
+    const double beta = k == 0 ? 0.0 : (FLEX ? (zTr - zTr_cross) : zTr) / sc->zrec[(k - 1) & 1];
     if (lead) {
         sc->zrec[k & 1] = zTr;
         sc->zTr = zTr;
@@ -376,7 +388,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64
 
 // B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p_k ;
 // r_{k+1} = r_k - alpha Ap_k ; z_{k+1} = P r_{k+1} ; partial zTr, |r|^2
-template <int G, bool JAC, class VT>
+template <int G, bool JAC, bool FLEX, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P, int64_t k) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
@@ -457,7 +469,116 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P, int6
                 finish(v, r0, r1, r2, double(d[0] * VT(r0)), double(d[1] * VT(r1)), double(d[2] * VT(r2)));
             });
     }
-    block_partials<2>(part, P.part_b);
+    if (!FLEX) block_partials<2>(part, P.part_b);
+}
+
+// Learned coarse correction, one work-group per aggregate: g = Phi_a^T r_new,
+// c = kappa |g| MLP(bf16(g / |g|)), z += Phi_a c on the aggregate's DOFs (z holds P r_new), then the
+// partial sums z.r_new, z.r_old and |r_new|^2 of the flexible iteration. Every DOF belongs to
+// exactly one aggregate, so the work-groups' partials add up to the full dot products.
+__global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* __restrict__ ra_new,
+                                                  const double* __restrict__ ra_old, double* __restrict__ zp,
+                                                  double* __restrict__ partials) {
+    __shared__ uint16_t s_w[LC_PARAMS - LC_HID - LC_HID - LC_OUT];  // W1 | W2 | W3 (BF16)
+    __shared__ double s_r[LC_MAX_AGG_NODES * 3];
+    __shared__ float s_g[LC_IN];
+    __shared__ float s_h1[LC_HID];
+    __shared__ float s_h2[LC_HID];
+    __shared__ float s_c[LC_OUT];
+    __shared__ double s_nrm;
+    const int a = blockIdx.x;
+    const int o = lv.agg_ptr[a], nd = lv.agg_ptr[a + 1] - o;
+    const int t = threadIdx.x;
+    // stage the weights (coalesced); skip the FP32 bias slots between the weight matrices
+    const uint32_t* wsrc1 = reinterpret_cast<const uint32_t*>(lv.wbf + LC_W1);
+    const uint32_t* wsrc2 = reinterpret_cast<const uint32_t*>(lv.wbf + LC_W2);
+    const uint32_t* wsrc3 = reinterpret_cast<const uint32_t*>(lv.wbf + LC_W3);
+    uint32_t* wdst = reinterpret_cast<uint32_t*>(s_w);
+    constexpr int n1 = LC_HID * LC_IN / 2, n2 = LC_HID * LC_HID / 2, n3 = LC_OUT * LC_HID / 2;
+    for (int i = t; i < n1; i += 256) wdst[i] = wsrc1[i];
+    for (int i = t; i < n2; i += 256) wdst[n1 + i] = wsrc2[i];
+    for (int i = t; i < n3; i += 256) wdst[n1 + n2 + i] = wsrc3[i];
+    for (int d = t; d < nd; d += 256) {
+        const int dof = lv.agg_dofs[o + d];
+        s_r[d] = ra_new[6 * (dof / 3) + dof % 3];
+    }
+    __syncthreads();
+    const float* B = lv.basis + (int64_t)LC_IN * o;
+    {
+        const int j = t / 4;
+        double c = 0.0;
+        for (int d = t % 4; d < nd; d += 4) c += (double)B[j * nd + d] * s_r[d];
+        c += __shfl_xor(c, 1, 4);
+        c += __shfl_xor(c, 2, 4);
+        if (t % 4 == 0) s_g[j] = (float)c;
+    }
+    __syncthreads();
+    if (t < 64) {
+        double q = (double)s_g[t] * (double)s_g[t];
+        for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+        if (t == 0) s_nrm = sqrt(q);
+    }
+    __syncthreads();
+    const double nrm = s_nrm;
+    const float inv = nrm > 0.0 ? (float)(1.0 / nrm) : 0.0f;
+    if (t < LC_IN) s_g[t] = bf16_round(s_g[t] * inv);
+    __syncthreads();
+    if (t < LC_HID) {
+        float acc = 0.0f;
+        for (int i = 0; i < LC_IN; ++i) acc += bf16_float(s_w[t * LC_IN + i]) * s_g[i];
+        s_h1[t] = bf16_round(gelu(acc + lv.params[LC_B1 + t]));
+    }
+    __syncthreads();
+    if (t < LC_HID) {
+        float acc = 0.0f;
+        for (int i = 0; i < LC_HID; ++i) acc += bf16_float(s_w[n1 * 2 + t * LC_HID + i]) * s_h1[i];
+        s_h2[t] = bf16_round(gelu(acc + lv.params[LC_B2 + t]));
+    }
+    __syncthreads();
+    if (t < LC_OUT) {
+        float acc = 0.0f;
+        for (int i = 0; i < LC_HID; ++i) acc += bf16_float(s_w[(n1 + n2) * 2 + t * LC_HID + i]) * s_h2[i];
+        s_c[t] = (acc + lv.params[LC_B3 + t]) * (float)nrm * lv.kappa;
+    }
+    __syncthreads();
+    // safeguard: the correction may lower the aggregate's share of z.r to at most half of the
+    // Schwarz share (and may not lower a non-positive share), so z.r stays positive in practice;
+    // the flexible solve still reports an indefinite preconditioner if it is not
+    __shared__ double s_corr[LC_MAX_AGG_NODES * 3];
+    __shared__ double s_red2[2][4];
+    double zs_r = 0.0, c_r = 0.0;
+    for (int d = t; d < nd; d += 256) {
+        double corr = 0.0;
+        for (int j = 0; j < LC_IN; ++j) corr += (double)B[j * nd + d] * (double)s_c[j];
+        s_corr[d] = corr;
+        const int dof = lv.agg_dofs[o + d];
+        zs_r += zp[6 * (int64_t)(dof / 3) + dof % 3] * s_r[d];
+        c_r += corr * s_r[d];
+    }
+    zs_r = wave_sum(zs_r);
+    c_r = wave_sum(c_r);
+    if ((t & 63) == 0) {
+        s_red2[0][t >> 6] = zs_r;
+        s_red2[1][t >> 6] = c_r;
+    }
+    __syncthreads();
+    const double sz = s_red2[0][0] + s_red2[0][1] + s_red2[0][2] + s_red2[0][3];
+    const double sc_ = s_red2[1][0] + s_red2[1][1] + s_red2[1][2] + s_red2[1][3];
+    double omega = 1.0;
+    if (sc_ < 0.0) omega = sz > 0.0 ? fmin(1.0, 0.5 * sz / -sc_) : 0.0;
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int d = t; d < nd; d += 256) {
+        const int dof = lv.agg_dofs[o + d];
+        const int64_t zi = 6 * (int64_t)(dof / 3) + dof % 3;
+        const double z = zp[zi] + omega * s_corr[d];
+        zp[zi] = z;
+        const double rn = s_r[d];
+        const double ro = ra_old ? ra_old[zi] : 0.0;
+        part[0] += z * rn;
+        part[1] += z * ro;
+        part[2] += rn * rn;
+    }
+    block_partials<3>(part, partials);
 }
 
 __global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
@@ -505,7 +626,7 @@ void CgWork::ensure(int64_t nn_dof, int nblocks) {
     if (!part_pre.p) {
         part_pre.alloc(CG_MAX_BLOCKS);
         part_a.alloc(CG_MAX_BLOCKS);
-        part_b.alloc(2 * CG_MAX_BLOCKS);
+        part_b.alloc(3 * CG_MAX_BLOCKS);
     }
     if ((int64_t)partials.n < (int64_t)nblocks) partials.alloc((size_t)nblocks);
     if (!scalars.p) {
@@ -529,9 +650,19 @@ static Csr3<VT> view(const NodePattern& p, const VT* vals) {
                     p.d_entries.get(), p.n_block, p.n_diag};
 }
 
-template <int GA, int GB, bool JAC, class VT>
+// xs = x, rs = r (from the interleaved [r | Ap] buffer)
+__global__ void k_cg_snapshot(int64_t n, const double* __restrict__ x, const double* __restrict__ ra,
+                              double* __restrict__ xs, double* __restrict__ rs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    xs[i] = x[i];
+    rs[i] = ra[6 * (i / 3) + i % 3];
+}
+
+template <int GA, int GB, bool JAC, bool FLEX, class VT>
 static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>& Pm, const double* b, double* x,
-                          double tol, int64_t max_iter, int64_t* iterations, hoefem_status* out_status) {
+                          double tol, int64_t max_iter, int64_t* iterations, hoefem_status* out_status,
+                          CgHarvest* h, const LearnedView* lv) {
     const hipStream_t s = m.stream;
     const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA);
     const unsigned blkB = JAC ? cg_grid(0, Pm.nn, GB) : cg_grid(Pm.n_block, Pm.n_diag, GB);
@@ -543,7 +674,7 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     P.part_a = w.part_a.get();
     P.part_b = w.part_b.get();
     P.nb_a = (int)blkA;
-    P.nb_b = (int)blkB;
+    P.nb_b = FLEX ? lv->na : (int)blkB;
     P.sc = w.scalars.get();
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
     HIP_CHECK(hipMemsetAsync(w.zp[0].get(), 0, sizeof(double) * 2 * w.n, s));  // p_{-1} = 0
@@ -551,8 +682,12 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     k_cg_pre1<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
     P.ra_old = w.ra[0].get();
     P.zp_new = w.zp[0].get();
-    k_cg_pre2<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+    k_cg_pre2<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+    if (FLEX)
+        k_lc_apply<<<dim3((unsigned)lv->na), dim3(256), 0, s>>>(*lv, w.ra[0].get(), nullptr, w.zp[0].get(),
+                                                               w.part_b.get());
     check_launch("cg preamble");
+    std::vector<int64_t> snap_k;  // iteration index of every snapshot taken
     int cur = 0;
     int64_t k_next = 0;  // iteration index of the next A/B pair
     size_t nev = 0;      // events used by this solve (profiling only)
@@ -571,15 +706,25 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
                     HIP_CHECK(hipEventCreate(&e));
                     m.prof_ev.push_back(e);
                 }
-                hipExtLaunchKernelGGL(k_cg_a<GA, VT>, dim3(blkA), dim3(CG_THREADS), 0, s, m.prof_ev[nev],
+                hipExtLaunchKernelGGL(k_cg_a<GA, FLEX, VT>, dim3(blkA), dim3(CG_THREADS), 0, s, m.prof_ev[nev],
                                       m.prof_ev[nev + 1], 0, A, P, k_next);
                 nev += 2;
             } else {
-                k_cg_a<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P, k_next);
+                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P, k_next);
             }
-            k_cg_b<GB, JAC, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P, k_next);
+            k_cg_b<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P, k_next);
+            if (FLEX)
+                k_lc_apply<<<dim3((unsigned)lv->na), dim3(256), 0, s>>>(*lv, w.ra[1 - cur].get(), w.ra[cur].get(),
+                                                                       w.zp[1 - cur].get(), w.part_b.get());
             cur ^= 1;
             ++k_next;
+            if (h && h->taken + (int)snap_k.size() < h->max && k_next % h->every == 0) {
+                const int q = (int)snap_k.size();
+                const int64_t nd = w.n;
+                k_cg_snapshot<<<dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, s>>>(
+                    nd, x, w.ra[cur].get(), h->xs + (int64_t)q * nd, h->rs + (int64_t)q * nd);
+                snap_k.push_back(k_next);
+            }
         }
         check_launch("cg iteration");
     };
@@ -606,6 +751,12 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     HIP_CHECK(hipMemcpyAsync(hs, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     *iterations = hs->iters;
+    if (h) {  // keep the snapshots of iterations the solve actually reached
+        int keep = 0;
+        for (int64_t kk : snap_k)
+            if (kk < hs->iters) ++keep;
+        h->taken = keep;
+    }
     if (m.prof) {
         // launches [0, iters) did a full iteration; the rest found the solve finished and returned
         for (size_t k = 0; k + 1 < nev; k += 2) {
@@ -632,7 +783,8 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
 
 template <class VT>
 static hoefem_status pcg_solve_t(Model& m, CgWork& w, const VT* Avals, const VT* Pvals, Schwarz* Pre, const double* b,
-                                 double* x, double tol, int64_t max_iter, int64_t* iterations) {
+                                 double* x, double tol, int64_t max_iter, int64_t* iterations, CgHarvest* h = nullptr,
+                                 const LearnedView* lv = nullptr) {
     const NodePattern& pat = m.pattern;
     const int64_t nn = pat.num_nodes;
     const int GA = spmv_group_size(pat);
@@ -649,15 +801,22 @@ static hoefem_status pcg_solve_t(Model& m, CgWork& w, const VT* Avals, const VT*
     }
     hoefem_status st = HOEFEM_OK;
     const bool JAC = (Pre == nullptr);
-#define CG_CASE(ga, gb, j)                                                                               \
-    if (GA == ga && GB == gb && JAC == j) {                                                              \
-        cg_launch_all<ga, gb, j, VT>(m, w, A, Pm, b, x, tol, max_iter, iterations, &st);                   \
+    const bool FLEX = (lv != nullptr);
+    if (FLEX && JAC) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "the learned correction needs the Schwarz preconditioner");
+    if (FLEX && (lv->na < 1 || lv->na > CG_MAX_BLOCKS))
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "number of aggregates out of range");
+#define CG_CASE(ga, gb, j, f)                                                                            \
+    if (GA == ga && GB == gb && JAC == j && FLEX == f) {                                                 \
+        cg_launch_all<ga, gb, j, f, VT>(m, w, A, Pm, b, x, tol, max_iter, iterations, &st, h, lv);        \
         return st;                                                                                       \
     }
-    CG_CASE(8, 8, true) CG_CASE(16, 8, true) CG_CASE(32, 8, true)
-    CG_CASE(8, 8, false) CG_CASE(8, 16, false) CG_CASE(8, 32, false)
-    CG_CASE(16, 8, false) CG_CASE(16, 16, false) CG_CASE(16, 32, false)
-    CG_CASE(32, 8, false) CG_CASE(32, 16, false) CG_CASE(32, 32, false)
+    CG_CASE(8, 8, true, false) CG_CASE(16, 8, true, false) CG_CASE(32, 8, true, false)
+    CG_CASE(8, 8, false, false) CG_CASE(8, 16, false, false) CG_CASE(8, 32, false, false)
+    CG_CASE(16, 8, false, false) CG_CASE(16, 16, false, false) CG_CASE(16, 32, false, false)
+    CG_CASE(32, 8, false, false) CG_CASE(32, 16, false, false) CG_CASE(32, 32, false, false)
+    CG_CASE(8, 8, false, true) CG_CASE(8, 16, false, true) CG_CASE(8, 32, false, true)
+    CG_CASE(16, 8, false, true) CG_CASE(16, 16, false, true) CG_CASE(16, 32, false, true)
+    CG_CASE(32, 8, false, true) CG_CASE(32, 16, false, true) CG_CASE(32, 32, false, true)
 #undef CG_CASE
     throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unsupported CG configuration");
 }
@@ -672,6 +831,30 @@ hoefem_status pcg_solve(Model& m, CgWork& w, const float* A, Schwarz* P, const d
                         int64_t max_iter, int64_t* iterations) {
     if (P && !P->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "FP64 preconditioner with an FP32 operator");
     return pcg_solve_t<float>(m, w, A, P ? P->d_vals32.get() : nullptr, P, b, x, tol, max_iter, iterations);
+}
+
+hoefem_status pcg_solve_harvest(Model& m, CgWork& w, const double* A, Schwarz* P, const double* b, double* x,
+                                double tol, int64_t max_iter, int64_t* iterations, CgHarvest* h) {
+    if (P && P->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "FP32 preconditioner with an FP64 operator");
+    return pcg_solve_t<double>(m, w, A, P ? P->d_vals.get() : nullptr, P, b, x, tol, max_iter, iterations, h);
+}
+
+hoefem_status pcg_solve_harvest(Model& m, CgWork& w, const float* A, Schwarz* P, const double* b, double* x,
+                                double tol, int64_t max_iter, int64_t* iterations, CgHarvest* h) {
+    if (P && !P->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "FP64 preconditioner with an FP32 operator");
+    return pcg_solve_t<float>(m, w, A, P ? P->d_vals32.get() : nullptr, P, b, x, tol, max_iter, iterations, h);
+}
+
+hoefem_status pcg_solve_learned(Model& m, CgWork& w, const double* A, Schwarz* P, const LearnedView& lv,
+                                const double* b, double* x, double tol, int64_t max_iter, int64_t* iterations) {
+    if (!P || P->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "learned solve needs the FP64 Schwarz operator");
+    return pcg_solve_t<double>(m, w, A, P->d_vals.get(), P, b, x, tol, max_iter, iterations, nullptr, &lv);
+}
+
+hoefem_status pcg_solve_learned(Model& m, CgWork& w, const float* A, Schwarz* P, const LearnedView& lv,
+                                const double* b, double* x, double tol, int64_t max_iter, int64_t* iterations) {
+    if (!P || !P->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "learned solve needs the FP32 Schwarz operator");
+    return pcg_solve_t<float>(m, w, A, P->d_vals32.get(), P, b, x, tol, max_iter, iterations, nullptr, &lv);
 }
 
 // ================================================================== Newton / BE kernels

@@ -21,11 +21,11 @@ from ._lib import check
 from .scenes import Scene, lame_from_young_poisson
 
 
-def _dev(t: torch.Tensor) -> int:
+def _dev(t: torch.Tensor, dtype=torch.float64) -> int:
     if not t.is_cuda:
         raise ValueError("expected a device (cuda/hip) tensor")
-    if t.dtype != torch.float64:
-        raise ValueError("expected float64")
+    if t.dtype != dtype:
+        raise ValueError(f"expected {dtype}")
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return t.data_ptr()
@@ -271,3 +271,73 @@ class NewtonCgIntegrator:
         st = _lib.StepStats()
         check(_lib.load().hoefem_sim_step(self.h, t, dt, _dev(u), _dev(v), C.byref(st)))
         return {k: getattr(st, k) for k, _ in _lib.StepStats._fields_}
+
+
+class LearnedCoarseCorrection:
+    """Learned correction on top of the Schwarz preconditioner of a NewtonCgIntegrator
+    (no counterpart in the reference; hoefem_learned_* in include/hoefem.h).
+
+    Workflow: ``harvest(True)``, run plain steps (their CG solves store error / residual
+    snapshots), ``train(...)``, ``enable(True)``; later solves use flexible PCG with the
+    correction (under the plain path's rules: the CG iteration limit is tolerated and the Newton
+    tolerance is unchanged) and fall back to the plain Schwarz-CG on a breakdown.
+    The correction holds a reference to the integrator, which must outlive it.
+    """
+
+    PARAMS = None
+
+    def __init__(self, integrator: "NewtonCgIntegrator", num_aggregates: int = 512, seed: int = 0):
+        L = _lib.load()
+        d = _lib.LearnedDesc()
+        d.num_aggregates, d.seed = num_aggregates, seed
+        h = C.c_void_p()
+        check(L.hoefem_learned_create(integrator.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.integrator = integrator
+        LearnedCoarseCorrection.PARAMS = int(L.hoefem_learned_param_count())
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib._lib.hoefem_learned_destroy(self.h)
+            self.h = None
+
+    def harvest(self, enable: bool = True, capacity: int = 256, every: int = 32):
+        check(_lib.load().hoefem_learned_harvest(self.h, 1 if enable else 0, capacity, every))
+
+    def info(self) -> dict:
+        s = _lib.LearnedStats()
+        check(_lib.load().hoefem_learned_info(self.h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in s._fields_}
+
+    def train(self, steps: int = 300, learning_rate: float = 1e-3, batch: int = 8):
+        a, b = C.c_double(), C.c_double()
+        check(_lib.load().hoefem_learned_train(self.h, steps, learning_rate, batch, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def enable(self, on: bool = True):
+        check(_lib.load().hoefem_learned_enable(self.h, 1 if on else 0))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.PARAMS, dtype=np.float32)
+        check(_lib.load().hoefem_learned_get_params(self.h, p.ctypes.data))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.PARAMS
+        check(_lib.load().hoefem_learned_set_params(self.h, p.ctypes.data))
+
+    def mlp_forward(self, x: torch.Tensor) -> torch.Tensor:
+        x = x.contiguous()
+        assert x.dtype == torch.float32 and x.dim() == 2 and x.shape[1] == 64 and x.is_cuda
+        y = torch.empty_like(x)
+        check(_lib.load().hoefem_learned_mlp_forward(self.h, x.shape[0], _dev(x, torch.float32), _dev(y, torch.float32)))
+        return y
+
+    def mlp_grad(self, x: torch.Tensor, dy: torch.Tensor) -> torch.Tensor:
+        x, dy = x.contiguous(), dy.contiguous()
+        assert x.dtype == torch.float32 and dy.dtype == torch.float32 and x.shape == dy.shape
+        g = torch.zeros(self.PARAMS, dtype=torch.float32, device=x.device)
+        check(_lib.load().hoefem_learned_mlp_grad(self.h, x.shape[0], _dev(x, torch.float32), _dev(dy, torch.float32),
+                                                   _dev(g, torch.float32)))
+        return g

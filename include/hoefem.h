@@ -211,6 +211,61 @@ hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const do
 /* one time step: u, v device vectors updated in place */
 hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, double* v, hoefem_step_stats* stats);
 
+/* ------------------------------------------------------------------ learned coarse correction
+ * No counterpart in the reference: an extra correction on top of its Schwarz preconditioner
+ * (SystemPreconditioner::apply, simulation_toolbox/src/fem/newton_cg.rs:397-401), used by the
+ * simulation's linear solves when enabled.
+ *
+ * The nodes are split into aggregates; each aggregate carries 64 orthonormal modes taken from
+ * error snapshots x* - x_k of the plain Schwarz-CG's own solves. A network shared by all
+ * aggregates (64 -> 128 -> 128 -> 64, GELU, BF16 weights, FP32 accumulation) maps the
+ * normalized projection of the residual on an aggregate's modes to correction coefficients:
+ *     z = P r + sum_a Phi_a |g_a| MLP(g_a / |g_a|),   g_a = Phi_a^T r.
+ * Training minimizes -(r^T z)^2 / (z^T J z) (the energy left after the best step along z) over
+ * harvested residuals (Adam). A learned solve is a flexible (Polak-Ribiere) PCG under the plain
+ * path's rules (the CG iteration limit is tolerated; the Newton residual tolerance is
+ * unchanged); on a breakdown the plain Schwarz-CG solves the same system from zero.
+ *
+ * Lifetime: the correction refers to its simulation; destroying the simulation first detaches
+ * it (training then fails with HOEFEM_ERROR_INVALID_ARGUMENT). */
+typedef struct hoefem_learned hoefem_learned;
+
+typedef struct {
+    int32_t num_aggregates; /* target count (greedy growth may end a few above it) */
+    uint64_t seed;          /* weight initialization */
+} hoefem_learned_desc;
+
+typedef struct {
+    int32_t num_aggregates, max_aggregate_dofs;
+    int64_t harvested_pairs; /* (residual, error) snapshot pairs taken so far */
+    int64_t stored_pairs;    /* pairs held in the ring */
+    int32_t trained, enabled;
+    int64_t learned_solves, fallbacks, learned_iterations;
+    int32_t last_status; /* of the last learned solve: HOEFEM_OK or the CG status that caused the fallback */
+} hoefem_learned_stats;
+
+/* number of network parameters (FP32 master copy): W1[128][64] b1[128] W2[128][128] b2[128] W3[64][128] b3[64] */
+int64_t hoefem_learned_param_count(void);
+hoefem_status hoefem_learned_create(hoefem_sim* sim, const hoefem_learned_desc* desc, hoefem_learned** out);
+hoefem_status hoefem_learned_destroy(hoefem_learned* l);
+/* while enabled, every plain Schwarz-CG solve of the simulation keeps (x_k, r_k) every `every`
+ * iterations and stores (x* - x_k, r_k) in a ring of `capacity` (>= 64) pairs */
+hoefem_status hoefem_learned_harvest(hoefem_learned* l, int32_t enable, int32_t capacity, int32_t every);
+hoefem_status hoefem_learned_info(hoefem_learned* l, hoefem_learned_stats* out);
+/* builds the aggregate modes from the stored error snapshots (first call) and runs `steps` Adam
+ * steps on batches of `batch` stored residuals against the simulation's current Jacobian and
+ * Schwarz operator; losses are averaged over the batch (initial: before the first step) */
+hoefem_status hoefem_learned_train(hoefem_learned* l, int32_t steps, double learning_rate, int32_t batch,
+                                   double* initial_loss, double* final_loss);
+hoefem_status hoefem_learned_enable(hoefem_learned* l, int32_t enable);
+/* host arrays of hoefem_learned_param_count() floats */
+hoefem_status hoefem_learned_get_params(hoefem_learned* l, float* params);
+hoefem_status hoefem_learned_set_params(hoefem_learned* l, const float* params);
+/* network on n device samples: y[n][64] = MLP(x[n][64]); grad (device, param_count floats) =
+ * d(sum_s dy[s] . MLP(x[s])) / d params */
+hoefem_status hoefem_learned_mlp_forward(hoefem_learned* l, int64_t n, const float* x, float* y);
+hoefem_status hoefem_learned_mlp_grad(hoefem_learned* l, int64_t n, const float* x, const float* dy, float* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,6 +138,9 @@ def lib():
         L.orc_schwarz_update.argtypes = [C.c_void_p, i32p, i32p, f64p]
         L.orc_schwarz_values.argtypes = [C.c_void_p, f64p]
         L.orc_invert_block.argtypes = [C.c_int, f64p, f64p]
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        L.orc_mlp_forward.argtypes = [f32p, C.c_int64, f32p, C.c_int, f64p]
+        L.orc_mlp_grad.argtypes = [f32p, C.c_int64, f32p, f32p, C.c_int, f64p]
         L.orc_symmetric_eigen.argtypes = [C.c_int, f64p, f64p, f64p]
         L.orc_pcg_csr.argtypes = [C.c_int64, i32p, i32p, f64p, C.c_void_p, C.c_void_p, C.c_void_p, f64p, f64p,
                                   C.c_double, C.c_int64, C.POINTER(C.c_int64)]
@@ -424,6 +427,29 @@ def invert_block(B):
     out = np.empty_like(B)
     check(lib().orc_invert_block(B.shape[0], B.ravel(), out.ravel()))
     return out.T.copy()
+
+
+MLP_PARAMS = 128 * 64 + 128 + 128 * 128 + 128 + 64 * 128 + 64
+
+
+def mlp_forward(params, x, emulate_bf16=True):
+    """The learned correction's network on samples x[n][64] (FP64 arithmetic; with
+    emulate_bf16, BF16 rounding of inputs, weights and hidden activations as on the GPU)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((x.shape[0], 64))
+    check(lib().orc_mlp_forward(p, x.shape[0], x.ravel(), 1 if emulate_bf16 else 0, y.ravel()))
+    return y
+
+
+def mlp_grad(params, x, dy, emulate_bf16=True):
+    """d(sum_s dy[s] . y[s]) / d params."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    dy = np.ascontiguousarray(dy, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS)
+    check(lib().orc_mlp_grad(p, x.shape[0], x.ravel(), dy.ravel(), 1 if emulate_bf16 else 0, g))
+    return g
 
 
 def pcg(rp, col, vals, b, x0=None, prec=None, tol=1e-5, max_iter=500):

@@ -1767,3 +1767,107 @@ int orc_sim_step(orc_sim* s, double t, double dt, double* u, double* v, orc_step
 }
 
 }  // extern "C"
+
+// ====================================================================== learned correction network
+namespace {
+constexpr int MI = 64, MH = 128, MO = 64;
+constexpr int P_W1 = 0, P_B1 = P_W1 + MH * MI, P_W2 = P_B1 + MH, P_B2 = P_W2 + MH * MH, P_W3 = P_B2 + MH,
+              P_B3 = P_W3 + MO * MH, P_N = P_B3 + MO;
+
+float bf16r(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u = ((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16) << 16;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+double gelu_d(double x) { return 0.5 * x * (1.0 + std::erf(x / std::sqrt(2.0))); }
+double gelu_grad_d(double x) {
+    return 0.5 * (1.0 + std::erf(x / std::sqrt(2.0))) + x * std::exp(-0.5 * x * x) / std::sqrt(2.0 * M_PI);
+}
+
+struct MlpTape {
+    std::vector<double> x, z1, h1, z2, h2, y;
+};
+
+void mlp_fwd_one(const float* p, const float* xin, bool em, MlpTape& t) {
+    auto w = [&](int i) { return em ? (double)bf16r(p[i]) : (double)p[i]; };
+    t.x.resize(MI);
+    t.z1.resize(MH);
+    t.h1.resize(MH);
+    t.z2.resize(MH);
+    t.h2.resize(MH);
+    t.y.resize(MO);
+    for (int i = 0; i < MI; ++i) t.x[i] = em ? (double)bf16r(xin[i]) : (double)xin[i];
+    for (int o = 0; o < MH; ++o) {
+        double a = 0.0;
+        for (int i = 0; i < MI; ++i) a += w(P_W1 + o * MI + i) * t.x[i];
+        a += p[P_B1 + o];
+        t.z1[o] = a;
+        t.h1[o] = em ? (double)bf16r((float)gelu_d(a)) : gelu_d(a);
+    }
+    for (int o = 0; o < MH; ++o) {
+        double a = 0.0;
+        for (int i = 0; i < MH; ++i) a += w(P_W2 + o * MH + i) * t.h1[i];
+        a += p[P_B2 + o];
+        t.z2[o] = a;
+        t.h2[o] = em ? (double)bf16r((float)gelu_d(a)) : gelu_d(a);
+    }
+    for (int o = 0; o < MO; ++o) {
+        double a = 0.0;
+        for (int i = 0; i < MH; ++i) a += w(P_W3 + o * MH + i) * t.h2[i];
+        t.y[o] = a + p[P_B3 + o];
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int orc_mlp_forward(const float* params, int64_t n, const float* x, int emulate_bf16, double* y) {
+    if (!params || (n > 0 && (!x || !y)) || n < 0) return fail(ORC_ERR_INVALID, "bad network arguments");
+    MlpTape t;
+    for (int64_t s = 0; s < n; ++s) {
+        mlp_fwd_one(params, x + s * MI, emulate_bf16 != 0, t);
+        std::copy(t.y.begin(), t.y.end(), y + s * MO);
+    }
+    return ORC_OK;
+}
+
+int orc_mlp_grad(const float* params, int64_t n, const float* x, const float* dy, int emulate_bf16, double* grad) {
+    if (!params || n < 1 || !x || !dy || !grad) return fail(ORC_ERR_INVALID, "bad network arguments");
+    const bool em = emulate_bf16 != 0;
+    auto w = [&](int i) { return em ? (double)bf16r(params[i]) : (double)params[i]; };
+    std::fill(grad, grad + P_N, 0.0);
+    MlpTape t;
+    std::vector<double> d3(MO), d2(MH), d1(MH);
+    for (int64_t s = 0; s < n; ++s) {
+        mlp_fwd_one(params, x + s * MI, em, t);
+        for (int o = 0; o < MO; ++o) d3[o] = dy[s * MO + o];
+        for (int o = 0; o < MO; ++o) {
+            for (int i = 0; i < MH; ++i) grad[P_W3 + o * MH + i] += d3[o] * t.h2[i];
+            grad[P_B3 + o] += d3[o];
+        }
+        for (int i = 0; i < MH; ++i) {
+            double a = 0.0;
+            for (int o = 0; o < MO; ++o) a += w(P_W3 + o * MH + i) * d3[o];
+            d2[i] = a * gelu_grad_d(t.z2[i]);
+        }
+        for (int o = 0; o < MH; ++o) {
+            for (int i = 0; i < MH; ++i) grad[P_W2 + o * MH + i] += d2[o] * t.h1[i];
+            grad[P_B2 + o] += d2[o];
+        }
+        for (int i = 0; i < MH; ++i) {
+            double a = 0.0;
+            for (int o = 0; o < MH; ++o) a += w(P_W2 + o * MH + i) * d2[o];
+            d1[i] = a * gelu_grad_d(t.z1[i]);
+        }
+        for (int o = 0; o < MH; ++o) {
+            for (int i = 0; i < MI; ++i) grad[P_W1 + o * MI + i] += d1[o] * t.x[i];
+            grad[P_B1 + o] += d1[o];
+        }
+    }
+    return ORC_OK;
+}
+
+}  // extern "C"

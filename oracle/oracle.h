@@ -192,6 +192,16 @@ int orc_sim_jacobian_values(const orc_sim* s, double* values);
 int orc_sim_residual(orc_sim* s, double t, double dt, const double* u, const double* v, const double* w,
                      double* F);
 
+/* The per-aggregate network of the learned coarse correction (no reference counterpart):
+ * 64 -> 128 -> 128 -> 64, GELU (erf form) after the two hidden layers. Parameter layout
+ * W1[128][64] b1[128] W2[128][128] b2[128] W3[64][128] b3[64] (33088 floats). FP64 arithmetic;
+ * with emulate_bf16 the inputs, weights and hidden activations are rounded to BF16
+ * (round-to-nearest-even) before they enter a product, as the GPU kernels do. */
+int orc_mlp_forward(const float* params, int64_t n, const float* x, int emulate_bf16, double* y);
+/* grad = d(sum_s dy[s] . y[s]) / d params; with emulate_bf16 the derivative treats the BF16
+ * rounding of the weights as the identity (gradients land on the FP32 master copy) */
+int orc_mlp_grad(const float* params, int64_t n, const float* x, const float* dy, int emulate_bf16, double* grad);
+
 #ifdef __cplusplus
 }
 #endif
