@@ -267,9 +267,7 @@ struct LearnedView {  // what the CG apply kernel reads
     const int32_t* agg_ptr;   // [na + 1] offsets into agg_dofs (3 per node)
     const int32_t* agg_dofs;  // DOFs grouped by aggregate
     const float* basis;       // aggregate a: 64 modes x n_a DOFs at 64 * agg_ptr[a]
-    const uint16_t* wbf;      // BF16 weights
-    const float* params;      // FP32 master parameters (biases)
-    float kappa;              // output scale: c_a = kappa |g_a| MLP(g_a / |g_a|)
+    const float* gain;        // [na][64] mode gains kappa * MLP(f_a)
 };
 
 struct Learned {
@@ -281,6 +279,7 @@ struct Learned {
     DevBuf<float> d_basis;
     bool has_basis = false, trained = false, enabled = false, harvesting = false;
     float kappa = 0.0f;  // mean |P r| / |r| over the first training batch
+    DevBuf<float> d_feat, d_gain;  // [na][64] profile features and mode gains
     // network
     DevBuf<float> d_params, d_m, d_v, d_grad, d_gpart;
     DevBuf<uint16_t> d_wbf;
@@ -308,6 +307,7 @@ double learned_train(Learned& L, const float* J, Schwarz* P, int steps, double l
 void learned_mlp_forward(Learned& L, int64_t N, const float* X, float* Y);
 void learned_mlp_grad(Learned& L, int64_t N, const float* X, const float* dY, float* grad, float scale);
 LearnedView learned_view(const Learned& L);
+void learned_refresh_gains(Learned& L);
 
 // Snapshot hook of the plain CG: every `every` iterations x_k and r_k are copied into
 // xs / rs (at most max snapshots); *taken receives the count.

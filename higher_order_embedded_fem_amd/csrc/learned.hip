@@ -3,23 +3,26 @@
 // The nodes are split into aggregates (greedy breadth-first growth over the node graph, node
 // order seeds). Each aggregate a carries 64 orthonormal modes Phi_a: the restrictions to its
 // DOFs of 64 error snapshots x* - x_k harvested from the plain Schwarz-CG solver's own solves
-// (orthonormalized per aggregate by Gram-Schmidt, Dirichlet DOFs zeroed). One network shared by
-// all aggregates maps the normalized residual projection g_a / |g_a| (g_a = Phi_a^T r) to
-// coefficients y_a; the correction is
-//     z = P r + sum_a Phi_a c_a,   c_a = kappa |g_a| y_a,
-// positively homogeneous in r, as a preconditioner must be for CG; kappa = mean |P r| / |r|
-// over the first training batch keeps the network's output O(1). The network is trained on
-// harvested residuals r (unit norm) by minimizing
+// (orthonormalized per aggregate by Gram-Schmidt, Dirichlet DOFs zeroed). The correction is
+// linear, symmetric and positive semidefinite in the residual,
+//     z = P r + sum_a Phi_a diag(gamma_a) Phi_a^T r,     gamma_a = kappa softplus(MLP(f_a)),
+// so the preconditioner stays symmetric positive definite and CG keeps its Krylov optimality
+// (a correction that depends nonlinearly on r, or negative gains, lose that).
+// One network shared by all aggregates maps the aggregate's residual projection profile
+// f_a (64 values: the RMS of Phi_a^T r over harvested residuals, normalized) to its 64 mode
+// gains; kappa = mean |P r| / |r| over the first training batch keeps the output O(1).
+// Training minimizes, over harvested residuals r (unit norm),
 //     L(z) = -(r^T z)^2 / (z^T J z)  = min over s of |J^-1 r - s z|_J^2 - |J^-1 r|_J^2,
 // the energy left after the best step along z: the quality of z as a CG search direction,
-// independent of the scale of z (to which CG is blind). The gradient reaches the network
-// through dL/dz = (2 a / q) ((a / q) J z - r), a = r^T z, q = z^T J z, and Phi_a^T. The last
-// layer starts at zero: the untrained correction is exactly the plain Schwarz preconditioner.
+// independent of the scale of z (to which CG is blind). The gradient reaches the gains through
+// dL/dz = (2 a / q) ((a / q) J z - r), a = r^T z, q = z^T J z, and Phi_a^T. The last layer
+// starts with zero weights and bias -4: the untrained gains are softplus(-4) kappa = 0.018 kappa,
+// a small perturbation of the plain Schwarz preconditioner with gradients that still flow.
 //
-// Kernels: per-aggregate Gram-Schmidt basis, projection and prolongation, batched network
-// forward / backward (FP32 accumulation over BF16-rounded weights and activations),
-// deterministic gradient reduction, Adam. The fused per-iteration apply kernel lives with the
-// CG kernels.
+// Kernels: per-aggregate Gram-Schmidt basis, projection and prolongation, profile features,
+// batched network forward / backward (FP32 accumulation over BF16-rounded weights and
+// activations), deterministic gradient reduction, Adam. The per-iteration apply kernel lives
+// with the CG kernels.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -93,6 +96,7 @@ void learned_setup(Learned& L, Model& m, int target_aggregates, const std::vecto
     for (int i = 0; i < LC_HID * LC_IN; ++i) p[LC_W1 + i] = (float)((2.0 * unif(LC_W1 + i) - 1.0) / std::sqrt((double)LC_IN));
     for (int i = 0; i < LC_HID * LC_HID; ++i)
         p[LC_W2 + i] = (float)((2.0 * unif(LC_W2 + i) - 1.0) / std::sqrt((double)LC_HID));
+    for (int i = 0; i < LC_OUT; ++i) p[LC_B3 + i] = -4.0f;
     learned_set_params(L, p.data(), false);
     L.d_m.alloc(LC_PARAMS);
     L.d_v.alloc(LC_PARAMS);
@@ -101,6 +105,8 @@ void learned_setup(Learned& L, Model& m, int target_aggregates, const std::vecto
     L.adam_t = 0;
     L.d_grad.alloc(LC_PARAMS);
 }
+
+void learned_refresh_gains(Learned& L);
 
 __global__ void k_params_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,6 +126,7 @@ void learned_set_params(Learned& L, const float* host_or_dev, bool on_device) {
     }
     k_params_to_bf16<<<dim3((LC_PARAMS + 255) / 256), dim3(256), 0, m.stream>>>(L.d_params.get(), L.d_wbf.get());
     check_launch("k_params_to_bf16");
+    learned_refresh_gains(L);
 }
 
 // ------------------------------------------------------------------ harvesting
@@ -603,11 +610,82 @@ __global__ __launch_bounds__(256) void k_lc_dots(int64_t n, const double* __rest
     }
 }
 
-// dY[s][j] = |g_s| * dC[s][j]   (c = |g| y)
-__global__ void k_lc_scale_rows(int64_t N, const float* __restrict__ gn, const float* __restrict__ dC,
-                                float* __restrict__ dY, float scale) {
+// f[a][j] = sqrt(mean_b G[b][a][j]^2) / sqrt(sum_j mean_b G[b][a][j]^2): how the harvested
+// residuals' projections spread over the aggregate's modes (zero for an empty aggregate)
+__global__ void k_lc_features(int na, int B, const float* __restrict__ G, float* __restrict__ F) {
+    const int a = blockIdx.x, j = threadIdx.x;  // 64 threads
+    double acc = 0.0;
+    for (int b = 0; b < B; ++b) {
+        const double g = G[((int64_t)b * na + a) * LC_IN + j];
+        acc += g * g;
+    }
+    double tot = acc;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+    F[(int64_t)a * LC_IN + j] = tot > 0.0 ? (float)sqrt(acc / tot) : 0.0f;
+}
+
+__device__ inline float softplus(float y) { return y > 20.0f ? y : log1pf(expf(y)); }
+__device__ inline float sigmoid(float y) { return 1.0f / (1.0f + expf(-y)); }
+
+// C[b][a][j] = softplus(Y[a][j]) * G[b][a][j]
+__global__ void k_lc_apply_gain(int64_t total, int na, const float* __restrict__ Y, const float* __restrict__ G,
+                                float* __restrict__ C) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N * LC_OUT) dY[i] = dC[i] * gn[i / LC_OUT] * scale;
+    if (i >= total) return;
+    C[i] = softplus(Y[i % ((int64_t)na * LC_IN)]) * G[i];
+}
+
+// dY[a][j] = kappa sigmoid(Y[a][j]) sum_b G[b][a][j] dC[b][a][j]  (gain = kappa softplus(Y))
+__global__ void k_lc_gain_grad(int na, int B, const float* __restrict__ Y, const float* __restrict__ G,
+                               const float* __restrict__ dC, float kappa, float* __restrict__ dY) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)na * LC_IN) return;
+    double acc = 0.0;
+    for (int b = 0; b < B; ++b) acc += (double)G[(int64_t)b * na * LC_IN + i] * (double)dC[(int64_t)b * na * LC_IN + i];
+    dY[i] = (float)(acc * kappa) * sigmoid(Y[i]);
+}
+
+__global__ void k_lc_scale(int64_t n, const float* __restrict__ y, float kappa, float* __restrict__ gain) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) gain[i] = kappa * softplus(y[i]);
+}
+
+// profile features from up to 64 stored residuals
+static void learned_features(Learned& L) {
+    Model& m = *L.model;
+    const hipStream_t s = m.stream;
+    const int64_t n = 3 * m.nv;
+    const int B = (int)std::min<int64_t>(64, L.count);
+    DevBuf<double> r;
+    r.alloc((size_t)B * n);
+    const int64_t oldest = (L.head - L.count + L.cap) % L.cap;
+    for (int b = 0; b < B; ++b) {
+        const int64_t slot = (oldest + (int64_t)b * (L.count - 1) / std::max(1, B - 1)) % L.cap;
+        HIP_CHECK(hipMemcpyAsync(r.get() + (int64_t)b * n, L.d_res.get() + slot * n, sizeof(double) * n,
+                                 hipMemcpyDeviceToDevice, s));
+    }
+    DevBuf<float> G;
+    G.alloc((size_t)B * L.na * LC_IN);
+    k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(),
+                                                                         L.d_basis.get(), r.get(), n, G.get(), nullptr, 0);
+    L.d_feat.alloc((size_t)L.na * LC_IN);
+    k_lc_features<<<dim3((unsigned)L.na), dim3(64), 0, s>>>(L.na, B, G.get(), L.d_feat.get());
+    check_launch("learned features");
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// gains = kappa softplus(MLP(features)) for the apply kernel
+void learned_refresh_gains(Learned& L) {
+    Model& m = *L.model;
+    if (!L.d_feat.p) return;
+    L.ensure_batch(L.na);
+    L.d_gain.ensure((size_t)L.na * LC_IN);
+    const unsigned nwg = (unsigned)((L.na + MLP_TS - 1) / MLP_TS);
+    k_mlp_fwd<<<dim3(nwg), dim3(256), 0, m.stream>>>(L.na, L.d_feat.get(), L.d_wbf.get(), L.d_params.get(),
+                                                     L.d_z1.get(), L.d_h1.get(), L.d_z2.get(), L.d_h2.get(), L.d_y.get());
+    const int64_t tot = (int64_t)L.na * LC_IN;
+    k_lc_scale<<<dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, m.stream>>>(tot, L.d_y.get(), L.kappa, L.d_gain.get());
+    check_launch("learned gains");
 }
 
 template <class VT>
@@ -618,25 +696,28 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
     const int64_t n = 3 * m.nv;
     if (!L.has_basis) learned_build_basis(L);
     if (L.count < 1) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "no harvested residuals");
+    if (!L.d_feat.p) learned_features(L);
     B = std::max(1, std::min<int>(B, (int)L.count));
-    const int64_t N = (int64_t)B * L.na;
-    L.ensure_batch(N);
-    DevBuf<double> r, zs, y, Jz, dz, loss, aq;
+    const int64_t NG = (int64_t)B * L.na * LC_IN;  // projections of a batch
+    L.ensure_batch(L.na);
+    DevBuf<double> r, zs, y, Jz, dz, aq;
     r.alloc((size_t)B * n);
     zs.alloc((size_t)B * n);
     y.alloc((size_t)B * n);
     Jz.alloc((size_t)B * n);
     dz.alloc((size_t)B * n);
-    loss.alloc(B);
     aq.alloc(2 * (size_t)B);
-    DevBuf<float> dC;
-    dC.alloc((size_t)N * LC_OUT);
+    DevBuf<float> G, C, dC;
+    G.alloc(NG);
+    C.alloc(NG);
+    dC.alloc(NG);
     DevBuf<int32_t> d_slots;
     d_slots.alloc(B);
     DevBuf<double> d_inv;
     d_inv.alloc(B);
     // residual norms of the ring (host, once)
     std::vector<double> rnorm(L.cap, 0.0);
+    std::vector<int64_t> usable;
     {
         std::vector<double> h((size_t)n);
         const int64_t oldest = (L.head - L.count + L.cap) % L.cap;
@@ -647,23 +728,18 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
             double q = 0.0;
             for (double v : h) q += v * v;
             rnorm[slot] = std::sqrt(q);
-        }
-    }
-    std::vector<int64_t> usable;
-    {
-        const int64_t oldest = (L.head - L.count + L.cap) % L.cap;
-        for (int64_t k = 0; k < L.count; ++k) {
-            const int64_t slot = (oldest + k) % L.cap;
             if (rnorm[slot] > 0.0 && std::isfinite(rnorm[slot])) usable.push_back(slot);
         }
     }
     if (usable.empty()) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "all harvested residuals are zero");
     const unsigned gn = (unsigned)((n + 255) / 256);
     const unsigned gBn = (unsigned)((B * n + 255) / 256);
+    const unsigned gG = (unsigned)((NG + 255) / 256);
+    const unsigned gNa = (unsigned)(((int64_t)L.na * LC_IN + 255) / 256);
+    const unsigned nwg = (unsigned)((L.na + MLP_TS - 1) / MLP_TS);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     std::vector<int32_t> slots(B);
-    std::vector<double> inv(B);
-    std::vector<double> hl(B), haq(2 * (size_t)B);
+    std::vector<double> inv(B), haq(2 * (size_t)B), hl(B);
     double last = 0.0;
     uint64_t cursor = 0;
     for (int step = 0; step <= steps; ++step) {
@@ -683,21 +759,22 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
                 launch_spmv(P->P, P->d_vals.get(), 0.0, zs.get() + (int64_t)b * n, 1.0, r.get() + (int64_t)b * n, s);
         }
         if (L.kappa == 0.0f) {  // |r_b| = 1
-            k_lc_norm2<<<dim3((unsigned)B), dim3(256), 0, s>>>(n, zs.get(), loss.get());
-            HIP_CHECK(hipMemcpyAsync(hl.data(), loss.get(), sizeof(double) * B, hipMemcpyDeviceToHost, s));
+            k_lc_norm2<<<dim3((unsigned)B), dim3(256), 0, s>>>(n, zs.get(), aq.get());
+            HIP_CHECK(hipMemcpyAsync(hl.data(), aq.get(), sizeof(double) * B, hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
             double k = 0.0;
             for (double q : hl) k += std::sqrt(q);
             L.kappa = (float)(k / B);
             if (!(L.kappa > 0.0f) || !std::isfinite(L.kappa)) throw Error(HOEFEM_ERROR_NOT_FINITE, "bad output scale");
         }
-        k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
-            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), r.get(), n, L.d_x.get(), L.d_gn.get(), 1);
-        const unsigned nwg = (unsigned)((N + MLP_TS - 1) / MLP_TS);
-        k_mlp_fwd<<<dim3(nwg), dim3(256), 0, s>>>(N, L.d_x.get(), L.d_wbf.get(), L.d_params.get(), L.d_z1.get(),
+        // gains = kappa softplus(MLP(features)); z = P r + Phi (gains . Phi^T r)
+        k_mlp_fwd<<<dim3(nwg), dim3(256), 0, s>>>(L.na, L.d_feat.get(), L.d_wbf.get(), L.d_params.get(), L.d_z1.get(),
                                                   L.d_h1.get(), L.d_z2.get(), L.d_h2.get(), L.d_y.get());
+        k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
+            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), r.get(), n, G.get(), nullptr, 0);
+        k_lc_apply_gain<<<dim3(gG), dim3(256), 0, s>>>(NG, L.na, L.d_y.get(), G.get(), C.get());
         k_lc_prolong<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
-            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), L.d_y.get(), L.d_gn.get(), L.kappa, n, y.get());
+            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), C.get(), nullptr, L.kappa, n, y.get());
         k_lc_add<<<dim3(gBn), dim3(256), 0, s>>>(B * n, zs.get(), y.get());  // y <- z = zs + y
         for (int b = 0; b < B; ++b) launch_spmv(m.pattern, Jvals, 0.0, Jz.get() + (int64_t)b * n, 1.0, y.get() + (int64_t)b * n, s);
         check_launch("learned training forward");
@@ -715,9 +792,9 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
         k_lc_graddz<<<dim3(gBn), dim3(256), 0, s>>>(n, B, aq.get(), Jz.get(), r.get(), dz.get());
         k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
             L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), dz.get(), n, dC.get(), nullptr, 0);
-        k_lc_scale_rows<<<dim3((unsigned)((N * LC_OUT + 255) / 256)), dim3(256), 0, s>>>(N, L.d_gn.get(), dC.get(),
-                                                                                        L.d_dy.get(), L.kappa);
-        k_mlp_bwd<<<dim3(nwg), dim3(256), 0, s>>>(N, L.d_x.get(), L.d_wbf.get(), L.d_z1.get(), L.d_h1.get(),
+        k_lc_gain_grad<<<dim3(gNa), dim3(256), 0, s>>>(L.na, B, L.d_y.get(), G.get(), dC.get(), L.kappa,
+                                                       L.d_dy.get());
+        k_mlp_bwd<<<dim3(nwg), dim3(256), 0, s>>>(L.na, L.d_feat.get(), L.d_wbf.get(), L.d_z1.get(), L.d_h1.get(),
                                                   L.d_z2.get(), L.d_h2.get(), L.d_dy.get(), L.d_gpart.get());
         k_mlp_reduce<<<dim3((LC_PARAMS + 255) / 256), dim3(256), 0, s>>>((int)nwg, L.d_gpart.get(), 1.0f / (float)B,
                                                                         L.d_grad.get());
@@ -728,6 +805,8 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
                                                                    c1, c2);
         check_launch("learned training step");
     }
+    learned_refresh_gains(L);
+    HIP_CHECK(hipStreamSynchronize(s));
     L.trained = true;
     return last;
 }
@@ -743,8 +822,7 @@ double learned_train(Learned& L, const float* J, Schwarz* P, int steps, double l
 }
 
 LearnedView learned_view(const Learned& L) {
-    return LearnedView{L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), L.d_wbf.get(), L.d_params.get(),
-                       L.kappa};
+    return LearnedView{L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), L.d_gain.get()};
 }
 
 }  // namespace hoefem

@@ -473,31 +473,17 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P, int6
 }
 
 // Learned coarse correction, one work-group per aggregate: g = Phi_a^T r_new,
-// c = kappa |g| MLP(bf16(g / |g|)), z += Phi_a c on the aggregate's DOFs (z holds P r_new), then the
-// partial sums z.r_new, z.r_old and |r_new|^2 of the flexible iteration. Every DOF belongs to
-// exactly one aggregate, so the work-groups' partials add up to the full dot products.
+// z += Phi_a (gain_a . g) on the aggregate's DOFs (z holds P r_new), then the partial sums
+// z.r_new, z.r_old and |r_new|^2 of the iteration. Every DOF belongs to exactly one aggregate,
+// so the work-groups' partials add up to the full dot products.
 __global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* __restrict__ ra_new,
                                                   const double* __restrict__ ra_old, double* __restrict__ zp,
                                                   double* __restrict__ partials) {
-    __shared__ uint16_t s_w[LC_PARAMS - LC_HID - LC_HID - LC_OUT];  // W1 | W2 | W3 (BF16)
     __shared__ double s_r[LC_MAX_AGG_NODES * 3];
-    __shared__ float s_g[LC_IN];
-    __shared__ float s_h1[LC_HID];
-    __shared__ float s_h2[LC_HID];
-    __shared__ float s_c[LC_OUT];
-    __shared__ double s_nrm;
+    __shared__ double s_c[LC_IN];
     const int a = blockIdx.x;
     const int o = lv.agg_ptr[a], nd = lv.agg_ptr[a + 1] - o;
     const int t = threadIdx.x;
-    // stage the weights (coalesced); skip the FP32 bias slots between the weight matrices
-    const uint32_t* wsrc1 = reinterpret_cast<const uint32_t*>(lv.wbf + LC_W1);
-    const uint32_t* wsrc2 = reinterpret_cast<const uint32_t*>(lv.wbf + LC_W2);
-    const uint32_t* wsrc3 = reinterpret_cast<const uint32_t*>(lv.wbf + LC_W3);
-    uint32_t* wdst = reinterpret_cast<uint32_t*>(s_w);
-    constexpr int n1 = LC_HID * LC_IN / 2, n2 = LC_HID * LC_HID / 2, n3 = LC_OUT * LC_HID / 2;
-    for (int i = t; i < n1; i += 256) wdst[i] = wsrc1[i];
-    for (int i = t; i < n2; i += 256) wdst[n1 + i] = wsrc2[i];
-    for (int i = t; i < n3; i += 256) wdst[n1 + n2 + i] = wsrc3[i];
     for (int d = t; d < nd; d += 256) {
         const int dof = lv.agg_dofs[o + d];
         s_r[d] = ra_new[6 * (dof / 3) + dof % 3];
@@ -510,67 +496,16 @@ __global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* 
         for (int d = t % 4; d < nd; d += 4) c += (double)B[j * nd + d] * s_r[d];
         c += __shfl_xor(c, 1, 4);
         c += __shfl_xor(c, 2, 4);
-        if (t % 4 == 0) s_g[j] = (float)c;
+        if (t % 4 == 0) s_c[j] = c * (double)lv.gain[(int64_t)a * LC_IN + j];
     }
     __syncthreads();
-    if (t < 64) {
-        double q = (double)s_g[t] * (double)s_g[t];
-        for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
-        if (t == 0) s_nrm = sqrt(q);
-    }
-    __syncthreads();
-    const double nrm = s_nrm;
-    const float inv = nrm > 0.0 ? (float)(1.0 / nrm) : 0.0f;
-    if (t < LC_IN) s_g[t] = bf16_round(s_g[t] * inv);
-    __syncthreads();
-    if (t < LC_HID) {
-        float acc = 0.0f;
-        for (int i = 0; i < LC_IN; ++i) acc += bf16_float(s_w[t * LC_IN + i]) * s_g[i];
-        s_h1[t] = bf16_round(gelu(acc + lv.params[LC_B1 + t]));
-    }
-    __syncthreads();
-    if (t < LC_HID) {
-        float acc = 0.0f;
-        for (int i = 0; i < LC_HID; ++i) acc += bf16_float(s_w[n1 * 2 + t * LC_HID + i]) * s_h1[i];
-        s_h2[t] = bf16_round(gelu(acc + lv.params[LC_B2 + t]));
-    }
-    __syncthreads();
-    if (t < LC_OUT) {
-        float acc = 0.0f;
-        for (int i = 0; i < LC_HID; ++i) acc += bf16_float(s_w[(n1 + n2) * 2 + t * LC_HID + i]) * s_h2[i];
-        s_c[t] = (acc + lv.params[LC_B3 + t]) * (float)nrm * lv.kappa;
-    }
-    __syncthreads();
-    // safeguard: the correction may lower the aggregate's share of z.r to at most half of the
-    // Schwarz share (and may not lower a non-positive share), so z.r stays positive in practice;
-    // the flexible solve still reports an indefinite preconditioner if it is not
-    __shared__ double s_corr[LC_MAX_AGG_NODES * 3];
-    __shared__ double s_red2[2][4];
-    double zs_r = 0.0, c_r = 0.0;
-    for (int d = t; d < nd; d += 256) {
-        double corr = 0.0;
-        for (int j = 0; j < LC_IN; ++j) corr += (double)B[j * nd + d] * (double)s_c[j];
-        s_corr[d] = corr;
-        const int dof = lv.agg_dofs[o + d];
-        zs_r += zp[6 * (int64_t)(dof / 3) + dof % 3] * s_r[d];
-        c_r += corr * s_r[d];
-    }
-    zs_r = wave_sum(zs_r);
-    c_r = wave_sum(c_r);
-    if ((t & 63) == 0) {
-        s_red2[0][t >> 6] = zs_r;
-        s_red2[1][t >> 6] = c_r;
-    }
-    __syncthreads();
-    const double sz = s_red2[0][0] + s_red2[0][1] + s_red2[0][2] + s_red2[0][3];
-    const double sc_ = s_red2[1][0] + s_red2[1][1] + s_red2[1][2] + s_red2[1][3];
-    double omega = 1.0;
-    if (sc_ < 0.0) omega = sz > 0.0 ? fmin(1.0, 0.5 * sz / -sc_) : 0.0;
     double part[3] = {0.0, 0.0, 0.0};
     for (int d = t; d < nd; d += 256) {
+        double corr = 0.0;
+        for (int j = 0; j < LC_IN; ++j) corr += (double)B[j * nd + d] * s_c[j];
         const int dof = lv.agg_dofs[o + d];
         const int64_t zi = 6 * (int64_t)(dof / 3) + dof % 3;
-        const double z = zp[zi] + omega * s_corr[d];
+        const double z = zp[zi] + corr;
         zp[zi] = z;
         const double rn = s_r[d];
         const double ro = ra_old ? ra_old[zi] : 0.0;

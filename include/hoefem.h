@@ -217,14 +217,17 @@ hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, d
  * simulation's linear solves when enabled.
  *
  * The nodes are split into aggregates; each aggregate carries 64 orthonormal modes taken from
- * error snapshots x* - x_k of the plain Schwarz-CG's own solves. A network shared by all
- * aggregates (64 -> 128 -> 128 -> 64, GELU, BF16 weights, FP32 accumulation) maps the
- * normalized projection of the residual on an aggregate's modes to correction coefficients:
- *     z = P r + sum_a Phi_a |g_a| MLP(g_a / |g_a|),   g_a = Phi_a^T r.
- * Training minimizes -(r^T z)^2 / (z^T J z) (the energy left after the best step along z) over
- * harvested residuals (Adam). A learned solve is a flexible (Polak-Ribiere) PCG under the plain
- * path's rules (the CG iteration limit is tolerated; the Newton residual tolerance is
- * unchanged); on a breakdown the plain Schwarz-CG solves the same system from zero.
+ * error snapshots x* - x_k of the plain Schwarz-CG's own solves. The correction is linear,
+ * symmetric and positive semidefinite in the residual, so the preconditioner stays SPD and CG
+ * keeps its Krylov properties:
+ *     z = P r + sum_a Phi_a diag(gamma_a) Phi_a^T r,   gamma_a = kappa softplus(MLP(f_a)),
+ * where one network shared by all aggregates (64 -> 128 -> 128 -> 64, GELU, BF16 weights, FP32
+ * accumulation) maps the aggregate's residual projection profile f_a (RMS of Phi_a^T r over
+ * harvested residuals, normalized) to its 64 mode gains. Training minimizes
+ * -(r^T z)^2 / (z^T J z) (the energy left after the best step along z) over harvested residuals
+ * (Adam). A learned solve runs the flexible (Polak-Ribiere) PCG under the plain path's rules
+ * (the CG iteration limit is tolerated; the Newton residual tolerance is unchanged); on a
+ * breakdown the plain Schwarz-CG solves the same system from zero.
  *
  * Lifetime: the correction refers to its simulation; destroying the simulation first detaches
  * it (training then fails with HOEFEM_ERROR_INVALID_ARGUMENT). */

@@ -9,7 +9,7 @@
   within 1e-6 (the bar of the mixed-precision tests).
 - From the same state, CG iterations with and without the correction are printed (the gain),
   with a CG iteration limit high enough for both to converge.
-- With an all-zero network the flexible iteration reproduces the plain one.
+- With vanishing gains the flexible iteration reproduces the plain one.
 """
 
 import numpy as np
@@ -105,7 +105,7 @@ def test_training_lowers_loss_and_solves_reach_tolerance(small):
     lc.harvest(False)
     info = lc.info()
     assert info["stored_pairs"] >= 64, info
-    l0, l1 = lc.train(steps=300, learning_rate=1e-3, batch=8)
+    l0, l1 = lc.train(steps=600, learning_rate=3e-3, batch=8)
     print(f"training: loss {l0:.6e} -> {l1:.6e}; aggregates {info['num_aggregates']}, pairs {info['stored_pairs']}")
     assert l1 < l0
     # same state, one step without and one with the correction
@@ -130,7 +130,7 @@ def test_training_lowers_loss_and_solves_reach_tolerance(small):
 
 
 def test_flexible_solver_with_zero_correction_matches_plain(small):
-    """With every network parameter zero the correction vanishes: the flexible (Polak-Ribiere)
+    """With zero weights and output bias -30 the gains vanish: the flexible (Polak-Ribiere)
     iteration then equals plain Schwarz-CG in exact arithmetic, so the iteration counts agree
     to within rounding."""
     fem, sc, gm, _ = small
@@ -143,7 +143,9 @@ def test_flexible_solver_with_zero_correction_matches_plain(small):
         sim.step(k * sc.dt, sc.dt, u, v)
     lc.harvest(False)
     lc.train(steps=0)
-    lc.set_params(np.zeros(O.MLP_PARAMS, dtype=np.float32))
+    p = np.zeros(O.MLP_PARAMS, dtype=np.float32)
+    p[-64:] = -30.0  # gains kappa softplus(-30) ~ 1e-13 kappa: the correction vanishes
+    lc.set_params(p)
     u0, v0 = u.clone(), v.clone()
     plain = sim.step(2 * sc.dt, sc.dt, u0, v0)
     lc.enable(True)
