@@ -199,7 +199,8 @@ struct Schwarz {
 struct CgScalars {
     double zTr, pAp, alpha, beta, rr, bnorm, tol;
     double zrec[2];  // zTr of iteration k in zrec[k & 1] (read by the next iteration's beta)
-    int64_t iters, max_iter;
+    int64_t iters, max_iter;  // iters: completed iterations (the A-kernel's k)
+    int64_t kcur;             // k of the running iteration (written by the A-kernel for the B-kernel)
     int status;  // 0 running, 1 converged, hoefem_status codes for errors
     int pad;
 };
@@ -215,6 +216,13 @@ struct CgWork {
     DevBuf<double> part_pre, part_a, part_b;  // b.b (preamble 1), pAp (A-kernel), zTr and rr (B-kernel)
     DevBuf<double> partials;                  // residual kernel partials
     DevBuf<CgScalars> scalars;
+    // every solve runs on its own stream (joined to the caller's by events); a batch of
+    // iterations is captured once as a graph and replayed while its arguments stay the same
+    hipStream_t cs = nullptr;
+    hipEvent_t join[2] = {nullptr, nullptr};
+    hipGraphExec_t batch_exec = nullptr;
+    std::vector<const void*> batch_key;
+    bool use_graphs = true;
     PinnedBuf<CgScalars> h_scalars{3};  // two polling slots + final readback
     hipEvent_t ev[2] = {nullptr, nullptr};
     void ensure(int64_t n, int nblocks);

@@ -306,9 +306,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
 // A-kernel of iteration k: stopping tests on r_k, beta_k = zTr_k / zTr_{k-1},
 // p_k = z_k + beta p_{k-1} ; Ap_k = A p_k ; partial pAp
 template <int G, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64_t k) {
+__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
+    const int64_t k = sc->iters;  // written by the previous B-kernel (0 after the preamble)
     double zTr, rr, zTr_cross = 0.0;  // z_k.r_k, |r_k|^2, z_k.r_{k-1} (flexible variant)
     if (FLEX) {
         double tot[3];
@@ -346,6 +347,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64
         sc->zTr = zTr;
         sc->rr = rr;
         sc->beta = beta;
+        sc->kcur = k;
     }
     const double* zp = P.zp_old;
     double part[1] = {0.0};
@@ -389,9 +391,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P, int64
 // B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p_k ;
 // r_{k+1} = r_k - alpha Ap_k ; z_{k+1} = P r_{k+1} ; partial zTr, |r|^2
 template <int G, bool JAC, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P, int64_t k) {
+__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
+    const int64_t k = sc->kcur;  // written by this iteration's A-kernel
     double tot[1];
     sum_partials<1>(P.part_a, P.nb_a, tot);
     const double pAp = tot[0];
@@ -521,6 +524,7 @@ __global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
     sc->max_iter = max_iter;
     sc->status = CG_RUNNING;
     sc->iters = 0;
+    sc->kcur = 0;
     sc->beta = 0.0;
     sc->zrec[0] = sc->zrec[1] = 0.0;
 }
@@ -571,12 +575,19 @@ void CgWork::ensure(int64_t nn_dof, int nblocks) {
     if (!ev[0]) {
         HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&join[0], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&join[1], hipEventDisableTiming));
+        HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     }
 }
 
 CgWork::~CgWork() {
+    if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : join)
+        if (e) (void)hipEventDestroy(e);
+    if (cs) (void)hipStreamDestroy(cs);
 }
 
 template <class VT>
@@ -598,7 +609,9 @@ template <int GA, int GB, bool JAC, bool FLEX, class VT>
 static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>& Pm, const double* b, double* x,
                           double tol, int64_t max_iter, int64_t* iterations, hoefem_status* out_status,
                           CgHarvest* h, const LearnedView* lv) {
-    const hipStream_t s = m.stream;
+    const hipStream_t s = w.cs;
+    HIP_CHECK(hipEventRecord(w.join[0], m.stream));
+    HIP_CHECK(hipStreamWaitEvent(s, w.join[0], 0));
     const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA);
     const unsigned blkB = JAC ? cg_grid(0, Pm.nn, GB) : cg_grid(Pm.n_block, Pm.n_diag, GB);
     CgPtrs P{};
@@ -626,7 +639,16 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     int cur = 0;
     int64_t k_next = 0;  // iteration index of the next A/B pair
     size_t nev = 0;      // events used by this solve (profiling only)
-    auto enqueue_batch = [&]() {
+    // graph replay of whole batches: every argument of a batch is fixed (the iteration index
+    // lives on the device and CG_BATCH is even, so the buffer parity repeats); per-iteration
+    // hooks (kernel timing, snapshots) need the direct launches
+    const bool graph = w.use_graphs && !m.prof && !h;
+    const std::vector<const void*> key = {A.vals, Pm.vals, x, A.ent, Pm.ent, w.zp[0].get(), w.ra[0].get(),
+                                          (const void*)(intptr_t)GA, (const void*)(intptr_t)GB,
+                                          (const void*)(intptr_t)JAC, (const void*)(intptr_t)FLEX,
+                                          lv ? lv->basis : nullptr, lv ? lv->gain : nullptr,
+                                          (const void*)(intptr_t)(lv ? lv->na : 0), P.dinv, P.sc};
+    auto launch_batch = [&]() {
         for (int q = 0; q < CG_BATCH; ++q) {
             P.zp_old = w.zp[cur].get();
             P.zp_new = w.zp[1 - cur].get();
@@ -642,12 +664,12 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
                     m.prof_ev.push_back(e);
                 }
                 hipExtLaunchKernelGGL(k_cg_a<GA, FLEX, VT>, dim3(blkA), dim3(CG_THREADS), 0, s, m.prof_ev[nev],
-                                      m.prof_ev[nev + 1], 0, A, P, k_next);
+                                      m.prof_ev[nev + 1], 0, A, P);
                 nev += 2;
             } else {
-                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P, k_next);
+                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
             }
-            k_cg_b<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P, k_next);
+            k_cg_b<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
             if (FLEX)
                 k_lc_apply<<<dim3((unsigned)lv->na), dim3(256), 0, s>>>(*lv, w.ra[1 - cur].get(), w.ra[cur].get(),
                                                                        w.zp[1 - cur].get(), w.part_b.get());
@@ -662,6 +684,31 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
             }
         }
         check_launch("cg iteration");
+    };
+    auto enqueue_batch = [&]() {
+        if (!graph) {
+            launch_batch();
+            return;
+        }
+        if (!w.batch_exec || w.batch_key != key) {
+            if (w.batch_exec) {
+                HIP_CHECK(hipGraphExecDestroy(w.batch_exec));
+                w.batch_exec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int cur0 = cur;
+            const int64_t k0 = k_next;
+            launch_batch();
+            HIP_CHECK(hipStreamEndCapture(s, &g));
+            HIP_CHECK(hipGraphInstantiate(&w.batch_exec, g, nullptr, nullptr, 0));
+            HIP_CHECK(hipGraphDestroy(g));
+            w.batch_key = key;
+            cur = cur0;  // the capture only recorded the batch
+            k_next = k0;
+        }
+        HIP_CHECK(hipGraphLaunch(w.batch_exec, s));
+        k_next += CG_BATCH;  // CG_BATCH is even: the buffer parity is back where it started
     };
     CgScalars* slots = w.h_scalars.p;  // [0], [1]: polling slots
     CgScalars* hs = w.h_scalars.p + 2;
@@ -685,6 +732,8 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     }
     HIP_CHECK(hipMemcpyAsync(hs, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipEventRecord(w.join[1], s));
+    HIP_CHECK(hipStreamWaitEvent(m.stream, w.join[1], 0));
     *iterations = hs->iters;
     if (h) {  // keep the snapshots of iterations the solve actually reached
         int keep = 0;
@@ -708,7 +757,7 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     switch (hs->status) {
         case CG_CONVERGED: *out_status = HOEFEM_OK; break;
         case CG_ZERO_RHS:
-            HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * w.n, s));
+            HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * w.n, m.stream));  // after the join
             *out_status = HOEFEM_OK;
             break;
         case CG_RUNNING: *out_status = HOEFEM_ERROR_CG_MAX_ITERATIONS; break;
