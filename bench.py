@@ -39,7 +39,12 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--config", type=int, default=2, choices=[1, 2, 3])
+    p.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
+                   help="1: Tet10 cube, 2: embedded Hex20 cylinder, 3: embedded Hex27 cylinder (48^3), "
+                        "4: config 2 with the learned coarse correction, 5: batched independent scenes")
+    p.add_argument("--harvest-steps", type=int, default=10, help="config 4: plain steps that harvest snapshots")
+    p.add_argument("--train-steps", type=int, default=600, help="config 4: Adam steps")
+    p.add_argument("--scenes", type=int, default=64, help="config 5: number of scenes")
     p.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
     p.add_argument("--no-profile", action="store_true", help="do not bracket the CG kernel with events")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -60,12 +65,125 @@ def cg_kernel_bytes(model, value_bytes: int = 8) -> int:
     return nnz * value_bytes + (nnz // 9) * 4 + (ndof + 1) * 4 + (nv + 1) * 4 + 4 * ndof * 8
 
 
+def common_line(args, metric, value, ms, dtype, config, **extra):
+    line = {"metric": metric, "value": round(value, 3), "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded scene generator)", "config": config}
+    line.update(extra)
+    return line
+
+
+def bench_learned(args):
+    """Config 4: the embedded Hex20 cylinder with the learned coarse correction. One simulation
+    harvests snapshots during its first plain steps, trains the network and continues with the
+    correction enabled; a second simulation runs the same steps plain. Both time the same K
+    steps after the harvest phase."""
+    import numpy as np
+    import torch
+
+    from higher_order_embedded_fem_amd import fem, scenes
+
+    sc = scenes.config_scene(2)
+    model = fem.ElasticityModel(sc)
+    sims = {name: fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mixed)
+            for name in ("plain", "learned")}
+    for sim in sims.values():
+        if not args.no_autotune:
+            sim.autotune()
+    lc = fem.LearnedCoarseCorrection(sims["learned"], num_aggregates=512, seed=0)
+    state = {}
+    for name, sim in sims.items():
+        u = torch.zeros(sc.ndof, dtype=torch.float64, device="cuda")
+        v = torch.zeros_like(u)
+        if name == "learned":
+            lc.harvest(True, capacity=max(64, 40 * args.harvest_steps), every=16)
+        for k in range(args.harvest_steps):
+            sim.step(k * sc.dt, sc.dt, u, v)
+        state[name] = (u, v)
+    lc.harvest(False)
+    t0 = time.perf_counter()
+    l0, l1 = lc.train(steps=args.train_steps, learning_rate=3e-3, batch=8)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    lc.enable(True)
+    res = {}
+    for name, sim in sims.items():
+        u, v = state[name]
+        cg = newton = 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.harvest_steps, args.harvest_steps + args.steps):
+            st = sim.step(k * sc.dt, sc.dt, u, v)
+            cg += st["cg_iterations"]
+            newton += st["newton_iterations"]
+        torch.cuda.synchronize()
+        res[name] = {"seconds": time.perf_counter() - t0, "cg": cg, "newton": newton, "u": u.cpu().numpy()}
+        print(f"[bench] {name}: {args.steps} steps in {res[name]['seconds']:.3f}s, cg {cg}, newton {newton}",
+              file=sys.stderr, flush=True)
+    info = lc.info()
+    d = res["learned"]["u"] - res["plain"]["u"]
+    value = args.steps / res["learned"]["seconds"]
+    line = common_line(
+        args, "timesteps/s at 1e-6 rel Newton residual", value, 1e3 * res["learned"]["seconds"] / args.steps,
+        "f32+f64" if args.mixed else "f64",
+        {"workload": sc.name + " + learned coarse correction", "dofs": sc.ndof, "nnz": model.nnz,
+         "harvest_steps": args.harvest_steps, "train_steps": args.train_steps, "aggregates": info["num_aggregates"],
+         "precision": "mixed" if args.mixed else "fp64", "parallelism": "single GPU"},
+        learned={"plain_value": round(args.steps / res["plain"]["seconds"], 3),
+                 "speedup_vs_plain": round(res["plain"]["seconds"] / res["learned"]["seconds"], 4),
+                 "cg_iterations_per_step": res["learned"]["cg"] / args.steps,
+                 "cg_iterations_per_step_plain": res["plain"]["cg"] / args.steps,
+                 "fallbacks": info["fallbacks"], "learned_solves": info["learned_solves"],
+                 "harvested_pairs": info["harvested_pairs"], "train_seconds": round(t_train, 3),
+                 "loss_before_after": [l0, l1],
+                 "u_rel_l2_vs_plain": float(np.linalg.norm(d) / max(np.linalg.norm(res["plain"]["u"]), 1e-300))},
+        roofline=None, cpu_baseline=None)
+    print(json.dumps(line))
+
+
+def bench_batched(args):
+    """Config 5: independent scenes of uneven size stepped together (one stream and one
+    captured CG batch graph per scene, host threads overlap the scenes)."""
+    import numpy as np
+    import torch
+
+    from higher_order_embedded_fem_amd import batch
+
+    t0 = time.time()
+    scs = batch.batched_scenes(args.scenes)
+    stepper = batch.BatchedStepper(scs, newton_tolerance_factor=1e-6, mixed_precision=args.mixed,
+                                   autotune=not args.no_autotune)
+    print(f"[bench] {len(scs)} scenes, {sum(s.ndof for s in scs)} dofs in total, setup {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    if args.warmup:
+        stepper.run(args.warmup)
+    stepper.reset()
+    r = stepper.run(args.steps)
+    print(f"[bench] {r['scene_steps']} scene steps in {r['wall_s']:.3f}s", file=sys.stderr, flush=True)
+    line = common_line(
+        args, "aggregate timesteps/s at 1e-6 rel Newton residual", r["aggregate_timesteps_per_s"],
+        1e3 * r["wall_s"] / args.steps, "f32+f64" if args.mixed else "f64",
+        {"workload": f"{len(scs)} independent scenes (Tet10 cubes and Hex20 cylinders, DOFs log-uniform in [2e3, 2e5])",
+         "total_dofs": r["total_dofs"], "precision": "mixed" if args.mixed else "fp64", "parallelism": "single GPU"},
+        batched={"p50_step_latency_ms": round(r["p50_step_latency_ms"], 3),
+                 "p99_step_latency_ms": round(r["p99_step_latency_ms"], 3),
+                 "newton_iterations": r["newton_iterations"], "cg_iterations": r["cg_iterations"]},
+        roofline=None, cpu_baseline=None)
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
+    if args.config in (4, 5):
+        import torch
+
+        assert torch.cuda.is_available(), "bench.py needs a HIP device"
+        (bench_learned if args.config == 4 else bench_batched)(args)
+        return
     import ctypes as C
 
     import numpy as np
