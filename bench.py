@@ -51,6 +51,8 @@ def parse():
                    help="run a plain 256 MiB device copy 3x first (reference kernel for HBM counters)")
     p.add_argument("--mixed", action="store_true",
                    help="FP32 element tangents / Jacobian SpMV / Schwarz operator inside the FP64 Newton refinement")
+    p.add_argument("--mixed-refined", action="store_true",
+                   help="FP64 tangents and Jacobian, CG on FP32 copies inside FP64 iterative refinement")
     p.add_argument("--no-autotune", action="store_true", help="keep the heuristic SpMV lanes per node")
     return p.parse_args()
 
@@ -85,7 +87,7 @@ def bench_learned(args):
 
     sc = scenes.config_scene(2)
     model = fem.ElasticityModel(sc)
-    sims = {name: fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mixed)
+    sims = {name: fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mode)
             for name in ("plain", "learned")}
     for sim in sims.values():
         if not args.no_autotune:
@@ -151,7 +153,7 @@ def bench_batched(args):
 
     t0 = time.time()
     scs = batch.batched_scenes(args.scenes)
-    stepper = batch.BatchedStepper(scs, newton_tolerance_factor=1e-6, mixed_precision=args.mixed,
+    stepper = batch.BatchedStepper(scs, newton_tolerance_factor=1e-6, mixed_precision=args.mode,
                                    autotune=not args.no_autotune)
     print(f"[bench] {len(scs)} scenes, {sum(s.ndof for s in scs)} dofs in total, setup {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
@@ -174,6 +176,8 @@ def bench_batched(args):
 
 def main():
     args = parse()
+    args.mode = 2 if args.mixed_refined else (1 if args.mixed else 0)
+    args.mixed = args.mode != 0
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no multi-GPU path.",
               file=sys.stderr)
@@ -203,12 +207,12 @@ def main():
     t_setup = time.time()
     sc = scenes.config_scene(args.config)
     model = fem.ElasticityModel(sc)
-    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mixed)
+    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mode)
     L = _lib.load()
     lanes = None if args.no_autotune else sim.autotune()
     print(f"[bench] {sc.name}: {sc.num_elements} elements, {sc.ndof} dofs, nnz {model.nnz}, "
           f"setup {time.time() - t_setup:.1f}s, SpMV lanes per node (J, P) {lanes}", file=sys.stderr, flush=True)
-    precision = "mixed" if args.mixed else "fp64"
+    precision = {0: "fp64", 1: "mixed", 2: "mixed-refined"}[args.mode]
 
     def run(steps, record=None):
         u = torch.zeros(sc.ndof, dtype=torch.float64, device="cuda")

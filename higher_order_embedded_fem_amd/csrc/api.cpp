@@ -21,6 +21,8 @@ void launch_finite_check(Model& m, const double* a, const double* b);
 void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n);
 void launch_f64_to_f32(Model& m, const double* a, float* b, int64_t n);
 void launch_fill(Model& m, double* a, int64_t n, double value);
+void launch_add(Model& m, double* a, const double* b, int64_t n);
+double device_dot(Model& m, CgWork& w, const double* a, const double* b, double* d_out, double* h_out);
 
 thread_local std::string g_last_error;
 
@@ -137,8 +139,12 @@ struct Sim {
     DevBuf<int32_t> d_bc_nodes;
     int64_t n_bc = 0;
     DevBuf<double> d_M, d_J, d_gforce, d_up, d_vn, d_w, d_F, d_sol, d_uu, d_us, d_vs, d_scale, d_norm2;
-    bool mixed = false;   // FP32 element tangents, J and Schwarz operator
+    bool mixed = false;   // FP32 CG operator (J and Schwarz values)
+    int mixed_mode = 0;   // 1: FP32 element tangents, the Newton loop refines; 2: FP64 tangents and J, FP32
+                          //    copy for the CG, FP64 iterative refinement of every linear solve
     DevBuf<float> d_J32;
+    DevBuf<double> d_res, d_dx;  // iterative refinement (mode 2)
+    int64_t refinements = 0;
     PinnedBuf<double> h_norm2{1};
     CgWork cg;
     Learned* learned = nullptr;                 // optional learned coarse correction (not owned)
@@ -178,12 +184,16 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
     const int64_t n = 3 * m.nv, nnz = m.pattern.nnz();
     for (DevBuf<double>* b : {&s->d_gforce, &s->d_up, &s->d_vn, &s->d_w, &s->d_F, &s->d_sol, &s->d_uu, &s->d_us, &s->d_vs})
         b->ensure(n);
+    if (d->mixed_precision < 0 || d->mixed_precision > 2) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "mixed_precision must be 0, 1 or 2");
     s->mixed = d->mixed_precision != 0;
+    s->mixed_mode = d->mixed_precision;
     s->d_M.ensure(nnz);
-    if (s->mixed)
+    s->d_J.ensure(nnz);
+    if (s->mixed) {
         s->d_J32.ensure(nnz);
-    else
-        s->d_J.ensure(nnz);
+        s->d_res.ensure(n);
+        s->d_dx.ensure(n);
+    }
     s->d_scale.ensure(1);
     s->d_norm2.ensure(1);
     // mass matrix with Dirichlet conditions (fe_model.rs:490-515)
@@ -238,13 +248,21 @@ static double sim_eval(Sim& s, double dt, const double* w) {
 static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double* rhs, double* sol) {
     Model& m = *s.model;
     launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
-    launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_uu.get(), s.mixed);
+    launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_uu.get(), s.mixed_mode == 1);
     const double alpha = -dt * dt;
-    if (s.mixed) {
+    if (s.mixed_mode == 1) {
         launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -alpha, s.d_M.get(), true, m.stream);
         launch_dirichlet(m.pattern, s.d_J32.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
                          m.stream);
         if (s.schwarz) schwarz_update(*s.schwarz, s.d_J32.get());
+    } else if (s.mixed_mode == 2) {
+        // FP64 Jacobian (for the refinement residual and the Schwarz factorizations), FP32 copy
+        // of its values for the CG operator; P is stored in FP32
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream);
+        launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
+                         m.stream);
+        launch_f64_to_f32(m, s.d_J.get(), s.d_J32.get(), m.pattern.nnz());
+        if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
     } else {
         launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream);
         launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
@@ -292,11 +310,40 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
                      : pcg_solve_harvest(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
                                          s.d.cg_max_iter, &it, &h);
         learned_harvest_solve(*L, sol, h.taken);
+    } else if (s.mixed_mode == 2) {
+        // iterative refinement: the CG on the FP32 operator solves for a correction, the residual
+        // is recomputed in FP64 (rhs - J sol) until it meets the same relative tolerance as the
+        // plain solve; the CG iterations of all passes share the plain solve's iteration limit
+        const int64_t n = 3 * m.nv;
+        const double bnorm = std::sqrt(device_dot(m, s.cg, rhs, rhs, s.d_norm2.get(), s.h_norm2.p));
+        HIP_CHECK(hipMemcpyAsync(s.d_res.get(), rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+        st = HOEFEM_OK;
+        for (int pass = 0; pass < 8; ++pass) {
+            int64_t ip = 0;
+            HIP_CHECK(hipMemsetAsync(s.d_dx.get(), 0, sizeof(double) * n, m.stream));
+            const int64_t budget = s.d.cg_max_iter >= 0 ? std::max<int64_t>(1, s.d.cg_max_iter - it) : -1;
+            st = pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), s.d_res.get(), s.d_dx.get(), s.d.cg_tolerance,
+                           budget, &ip);
+            it += ip;
+            launch_add(m, sol, s.d_dx.get(), n);
+            if (st != HOEFEM_OK && st != HOEFEM_ERROR_CG_MAX_ITERATIONS) break;
+            HIP_CHECK(hipMemcpyAsync(s.d_res.get(), rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+            launch_spmv(m.pattern, s.d_J.get(), 1.0, s.d_res.get(), -1.0, sol, m.stream);
+            s.refinements += 1;
+            const double rn = std::sqrt(device_dot(m, s.cg, s.d_res.get(), s.d_res.get(), s.d_norm2.get(), s.h_norm2.p));
+            if (rn <= s.d.cg_tolerance * bnorm || bnorm == 0.0) {
+                st = HOEFEM_OK;
+                break;
+            }
+            if (s.d.cg_max_iter >= 0 && it >= s.d.cg_max_iter) {
+                st = HOEFEM_ERROR_CG_MAX_ITERATIONS;
+                break;
+            }
+        }
+    } else if (s.mixed_mode == 1) {
+        st = pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter, &it);
     } else {
-        st = s.mixed ? pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance,
-                                 s.d.cg_max_iter, &it)
-                     : pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter,
-                                 &it);
+        st = pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter, &it);
     }
     s.cg_total += it;
     if (st == HOEFEM_ERROR_CG_MAX_ITERATIONS || st == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) st = HOEFEM_OK;

@@ -542,6 +542,8 @@ template void launch_block_gather<float, float>(const NodePattern&, const BlockS
                                                 double, const double*, bool, hipStream_t);
 template void launch_block_gather<double, float>(const NodePattern&, const BlockScatter&, const double*, float*, int,
                                                  double, const double*, bool, hipStream_t);
+template void launch_block_gather<float, double>(const NodePattern&, const BlockScatter&, const float*, double*, int,
+                                                 double, const double*, bool, hipStream_t);
 
 void launch_force_gather(Model& m, double* f) {
     if (m.nv == 0) return;

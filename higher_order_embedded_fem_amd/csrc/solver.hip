@@ -959,6 +959,39 @@ __global__ void k_fill(int64_t n, double* __restrict__ a, double value) {
     if (i < n) a[i] = value;
 }
 
+// partial sums of a.b over a grid-stride loop (one per block)
+__global__ __launch_bounds__(CG_THREADS) void k_dot_partials(int64_t n, const double* __restrict__ a,
+                                                             const double* __restrict__ b, double* __restrict__ partials) {
+    double part[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * CG_THREADS)
+        part[0] += a[i] * b[i];
+    block_partials<1>(part, partials);
+}
+
+// a.b on the model stream (deterministic), returned to the host
+double device_dot(Model& m, CgWork& w, const double* a, const double* b, double* d_out, double* h_out) {
+    const int64_t n = 3 * m.nv;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + CG_THREADS - 1) / CG_THREADS, 512));
+    w.ensure(n, (int)g + 1);
+    k_dot_partials<<<dim3(g), dim3(CG_THREADS), 0, m.stream>>>(n, a, b, w.partials.get());
+    k_sum_partials<<<dim3(1), dim3(CG_THREADS), 0, m.stream>>>(w.partials.get(), (int)g, d_out);
+    check_launch("device_dot");
+    HIP_CHECK(hipMemcpyAsync(h_out, d_out, sizeof(double), hipMemcpyDeviceToHost, m.stream));
+    HIP_CHECK(hipStreamSynchronize(m.stream));
+    return *h_out;
+}
+
+__global__ void k_add(int64_t n, double* __restrict__ a, const double* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] += b[i];
+}
+
+void launch_add(Model& m, double* a, const double* b, int64_t n) {
+    if (n <= 0) return;
+    k_add<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream>>>(n, a, b);
+    check_launch("k_add");
+}
+
 void launch_f64_to_f32(Model& m, const double* a, float* b, int64_t n) {
     if (n <= 0) return;
     k_f64_to_f32<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream>>>(n, a, b);

@@ -230,7 +230,7 @@ class NewtonCgIntegrator:
         d.max_newton_iter = max_newton_iter
         d.cg_tolerance = cg_tolerance
         d.cg_max_iter = cg_max_iter
-        d.mixed_precision = 1 if mixed_precision else 0
+        d.mixed_precision = int(mixed_precision)  # False/0, True/1 or 2 (see hoefem_sim_desc)
         h = C.c_void_p()
         check(L.hoefem_sim_create(model.h, C.byref(d), C.byref(h)))
         self.h = h

@@ -186,7 +186,11 @@ typedef struct hoefem_sim_desc {
                                               the Jacobian values, its SpMV and the Schwarz operator in FP32
                                               (FP32 accumulation); residual, forces, line search, CG dots and
                                               norms and the Schwarz eigensolves stay FP64 (Newton is the FP64
-                                              refinement loop around the FP32 linear operator). */
+                                              refinement loop around the FP32 linear operator). 2: FP64
+                                              tangents, Jacobian and Schwarz factorizations; the CG runs on
+                                              FP32 copies of the Jacobian and Schwarz values inside an FP64
+                                              iterative refinement of each linear solve (same relative
+                                              tolerance and iteration limit as the plain CG). */
 } hoefem_sim_desc;
 
 typedef struct hoefem_step_stats {

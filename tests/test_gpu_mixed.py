@@ -1,5 +1,6 @@
-"""Mixed precision (FP32 element tangents, FP32 Jacobian values / SpMV / Schwarz operator; FP64
-residual, forces, line search, CG dots and Schwarz eigensolves) against the FP64 oracle.
+"""Mixed precision against the FP64 oracle. Mode 1: FP32 element tangents, FP32 Jacobian values /
+SpMV / Schwarz operator; FP64 residual, forces, line search, CG dots and Schwarz eigensolves.
+Mode 2: FP64 tangents and Jacobian, the CG on FP32 copies inside FP64 iterative refinement.
 
 The Newton residual is evaluated in FP64, so the mixed step reaches the same residual
 tolerance as the FP64 step; the displacement then differs from the oracle by at most what
@@ -33,17 +34,18 @@ def rel_mass(om, sc, a, b):
     return float(np.sqrt(max(d @ Md, 0.0)) / max(np.sqrt(b @ Mb), 1e-300))
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("make", [lambda: scenes.cube_scene(3, scenes.TET10),
                                   lambda: scenes.cylinder_scene(8, scenes.HEX20, 8),
                                   lambda: scenes.cylinder_scene(6, scenes.HEX27, 27)])
-def test_mixed_step_matches_fp64_oracle(make):
+def test_mixed_step_matches_fp64_oracle(make, mode):
     from higher_order_embedded_fem_amd import fem
 
     sc = make()
     om = O.Model(sc)
     osim = O.Sim(om, newton_tolerance_factor=1e-8)
     gm = fem.ElasticityModel(sc)
-    gsim = fem.NewtonCgIntegrator(gm, sc, newton_tolerance_factor=1e-8, mixed_precision=True)
+    gsim = fem.NewtonCgIntegrator(gm, sc, newton_tolerance_factor=1e-8, mixed_precision=mode)
     u_o, v_o = np.zeros(sc.ndof), np.zeros(sc.ndof)
     u_g = torch.zeros(sc.ndof, dtype=torch.float64, device="cuda")
     v_g = torch.zeros_like(u_g)
@@ -53,7 +55,7 @@ def test_mixed_step_matches_fp64_oracle(make):
         assert st["final_residual"] <= st["tolerance"]
         ug = u_g.cpu().numpy()
         err, err_m = rel(ug, u_o), rel_mass(om, sc, ug, u_o)
-        print(f"{sc.name} step {k}: mixed-vs-oracle u rel-L2 {err:.3e}, M-norm {err_m:.3e}, "
+        print(f"{sc.name} mode {mode} step {k}: mixed-vs-oracle u rel-L2 {err:.3e}, M-norm {err_m:.3e}, "
               f"residual {st['final_residual']:.3e} (tol {st['tolerance']:.3e}), newton {st['newton_iterations']}, "
               f"cg {st['cg_iterations']}")
         assert err_m < 1e-6
