@@ -89,7 +89,22 @@ __global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t*
 // (invert_element_block, schwarz_precond.rs:329-342). Rotations follow Golub & Van Loan,
 // "Matrix Computations", sym.schur2.
 constexpr int EIG_THREADS = 256;
-constexpr double SPD_COND_LIMIT = 1e11;
+// kappa_2 bound below which the eigenvalue clamp max(|lambda|, 1e-12 max|lambda|) is provably
+// inactive, with a 2x margin for the rounding of the computed inverse (relative error ~kappa eps)
+constexpr double SPD_COND_LIMIT = 5e11;
+
+__device__ inline double block_max(double v, double* s_red) {
+    const int t = threadIdx.x;
+    s_red[t] = v;
+    __syncthreads();
+    for (int s = EIG_THREADS / 2; s > 0; s >>= 1) {
+        if (t < s) s_red[t] = fmax(s_red[t], s_red[t + s]);
+        __syncthreads();
+    }
+    const double r = s_red[0];
+    __syncthreads();
+    return r;
+}
 
 __device__ inline double block_sum(double v, double* s_red) {
     const int t = threadIdx.x;
@@ -106,10 +121,11 @@ __device__ inline double block_sum(double v, double* s_red) {
 
 // Gauss-Jordan inversion of a symmetric block in LDS without pivoting. Each pivot is a
 // diagonal entry of a Schur complement, so all pivots are positive iff B is positive
-// definite. The block is accepted when they are and |B|_F |B^-1|_F < SPD_COND_LIMIT, which
-// bounds kappa_2(B) below the 1e12 of the eigenvalue clamp: then B^+ = B^-1 and the
-// symmetrized inverse replaces the block. Otherwise fallback[b] = 1 and the block is kept
-// for the eigen path.
+// definite. The block is accepted when they are and
+//     min(|B|_F |B^-1|_F, |B|_inf |B^-1|_inf) < SPD_COND_LIMIT,
+// both upper bounds of kappa_2(B) for symmetric B, which keeps kappa_2 below the 1e12 of the
+// eigenvalue clamp: then B^+ = B^-1 and the symmetrized inverse replaces the block. Otherwise
+// fallback[b] = 1 and the block is kept for the eigen path.
 __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __restrict__ msz,
                                                              const int64_t* __restrict__ val_off,
                                                              double* __restrict__ blocks, int* __restrict__ fallback,
@@ -132,6 +148,13 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
             loc += v * v;
         }
     const double nb2 = block_sum(loc, s_red);
+    double rs = 0.0;  // max absolute row sum (infinity norm)
+    for (int i = t; i < dim; i += EIG_THREADS) {
+        double a = 0.0;
+        for (int j = 0; j < dim; ++j) a += fabs(A[i * ld + j]);
+        rs = fmax(rs, a);
+    }
+    const double nbinf = block_max(rs, s_red);
     bool bad = false;
     for (int k = 0; k < dim; ++k) {
         for (int i = t; i < dim; i += EIG_THREADS) {
@@ -166,7 +189,15 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
     for (int i = wid; i < dim; i += NW)
         for (int j = lane; j < dim; j += 64) loc += A[i * ld + j] * A[i * ld + j];
     const double ni2 = block_sum(loc, s_red);
-    if (!(sqrt(nb2) * sqrt(ni2) < SPD_COND_LIMIT)) {
+    rs = 0.0;
+    for (int i = t; i < dim; i += EIG_THREADS) {
+        double a = 0.0;
+        for (int j = 0; j < dim; ++j) a += fabs(A[i * ld + j]);
+        rs = fmax(rs, a);
+    }
+    const double niinf = block_max(rs, s_red);
+    const double cert = fmin(sqrt(nb2) * sqrt(ni2), nbinf * niinf);
+    if (!(cert < SPD_COND_LIMIT)) {
         if (t == 0) fallback[b] = 1;
         return;
     }
