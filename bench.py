@@ -212,6 +212,9 @@ def main():
     lanes = None if args.no_autotune else sim.autotune()
     print(f"[bench] {sc.name}: {sc.num_elements} elements, {sc.ndof} dofs, nnz {model.nnz}, "
           f"setup {time.time() - t_setup:.1f}s, SpMV lanes per node (J, P) {lanes}", file=sys.stderr, flush=True)
+    if lanes is not None:
+        print(f"[bench] autotune us per CG iteration, rows J lanes 8/16/32, columns P lanes 8/16/32:\n"
+              f"{np.array2string(sim.autotune_us, precision=2)}", file=sys.stderr, flush=True)
     precision = {0: "fp64", 1: "mixed", 2: "mixed-refined"}[args.mode]
 
     def run(steps, record=None):

@@ -219,7 +219,7 @@ def load(path: str = LIB_PATH):
     L.hoefem_pcg.argtypes = [vp, vp, vp, vp, vp, f64, i64, C.POINTER(i64)]
     L.hoefem_model_autotune.argtypes = [vp, C.POINTER(i32), vp]
     L.hoefem_schwarz_autotune.argtypes = [vp, C.POINTER(i32), vp]
-    L.hoefem_sim_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+    L.hoefem_sim_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), vp]
     L.hoefem_profile_enable.argtypes = [vp, i32]
     L.hoefem_profile_read.argtypes = [vp, C.POINTER(i64), C.POINTER(f64), C.POINTER(i64), C.POINTER(f64)]
     L.hoefem_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]

@@ -245,7 +245,9 @@ class NewtonCgIntegrator:
         a different choice changes the reduction order, so runs stay bitwise reproducible only
         with a fixed choice)."""
         a, b = C.c_int32(), C.c_int32()
-        check(_lib.load().hoefem_sim_autotune(self.h, C.byref(a), C.byref(b)))
+        t = np.zeros(9)
+        check(_lib.load().hoefem_sim_autotune(self.h, C.byref(a), C.byref(b), t.ctypes.data))
+        self.autotune_us = t.reshape(3, 3)  # [lanes of J in (8, 16, 32)][lanes of P in (8, 16, 32)]
         return a.value, b.value
 
     @property

@@ -152,8 +152,13 @@ hoefem_status hoefem_pcg(hoefem_model* model, const double* A_values, hoefem_sch
  * launch. candidate_us (optional, [3]) receives the measured average launch times. */
 hoefem_status hoefem_model_autotune(hoefem_model* model, int32_t* lanes_per_node, double* candidate_us);
 hoefem_status hoefem_schwarz_autotune(hoefem_schwarz* s, int32_t* lanes_per_node, double* candidate_us);
-/* the same for the Jacobian and Schwarz patterns of a simulation (lanes_preconditioner = 0 without Schwarz) */
-hoefem_status hoefem_sim_autotune(hoefem_sim* sim, int32_t* lanes_matrix, int32_t* lanes_preconditioner);
+/* the same for a simulation's CG: every (lanes of the Jacobian kernel, lanes of the preconditioner
+ * kernel) pair in {8, 16, 32}^2 runs the best of three 96-iteration solves of J = M with the Schwarz
+ * operator of M; the fastest pair is kept (lanes_preconditioner = 0 without Schwarz). candidate_us
+ * (optional, [9], index 3 * a + b) receives the time per CG iteration of each pair. J and P are
+ * overwritten (the next step assembles them again). */
+hoefem_status hoefem_sim_autotune(hoefem_sim* sim, int32_t* lanes_matrix, int32_t* lanes_preconditioner,
+                                  double* candidate_us);
 
 /* Kernel timing (no counterpart in the reference, which times phases with coarse_prof scopes,
  * simulation_toolbox/src/fem/newton_cg.rs:685-764): when enabled, every launch of the CG
