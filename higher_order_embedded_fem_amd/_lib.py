@@ -18,9 +18,12 @@ EXPORTED = [
     "hoefem_nodes_per_element",
     "hoefem_lame_from_young_poisson",
     "hoefem_material_eval",
+    "hoefem_material_eval_kind",
+    "hoefem_rotation_svd",
     "hoefem_model_create",
     "hoefem_model_destroy",
     "hoefem_model_set_stream",
+    "hoefem_model_set_material",
     "hoefem_model_ndof",
     "hoefem_model_nnz",
     "hoefem_model_pattern",
@@ -190,10 +193,13 @@ def load(path: str = LIB_PATH):
     L.hoefem_lame_from_young_poisson.argtypes = [f64, f64, C.POINTER(f64), C.POINTER(f64)]
     L.hoefem_lame_from_young_poisson.restype = None
     L.hoefem_material_eval.argtypes = [f64, f64, i64, vp, vp, vp]
+    L.hoefem_material_eval_kind.argtypes = [i32, f64, f64, f64, i64, vp, vp, vp]
+    L.hoefem_rotation_svd.argtypes = [i64, vp, vp, vp, vp]
     L.hoefem_model_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(vp)]
     L.hoefem_model_destroy.argtypes = [vp]
     L.hoefem_model_destroy.restype = None
     L.hoefem_model_set_stream.argtypes = [vp, vp]
+    L.hoefem_model_set_material.argtypes = [vp, i32, f64]
     L.hoefem_model_ndof.argtypes = [vp]
     L.hoefem_model_ndof.restype = i64
     L.hoefem_model_nnz.argtypes = [vp]

@@ -494,6 +494,13 @@ static hoefem_status guarded(F&& f) {
     }
 }
 
+static void check_material(int32_t kind, double threshold) {
+    if (kind != dev::MAT_SNH && kind != dev::MAT_INVERTIBLE_SNH && kind != dev::MAT_PROJECTED_SNH)
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unknown material kind");
+    if (kind == dev::MAT_INVERTIBLE_SNH && !(threshold >= 0.0 && std::isfinite(threshold)))
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "inversion threshold must be finite and >= 0");
+}
+
 static Model& M_(hoefem_model* m) {
     if (!m) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null model");
     return m->m;
@@ -516,7 +523,25 @@ hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const d
         if (nq < 0 || (nq > 0 && !F)) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
         double mr, lr;
         dev::snh_reparam(mu, lambda, &mr, &lr);
-        material_eval(mr, lr, nq, F, P, dPdF);
+        material_eval(dev::MAT_SNH, 0.0, mr, lr, nq, F, P, dPdF);
+    });
+}
+
+hoefem_status hoefem_material_eval_kind(int32_t kind, double inversion_threshold, double mu, double lambda, int64_t nq,
+                                       const double* F, double* P, double* dPdF) {
+    return guarded([&] {
+        if (nq < 0 || (nq > 0 && !F)) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
+        check_material(kind, inversion_threshold);
+        double mr, lr;
+        dev::snh_reparam(mu, lambda, &mr, &lr);
+        material_eval(kind, inversion_threshold, mr, lr, nq, F, P, dPdF);
+    });
+}
+
+hoefem_status hoefem_rotation_svd(int64_t nq, const double* F, double* U, double* S, double* Vt) {
+    return guarded([&] {
+        if (nq < 0 || (nq > 0 && (!F || !U || !S || !Vt))) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
+        rotation_svd_eval(nq, F, U, S, Vt);
     });
 }
 
@@ -534,6 +559,15 @@ void hoefem_model_destroy(hoefem_model* model) { delete model; }
 
 hoefem_status hoefem_model_set_stream(hoefem_model* model, void* stream) {
     return guarded([&] { M_(model).stream = (hipStream_t)stream; });
+}
+
+hoefem_status hoefem_model_set_material(hoefem_model* model, int32_t kind, double inversion_threshold) {
+    return guarded([&] {
+        Model& m = M_(model);
+        check_material(kind, inversion_threshold);
+        m.material = kind;
+        m.inversion_threshold = kind == dev::MAT_INVERTIBLE_SNH ? inversion_threshold : 0.0;
+    });
 }
 
 int64_t hoefem_model_ndof(const hoefem_model* m) { return m ? 3 * m->m.nv : 0; }

@@ -18,14 +18,14 @@ using namespace dev;
 // assemble_generalized_element_elliptic_term (fenris/src/assembly.rs:674-711) with
 // g = -P(F)^T (fenris/src/solid/assembly.rs:33-38). One 64-lane work-group per element:
 // lanes = quadrature points (phase 1), then lanes = element nodes (phase 2).
-template <int ET>
+template <int ET, int MAT>
 __global__ __launch_bounds__(64) void k_element_forces(int64_t ne, const int32_t* __restrict__ conn,
                                                        const double* __restrict__ verts,
                                                        const int32_t* __restrict__ qoff,
                                                        const double* __restrict__ qw,
-                                                       const double* __restrict__ qxi, double mu, double lambda,
-                                                       const double* __restrict__ u, double* __restrict__ fe,
-                                                       int* err) {
+                                                       const double* __restrict__ qxi, double thr,
+                                                       double mu, double lambda, const double* __restrict__ u,
+                                                       double* __restrict__ fe, int* err) {
     constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN;
     const int64_t e = blockIdx.x;
     const int t = threadIdx.x;
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(64) void k_element_forces(int64_t ne, const int32_t
                 F[4] += 1.0;
                 F[8] += 1.0;
                 double P[9];
-                ok = snh_stress(mu, lambda, F, P);
+                ok = material_stress(MAT, thr, mu, lambda, F, P);
                 if (ok) {
                     // A = (-P) * J^-T  : A(i, a) = sum_c -P(i, c) * Jinv(a, c)
                     for (int a = 0; a < 3; ++a)
@@ -115,10 +115,10 @@ __global__ __launch_bounds__(64) void k_element_forces(int64_t ne, const int32_t
 constexpr int STIFF_THREADS = 256;
 constexpr int STIFF_QC = 16;
 
-template <int ET, class T>
+template <int ET, class T, int MAT>
 __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
     int64_t ne, const int32_t* __restrict__ conn, const double* __restrict__ verts, const int32_t* __restrict__ qoff,
-    const double* __restrict__ qw, const double* __restrict__ qxi, T mu, T lambda,
+    const double* __restrict__ qw, const double* __restrict__ qxi, T thr, T mu, T lambda,
     const double* __restrict__ u, T* __restrict__ ke, int* err) {
     constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN, M3 = 3 * N;
     constexpr int PAIRS = N * (N + 1) / 2;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
                 F[0] += T(1);
                 F[4] += T(1);
                 F[8] += T(1);
-                ok = snh_dpdf(mu, lambda, F, s_C[t]);
+                ok = material_dpdf(MAT, thr, mu, lambda, F, s_C[t]);
             }
             if (!ok) {
                 atomicOr(err, KERR_SINGULAR);
@@ -452,15 +452,15 @@ __global__ void k_dirichlet_rows(const int32_t* __restrict__ bc_nodes, int64_t n
 }
 
 // ------------------------------------------------------------------ material (batched)
-__global__ void k_material(double mu, double lambda, int64_t nq, const double* __restrict__ F, double* __restrict__ P,
-                           double* __restrict__ dP, int* err) {
+__global__ void k_material(int mat, double thr, double mu, double lambda, int64_t nq, const double* __restrict__ F,
+                           double* __restrict__ P, double* __restrict__ dP, int* err) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     double f[9];
     for (int k = 0; k < 9; ++k) f[k] = F[9 * q + k];
     if (P) {
         double p[9];
-        if (!snh_stress(mu, lambda, f, p)) {
+        if (!material_stress(mat, thr, mu, lambda, f, p)) {
             atomicOr(err, KERR_SINGULAR);
             for (int k = 0; k < 9; ++k) p[k] = 0.0;
         }
@@ -468,12 +468,26 @@ __global__ void k_material(double mu, double lambda, int64_t nq, const double* _
     }
     if (dP) {
         double d[81];
-        if (!snh_dpdf(mu, lambda, f, d)) {
+        if (!material_dpdf(mat, thr, mu, lambda, f, d)) {
             atomicOr(err, KERR_SINGULAR);
             for (int k = 0; k < 81; ++k) d[k] = 0.0;
         }
         for (int k = 0; k < 81; ++k) dP[81 * q + k] = d[k];
     }
+}
+
+// rotation_svd (fenris/src/util.rs:68-94) on nq matrices: U, Vt column-major, S[3] signed.
+__global__ void k_rotation_svd(int64_t nq, const double* __restrict__ F, double* __restrict__ U,
+                               double* __restrict__ S, double* __restrict__ Vt) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    double f[9], u[9], sv[3], v[9];
+    for (int k = 0; k < 9; ++k) f[k] = F[9 * q + k];
+    rotation_svd3(f, u, sv, v);
+    for (int k = 0; k < 9; ++k) U[9 * q + k] = u[k];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) Vt[9 * q + r + 3 * c] = v[c + 3 * r];
+    for (int k = 0; k < 3; ++k) S[3 * q + k] = sv[k];
 }
 
 // ================================================================== launchers
@@ -488,12 +502,23 @@ __global__ void k_material(double mu, double lambda, int64_t nq, const double* _
         default: throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad element type");  \
     }
 
+#define HOEFEM_MATERIAL_DISPATCH(kind, ...)                                          \
+    switch (kind) {                                                                  \
+        case MAT_SNH: { constexpr int MK = MAT_SNH; __VA_ARGS__; } break;            \
+        case MAT_INVERTIBLE_SNH: { constexpr int MK = MAT_INVERTIBLE_SNH; __VA_ARGS__; } break; \
+        case MAT_PROJECTED_SNH: { constexpr int MK = MAT_PROJECTED_SNH; __VA_ARGS__; } break;   \
+        default: throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad material kind");   \
+    }
+
 void launch_element_forces(Model& m, double mu_r, double lambda_r, const double* u) {
     m.d_fe.ensure((size_t)m.ne * 3 * m.n);
     if (m.ne == 0) return;
-    HOEFEM_DISPATCH(m.type, k_element_forces<ET><<<dim3((unsigned)m.ne), dim3(64), 0, m.stream>>>(
-                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_ell.off.get(), m.q_ell.w.get(),
-                                m.q_ell.xi.get(), mu_r, lambda_r, u, m.d_fe.get(), m.d_err.get()));
+    HOEFEM_MATERIAL_DISPATCH(
+        m.material,
+        HOEFEM_DISPATCH(m.type, k_element_forces<ET, MK><<<dim3((unsigned)m.ne), dim3(64), 0, m.stream>>>(
+                                    m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_ell.off.get(), m.q_ell.w.get(),
+                                    m.q_ell.xi.get(), m.inversion_threshold, mu_r, lambda_r, u, m.d_fe.get(),
+                                    m.d_err.get())));
     check_launch("k_element_forces");
 }
 
@@ -501,14 +526,22 @@ void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const doub
     if (m.ne == 0) return;
     if (fp32) {
         m.d_ke32.ensure((size_t)m.ne * 9 * m.n * m.n);
-        HOEFEM_DISPATCH(m.type, k_element_stiffness<ET, float><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
-                                    m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
-                                    m.q_stiff.xi.get(), (float)mu_r, (float)lambda_r, u, m.d_ke32.get(), m.d_err.get()));
+        HOEFEM_MATERIAL_DISPATCH(
+            m.material,
+            HOEFEM_DISPATCH(m.type,
+                            k_element_stiffness<ET, float, MK><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
+                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
+                                m.q_stiff.xi.get(), (float)m.inversion_threshold, (float)mu_r, (float)lambda_r, u,
+                                m.d_ke32.get(), m.d_err.get())));
     } else {
         m.d_ke.ensure((size_t)m.ne * 9 * m.n * m.n);
-        HOEFEM_DISPATCH(m.type, k_element_stiffness<ET, double><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
-                                    m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
-                                    m.q_stiff.xi.get(), mu_r, lambda_r, u, m.d_ke.get(), m.d_err.get()));
+        HOEFEM_MATERIAL_DISPATCH(
+            m.material,
+            HOEFEM_DISPATCH(m.type,
+                            k_element_stiffness<ET, double, MK><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
+                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
+                                m.q_stiff.xi.get(), m.inversion_threshold, mu_r, lambda_r, u, m.d_ke.get(),
+                                m.d_err.get())));
     }
     check_launch("k_element_stiffness");
 }
@@ -645,18 +678,26 @@ int autotune_spmv(NodePattern& pat, hipStream_t s, double* times_us) {
     return best;
 }
 
-void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF) {
+void material_eval(int mat, double thr, double mu_r, double lambda_r, int64_t nq, const double* F, double* P,
+                   double* dPdF) {
     DevBuf<int> err;
     err.alloc(1);
     HIP_CHECK(hipMemset(err.get(), 0, sizeof(int)));
     if (nq > 0) {
-        hipLaunchKernelGGL(k_material, dim3((unsigned)((nq + 127) / 128)), dim3(128), 0, nullptr, mu_r, lambda_r, nq, F,
+        hipLaunchKernelGGL(k_material, dim3((unsigned)((nq + 127) / 128)), dim3(128), 0, nullptr, mat, thr, mu_r, lambda_r, nq, F,
                            P, dPdF, err.get());
         check_launch("k_material");
     }
     int h = 0;
     HIP_CHECK(hipMemcpy(&h, err.get(), sizeof(int), hipMemcpyDeviceToHost));
     if (h & KERR_SINGULAR) throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient");
+}
+
+void rotation_svd_eval(int64_t nq, const double* F, double* U, double* S, double* Vt) {
+    if (nq == 0) return;
+    hipLaunchKernelGGL(k_rotation_svd, dim3((unsigned)((nq + 127) / 128)), dim3(128), 0, nullptr, nq, F, U, S, Vt);
+    check_launch("k_rotation_svd");
+    HIP_CHECK(hipDeviceSynchronize());
 }
 
 int read_and_clear_err(Model& m) {

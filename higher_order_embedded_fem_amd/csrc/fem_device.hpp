@@ -212,6 +212,25 @@ __device__ inline bool inv3(const T* m, T* out) {
     return true;
 }
 
+// Cofactor matrix cof(m) = det(m) m^-T (column-major), computed from the 2x2 minors without a
+// division: J F^-T enters the stress and tangent, and on a nearly singular F the two
+// determinants of inverse-times-determinant would not cancel exactly once contracted to FMAs.
+template <class T>
+__device__ inline void cof3(const T* m, T* C) {
+    const T m11 = m[0], m12 = m[3], m13 = m[6];
+    const T m21 = m[1], m22 = m[4], m23 = m[7];
+    const T m31 = m[2], m32 = m[5], m33 = m[8];
+    C[0] = m22 * m33 - m32 * m23;
+    C[3] = -(m21 * m33 - m31 * m23);
+    C[6] = m21 * m32 - m31 * m22;
+    C[1] = m13 * m32 - m33 * m12;
+    C[4] = m11 * m33 - m31 * m13;
+    C[7] = m12 * m31 - m32 * m11;
+    C[2] = m12 * m23 - m22 * m13;
+    C[5] = m13 * m21 - m23 * m11;
+    C[8] = m11 * m22 - m21 * m12;
+}
+
 // reparametrize_lame_for_stable_neo_hookean (fenris/src/solid/materials.rs:243-253), d = 3
 __host__ __device__ inline void snh_reparam(double mu, double lambda, double* mu_r, double* lambda_r) {
     const double d = 3.0;
@@ -226,12 +245,12 @@ __device__ inline bool snh_stress(T mu, T lambda, const T* F, T* P) {
     T I_C = T(0);
     for (int i = 0; i < 9; ++i) I_C += F[i] * F[i];
     const T J = det3(F);
-    T Finv[9];
-    if (!inv3(F, Finv)) return false;
+    if (J == T(0)) return false;  // singular F (the reference's try_inverse fails)
+    T g[9];                       // J F^-T
+    cof3(F, g);
     const T s1 = mu * (T(1) - T(1) / (I_C + T(1)));
     const T s2 = lambda * (J - alpha);
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) P[r + 3 * c] = F[r + 3 * c] * s1 + (Finv[c + 3 * r] * J) * s2;
+    for (int k = 0; k < 9; ++k) P[k] = F[k] * s1 + g[k] * s2;
     return true;
 }
 
@@ -245,11 +264,9 @@ __device__ inline bool snh_dpdf(T mu, T lambda, const T* F, T* dp) {
     const T J = det3(F);
     const T beta = I_C + T(1);
     const T beta2 = beta * beta;
-    T Finv[9];
-    if (!inv3(F, Finv)) return false;
-    T g[9];
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) g[3 * c + r] = Finv[c + 3 * r] * J;
+    if (J == T(0)) return false;
+    T g[9];  // vec(J F^-T)
+    cof3(F, g);
     const T diag = mu * (T(1) - T(1) / beta);
     const T cf = mu * T(2) / beta2;
     for (int c = 0; c < 9; ++c)
@@ -274,6 +291,236 @@ __device__ inline bool snh_dpdf(T mu, T lambda, const T* F, T* dp) {
                 dp[(r0b + r) + 9 * (c0b + c)] -= h[r + 3 * c];
             }
     }
+    return true;
+}
+
+
+// ------------------------------------------------------------------ material variants
+// The elastic models of simulation_toolbox/src/fem/fe_model.rs:35-45 that wrap Stable
+// Neo-Hookean: plain, InvertibleMaterial<StableNeoHookean> (F projected before the material,
+// fenris/src/solid/materials.rs:862-945) and ProjectedStableNeoHookean (dP/dF projected onto
+// the positive semi-definite cone, :387-441).
+constexpr int MAT_SNH = 0, MAT_INVERTIBLE_SNH = 1, MAT_PROJECTED_SNH = 2;
+
+template <class T>
+struct Eps;
+template <> struct Eps<double> { static constexpr double v = 2.220446049250313e-16; };
+template <> struct Eps<float> { static constexpr float v = 1.1920929e-7f; };
+
+// One Jacobi rotation zeroing a(p,q) of the symmetric 3x3 matrix held as a[3][3] (only the
+// entries touched are updated), accumulated into the columns p, q of V (column-major).
+template <class T>
+__device__ inline void jacobi3_rotate(T (&a)[3][3], T* V, int p, int q) {
+    const T apq = a[p][q];
+    if (apq == T(0)) return;
+    const T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
+    const T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+    const T c = T(1) / sqrt(t * t + T(1));
+    const T s = t * c;
+    a[p][p] -= t * apq;
+    a[q][q] += t * apq;
+    a[p][q] = a[q][p] = T(0);
+    const int r = 3 - p - q;
+    const T arp = a[r][p], arq = a[r][q];
+    a[r][p] = a[p][r] = c * arp - s * arq;
+    a[r][q] = a[q][r] = s * arp + c * arq;
+    for (int k = 0; k < 3; ++k) {
+        const T vp = V[k + 3 * p], vq = V[k + 3 * q];
+        V[k + 3 * p] = c * vp - s * vq;
+        V[k + 3 * q] = s * vp + c * vq;
+    }
+}
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi sweeps. A column-major (symmetric);
+// w = eigenvalues (unsorted), Q = eigenvectors as columns.
+template <class T>
+__device__ inline void sym3_eigen(const T* A, T* w, T* Q) {
+    T a[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) a[r][c] = T(0.5) * (A[r + 3 * c] + A[c + 3 * r]);
+    for (int k = 0; k < 9; ++k) Q[k] = (k % 4 == 0) ? T(1) : T(0);
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        const T off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        const T dia = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (off <= Eps<T>::v * Eps<T>::v * T(1e-2) * dia || off == T(0)) break;
+        jacobi3_rotate(a, Q, 0, 1);
+        jacobi3_rotate(a, Q, 0, 2);
+        jacobi3_rotate(a, Q, 1, 2);
+    }
+    w[0] = a[0][0];
+    w[1] = a[1][1];
+    w[2] = a[2][2];
+}
+
+// rotation_svd (fenris/src/util.rs:68-94): F = U diag(s) V^T with U, V proper rotations and
+// s sorted by magnitude, s[0] >= s[1] >= |s[2]|, the sign of det F carried by s[2].
+// V from the eigenvectors of F^T F; U from F V (Gram-Schmidt on the two largest columns,
+// the third as their cross product), so s[2] = u2 . (F v2) keeps its sign and absolute accuracy.
+template <class T>
+__device__ inline void rotation_svd3(const T* F, T* U, T* s, T* V) {
+    T C[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            C[r + 3 * c] = F[3 * r] * F[3 * c] + F[3 * r + 1] * F[3 * c + 1] + F[3 * r + 2] * F[3 * c + 2];
+    T w[3];
+    sym3_eigen(C, w, V);
+    // sort eigenpairs descending
+    auto swap_cols = [&](int i, int j) {
+        const T tw = w[i];
+        w[i] = w[j];
+        w[j] = tw;
+        for (int k = 0; k < 3; ++k) {
+            const T tv = V[k + 3 * i];
+            V[k + 3 * i] = V[k + 3 * j];
+            V[k + 3 * j] = tv;
+        }
+    };
+    if (w[0] < w[1]) swap_cols(0, 1);
+    if (w[0] < w[2]) swap_cols(0, 2);
+    if (w[1] < w[2]) swap_cols(1, 2);
+    if (det3(V) < T(0))
+        for (int k = 0; k < 3; ++k) V[k + 6] = -V[k + 6];
+    T b[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            b[r + 3 * c] = F[r] * V[3 * c] + F[r + 3] * V[3 * c + 1] + F[r + 6] * V[3 * c + 2];
+    // u0
+    T n0 = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    if (n0 > T(0)) {
+        for (int k = 0; k < 3; ++k) U[k] = b[k] / n0;
+    } else {
+        U[0] = T(1);
+        U[1] = T(0);
+        U[2] = T(0);
+    }
+    // u1 = b1 - (u0 . b1) u0, normalized; any unit vector orthogonal to u0 if that vanishes
+    const T d01 = U[0] * b[3] + U[1] * b[4] + U[2] * b[5];
+    T u1[3] = {b[3] - d01 * U[0], b[4] - d01 * U[1], b[5] - d01 * U[2]};
+    T n1 = sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    if (!(n1 > Eps<T>::v * n0) || n1 == T(0)) {
+        // the coordinate axis least aligned with u0, orthogonalized
+        int ax = 0;
+        if (fabs(U[1]) < fabs(U[ax])) ax = 1;
+        if (fabs(U[2]) < fabs(U[ax])) ax = 2;
+        T e[3] = {T(0), T(0), T(0)};
+        e[ax] = T(1);
+        const T d = U[ax];
+        for (int k = 0; k < 3; ++k) u1[k] = e[k] - d * U[k];
+        n1 = sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    }
+    for (int k = 0; k < 3; ++k) U[3 + k] = u1[k] / n1;
+    U[6] = U[1] * U[5] - U[2] * U[4];
+    U[7] = U[2] * U[3] - U[0] * U[5];
+    U[8] = U[0] * U[4] - U[1] * U[3];
+    for (int i = 0; i < 3; ++i) s[i] = U[3 * i] * b[3 * i] + U[3 * i + 1] * b[3 * i + 1] + U[3 * i + 2] * b[3 * i + 2];
+}
+
+// project_deformation_gradient (fenris/src/solid/materials.rs:876-915): signed singular
+// values below the threshold are raised to it (an inverted F has its smallest one negative);
+// F is recomposed only when something changed. Returns whether it did.
+template <class T>
+__device__ inline bool project_deformation_gradient(const T* F, T threshold, T* Fp) {
+    T U[9], s[3], V[9];
+    rotation_svd3(F, U, s, V);
+    bool changed = false;
+    for (int i = 0; i < 3; ++i)
+        if (s[i] < threshold) {
+            s[i] = threshold;
+            changed = true;
+        }
+    if (!changed) {
+        for (int k = 0; k < 9; ++k) Fp[k] = F[k];
+        return false;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            Fp[r + 3 * c] = U[r] * s[0] * V[c] + U[r + 3] * s[1] * V[c + 3] + U[r + 6] * s[2] * V[c + 6];
+    return true;
+}
+
+// ProjectedStableNeoHookeanMaterial tangent (fenris/src/solid/materials.rs:427-441 with the
+// analytic eigensystem of :561-687): dp holds build_stable_neohookean_dp_df(F); every
+// eigenpair with a negative eigenvalue is removed, dp -= lambda_i q_i q_i^T. Eigenpairs, with
+// F = U diag(s) V^T (rotation SVD), mu_t = mu (1 - 1/(I_C + 1)), k the index not in {i, j}:
+//   twist  mu_t + lambda (J - alpha) s_k,  q = vec(u_i v_j^T - u_j v_i^T) / sqrt 2
+//   flip   mu_t - lambda (J - alpha) s_k,  q = vec(u_i v_j^T + u_j v_i^T) / sqrt 2
+//   scale  eigenpairs (w, e) of the 3x3 matrix A of the scaling directions, q = vec(U diag(e) V^T)
+template <class T>
+__device__ inline void snh_dpdf_project_psd(T mu, T lambda, const T* F, T* dp) {
+    T U[9], s[3], V[9];
+    rotation_svd3(F, U, s, V);
+    const T alpha = T(1) + mu / lambda * T(0.75);
+    T I_C = T(0);
+    for (int i = 0; i < 9; ++i) I_C += F[i] * F[i];
+    const T J = det3(F);
+    const T mu_t = mu * (T(1) - T(1) / (I_C + T(1)));
+    const T sv_scale = lambda * (J - alpha);
+    auto remove = [&](T ev, const T* q) {
+        for (int c = 0; c < 9; ++c) {
+            const T f = ev * q[c];
+            for (int r = 0; r < 9; ++r) dp[r + 9 * c] -= f * q[r];
+        }
+    };
+    const int PI[3] = {1, 0, 0}, PJ[3] = {2, 2, 1};
+    const T h = T(0.70710678118654752440);
+    for (int k = 0; k < 3; ++k) {
+        const int i = PI[k], j = PJ[k];
+        for (int sign = 0; sign < 2; ++sign) {
+            const T ev = sign == 0 ? mu_t + s[k] * sv_scale : mu_t - s[k] * sv_scale;
+            if (!(ev < T(0))) continue;
+            T q[9];
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r) {
+                    const T a = U[r + 3 * i] * V[c + 3 * j], b = U[r + 3 * j] * V[c + 3 * i];
+                    q[r + 3 * c] = h * (sign == 0 ? a - b : a + b);
+                }
+            remove(ev, q);
+        }
+    }
+    const T beta = I_C + T(1);
+    const T m2 = T(2) * mu / (beta * beta);
+    const T gamma = lambda * (T(2) * J - alpha);
+    T A[9];
+    A[0] = mu_t + m2 * s[0] * s[0] + lambda * s[1] * s[1] * s[2] * s[2];
+    A[4] = mu_t + m2 * s[1] * s[1] + lambda * s[0] * s[0] * s[2] * s[2];
+    A[8] = mu_t + m2 * s[2] * s[2] + lambda * s[0] * s[0] * s[1] * s[1];
+    A[3] = A[1] = s[2] * gamma + m2 * s[0] * s[1];
+    A[6] = A[2] = s[1] * gamma + m2 * s[0] * s[2];
+    A[7] = A[5] = s[0] * gamma + m2 * s[1] * s[2];
+    T w[3], E[9];
+    sym3_eigen(A, w, E);
+    for (int m = 0; m < 3; ++m) {
+        if (!(w[m] < T(0))) continue;
+        T q[9];
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r)
+                q[r + 3 * c] = U[r] * E[3 * m] * V[c] + U[r + 3] * E[3 * m + 1] * V[c + 3] +
+                               U[r + 6] * E[3 * m + 2] * V[c + 6];
+        remove(w[m], q);
+    }
+}
+
+// The stress of the selected model (kind as in MAT_*).
+template <class T>
+__device__ inline bool material_stress(int kind, T threshold, T mu, T lambda, const T* F, T* P) {
+    if (kind == MAT_INVERTIBLE_SNH) {
+        T Fp[9];
+        project_deformation_gradient(F, threshold, Fp);
+        return snh_stress(mu, lambda, Fp, P);
+    }
+    return snh_stress(mu, lambda, F, P);
+}
+
+// The tangent dP/dF of the selected model.
+template <class T>
+__device__ inline bool material_dpdf(int kind, T threshold, T mu, T lambda, const T* F, T* dp) {
+    if (kind == MAT_INVERTIBLE_SNH) {
+        T Fp[9];
+        project_deformation_gradient(F, threshold, Fp);
+        return snh_dpdf(mu, lambda, Fp, dp);
+    }
+    if (!snh_dpdf(mu, lambda, F, dp)) return false;
+    if (kind == MAT_PROJECTED_SNH) snh_dpdf_project_psd(mu, lambda, F, dp);
     return true;
 }
 

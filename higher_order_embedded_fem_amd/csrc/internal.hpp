@@ -157,6 +157,8 @@ struct Model {
     NodePattern pattern;
     BlockScatter scatter;                         // element blocks -> pattern, color order
     double mass_reg = 0.0;
+    int material = 0;                 // dev::MAT_* (elastic model wrapping Stable Neo-Hookean)
+    double inversion_threshold = 0.0; // InvertibleMaterial threshold
     hipStream_t stream = nullptr;
     DevBuf<double> d_vertices;
     DevBuf<int32_t> d_conn;  // SoA: [k * E + e]
@@ -248,7 +250,9 @@ int spmv_group_size(const NodePattern& pat);
 // Times the node-blocked SpMV with 8, 16 and 32 lanes per node on this pattern and keeps the
 // fastest in pat.tuned_group; returns it. Per-candidate averages (us) go to times_us[3] if given.
 int autotune_spmv(NodePattern& pat, hipStream_t s, double* times_us);
-void material_eval(double mu_r, double lambda_r, int64_t nq, const double* F, double* P, double* dPdF);
+void rotation_svd_eval(int64_t nq, const double* F, double* U, double* S, double* Vt);
+void material_eval(int mat, double thr, double mu_r, double lambda_r, int64_t nq, const double* F, double* P,
+                   double* dPdF);
 int read_and_clear_err(Model& m);  // synchronizes the model stream
 
 // ------------------------------------------------------------------ Schwarz (schwarz.hip)

@@ -38,6 +38,9 @@ def _np(a, dtype):
 class StableNeoHookeanMaterial:
     """StableNeoHookeanMaterial with Lame parameters from YoungPoisson (reference convention)."""
 
+    kind = 0  # HOEFEM_STABLE_NEO_HOOKEAN
+    threshold = 0.0
+
     def __init__(self, young: float = None, poisson: float = None, mu: float = None, lam: float = None):
         if mu is None:
             mu, lam = lame_from_young_poisson(young, poisson)
@@ -50,12 +53,43 @@ class StableNeoHookeanMaterial:
         Fc = F.transpose(1, 2).contiguous()  # column-major storage
         P = torch.empty_like(Fc) if stress else None
         D = torch.empty((nq, 9, 9), dtype=torch.float64, device=F.device) if tangent else None
-        check(L.hoefem_material_eval(self.mu, self.lam, nq, Fc.data_ptr(), P.data_ptr() if P is not None else None,
-                                     D.data_ptr() if D is not None else None))
+        check(L.hoefem_material_eval_kind(self.kind, self.threshold, self.mu, self.lam, nq, Fc.data_ptr(),
+                                          P.data_ptr() if P is not None else None,
+                                          D.data_ptr() if D is not None else None))
         torch.cuda.synchronize()
         Pout = P.transpose(1, 2) if P is not None else None
         Dout = D.transpose(1, 2) if D is not None else None  # [q, row, col]
         return Pout, Dout
+
+
+class InvertibleMaterial(StableNeoHookeanMaterial):
+    """InvertibleMaterial::new(StableNeoHookeanMaterial, threshold) (fenris/src/solid/materials.rs:862-945):
+    F is projected (signed singular values raised to the threshold) before the stress and the tangent.
+    The reference's default Material is this with threshold 1e-6 (simulation_toolbox/src/fem/fe_model.rs:145-158)."""
+
+    kind = 1  # HOEFEM_INVERTIBLE_STABLE_NEO_HOOKEAN
+
+    def __init__(self, material: StableNeoHookeanMaterial, threshold: float = 1e-6):
+        super().__init__(mu=material.mu, lam=material.lam)
+        self.threshold = float(threshold)
+
+
+class ProjectedStableNeoHookeanMaterial(StableNeoHookeanMaterial):
+    """ProjectedStableNeoHookeanMaterial (fenris/src/solid/materials.rs:387-441): Stable Neo-Hookean stress,
+    tangent projected onto the positive semi-definite cone."""
+
+    kind = 2  # HOEFEM_PROJECTED_STABLE_NEO_HOOKEAN
+
+
+def rotation_svd(F: torch.Tensor):
+    """rotation_svd (fenris/src/util.rs:68-94) on (nq, 3, 3) device float64: returns U, S, Vt."""
+    L = _lib.load()
+    nq = F.shape[0]
+    Fc = F.transpose(1, 2).contiguous()
+    U, Vt = torch.empty_like(Fc), torch.empty_like(Fc)
+    S = torch.empty((nq, 3), dtype=torch.float64, device=F.device)
+    check(L.hoefem_rotation_svd(nq, Fc.data_ptr(), U.data_ptr(), S.data_ptr(), Vt.data_ptr()))
+    return U.transpose(1, 2), S, Vt.transpose(1, 2)
 
 
 def _quad_struct(q, keep):
@@ -122,13 +156,19 @@ class ElasticityModel:
     def zeros_matrix(self):
         return torch.zeros(self.nnz, dtype=torch.float64, device=self.device)
 
+    def set_material(self, material: StableNeoHookeanMaterial):
+        """The elastic model used by this model's assembly and by simulations built on it."""
+        check(_lib.load().hoefem_model_set_material(self.h, material.kind, material.threshold))
+
     def assemble_elastic_pseudo_forces_into(self, f: torch.Tensor, u: torch.Tensor, material: StableNeoHookeanMaterial):
         """f += f_int(u) (assemble_elastic_pseudo_forces_into_par)."""
+        self.set_material(material)
         check(_lib.load().hoefem_model_forces_add(self.h, material.mu, material.lam, _dev(u), _dev(f)))
         return f
 
     def assemble_stiffness_into(self, values: torch.Tensor, u: torch.Tensor, material: StableNeoHookeanMaterial):
         """values += K(u) on the model pattern (assemble_transformed_stiffness_into_par)."""
+        self.set_material(material)
         check(_lib.load().hoefem_model_stiffness_add(self.h, material.mu, material.lam, _dev(u), _dev(values)))
         return values
 
@@ -211,10 +251,15 @@ class NewtonCgIntegrator:
 
     def __init__(self, model: ElasticityModel, scene: Scene, newton_tolerance_factor: float = 1e-5,
                  cg_tolerance: float = 1e-5, cg_max_iter: int = 500, max_newton_iter: int = 100,
-                 schwarz: bool = True, gravity_enabled: bool = True, mixed_precision: bool = False):
+                 schwarz: bool = True, gravity_enabled: bool = True, mixed_precision: bool = False,
+                 material: Optional[StableNeoHookeanMaterial] = None):
         L = _lib.load()
         self.model, self.scene = model, scene
         mu, lam = lame_from_young_poisson(scene.young, scene.poisson)
+        if material is not None:
+            # the scene's elastic model (fe_model.rs:35-45); plain Stable Neo-Hookean otherwise
+            mu, lam = material.mu, material.lam
+            model.set_material(material)
         d = _lib.SimDesc()
         d.mu, d.lam, d.density = mu, lam, scene.density
         d.gravity, d.gravity_enabled = scene.gravity, 1 if gravity_enabled else 0

@@ -91,6 +91,31 @@ void hoefem_lame_from_young_poisson(double young, double poisson, double* mu, do
  * F, P: [nq * 9] column-major; dPdF: [nq * 81]. Either output may be NULL. Device pointers. */
 hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const double* F, double* P, double* dPdF);
 
+/* The elastic models of simulation_toolbox/src/fem/fe_model.rs:35-45 built on Stable Neo-Hookean. */
+typedef enum hoefem_material_kind {
+    /* StableNeoHookeanMaterial (fenris/src/solid/materials.rs:229-377); the default */
+    HOEFEM_STABLE_NEO_HOOKEAN = 0,
+    /* InvertibleMaterial<StableNeoHookeanMaterial> (materials.rs:862-945): F is projected
+     * (signed singular values raised to the threshold, project_deformation_gradient) before
+     * both the stress and the tangent. The reference's default Material uses threshold 1e-6
+     * (simulation_toolbox/src/fem/fe_model.rs:145-158). */
+    HOEFEM_INVERTIBLE_STABLE_NEO_HOOKEAN = 1,
+    /* ProjectedStableNeoHookeanMaterial (materials.rs:387-441): Stable Neo-Hookean stress; the
+     * tangent dP/dF with its negative eigenvalues clamped to zero (analytic eigensystem,
+     * materials.rs:561-687). */
+    HOEFEM_PROJECTED_STABLE_NEO_HOOKEAN = 2
+} hoefem_material_kind;
+
+/* hoefem_material_eval for any hoefem_material_kind; inversion_threshold is used by
+ * HOEFEM_INVERTIBLE_STABLE_NEO_HOOKEAN only. Device pointers. */
+hoefem_status hoefem_material_eval_kind(int32_t kind, double inversion_threshold, double mu, double lambda,
+                                        int64_t nq, const double* F, double* P, double* dPdF);
+
+/* rotation_svd (fenris/src/util.rs:68-94) on nq 3x3 matrices: F = U diag(S) Vt with U, Vt
+ * rotations (det +1), |S| descending and the sign of det F on S[2]. F, U, Vt: [nq * 9]
+ * column-major; S: [nq * 3]. Device pointers. */
+hoefem_status hoefem_rotation_svd(int64_t nq, const double* F, double* U, double* S, double* Vt);
+
 /* ElasticityModelParallel for NodalModel / EmbeddedModel (fenris/src/solid/mod.rs:124-172,
  * fenris/src/solid/impl_model.rs:111-164, fenris/src/embedding/mod.rs:506-639).
  * Creation builds the greedy element coloring (paradis/src/coloring.rs:48-72), the CSR
@@ -98,6 +123,10 @@ hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const d
 hoefem_status hoefem_model_create(const hoefem_model_desc* desc, hoefem_model** out);
 void hoefem_model_destroy(hoefem_model* model);
 hoefem_status hoefem_model_set_stream(hoefem_model* model, void* hip_stream);
+/* The elastic model used by this model's forces and stiffness (and by every simulation built
+ * on it): the ElasticMaterialModel of the reference's Material (fe_model.rs:35-45, :131-158).
+ * kind is a hoefem_material_kind; inversion_threshold applies to the invertible kind only. */
+hoefem_status hoefem_model_set_material(hoefem_model* model, int32_t kind, double inversion_threshold);
 int64_t hoefem_model_ndof(const hoefem_model* model);
 int64_t hoefem_model_nnz(const hoefem_model* model);
 /* host outputs: rowptr [ndof + 1], col [nnz] */

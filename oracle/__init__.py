@@ -109,6 +109,15 @@ def lib():
         L.orc_snh_energy.argtypes = [C.c_double, C.c_double, C.c_int64, f64p, f64p]
         L.orc_snh_contract_with.argtypes = [C.c_double, C.c_double, f64p, f64p, f64p, f64p]
         L.orc_snh_contract_multiple.argtypes = [C.c_double, C.c_double, f64p, C.c_int, f64p, f64p]
+        L.orc_svd3.argtypes = [C.c_int64, f64p, f64p, f64p, f64p]
+        L.orc_rotation_svd.argtypes = [C.c_int64, f64p, f64p, f64p, f64p]
+        L.orc_project_deformation_gradient.argtypes = [C.c_double, C.c_int64, f64p, f64p]
+        L.orc_snh_dpdf_eigen.argtypes = [C.c_double, C.c_double, f64p, f64p, f64p]
+        L.orc_material_stress.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64, f64p, f64p]
+        L.orc_material_dpdf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int64, f64p, f64p]
+        L.orc_material_contract_multiple.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, f64p, C.c_int,
+                                                     f64p, f64p]
+        L.orc_model_set_material.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.orc_element_basis.argtypes = [C.c_int, C.c_int64, f64p, f64p]
         L.orc_element_gradients.argtypes = [C.c_int, C.c_int64, f64p, f64p]
         L.orc_element_reference_jacobian.argtypes = [C.c_int, f64p, f64p, f64p]
@@ -224,6 +233,77 @@ def snh_contract_multiple(mu, lam, F, a, out=None):
     return buf.T.copy()
 
 
+# elastic model kinds (simulation_toolbox/src/fem/fe_model.rs:35-45)
+MAT_SNH, MAT_INVERTIBLE_SNH, MAT_PROJECTED_SNH = 0, 1, 2
+
+
+def _colmajor(F):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1, 3, 3)
+    return F.shape[0], np.ascontiguousarray(F.transpose(0, 2, 1)).ravel()
+
+
+def _rowmajor(buf, nq):
+    return buf.reshape(nq, 3, 3).transpose(0, 2, 1).copy()
+
+
+def svd3(F):
+    """Oracle SVD (sigma >= 0, U, V orthogonal): returns U, S, V with F = U diag(S) V^T."""
+    nq, Fc = _colmajor(F)
+    U, S, V = np.empty(9 * nq), np.empty(3 * nq), np.empty(9 * nq)
+    check(lib().orc_svd3(nq, Fc, U, S, V))
+    return _rowmajor(U, nq), S.reshape(nq, 3), _rowmajor(V, nq)
+
+
+def rotation_svd(F):
+    """rotation_svd (fenris/src/util.rs:68-94): returns U, S, Vt with U, Vt rotations."""
+    nq, Fc = _colmajor(F)
+    U, S, Vt = np.empty(9 * nq), np.empty(3 * nq), np.empty(9 * nq)
+    check(lib().orc_rotation_svd(nq, Fc, U, S, Vt))
+    return _rowmajor(U, nq), S.reshape(nq, 3), _rowmajor(Vt, nq)
+
+
+def project_deformation_gradient(F, threshold):
+    """project_deformation_gradient (fenris/src/solid/materials.rs:876-915)."""
+    nq, Fc = _colmajor(F)
+    out = np.empty(9 * nq)
+    check(lib().orc_project_deformation_gradient(threshold, nq, Fc, out))
+    return _rowmajor(out, nq)
+
+
+def snh_dpdf_eigen(mu, lam, F):
+    """build_stable_neohookean_dp_df_eigen_3d (materials.rs:561-687): eigenvalues (9,) and
+    eigenvectors as the columns of a (9, 9) matrix, vec() ordering."""
+    Fc = np.ascontiguousarray(np.asarray(F, dtype=np.float64).T).ravel()
+    ev, Q = np.empty(9), np.empty(81)
+    check(lib().orc_snh_dpdf_eigen(mu, lam, Fc, ev, Q))
+    return ev, Q.reshape(9, 9).T.copy()
+
+
+def material_stress(kind, threshold, mu, lam, F):
+    nq, Fc = _colmajor(F)
+    out = np.empty(9 * nq)
+    check(lib().orc_material_stress(kind, threshold, mu, lam, nq, Fc, out))
+    return _rowmajor(out, nq)
+
+
+def material_dpdf(kind, threshold, mu, lam, F):
+    nq, Fc = _colmajor(F)
+    out = np.empty(81 * nq)
+    check(lib().orc_material_dpdf(kind, threshold, mu, lam, nq, Fc, out))
+    return out.reshape(-1, 9, 9).transpose(0, 2, 1).copy()
+
+
+def material_contract_multiple(kind, threshold, mu, lam, F, a):
+    """The (3n, 3n) contraction of the elastic model kind with the columns a_I of a (3, n)."""
+    a = np.asarray(a, dtype=np.float64)
+    n = a.shape[1]
+    Fc = np.ascontiguousarray(np.asarray(F, dtype=np.float64).T).ravel()
+    buf = np.zeros(9 * n * n)
+    check(lib().orc_material_contract_multiple(kind, threshold, mu, lam, Fc, n, np.ascontiguousarray(a.T).ravel(),
+                                               buf))
+    return buf.reshape(3 * n, 3 * n).T.copy()
+
+
 def element_basis(etype, xi):
     xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, 3)
     n = lib().orc_nodes_per_element(etype)
@@ -311,6 +391,9 @@ class Model:
         vals = np.zeros(self.nnz)
         check(lib().orc_model_assemble_mass(self.h, self.scene.density if density is None else density, vals))
         return vals
+
+    def set_material(self, kind, threshold=0.0):
+        check(lib().orc_model_set_material(self.h, kind, threshold))
 
     def forces_add(self, mu, lam, u, f):
         check(lib().orc_model_forces_add(self.h, mu, lam, np.ascontiguousarray(u, dtype=np.float64), f))

@@ -601,6 +601,293 @@ void invert_block(int m, const double* block, double* out) {
         }
 }
 
+// ---------------------------------------------------------------- SVD-based material variants
+// F = U diag(sigma) V^T with sigma >= 0 and U, V orthogonal (determinant +-1), by one-sided
+// Jacobi (Hestenes) rotations on the columns of F. It stands in for nalgebra 0.21.1's
+// Matrix::svd(true, true) (Golub-Kahan bidiagonalization + implicit QR), a dependency of the
+// reference that is not vendored: the factors of both are unique up to the order and the
+// joint signs of the singular triplets, and every use below is invariant under both (ties
+// between singular values aside).
+void svd3(const double* F, double* U, double* sigma, double* V) {
+    double B[9];
+    for (int k = 0; k < 9; ++k) {
+        B[k] = F[k];
+        V[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    }
+    const int PP[3] = {0, 0, 1}, QQ[3] = {1, 2, 2};
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int k = 0; k < 3; ++k) {
+            const int p = PP[k], q = QQ[k];
+            double a = 0.0, b = 0.0, g = 0.0;
+            for (int r = 0; r < 3; ++r) {
+                a += B[r + 3 * p] * B[r + 3 * p];
+                b += B[r + 3 * q] * B[r + 3 * q];
+                g += B[r + 3 * p] * B[r + 3 * q];
+            }
+            if (g == 0.0 || std::fabs(g) <= 1e-16 * std::sqrt(a * b)) continue;
+            rotated = true;
+            const double zeta = (b - a) / (2.0 * g);
+            const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+            const double c = 1.0 / std::sqrt(1.0 + t * t), sn = c * t;
+            for (int r = 0; r < 3; ++r) {
+                const double bp = B[r + 3 * p], bq = B[r + 3 * q];
+                B[r + 3 * p] = c * bp - sn * bq;
+                B[r + 3 * q] = sn * bp + c * bq;
+                const double vp = V[r + 3 * p], vq = V[r + 3 * q];
+                V[r + 3 * p] = c * vp - sn * vq;
+                V[r + 3 * q] = sn * vp + c * vq;
+            }
+        }
+        if (!rotated) break;
+    }
+    bool have[3];
+    int nz = 0;
+    for (int i = 0; i < 3; ++i) {
+        sigma[i] = std::sqrt(B[3 * i] * B[3 * i] + B[3 * i + 1] * B[3 * i + 1] + B[3 * i + 2] * B[3 * i + 2]);
+        have[i] = sigma[i] > 0.0;
+        if (have[i]) {
+            ++nz;
+            for (int r = 0; r < 3; ++r) U[r + 3 * i] = B[r + 3 * i] / sigma[i];
+        }
+    }
+    // complete U to an orthonormal basis where F has a null space
+    if (nz == 0) {
+        for (int k = 0; k < 9; ++k) U[k] = (k % 4 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    for (int i = 0; i < 3 && nz < 3; ++i) {
+        if (have[i]) continue;
+        int a = -1, b = -1;
+        for (int j = 0; j < 3; ++j)
+            if (have[j]) (a < 0 ? a : b) = j;
+        double* u = U + 3 * i;
+        if (b >= 0) {
+            const double* x = U + 3 * a;
+            const double* y = U + 3 * b;
+            u[0] = x[1] * y[2] - x[2] * y[1];
+            u[1] = x[2] * y[0] - x[0] * y[2];
+            u[2] = x[0] * y[1] - x[1] * y[0];
+        } else {
+            const double* x = U + 3 * a;
+            int ax = 0;
+            for (int r = 1; r < 3; ++r)
+                if (std::fabs(x[r]) < std::fabs(x[ax])) ax = r;
+            for (int r = 0; r < 3; ++r) u[r] = (r == ax ? 1.0 : 0.0) - x[ax] * x[r];
+            const double n = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            for (int r = 0; r < 3; ++r) u[r] /= n;
+        }
+        have[i] = true;
+        ++nz;
+    }
+}
+
+int imin3(const double* s) {
+    int k = 0;
+    for (int i = 1; i < 3; ++i)
+        if (s[i] < s[k]) k = i;
+    return k;
+}
+
+// rotation_svd (fenris/src/util.rs:68-94): U, V^T proper rotations, the sign moved to the
+// smallest singular value.
+void rotation_svd3(const double* F, double* U, double* s, double* Vt) {
+    double V[9];
+    svd3(F, U, s, V);
+    transpose3(V, Vt);
+    const int k = imin3(s);
+    if (det3(U) < 0.0) {
+        for (int r = 0; r < 3; ++r) U[r + 3 * k] = -U[r + 3 * k];
+        s[k] = -s[k];
+    }
+    if (det3(Vt) < 0.0) {
+        for (int c = 0; c < 3; ++c) Vt[k + 3 * c] = -Vt[k + 3 * c];
+        s[k] = -s[k];
+    }
+}
+
+// project_deformation_gradient (fenris/src/solid/materials.rs:876-915)
+void project_deformation_gradient(const double* F, double threshold, double* Fout) {
+    double U[9], s[3], V[9];
+    svd3(F, U, s, V);
+    const bool inversion = (det3(U) < 0.0) != (det3(V) < 0.0);  // det V = det V^T
+    if (inversion) {
+        const int k = imin3(s);
+        for (int r = 0; r < 3; ++r) U[r + 3 * k] = -U[r + 3 * k];
+        s[k] = -s[k];
+    }
+    bool correction = false;
+    for (int i = 0; i < 3; ++i)
+        if (s[i] < threshold) {
+            s[i] = threshold;
+            correction = true;
+        }
+    if (!correction) {
+        for (int k = 0; k < 9; ++k) Fout[k] = F[k];
+        return;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            double v = 0.0;
+            for (int i = 0; i < 3; ++i) v += U[r + 3 * i] * s[i] * V[c + 3 * i];
+            Fout[r + 3 * c] = v;
+        }
+}
+
+// build_stable_neohookean_dp_df_eigen_3d (fenris/src/solid/materials.rs:561-687):
+// evals[9], evecs column-major 9x9 (eigenvectors as columns, vec() ordering).
+void snh_dpdf_eigen(double mu, double lambda, const double* F, double* evals, double* evecs) {
+    double U[9], s[3], Vt[9];
+    rotation_svd3(F, U, s, Vt);
+    const double J = det3(F);
+    double Ft[9], C[9];
+    transpose3(F, Ft);
+    matmul3(Ft, F, C);
+    const double I_C = C[0] + C[4] + C[8];
+    const double alpha = 1.0 + mu / lambda * (3.0 / 4.0);
+    const double mu_t = mu * (1.0 - 1.0 / (I_C + 1.0));
+    const double sv_scale = lambda * (J - alpha);
+    for (int k = 0; k < 3; ++k) {
+        evals[k] = s[k] * sv_scale + mu_t;
+        evals[3 + k] = -s[k] * sv_scale + mu_t;
+    }
+    // D = (u_i v_j^T -+ u_j v_i^T) / sqrt(2); v_j = row j of V^T
+    const double h = 1.0 / std::sqrt(2.0);
+    auto outer = [&](int i, int j, double* out) {
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) out[r + 3 * c] = U[r + 3 * i] * (Vt[j + 3 * c] * h);
+    };
+    double u0v1[9], u0v2[9], u1v0[9], u1v2[9], u2v0[9], u2v1[9];
+    outer(0, 1, u0v1);
+    outer(0, 2, u0v2);
+    outer(1, 0, u1v0);
+    outer(1, 2, u1v2);
+    outer(2, 0, u2v0);
+    outer(2, 1, u2v1);
+    for (int k = 0; k < 9; ++k) {
+        evecs[k + 9 * 0] = u1v2[k] - u2v1[k];
+        evecs[k + 9 * 1] = u0v2[k] - u2v0[k];
+        evecs[k + 9 * 2] = u0v1[k] - u1v0[k];
+        evecs[k + 9 * 3] = u1v2[k] + u2v1[k];
+        evecs[k + 9 * 4] = u0v2[k] + u2v0[k];
+        evecs[k + 9 * 5] = u0v1[k] + u1v0[k];
+    }
+    const double beta = I_C + 1.0, beta2 = beta * beta;
+    const double gamma = lambda * (2.0 * J - alpha);
+    std::vector<double> A(9), ev, E;
+    A[0] = mu_t + 2.0 * s[0] * s[0] * mu / beta2 + s[1] * s[1] * s[2] * s[2] * lambda;
+    A[4] = mu_t + 2.0 * s[1] * s[1] * mu / beta2 + s[0] * s[0] * s[2] * s[2] * lambda;
+    A[8] = mu_t + 2.0 * s[2] * s[2] * mu / beta2 + s[0] * s[0] * s[1] * s[1] * lambda;
+    A[3] = A[1] = s[2] * gamma + 2.0 * s[0] * s[1] * mu / beta2;
+    A[6] = A[2] = s[1] * gamma + 2.0 * s[0] * s[2] * mu / beta2;
+    A[7] = A[5] = s[0] * gamma + 2.0 * s[1] * s[2] * mu / beta2;
+    jacobi_eigen(3, A, ev, E);
+    for (int m = 0; m < 3; ++m) {
+        evals[6 + m] = ev[m];
+        // Q_m = U diag(e_m) V^T
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) {
+                double v = 0.0;
+                for (int k = 0; k < 3; ++k) v += U[r + 3 * k] * E[k + 3 * m] * Vt[k + 3 * c];
+                evecs[(r + 3 * c) + 9 * (6 + m)] = v;
+            }
+    }
+}
+
+// ProjectedStableNeoHookeanMaterial::contract_multiple_stress_tensors_into, 3D
+// (fenris/src/solid/materials.rs:413-441): eigenvalues clamped at zero, recomposed.
+void snh_dpdf_projected(double mu, double lambda, const double* F, double* dp) {
+    double ev[9], Q[81];
+    snh_dpdf_eigen(mu, lambda, F, ev, Q);
+    for (int k = 0; k < 81; ++k) dp[k] = 0.0;
+    for (int m = 0; m < 9; ++m) {
+        const double l = std::max(0.0, ev[m]);
+        for (int c = 0; c < 9; ++c)
+            for (int r = 0; r < 9; ++r) dp[r + 9 * c] += Q[r + 9 * m] * l * Q[c + 9 * m];
+    }
+}
+
+// StableNeoHookeanMaterial::contract_stress_tensor_with (fenris/src/solid/materials.rs:324-347),
+// reparametrized mu / lambda passed in.
+bool snh_contract_with_r(double mu, double lambda, const double* F, const double* a, const double* b, double* out) {
+    const double alpha = 1.0 + mu / lambda * 0.75;
+    double B[9];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) B[r + 3 * c] = a[r] * b[c];
+    double Ft[9], C[9];
+    transpose3(F, Ft);
+    matmul3(Ft, F, C);
+    const double I_C = C[0] + C[4] + C[8];
+    const double J = det3(F);
+    const double beta = I_C + 1.0;
+    const double beta2 = beta * beta;
+    const double G = 1.0 - 1.0 / beta;
+    const double H = (J - alpha) * J;
+    double Finv[9], FinvT[9];
+    if (!inv3(F, Finv)) return false;
+    transpose3(Finv, FinvT);
+    const double trB = B[0] + B[4] + B[8];
+    double FB[9], FBFt[9], t1[9], t2[9], Bt[9], t3[9], t4[9];
+    matmul3(F, B, FB);
+    matmul3(FB, Ft, FBFt);
+    matmul3(FinvT, B, t1);
+    matmul3(t1, Finv, t2);
+    transpose3(B, Bt);
+    matmul3(FinvT, Bt, t3);
+    matmul3(t3, Finv, t4);
+    for (int i = 0; i < 9; ++i) {
+        const double id = (i % 4 == 0) ? 1.0 : 0.0;
+        const double Q = FBFt[i] * 2.0 / beta2 + id * G * trB;
+        const double R = t2[i] * (2.0 * J - alpha) * J - t4[i] * H;
+        out[i] = Q * mu + R * lambda;
+    }
+    return true;
+}
+
+// The default ElasticMaterialModel::contract_multiple_stress_tensors_into
+// (fenris/src/solid/mod.rs:56-86), which InvertibleMaterial inherits: one closed-form
+// contraction per node pair (i <= j) at the projected F, the transpose mirrored.
+bool contract_pairwise(double mu, double lambda, const double* F, int n, const double* a, double* out) {
+    const int dim = 3 * n;
+    for (int i = 0; i < n; ++i)
+        for (int j = i; j < n; ++j) {
+            double c3[9];
+            if (!snh_contract_with_r(mu, lambda, F, a + 3 * i, a + 3 * j, c3)) return false;
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 3; ++r) {
+                    out[(3 * i + r) + dim * (3 * j + c)] += c3[r + 3 * c];
+                    if (i != j) out[(3 * j + c) + dim * (3 * i + r)] += c3[r + 3 * c];
+                }
+        }
+    return true;
+}
+
+// The stress of the elastic model kind (ORC_MAT_*), reparametrized mu / lambda.
+bool material_stress_r(int kind, double thr, double mu, double lambda, const double* F, double* P) {
+    if (kind == ORC_MAT_INVERTIBLE_SNH) {
+        double Fp[9];
+        project_deformation_gradient(F, thr, Fp);
+        return snh_stress_r(mu, lambda, Fp, P);
+    }
+    return snh_stress_r(mu, lambda, F, P);
+}
+
+// The tangent dP/dF of the elastic model kind (for the invertible kind: at the projected F).
+bool material_dpdf_r(int kind, double thr, double mu, double lambda, const double* F, double* dp) {
+    if (kind == ORC_MAT_INVERTIBLE_SNH) {
+        double Fp[9];
+        project_deformation_gradient(F, thr, Fp);
+        return snh_dpdf_r(mu, lambda, Fp, dp);
+    }
+    if (kind == ORC_MAT_PROJECTED_SNH) {
+        double Finv[9];
+        if (!inv3(F, Finv)) return false;
+        snh_dpdf_projected(mu, lambda, F, dp);
+        return true;
+    }
+    return snh_dpdf_r(mu, lambda, F, dp);
+}
+
 double norm2(const std::vector<double>& x) {
     double s = 0.0;
     for (double v : x) s += v * v;
@@ -669,40 +956,9 @@ int orc_snh_dpdf(double mu0, double lambda0, int64_t nq, const double* F, double
 // fenris/src/solid/materials.rs:324-347
 int orc_snh_contract_with(double mu0, double lambda0, const double* F, const double* a, const double* b,
                           double* out) {
-    const double d = 3.0;
     double mu, lambda;
     snh_reparam(mu0, lambda0, &mu, &lambda);
-    const double alpha = 1.0 + mu / lambda * (d / (d + 1.0));
-    double B[9];
-    for (int c = 0; c < 3; ++c)
-        for (int r = 0; r < 3; ++r) B[r + 3 * c] = a[r] * b[c];
-    double Ft[9], C[9];
-    transpose3(F, Ft);
-    matmul3(Ft, F, C);
-    const double I_C = C[0] + C[4] + C[8];
-    const double J = det3(F);
-    const double beta = I_C + 1.0;
-    const double beta2 = beta * beta;
-    const double G = 1.0 - 1.0 / beta;
-    const double H = (J - alpha) * J;
-    double Finv[9], FinvT[9];
-    if (!inv3(F, Finv)) return fail(ORC_ERR_SINGULAR, "singular F");
-    transpose3(Finv, FinvT);
-    const double trB = B[0] + B[4] + B[8];
-    double FB[9], FBFt[9], t1[9], t2[9], Bt[9], t3[9], t4[9];
-    matmul3(F, B, FB);
-    matmul3(FB, Ft, FBFt);
-    matmul3(FinvT, B, t1);
-    matmul3(t1, Finv, t2);
-    transpose3(B, Bt);
-    matmul3(FinvT, Bt, t3);
-    matmul3(t3, Finv, t4);
-    for (int i = 0; i < 9; ++i) {
-        const double id = (i % 4 == 0) ? 1.0 : 0.0;
-        const double Q = FBFt[i] * 2.0 / beta2 + id * G * trB;
-        const double R = t2[i] * (2.0 * J - alpha) * J - t4[i] * H;
-        out[i] = Q * mu + R * lambda;
-    }
+    if (!snh_contract_with_r(mu, lambda, F, a, b, out)) return fail(ORC_ERR_SINGULAR, "singular F");
     return ORC_OK;
 }
 
@@ -710,6 +966,66 @@ int orc_snh_contract_multiple(double mu0, double lambda0, const double* F, int n
     double mu, lambda, dp[81];
     snh_reparam(mu0, lambda0, &mu, &lambda);
     if (!snh_dpdf_r(mu, lambda, F, dp)) return fail(ORC_ERR_SINGULAR, "singular F");
+    contract_stiffness(dp, n, a, out);
+    return ORC_OK;
+}
+
+int orc_svd3(int64_t nq, const double* F, double* U, double* S, double* V) {
+    for (int64_t q = 0; q < nq; ++q) svd3(F + 9 * q, U + 9 * q, S + 3 * q, V + 9 * q);
+    return ORC_OK;
+}
+
+int orc_rotation_svd(int64_t nq, const double* F, double* U, double* S, double* Vt) {
+    for (int64_t q = 0; q < nq; ++q) rotation_svd3(F + 9 * q, U + 9 * q, S + 3 * q, Vt + 9 * q);
+    return ORC_OK;
+}
+
+int orc_project_deformation_gradient(double threshold, int64_t nq, const double* F, double* Fout) {
+    for (int64_t q = 0; q < nq; ++q) project_deformation_gradient(F + 9 * q, threshold, Fout + 9 * q);
+    return ORC_OK;
+}
+
+int orc_snh_dpdf_eigen(double mu0, double lambda0, const double* F, double* evals, double* evecs) {
+    double mu, lambda;
+    snh_reparam(mu0, lambda0, &mu, &lambda);
+    snh_dpdf_eigen(mu, lambda, F, evals, evecs);
+    return ORC_OK;
+}
+
+int orc_material_stress(int kind, double threshold, double mu0, double lambda0, int64_t nq, const double* F,
+                        double* P) {
+    if (kind < ORC_MAT_SNH || kind > ORC_MAT_PROJECTED_SNH) return fail(ORC_ERR_INVALID, "unknown material kind");
+    double mu, lambda;
+    snh_reparam(mu0, lambda0, &mu, &lambda);
+    for (int64_t q = 0; q < nq; ++q)
+        if (!material_stress_r(kind, threshold, mu, lambda, F + 9 * q, P + 9 * q))
+            return fail(ORC_ERR_SINGULAR, "singular F");
+    return ORC_OK;
+}
+
+int orc_material_dpdf(int kind, double threshold, double mu0, double lambda0, int64_t nq, const double* F,
+                      double* dpdf) {
+    if (kind < ORC_MAT_SNH || kind > ORC_MAT_PROJECTED_SNH) return fail(ORC_ERR_INVALID, "unknown material kind");
+    double mu, lambda;
+    snh_reparam(mu0, lambda0, &mu, &lambda);
+    for (int64_t q = 0; q < nq; ++q)
+        if (!material_dpdf_r(kind, threshold, mu, lambda, F + 9 * q, dpdf + 81 * q))
+            return fail(ORC_ERR_SINGULAR, "singular F");
+    return ORC_OK;
+}
+
+int orc_material_contract_multiple(int kind, double threshold, double mu0, double lambda0, const double* F, int n,
+                                   const double* a, double* out) {
+    if (kind < ORC_MAT_SNH || kind > ORC_MAT_PROJECTED_SNH) return fail(ORC_ERR_INVALID, "unknown material kind");
+    double mu, lambda, dp[81];
+    snh_reparam(mu0, lambda0, &mu, &lambda);
+    if (kind == ORC_MAT_INVERTIBLE_SNH) {
+        double Fp[9];
+        project_deformation_gradient(F, threshold, Fp);
+        if (!contract_pairwise(mu, lambda, Fp, n, a, out)) return fail(ORC_ERR_SINGULAR, "singular F");
+        return ORC_OK;
+    }
+    if (!material_dpdf_r(kind, threshold, mu, lambda, F, dp)) return fail(ORC_ERR_SINGULAR, "singular F");
     contract_stiffness(dp, n, a, out);
     return ORC_OK;
 }
@@ -748,6 +1064,8 @@ struct orc_model {
         std::vector<double> fw, fxi;
     } mass, stiff, ell;
     double mass_reg = 0.0;
+    int material = ORC_MAT_SNH;  // elastic model kind
+    double threshold = 0.0;      // InvertibleMaterial threshold
     std::vector<std::vector<int32_t>> colors[2];  // per pass: color -> global element ids
     Csr pattern;                                  // values unused
 
@@ -822,7 +1140,7 @@ bool element_force(const orc_model& m, int64_t e, double mu_r, double lambda_r, 
         F[0] += 1.0;
         F[4] += 1.0;
         F[8] += 1.0;
-        if (!snh_stress_r(mu_r, lambda_r, F, P)) return false;
+        if (!material_stress_r(m.material, m.threshold, mu_r, lambda_r, F, P)) return false;
         // A = g^T * J_inv^T ; f_e += (A * G) * w * |detJ|
         double gT[9];
         for (int i = 0; i < 9; ++i) gT[i] = -P[i];
@@ -879,7 +1197,13 @@ bool element_stiffness(const orc_model& m, int64_t e, double mu_r, double lambda
         F[0] += 1.0;
         F[4] += 1.0;
         F[8] += 1.0;
-        if (!snh_dpdf_r(mu_r, lambda_r, F, dp)) return false;
+        if (m.material == ORC_MAT_INVERTIBLE_SNH) {
+            double Fp[9];
+            project_deformation_gradient(F, m.threshold, Fp);
+            if (!contract_pairwise(mu_r, lambda_r, Fp, n, G.data(), Ke)) return false;
+            continue;
+        }
+        if (!material_dpdf_r(m.material, m.threshold, mu_r, lambda_r, F, dp)) return false;
         contract_stiffness(dp, n, G.data(), Ke);
     }
     return true;
@@ -963,6 +1287,14 @@ int orc_model_create(const orc_model_desc* d, orc_model** out) {
 }
 
 void orc_model_destroy(orc_model* m) { delete m; }
+
+int orc_model_set_material(orc_model* m, int kind, double threshold) {
+    if (kind != ORC_MAT_SNH && kind != ORC_MAT_INVERTIBLE_SNH && kind != ORC_MAT_PROJECTED_SNH)
+        return fail(ORC_ERR_INVALID, "unknown material kind");
+    m->material = kind;
+    m->threshold = kind == ORC_MAT_INVERTIBLE_SNH ? threshold : 0.0;
+    return ORC_OK;
+}
 int64_t orc_model_ndof(const orc_model* m) { return 3 * m->nv; }
 int orc_model_num_colors(const orc_model* m, int pass) { return (int)m->colors[pass].size(); }
 

@@ -35,6 +35,13 @@ enum {
     ORC_HEX27 = 5
 };
 
+/* elastic model kinds (simulation_toolbox/src/fem/fe_model.rs:35-45) */
+enum {
+    ORC_MAT_SNH = 0,            /* StableNeoHookeanMaterial */
+    ORC_MAT_INVERTIBLE_SNH = 1, /* InvertibleMaterial<StableNeoHookeanMaterial> */
+    ORC_MAT_PROJECTED_SNH = 2   /* ProjectedStableNeoHookeanMaterial */
+};
+
 /* status codes (same numbering as the product library's hoefem_status) */
 enum {
     ORC_OK = 0,
@@ -66,6 +73,19 @@ int orc_snh_contract_with(double mu, double lambda, const double* F, const doubl
 int orc_snh_contract_multiple(double mu, double lambda, const double* F, int n, const double* a, double* out);
 
 /* ---------------- elements (fenris/src/element.rs) ---------------- */
+/* SVD-based variants of the material (fenris/src/solid/materials.rs:387-441, :862-945,
+ * fenris/src/util.rs:68-94). svd3: sigma >= 0, U and V orthogonal (one-sided Jacobi);
+ * rotation_svd: U, Vt rotations, signed S. kind: ORC_MAT_*; threshold for the invertible kind. */
+int orc_svd3(int64_t nq, const double* F, double* U, double* S, double* V);
+int orc_rotation_svd(int64_t nq, const double* F, double* U, double* S, double* Vt);
+int orc_project_deformation_gradient(double threshold, int64_t nq, const double* F, double* Fout);
+int orc_snh_dpdf_eigen(double mu, double lambda, const double* F, double* evals, double* evecs);
+int orc_material_stress(int kind, double threshold, double mu, double lambda, int64_t nq, const double* F, double* P);
+int orc_material_dpdf(int kind, double threshold, double mu, double lambda, int64_t nq, const double* F,
+                      double* dpdf);
+int orc_material_contract_multiple(int kind, double threshold, double mu, double lambda, const double* F, int n,
+                                   const double* a, double* out);
+
 int orc_element_basis(int element_type, int64_t npts, const double* xi, double* phi);
 int orc_element_gradients(int element_type, int64_t npts, const double* xi, double* grad);
 int orc_element_reference_jacobian(int element_type, const double* nodes, const double* xi, double* J);
@@ -97,6 +117,8 @@ typedef struct orc_model orc_model;
 
 int orc_model_create(const orc_model_desc* desc, orc_model** out);
 void orc_model_destroy(orc_model* m);
+/* the elastic model used by forces / stiffness / simulations of this model (ORC_MAT_*) */
+int orc_model_set_material(orc_model* m, int kind, double threshold);
 int64_t orc_model_ndof(const orc_model* m);
 int orc_model_num_colors(const orc_model* m, int pass);
 /* colors of a pass as NestedVec: offsets[ncolors+1], labels (global element indices) */
