@@ -55,6 +55,36 @@ __device__ inline void sum_partials(const double* __restrict__ partials, int nb,
     }
 }
 
+// The two halves of sum_partials, so that the loads can be issued early and the reduction
+// done once they are needed: acc = this thread's strided share of the partials ...
+template <int NV>
+__device__ inline void partials_load(const double* __restrict__ partials, int nb, double (&acc)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += CG_THREADS)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
+}
+
+// ... and the block-wide total, the same in every thread of every block (sum_partials' order).
+template <int NV>
+__device__ inline void partials_reduce(const double (&acc)[NV], double (&tot)[NV]) {
+    __shared__ double s_w[CG_THREADS / 64][NV];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (lane == 0) s_w[wid][k] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
+        tot[k] = s;
+    }
+}
+
 // partials[blockIdx.x * NV + k] = sum of v[k] over the block's threads.
 template <int NV>
 __device__ inline void block_partials(const double (&v)[NV], double* __restrict__ partials) {
@@ -188,6 +218,105 @@ __device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb
         diag(ent[nb + i]);
 }
 
+// The lane's first 3x3 block of a row (j = lane) and its column, loaded a pass ahead.
+template <class VT>
+struct RowPf {
+    VT a[9];
+    int32_t col;
+};
+
+template <class VT>
+__device__ inline void row_pf_issue(const Csr3<VT>& A, const NodeEntry& e, int lane, bool valid, RowPf<VT>& pf) {
+    if (valid && lane < e.count) {
+        const int k = e.count;
+        const VT* R0 = A.vals + e.rowoff + 3 * lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            pf.a[c] = R0[c];
+            pf.a[3 + c] = R0[3 * k + c];
+            pf.a[6 + c] = R0[6 * k + c];
+        }
+        pf.col = A.ncol[e.first + lane];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) pf.a[c] = VT(0);
+        pf.col = -1;
+    }
+}
+
+// entry_row_dot with the first block of each lane taken from pf.
+template <int G, class VT, class XN>
+__device__ inline void entry_row_dot_pf(const Csr3<VT>& A, const NodeEntry& e, int lane, bool valid,
+                                        const RowPf<VT>& pf, XN xnode, double& r0, double& r1, double& r2) {
+    VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
+    if (valid && pf.col >= 0) {
+        double y0, y1, y2;
+        xnode((int64_t)pf.col, y0, y1, y2);
+        const VT x0 = VT(y0), x1 = VT(y1), x2 = VT(y2);
+        a0 = pf.a[0] * x0 + pf.a[1] * x1 + pf.a[2] * x2;
+        a1 = pf.a[3] * x0 + pf.a[4] * x1 + pf.a[5] * x2;
+        a2 = pf.a[6] * x0 + pf.a[7] * x1 + pf.a[8] * x2;
+        const int k = e.count;
+        const VT* R0 = A.vals + e.rowoff;
+        const VT* R1 = R0 + 3 * k;
+        const VT* R2 = R1 + 3 * k;
+        const int32_t* nc = A.ncol + e.first;
+        for (int j = lane + G; j < k; j += G) {
+            xnode((int64_t)nc[j], y0, y1, y2);
+            const VT z0 = VT(y0), z1 = VT(y1), z2 = VT(y2);
+            a0 += R0[3 * j] * z0 + R0[3 * j + 1] * z1 + R0[3 * j + 2] * z2;
+            a1 += R1[3 * j] * z0 + R1[3 * j + 1] * z1 + R1[3 * j + 2] * z2;
+            a2 += R2[3 * j] * z0 + R2[3 * j + 1] * z1 + R2[3 * j + 2] * z2;
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off, G);
+        a1 += __shfl_xor(a1, off, G);
+        a2 += __shfl_xor(a2, off, G);
+    }
+    r0 = double(a0);
+    r1 = double(a1);
+    r2 = double(a2);
+}
+
+// for_entries with the memory latency of the iteration's start taken off the critical path:
+// the first entry is loaded before pro_issue() (loads that need no entry, e.g. the partial
+// sums), the first pass's row data right after it, and pro_finish() (status tests, reductions;
+// returns false to stop, uniformly over the grid) runs while those are in flight. In the loop
+// the row data of pass n + 1 and the entry of pass n + 2 are loaded before pass n is computed.
+template <int G, class VT, class PI, class PF, class FB, class FD>
+__device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_finish, FB block, FD diag) {
+    constexpr int per = CG_THREADS / G;
+    const int lane = (int)(threadIdx.x % G);
+    const int64_t nb = A.n_block, nd = A.n_diag;
+    const int64_t groups = (nb + per - 1) / per;
+    const int64_t sub = threadIdx.x / G;
+    const int64_t stride = gridDim.x;
+    auto has = [&](int64_t g) { return g < groups && g * per + sub < nb; };
+    int64_t g = blockIdx.x;
+    NodeEntry e{0, 0, 0, 0}, e1{0, 0, 0, 0};
+    if (has(g)) e = A.ent[g * per + sub];
+    pro_issue();
+    RowPf<VT> pf;
+    row_pf_issue(A, e, lane, has(g), pf);
+    if (has(g + stride)) e1 = A.ent[(g + stride) * per + sub];
+    if (!pro_finish()) return false;
+    for (; g < groups; g += stride) {
+        const NodeEntry cur = e;
+        const bool valid = has(g);
+        RowPf<VT> pn;
+        row_pf_issue(A, e1, lane, has(g + stride), pn);
+        e = e1;
+        if (has(g + 2 * stride)) e1 = A.ent[(g + 2 * stride) * per + sub];
+        block(cur, lane, valid, pf);
+        pf = pn;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; i < nd; i += (int64_t)gridDim.x * CG_THREADS)
+        diag(A.ent[nb + i]);
+    return true;
+}
+
 inline unsigned cg_grid(int64_t n_block, int64_t n_diag, int G) {
     const int64_t groups = (n_block + CG_THREADS / G - 1) / (CG_THREADS / G) + (n_diag + CG_THREADS - 1) / CG_THREADS;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(groups, CG_MAX_BLOCKS));
@@ -308,47 +437,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
 template <int G, bool FLEX, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
     CgScalars* sc = P.sc;
-    if (sc->status != CG_RUNNING) return;
-    const int64_t k = sc->iters;  // written by the previous B-kernel (0 after the preamble)
-    double zTr, rr, zTr_cross = 0.0;  // z_k.r_k, |r_k|^2, z_k.r_{k-1} (flexible variant)
-    if (FLEX) {
-        double tot[3];
-        sum_partials<3>(P.part_b, P.nb_b, tot);
-        zTr = tot[0];
-        zTr_cross = tot[1];
-        rr = tot[2];
-    } else {
-        double tot[2];
-        sum_partials<2>(P.part_b, P.nb_b, tot);
-        zTr = tot[0];
-        rr = tot[1];
-    }
+    constexpr int NV = FLEX ? 3 : 2;
+    double acc[NV];
+    double beta = 0.0;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    int decided = CG_RUNNING;
-    if (sc->bnorm == 0.0)
-        decided = CG_ZERO_RHS;
-    else if (sqrt(rr) <= sc->tol * sc->bnorm)
-        decided = CG_CONVERGED;
-    else if (sc->max_iter >= 0 && k >= sc->max_iter)
-        decided = HOEFEM_ERROR_CG_MAX_ITERATIONS;
-    if (decided != CG_RUNNING) {
-        if (lead) {
-            sc->zTr = zTr;
-            sc->rr = rr;
-            sc->status = decided;
-        }
-        return;
-    }
-    // Fletcher-Reeves form for a fixed preconditioner; Polak-Ribiere z_k.(r_k - r_{k-1}) when
-    // the preconditioner depends on its input (Notay, SIAM J. Sci. Comput. 22 (2000) 1444)
-    const double beta = k == 0 ? 0.0 : (FLEX ? (zTr - zTr_cross) : zTr) / sc->zrec[(k - 1) & 1];
-    if (lead) {
-        sc->zrec[k & 1] = zTr;
-        sc->zTr = zTr;
-        sc->rr = rr;
-        sc->beta = beta;
-        sc->kcur = k;
-    }
     const double* zp = P.zp_old;
     double part[1] = {0.0};
     auto pvec = [&](int64_t b, double& x0, double& x1, double& x2) {
@@ -367,11 +459,46 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
         P.ap_out[6 * v + 5] = a2;
         part[0] += p0 * a0 + p1 * a1 + p2 * a2;
     };
-    for_entries<G>(
-        A.ent, A.n_block, A.n_diag,
-        [&](const NodeEntry& e, int lane, bool valid) {
+    const bool ran = for_entries_pf<G>(
+        A, [&] { partials_load<NV>(P.part_b, P.nb_b, acc); },
+        [&]() -> bool {
+            if (sc->status != CG_RUNNING) return false;
+            const int64_t k = sc->iters;  // written by the previous B-kernel (0 after the preamble)
+            double tot[NV];
+            partials_reduce<NV>(acc, tot);
+            // z_k.r_k, |r_k|^2 and (flexible variant) z_k.r_{k-1}
+            const double zTr = tot[0], zTr_cross = FLEX ? tot[1] : 0.0, rr = tot[NV - 1];
+            int decided = CG_RUNNING;
+            if (sc->bnorm == 0.0)
+                decided = CG_ZERO_RHS;
+            else if (sqrt(rr) <= sc->tol * sc->bnorm)
+                decided = CG_CONVERGED;
+            else if (sc->max_iter >= 0 && k >= sc->max_iter)
+                decided = HOEFEM_ERROR_CG_MAX_ITERATIONS;
+            if (decided != CG_RUNNING) {
+                if (lead) {
+                    sc->zTr = zTr;
+                    sc->rr = rr;
+                    sc->status = decided;
+                }
+                return false;
+            }
+            // Fletcher-Reeves form for a fixed preconditioner; Polak-Ribiere z_k.(r_k - r_{k-1})
+            // when the preconditioner depends on its input (Notay, SIAM J. Sci. Comput. 22
+            // (2000) 1444)
+            beta = k == 0 ? 0.0 : (FLEX ? (zTr - zTr_cross) : zTr) / sc->zrec[(k - 1) & 1];
+            if (lead) {
+                sc->zrec[k & 1] = zTr;
+                sc->zTr = zTr;
+                sc->rr = rr;
+                sc->beta = beta;
+                sc->kcur = k;
+            }
+            return true;
+        },
+        [&](const NodeEntry& e, int lane, bool valid, const RowPf<VT>& pf) {
             double a0, a1, a2;
-            entry_row_dot<G>(A, e, lane, valid, pvec, a0, a1, a2);
+            entry_row_dot_pf<G>(A, e, lane, valid, pf, pvec, a0, a1, a2);
             if (valid && lane == 0) {
                 double p0, p1, p2;
                 pvec(e.node, p0, p1, p2);
@@ -385,7 +512,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
             pvec(v, p0, p1, p2);
             finish(v, p0, p1, p2, double(d[0] * VT(p0)), double(d[1] * VT(p1)), double(d[2] * VT(p2)));
         });
-    block_partials<1>(part, P.part_a);
+    if (ran) block_partials<1>(part, P.part_a);
 }
 
 // B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p_k ;
@@ -393,31 +520,37 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
 template <int G, bool JAC, bool FLEX, class VT>
 __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
     CgScalars* sc = P.sc;
-    if (sc->status != CG_RUNNING) return;
-    const int64_t k = sc->kcur;  // written by this iteration's A-kernel
-    double tot[1];
-    sum_partials<1>(P.part_a, P.nb_a, tot);
-    const double pAp = tot[0];
-    const double zTr = sc->zrec[k & 1];
+    double acc[1];
+    double alpha = 0.0;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    int decided = CG_RUNNING;
-    if (pAp <= 0.0)
-        decided = HOEFEM_ERROR_CG_INDEFINITE_OPERATOR;
-    else if (zTr <= 0.0)
-        decided = HOEFEM_ERROR_CG_INDEFINITE_PRECONDITIONER;
-    if (decided != CG_RUNNING) {
+    auto pro_issue = [&] { partials_load<1>(P.part_a, P.nb_a, acc); };
+    auto pro_finish = [&]() -> bool {
+        if (sc->status != CG_RUNNING) return false;
+        const int64_t k = sc->kcur;  // written by this iteration's A-kernel
+        double tot[1];
+        partials_reduce<1>(acc, tot);
+        const double pAp = tot[0];
+        const double zTr = sc->zrec[k & 1];
+        int decided = CG_RUNNING;
+        if (pAp <= 0.0)
+            decided = HOEFEM_ERROR_CG_INDEFINITE_OPERATOR;
+        else if (zTr <= 0.0)
+            decided = HOEFEM_ERROR_CG_INDEFINITE_PRECONDITIONER;
+        if (decided != CG_RUNNING) {
+            if (lead) {
+                sc->pAp = pAp;
+                sc->status = decided;
+            }
+            return false;
+        }
+        alpha = zTr / pAp;
         if (lead) {
             sc->pAp = pAp;
-            sc->status = decided;
+            sc->alpha = alpha;
+            sc->iters = k + 1;
         }
-        return;
-    }
-    const double alpha = zTr / pAp;
-    if (lead) {
-        sc->pAp = pAp;
-        sc->alpha = alpha;
-        sc->iters = k + 1;
-    }
+        return true;
+    };
     const double* ra = P.ra_old;
     double part[2] = {0.0, 0.0};
     auto rvec = [&](int64_t b, double& x0, double& x1, double& x2) {
@@ -447,17 +580,19 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
         part[1] += r0 * r0 + r1 * r1 + r2 * r2;
     };
     if (JAC) {
+        pro_issue();
+        if (!pro_finish()) return;
         for (int64_t v = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; v < Pm.nn; v += (int64_t)gridDim.x * CG_THREADS) {
             double r0, r1, r2;
             rvec(v, r0, r1, r2);
             finish(v, r0, r1, r2, 0.0, 0.0, 0.0);
         }
     } else {
-        for_entries<G>(
-            Pm.ent, Pm.n_block, Pm.n_diag,
-            [&](const NodeEntry& e, int lane, bool valid) {
+        const bool ran = for_entries_pf<G>(
+            Pm, pro_issue, pro_finish,
+            [&](const NodeEntry& e, int lane, bool valid, const RowPf<VT>& pf) {
                 double z0, z1, z2;
-                entry_row_dot<G>(Pm, e, lane, valid, rvec, z0, z1, z2);
+                entry_row_dot_pf<G>(Pm, e, lane, valid, pf, rvec, z0, z1, z2);
                 if (valid && lane == 0) {
                     double r0, r1, r2;
                     rvec(e.node, r0, r1, r2);
@@ -471,6 +606,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
                 rvec(v, r0, r1, r2);
                 finish(v, r0, r1, r2, double(d[0] * VT(r0)), double(d[1] * VT(r1)), double(d[2] * VT(r2)));
             });
+        if (!ran) return;
     }
     if (!FLEX) block_partials<2>(part, P.part_b);
 }
