@@ -52,6 +52,7 @@ EXPORTED = [
     "hoefem_sim_representative_force",
     "hoefem_sim_mass_values",
     "hoefem_sim_jacobian_values",
+    "hoefem_sim_damping_values",
     "hoefem_sim_residual",
     "hoefem_sim_step",
     "hoefem_learned_param_count",
@@ -159,6 +160,8 @@ class SimDesc(C.Structure):
         ("cg_tolerance", C.c_double),
         ("cg_max_iter", C.c_int64),
         ("mixed_precision", C.c_int32),
+        ("mass_damping_coefficient", C.c_double),
+        ("stiffness_damping_coefficient", C.c_double),
     ]
 
 
@@ -235,6 +238,7 @@ def load(path: str = LIB_PATH):
     L.hoefem_sim_representative_force.restype = f64
     L.hoefem_sim_mass_values.argtypes = [vp, vp]
     L.hoefem_sim_jacobian_values.argtypes = [vp, vp]
+    L.hoefem_sim_damping_values.argtypes = [vp, vp]
     L.hoefem_sim_residual.argtypes = [vp, f64, f64, vp, vp, vp, vp]
     L.hoefem_sim_step.argtypes = [vp, f64, f64, vp, vp, C.POINTER(StepStats)]
     L.hoefem_learned_param_count.argtypes = []

@@ -12,7 +12,8 @@ namespace hoefem {
 // solver.hip
 void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu);
 void launch_residual(Model& m, const double* Mvals, const double* gforce, const uint8_t* dirmask, double dt,
-                     const double* w, double* F, CgWork& cw, double* norm2);
+                     const double* w, const double* dv, double* F, CgWork& cw, double* norm2);
+void launch_state_v(Model& m, const double* vn, const double* w, const uint8_t* dirmask, double* vv);
 void launch_axpy_neg(Model& m, double* w, double delta, const double* sol);
 void launch_be_begin(Model& m, const double* u, const double* v, double dt, double* up, double* vn);
 void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask);
@@ -152,6 +153,8 @@ struct Sim {
     double rep_force = 0.0;
     int64_t cg_total = 0, evals = 0;
     int last_err_flags = 0;
+    bool damping = false;           // a damping matrix D is in use
+    DevBuf<double> d_D, d_vv, d_dv;  // D (model pattern), v = v_n + w, D v
 };
 
 static void build_bc(Model& m, const std::vector<int32_t>& dir, DevBuf<uint8_t>& dmask, DevBuf<int32_t>& dbc,
@@ -185,6 +188,16 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
     for (DevBuf<double>* b : {&s->d_gforce, &s->d_up, &s->d_vn, &s->d_w, &s->d_F, &s->d_sol, &s->d_uu, &s->d_us, &s->d_vs})
         b->ensure(n);
     if (d->mixed_precision < 0 || d->mixed_precision > 2) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "mixed_precision must be 0, 1 or 2");
+    if (!(d->mass_damping_coefficient >= 0.0 && std::isfinite(d->mass_damping_coefficient)) ||
+        !(d->stiffness_damping_coefficient >= 0.0 && std::isfinite(d->stiffness_damping_coefficient)))
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "damping coefficients must be finite and >= 0");
+    s->damping = d->mass_damping_coefficient > 0.0 || d->stiffness_damping_coefficient > 0.0;
+    if (s->damping) {
+        s->d_D.ensure(nnz);
+        s->d_vv.ensure(n);
+        s->d_dv.ensure(n);
+        HIP_CHECK(hipMemsetAsync(s->d_D.get(), 0, sizeof(double) * nnz, m.stream));
+    }
     s->mixed = d->mixed_precision != 0;
     s->mixed_mode = d->mixed_precision;
     s->d_M.ensure(nnz);
@@ -229,7 +242,12 @@ static double sim_eval(Sim& s, double dt, const double* w) {
     Model& m = *s.model;
     launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
     launch_element_forces(m, s.mu_r, s.lambda_r, s.d_uu.get());
-    launch_residual(m, s.d_M.get(), s.d_gforce.get(), s.d_dirmask.get(), dt, w, s.d_F.get(), s.cg, s.d_norm2.get());
+    if (s.damping) {  // D v with v = v_n + w (Dirichlet velocities zero)
+        launch_state_v(m, s.d_vn.get(), w, s.d_dirmask.get(), s.d_vv.get());
+        launch_spmv(m.pattern, s.d_D.get(), 0.0, s.d_dv.get(), 1.0, s.d_vv.get(), m.stream);
+    }
+    launch_residual(m, s.d_M.get(), s.d_gforce.get(), s.d_dirmask.get(), dt, w, s.damping ? s.d_dv.get() : nullptr,
+                    s.d_F.get(), s.cg, s.d_norm2.get());
     HIP_CHECK(hipMemcpyAsync(m.h_err.p, m.d_err.get(), sizeof(int), hipMemcpyDeviceToHost, m.stream));
     HIP_CHECK(hipMemcpyAsync(s.h_norm2.p, s.d_norm2.get(), sizeof(double), hipMemcpyDeviceToHost, m.stream));
     HIP_CHECK(hipStreamSynchronize(m.stream));
@@ -251,20 +269,23 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
     launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_uu.get(), s.mixed_mode == 1);
     const double alpha = -dt * dt;
     if (s.mixed_mode == 1) {
-        launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -alpha, s.d_M.get(), true, m.stream);
+        launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -alpha, s.d_M.get(), true, m.stream,
+                            s.damping ? s.d_D.get() : nullptr, dt, 1.0);
         launch_dirichlet(m.pattern, s.d_J32.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
                          m.stream);
         if (s.schwarz) schwarz_update(*s.schwarz, s.d_J32.get());
     } else if (s.mixed_mode == 2) {
         // FP64 Jacobian (for the refinement residual and the Schwarz factorizations), FP32 copy
         // of its values for the CG operator; P is stored in FP32
-        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream);
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream,
+                            s.damping ? s.d_D.get() : nullptr, dt, 1.0);
         launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
                          m.stream);
         launch_f64_to_f32(m, s.d_J.get(), s.d_J32.get(), m.pattern.nnz());
         if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
     } else {
-        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream);
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream,
+                            s.damping ? s.d_D.get() : nullptr, dt, 1.0);
         launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
                          m.stream);
         if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
@@ -352,9 +373,37 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
 
 // backward_euler_step (hamilton2/src/integrators/euler.rs:155-199) with newton_line_search and
 // BacktrackingLineSearch (hamilton2/src/newton.rs:76-131, :171-246). u, v: device state.
+// compute_damping_matrix_into (simulation_toolbox/src/fem/system_assembly.rs:145-194) at the
+// (sub)step's initial displacement: D = 0 + a_K K+(u) + a_M M, K+ without boundary conditions and
+// with every element matrix projected onto the positive semi-definite cone unless the material
+// is positive semi-definite by construction (ProjectedStableNeoHookean).
+static hoefem_status sim_damping(Sim& s, const double* u) {
+    Model& m = *s.model;
+    const double aK = s.d.stiffness_damping_coefficient, aM = s.d.mass_damping_coefficient;
+    const int64_t nnz = m.pattern.nnz();
+    if (aK > 0.0) {
+        launch_element_stiffness(m, s.mu_r, s.lambda_r, u, false);
+        if (m.material != dev::MAT_PROJECTED_SNH) launch_element_psd_projection(m);
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_D.get(), 1, aK, aM > 0.0 ? s.d_M.get() : nullptr,
+                            true, m.stream, nullptr, 0.0, aM);
+    } else {
+        launch_lincomb(m.stream, nnz, s.d_D.get(), aM, s.d_M.get());
+    }
+    const int e = read_and_clear_err(m);
+    if (e) {
+        s.last_err_flags |= e;
+        return HOEFEM_ERROR_SINGULAR;
+    }
+    return HOEFEM_OK;
+}
+
 static hoefem_status sim_be_step(Sim& s, double dt, double* u, double* v, int64_t* newton_iters, double* final_res,
                                  double* tol_out) {
     Model& m = *s.model;
+    if (s.damping) {
+        const hoefem_status ds = sim_damping(s, u);
+        if (ds != HOEFEM_OK) return ds;
+    }
     launch_be_begin(m, u, v, dt, s.d_up.get(), s.d_vn.get());
     HIP_CHECK(hipMemsetAsync(s.d_w.get(), 0, sizeof(double) * 3 * m.nv, m.stream));
     const double tol = dt * (s.d.newton_tolerance_factor * s.rep_force);
@@ -853,6 +902,19 @@ hoefem_status hoefem_sim_mass_values(hoefem_sim* sim, double* values) {
         if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
         Model& m = *sim->s.model;
         HIP_CHECK(hipMemcpyAsync(values, sim->s.d_M.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice, m.stream));
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+    });
+}
+
+hoefem_status hoefem_sim_damping_values(hoefem_sim* sim, double* values) {
+    return guarded([&] {
+        if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+        Model& m = *sim->s.model;
+        const int64_t nnz = m.pattern.nnz();
+        if (sim->s.damping)
+            HIP_CHECK(hipMemcpyAsync(values, sim->s.d_D.get(), sizeof(double) * nnz, hipMemcpyDeviceToDevice, m.stream));
+        else
+            HIP_CHECK(hipMemsetAsync(values, 0, sizeof(double) * nnz, m.stream));
         HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }

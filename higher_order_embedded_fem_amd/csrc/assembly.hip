@@ -281,7 +281,7 @@ __global__ __launch_bounds__(64) void k_block_gather(
     const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk, const int32_t* __restrict__ inc_loc,
     const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off, const int64_t* __restrict__ slot_off,
     const int32_t* __restrict__ slot, const TB* __restrict__ blocks, TV* __restrict__ values, int mode,
-    double c, const double* __restrict__ M, int zero_first) {
+    double c, const double* __restrict__ M, int zero_first, const double* __restrict__ D2, double cd, double cm) {
     extern __shared__ double s_rb[];
     const int64_t v = blockIdx.x;
     const int t = threadIdx.x;
@@ -312,9 +312,12 @@ __global__ __launch_bounds__(64) void k_block_gather(
         if (mode == 0)
             values[r0 + idx] = (TV)s;
         else {
+            // compute_jacobian_combination_into (system_assembly.rs:197-235): the terms in the order
+            // stiffness, damping, mass
             double v2 = 0.0;
             v2 += c * s;
-            v2 += 1.0 * M[r0 + idx];
+            if (D2) v2 += cd * D2[r0 + idx];
+            if (M) v2 += cm * M[r0 + idx];
             values[r0 + idx] = (TV)v2;
         }
     }
@@ -559,24 +562,29 @@ void launch_element_mass(Model& m, double density) {
 
 template <class TB, class TV>
 void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const TB* blocks, TV* values, int mode,
-                         double c, const double* M, bool zero_first, hipStream_t s) {
+                         double c, const double* M, bool zero_first, hipStream_t s, const double* D2, double cd,
+                         double cm) {
     if (pat.num_nodes == 0) return;
     const size_t lds = (size_t)9 * std::max(pat.max_nbr, 1) * sizeof(double);
     if (lds > 64 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "node row block exceeds 64 KiB of LDS");
     k_block_gather<TB, TV><<<dim3((unsigned)pat.num_nodes), dim3(64), lds, s>>>(
         pat.d_nptr.get(), pat.d_rowptr.get(), pat.d_diag_only.get(), sc.d_inc_ptr.get(), sc.d_inc_blk.get(),
         sc.d_inc_loc.get(), sc.d_m.get(), sc.d_val_off.get(), sc.d_slot_off.get(), sc.d_slot.get(), blocks, values, mode,
-        c, M, zero_first ? 1 : 0);
+        c, M, zero_first ? 1 : 0, D2, cd, cm);
     check_launch("k_block_gather");
 }
-template void launch_block_gather<double, double>(const NodePattern&, const BlockScatter&, const double*, double*, int,
-                                                  double, const double*, bool, hipStream_t);
-template void launch_block_gather<float, float>(const NodePattern&, const BlockScatter&, const float*, float*, int,
-                                                double, const double*, bool, hipStream_t);
-template void launch_block_gather<double, float>(const NodePattern&, const BlockScatter&, const double*, float*, int,
-                                                 double, const double*, bool, hipStream_t);
-template void launch_block_gather<float, double>(const NodePattern&, const BlockScatter&, const float*, double*, int,
-                                                 double, const double*, bool, hipStream_t);
+template void launch_block_gather<double, double>(const NodePattern&, const BlockScatter&, const double*,
+                                                  double*, int, double, const double*, bool, hipStream_t,
+                                                  const double*, double, double);
+template void launch_block_gather<float, float>(const NodePattern&, const BlockScatter&, const float*,
+                                                  float*, int, double, const double*, bool, hipStream_t,
+                                                  const double*, double, double);
+template void launch_block_gather<double, float>(const NodePattern&, const BlockScatter&, const double*,
+                                                  float*, int, double, const double*, bool, hipStream_t,
+                                                  const double*, double, double);
+template void launch_block_gather<float, double>(const NodePattern&, const BlockScatter&, const float*,
+                                                  double*, int, double, const double*, bool, hipStream_t,
+                                                  const double*, double, double);
 
 void launch_force_gather(Model& m, double* f) {
     if (m.nv == 0) return;
@@ -698,6 +706,20 @@ void rotation_svd_eval(int64_t nq, const double* F, double* U, double* S, double
     hipLaunchKernelGGL(k_rotation_svd, dim3((unsigned)((nq + 127) / 128)), dim3(128), 0, nullptr, nq, F, U, S, Vt);
     check_launch("k_rotation_svd");
     HIP_CHECK(hipDeviceSynchronize());
+}
+
+__global__ void k_lincomb(int64_t n, double* __restrict__ out, double a, const double* __restrict__ A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    v += a * A[i];
+    out[i] = v;
+}
+
+void launch_lincomb(hipStream_t s, int64_t n, double* out, double a, const double* A) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, out, a, A);
+    check_launch("k_lincomb");
 }
 
 int read_and_clear_err(Model& m) {

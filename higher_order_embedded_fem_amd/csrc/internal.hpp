@@ -167,6 +167,8 @@ struct Model {
     DevBuf<double> d_ke;  // element matrix staging [E * (3n)^2]
     DevBuf<float> d_ke32;  // FP32 element matrix staging (mixed precision)
     DevBuf<int> d_err;
+    DevBuf<int32_t> d_elem_m;   // per element: nodes (n), for the batched element-matrix kernels
+    DevBuf<int64_t> d_elem_off; // per element: offset of its (3n)^2 block in d_ke
     PinnedBuf<int> h_err{1};
     // kernel timing of the CG system-matrix kernel (HIP events on the model stream)
     bool prof = false;
@@ -237,8 +239,10 @@ void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const doub
 void launch_element_mass(Model& m, double density);  // -> m.d_ke (includes regularization)
 // values (+)= gathered blocks; mode 0: values += sum; mode 1: values = (0 + c*sum) + 1*M
 template <class TB, class TV>
+// mode 0: values = sum of the blocks; mode 1: values = c * sum + cd * D2 + cm * M (D2, M optional)
 void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const TB* blocks, TV* values, int mode,
-                         double c, const double* M, bool zero_first, hipStream_t s);
+                         double c, const double* M, bool zero_first, hipStream_t s, const double* D2 = nullptr,
+                         double cd = 0.0, double cm = 1.0);
 void launch_force_gather(Model& m, double* f);
 template <class VT>
 void launch_spmv(const NodePattern& pat, const VT* vals, double beta, double* y, double alpha, const double* x,
@@ -250,6 +254,8 @@ int spmv_group_size(const NodePattern& pat);
 // Times the node-blocked SpMV with 8, 16 and 32 lanes per node on this pattern and keeps the
 // fastest in pat.tuned_group; returns it. Per-candidate averages (us) go to times_us[3] if given.
 int autotune_spmv(NodePattern& pat, hipStream_t s, double* times_us);
+void launch_element_psd_projection(Model& m);  // in place on m.d_ke
+void launch_lincomb(hipStream_t s, int64_t n, double* out, double a, const double* A);  // out = 0 + a A
 void rotation_svd_eval(int64_t nq, const double* F, double* U, double* S, double* Vt);
 void material_eval(int mat, double thr, double mu_r, double lambda_r, int64_t nq, const double* F, double* P,
                    double* dPdF);

@@ -206,10 +206,15 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
     if (t == 0) fallback[b] = 0;
 }
 
+// Cyclic Jacobi eigen-decomposition of each symmetric block in LDS, then the block is replaced by
+// V f(lambda) V^T: project == 0, invert_element_block's clamped inverse (schwarz_precond.rs:329-342);
+// project == 1, the positive semi-definite projection max(lambda, 0) of DefaultSemidefiniteProjection
+// (fenris/src/assembly.rs:552-564).
 __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_inverse(const int32_t* __restrict__ msz,
                                                                 const int64_t* __restrict__ val_off,
                                                                 double* __restrict__ blocks,
-                                                                const int* __restrict__ fallback, int ldpad) {
+                                                                const int* __restrict__ fallback, int ldpad,
+                                                                int project) {
     if (fallback && fallback[blockIdx.x] == 0) return;
     extern __shared__ double lds[];
     const int64_t b = blockIdx.x;
@@ -341,8 +346,12 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_inverse(const int32_t* _
     const double lower = 1e-12 * s_red[0];
     __syncthreads();
     for (int k = t; k < dim; k += EIG_THREADS) {
-        const double ev = fabs(A[k * ld + k]);
-        cs[k] = ev >= lower ? 1.0 / ev : 1.0 / lower;
+        if (project) {
+            cs[k] = fmax(A[k * ld + k], 0.0);
+        } else {
+            const double ev = fabs(A[k * ld + k]);
+            cs[k] = ev >= lower ? 1.0 / ev : 1.0 / lower;
+        }
     }
     __syncthreads();
     for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
@@ -397,7 +406,7 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         const int ld = nn + 1;
         const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
         hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st, s.scatter.d_m.get(),
-                           s.scatter.d_val_off.get(), s.d_blocks.get(), (const int*)s.d_fallback.get(), ld);
+                           s.scatter.d_val_off.get(), s.d_blocks.get(), (const int*)s.d_fallback.get(), ld, 0);
         check_launch("k_jacobi_inverse");
         if (s.fp32)
             launch_block_gather(s.P, s.scatter, s.d_blocks.get(), s.d_vals32.get(), 0, 0.0, nullptr, true, st);
@@ -417,6 +426,29 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
                 s.d_vals.get(), m.d_err.get());
         check_launch("k_schwarz_untreated");
     }
+}
+
+// DefaultSemidefiniteProjection of every element matrix in m.d_ke (row-major (3n)^2 per element),
+// in place: one work-group per element runs the batched Jacobi kernel in projection mode.
+void launch_element_psd_projection(Model& m) {
+    if (m.ne == 0) return;
+    if (m.d_elem_m.n != (size_t)m.ne || m.d_elem_m.p == nullptr) {
+        std::vector<int32_t> msz((size_t)m.ne, m.n);
+        std::vector<int64_t> off((size_t)m.ne);
+        for (int64_t e = 0; e < m.ne; ++e) off[e] = e * 9 * (int64_t)m.n * m.n;
+        m.d_elem_m.upload(msz, m.stream);
+        m.d_elem_off.upload(off, m.stream);
+    }
+    const int dim = 3 * m.n;
+    const int nn = dim + (dim & 1);
+    const int ld = nn + 1;
+    const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
+    HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  120 * 1024));
+    if (lds > 120 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "element matrix too large for the projection");
+    hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)m.ne), dim3(EIG_THREADS), lds, m.stream, m.d_elem_m.get(),
+                       m.d_elem_off.get(), m.d_ke.get(), (const int*)nullptr, ld, 1);
+    check_launch("k_jacobi_inverse (projection)");
 }
 
 void schwarz_update(Schwarz& s, const double* J) { schwarz_update_t<double>(s, J); }

@@ -998,7 +998,8 @@ __global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3<double> M, const i
                                                          const double* __restrict__ fe,
                                                          const double* __restrict__ gforce,
                                                          const uint8_t* __restrict__ dirmask, double dt,
-                                                         const double* __restrict__ w, double* __restrict__ F,
+                                                         const double* __restrict__ w,
+                                                         const double* __restrict__ dv, double* __restrict__ F,
                                                          double* partials) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
@@ -1018,6 +1019,11 @@ __global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3<double> M, const i
             f0 += src[0];
             f1 += src[1];
             f2 += src[2];
+        }
+        if (dv) {  // f += -D v (spmv_csr with beta = 1, alpha = -1)
+            f0 = 1.0 * f0 + -1.0 * dv[3 * v];
+            f1 = 1.0 * f1 + -1.0 * dv[3 * v + 1];
+            f2 = 1.0 * f2 + -1.0 * dv[3 * v + 2];
         }
         if (dirmask[v]) f0 = f1 = f2 = 0.0;
         const double sc = dt * -1.0;
@@ -1146,6 +1152,21 @@ void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n) {
     check_launch("k_f32_to_f64");
 }
 
+// v = v_n + w with the Dirichlet velocities applied (zero): the velocity eval_f sees
+__global__ void k_state_v(int64_t ndof, const double* __restrict__ vn, const double* __restrict__ w,
+                          const uint8_t* __restrict__ dirmask, double* __restrict__ vv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndof) return;
+    const double val = vn[i] + w[i];
+    vv[i] = dirmask[i / 3] ? 0.0 : val;
+}
+
+void launch_state_v(Model& m, const double* vn, const double* w, const uint8_t* dirmask, double* vv) {
+    const int64_t n = 3 * m.nv;
+    hipLaunchKernelGGL(k_state_v, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, vn, w, dirmask, vv);
+    check_launch("k_state_v");
+}
+
 void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu) {
     const int64_t n = 3 * m.nv;
     hipLaunchKernelGGL(k_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, up, w, dt, dirmask, uu);
@@ -1153,7 +1174,7 @@ void launch_state(Model& m, const double* up, const double* w, double dt, const 
 }
 
 void launch_residual(Model& m, const double* Mvals, const double* gforce, const uint8_t* dirmask, double dt,
-                     const double* w, double* F, CgWork& cw, double* norm2) {
+                     const double* w, const double* dv, double* F, CgWork& cw, double* norm2) {
     const NodePattern& pat = m.pattern;
     const int G = spmv_group_size(pat);
     const Csr3<double> Mv = view(pat, Mvals);
@@ -1162,7 +1183,7 @@ void launch_residual(Model& m, const double* Mvals, const double* gforce, const 
 #define RES_CASE(GG)                                                                                              \
     hipLaunchKernelGGL(k_residual<GG>, dim3(blocks), dim3(CG_THREADS), 0, m.stream, Mv, m.scatter.d_inc_ptr.get(), \
                        m.scatter.d_inc_blk.get(), m.scatter.d_inc_loc.get(), m.n, m.d_fe.get(), gforce, dirmask, dt,  \
-                       w, F, cw.partials.get())
+                       w, dv, F, cw.partials.get())
     if (G == 32)
         RES_CASE(32);
     else if (G == 16)

@@ -252,7 +252,8 @@ class NewtonCgIntegrator:
     def __init__(self, model: ElasticityModel, scene: Scene, newton_tolerance_factor: float = 1e-5,
                  cg_tolerance: float = 1e-5, cg_max_iter: int = 500, max_newton_iter: int = 100,
                  schwarz: bool = True, gravity_enabled: bool = True, mixed_precision: bool = False,
-                 material: Optional[StableNeoHookeanMaterial] = None):
+                 material: Optional[StableNeoHookeanMaterial] = None, mass_damping: float = 0.0,
+                 stiffness_damping: float = 0.0):
         L = _lib.load()
         self.model, self.scene = model, scene
         mu, lam = lame_from_young_poisson(scene.young, scene.poisson)
@@ -276,6 +277,9 @@ class NewtonCgIntegrator:
         d.cg_tolerance = cg_tolerance
         d.cg_max_iter = cg_max_iter
         d.mixed_precision = int(mixed_precision)  # False/0, True/1 or 2 (see hoefem_sim_desc)
+        # Material::mass_damping_coefficient / stiffness_damping_coefficient (fe_model.rs:131-136); 0 = None
+        d.mass_damping_coefficient = float(mass_damping)
+        d.stiffness_damping_coefficient = float(stiffness_damping)
         h = C.c_void_p()
         check(L.hoefem_sim_create(model.h, C.byref(d), C.byref(h)))
         self.h = h
@@ -302,6 +306,12 @@ class NewtonCgIntegrator:
     def mass(self) -> torch.Tensor:
         out = self.model.zeros_matrix()
         check(_lib.load().hoefem_sim_mass_values(self.h, _dev(out)))
+        return out
+
+    def damping(self) -> torch.Tensor:
+        """The damping matrix of the last (sub)step on the model pattern (zeros without damping)."""
+        out = torch.empty(self.model.nnz, dtype=torch.float64, device=self.model.device)
+        check(_lib.load().hoefem_sim_damping_values(self.h, out.data_ptr()))
         return out
 
     def jacobian(self) -> torch.Tensor:

@@ -225,6 +225,14 @@ typedef struct hoefem_sim_desc {
                                               FP32 copies of the Jacobian and Schwarz values inside an FP64
                                               iterative refinement of each linear solve (same relative
                                               tolerance and iteration limit as the plain CG). */
+    /* Damping D = mass_damping_coefficient M + stiffness_damping_coefficient K+(u), K+ the stiffness
+       at the (sub)step's initial displacement with every element matrix projected onto the positive
+       semi-definite cone (unless the material is ProjectedStableNeoHookean), rebuilt at the start of
+       every (sub)step (compute_damping_matrix_into, simulation_toolbox/src/fem/system_assembly.rs:
+       145-194; integrate_model, newton_cg.rs:256-265). f gains -D v and the Newton matrix
+       M + dt D + dt^2 K. 0 leaves a term out (Material's None); both 0: no damping. */
+    double mass_damping_coefficient;
+    double stiffness_damping_coefficient;
 } hoefem_sim_desc;
 
 typedef struct hoefem_step_stats {
@@ -242,6 +250,9 @@ double hoefem_sim_representative_force(const hoefem_sim* sim);
 /* mass matrix with Dirichlet conditions applied (device, nnz) */
 hoefem_status hoefem_sim_mass_values(hoefem_sim* sim, double* values);
 /* J = M + dt^2 K with Dirichlet conditions of the last Newton iteration (device, nnz) */
+/* The damping matrix D of the current (sub)step on the model pattern (device output, nnz);
+ * zeros when the simulation has no damping. */
+hoefem_status hoefem_sim_damping_values(hoefem_sim* sim, double* values);
 hoefem_status hoefem_sim_jacobian_values(hoefem_sim* sim, double* values);
 /* F(w) of BackwardEulerImplicitFunction::eval_into at (u, v) (device vectors) */
 hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const double* u, const double* v,

@@ -66,6 +66,8 @@ class OrcSimDesc(C.Structure):
         ("cg_tolerance", C.c_double),
         ("cg_max_iter", C.c_int64),
         ("num_threads", C.c_int32),
+        ("mass_damping_coefficient", C.c_double),
+        ("stiffness_damping_coefficient", C.c_double),
     ]
 
 
@@ -133,6 +135,9 @@ def lib():
         L.orc_model_assemble_mass.argtypes = [C.c_void_p, C.c_double, f64p]
         L.orc_model_forces_add.argtypes = [C.c_void_p, C.c_double, C.c_double, f64p, f64p]
         L.orc_model_stiffness_add.argtypes = [C.c_void_p, C.c_double, C.c_double, f64p, f64p]
+        L.orc_model_stiffness_add_projected.argtypes = [C.c_void_p, C.c_double, C.c_double, f64p, f64p]
+        L.orc_psd_project.argtypes = [C.c_int, f64p, f64p]
+        L.orc_sim_damping_values.argtypes = [C.c_void_p, f64p]
         L.orc_greedy_coloring.argtypes = [C.c_int64, i64p, i32p, i32p, C.POINTER(C.c_int32)]
         L.orc_coo_pattern.argtypes = [C.c_int64, C.c_int64, i64p, i32p, C.c_int, C.POINTER(C.c_int64), C.c_void_p,
                                       C.c_void_p]
@@ -244,6 +249,15 @@ def _colmajor(F):
 
 def _rowmajor(buf, nq):
     return buf.reshape(nq, 3, 3).transpose(0, 2, 1).copy()
+
+
+def psd_project(A):
+    """DefaultSemidefiniteProjection (fenris/src/assembly.rs:552-564) of a symmetric matrix."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    out = np.empty(n * n)
+    check(lib().orc_psd_project(n, np.ascontiguousarray(A.T).ravel(), out))
+    return out.reshape(n, n).T.copy()
 
 
 def svd3(F):
@@ -398,6 +412,12 @@ class Model:
     def forces_add(self, mu, lam, u, f):
         check(lib().orc_model_forces_add(self.h, mu, lam, np.ascontiguousarray(u, dtype=np.float64), f))
         return f
+
+    def stiffness_add_projected(self, mu, lam, u, vals):
+        """values += the stiffness with every element matrix projected onto the PSD cone."""
+        check(lib().orc_model_stiffness_add_projected(self.h, mu, lam, np.ascontiguousarray(u, dtype=np.float64),
+                                                      vals))
+        return vals
 
     def stiffness_add(self, mu, lam, u, vals):
         check(lib().orc_model_stiffness_add(self.h, mu, lam, np.ascontiguousarray(u, dtype=np.float64), vals))
@@ -582,7 +602,7 @@ class Sim:
     """Oracle FE dynamic system: NewtonCgIntegrator3d semantics on the CPU."""
 
     def __init__(self, model, newton_tolerance_factor=1e-5, cg_tolerance=1e-5, cg_max_iter=500,
-                 max_newton_iter=100, schwarz=True, num_threads=0):
+                 max_newton_iter=100, schwarz=True, num_threads=0, mass_damping=0.0, stiffness_damping=0.0):
         sc = model.scene
         mu, lam = lame(sc.young, sc.poisson)
         d = OrcSimDesc()
@@ -606,6 +626,8 @@ class Sim:
         d.cg_tolerance = cg_tolerance
         d.cg_max_iter = cg_max_iter
         d.num_threads = num_threads
+        d.mass_damping_coefficient = float(mass_damping)
+        d.stiffness_damping_coefficient = float(stiffness_damping)
         h = C.c_void_p()
         check(lib().orc_sim_create(model.h, C.byref(d), C.byref(h)))
         self.h = h
@@ -625,6 +647,11 @@ class Sim:
     def mass(self):
         out = np.zeros(self.model.nnz)
         check(lib().orc_sim_mass_values(self.h, out))
+        return out
+
+    def damping(self):
+        out = np.zeros(self.model.nnz)
+        check(lib().orc_sim_damping_values(self.h, out))
         return out
 
     def jacobian(self):

@@ -1379,7 +1379,23 @@ int orc_model_forces_add(const orc_model* m, double mu0, double lambda0, const d
 
 // CsrParAssembler::assemble_into_csr (fenris/src/assembly.rs:253-313): per color, element
 // matrices added row by row with add_element_row_to_csr_row. values += K.
-int orc_model_stiffness_add(const orc_model* m, double mu0, double lambda0, const double* u, double* values) {
+}  // extern "C"
+
+namespace {
+// DefaultSemidefiniteProjection (fenris/src/assembly.rs:552-564) of a symmetric dim x dim matrix
+// (column-major, in place): eigen-decomposition, eigenvalues clamped at zero, recomposed.
+void psd_project(int dim, double* K) {
+    std::vector<double> a(K, K + (size_t)dim * dim), ev, V;
+    jacobi_eigen(dim, a, ev, V);
+    for (int c = 0; c < dim; ++c)
+        for (int r = 0; r < dim; ++r) {
+            double s = 0.0;
+            for (int k = 0; k < dim; ++k) s += V[r + (size_t)dim * k] * std::max(0.0, ev[k]) * V[c + (size_t)dim * k];
+            K[r + (size_t)dim * c] = s;
+        }
+}
+
+int stiffness_add_impl(const orc_model* m, double mu0, double lambda0, const double* u, double* values, bool project) {
     double mu, lambda;
     snh_reparam(mu0, lambda0, &mu, &lambda);
     const int n = m->n, dim = 3 * n;
@@ -1401,6 +1417,7 @@ int orc_model_stiffness_add(const orc_model* m, double mu0, double lambda0, cons
                     status = ORC_ERR_SINGULAR;
                     continue;
                 }
+                if (project) psd_project(dim, Ke.data());
                 const int32_t* g = m->conn.data() + e * n;
                 std::vector<int> perm(n);
                 std::iota(perm.begin(), perm.end(), 0);
@@ -1415,6 +1432,26 @@ int orc_model_stiffness_add(const orc_model* m, double mu0, double lambda0, cons
             if (status != ORC_OK) return fail(status, "stiffness assembly failed");
         }
     std::copy(csr.val.begin(), csr.val.end(), values);
+    return ORC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int orc_model_stiffness_add(const orc_model* m, double mu0, double lambda0, const double* u, double* values) {
+    return stiffness_add_impl(m, mu0, lambda0, u, values, false);
+}
+
+// assemble_transformed_stiffness_into_par with DefaultSemidefiniteProjection (fenris/src/solid/
+// impl_model.rs:146-164, fenris/src/assembly.rs:552-564): values += sum of projected K_e
+int orc_model_stiffness_add_projected(const orc_model* m, double mu0, double lambda0, const double* u,
+                                      double* values) {
+    return stiffness_add_impl(m, mu0, lambda0, u, values, true);
+}
+
+int orc_psd_project(int dim, const double* a, double* out) {
+    std::copy(a, a + (size_t)dim * dim, out);
+    psd_project(dim, out);
     return ORC_OK;
 }
 
@@ -1857,6 +1894,8 @@ struct orc_sim {
     std::vector<int32_t> dir;
     Csr M, K, J;
     std::vector<double> gforce;
+    Csr D;                 // damping matrix of the current (sub)step
+    bool damping = false;
     double rep_force = 0.0;
     orc_schwarz* schwarz = nullptr;
     std::vector<double> diag_inv;
@@ -1884,13 +1923,38 @@ int eval_f(orc_sim* s, const double* u_in, const double* v_in, double* f) {
     int st = orc_model_forces_add(s->model, s->d.mu, s->d.lambda, u.data(), f);
     if (s->stats) s->stats->t_forces += now_s() - t0;
     if (st != ORC_OK) return st;
+    // f += -D v
+    if (s->damping) orc_spmv(n, s->D.rowptr.data(), s->D.col.data(), s->D.val.data(), 1.0, f, -1.0, v.data());
     orc_dirichlet_rhs(f, (int64_t)s->dir.size(), s->dir.data());
+    return ORC_OK;
+}
+
+// compute_damping_matrix_into (simulation_toolbox/src/fem/system_assembly.rs:145-194), called by
+// integrate_model (newton_cg.rs:256-265) with the (sub)step's initial displacement:
+// D = 0 + a_K K+(u) + a_M M, K+ the stiffness without boundary conditions, element matrices
+// projected onto the positive semi-definite cone unless the material is positive semi-definite.
+int compute_damping(orc_sim* s, const double* u) {
+    const int64_t nnz = (int64_t)s->M.col.size();
+    const double aK = s->d.stiffness_damping_coefficient, aM = s->d.mass_damping_coefficient;
+    if (aK > 0.0) {
+        std::fill(s->K.val.begin(), s->K.val.end(), 0.0);
+        const int st = s->model->material == ORC_MAT_PROJECTED_SNH
+                           ? orc_model_stiffness_add(s->model, s->d.mu, s->d.lambda, u, s->K.val.data())
+                           : orc_model_stiffness_add_projected(s->model, s->d.mu, s->d.lambda, u, s->K.val.data());
+        if (st != ORC_OK) return st;
+    }
+    for (int64_t k = 0; k < nnz; ++k) {
+        double v = 0.0;
+        if (aK > 0.0) v += aK * s->K.val[k];
+        if (aM > 0.0) v += aM * s->M.val[k];
+        s->D.val[k] = v;
+    }
     return ORC_OK;
 }
 
 // NewtonCgDynamicSystem::solve_jacobian_combination (newton_cg.rs:680-802) with
 // (alpha, beta) = (-dt^2, -dt) from BackwardEulerImplicitFunction (hamilton2/src/integrators/euler.rs:122-140)
-int solve_jacobian(orc_sim* s, double alpha, const double* rhs, double* sol) {
+int solve_jacobian(orc_sim* s, double alpha, double beta, const double* rhs, double* sol) {
     const int64_t n = s->M.n;
     const int64_t nnz = (int64_t)s->M.col.size();
     double t0 = now_s();
@@ -1899,10 +1963,12 @@ int solve_jacobian(orc_sim* s, double alpha, const double* rhs, double* sol) {
     if (s->stats) s->stats->t_stiffness += now_s() - t0;
     if (st != ORC_OK) return st;
     t0 = now_s();
-    const double kc = -alpha;
+    // compute_jacobian_combination_into (system_assembly.rs:197-235): J = -alpha K - beta D + M
+    const double kc = -alpha, dc = -beta;
     for (int64_t k = 0; k < nnz; ++k) {
         double v = 0.0;
         v += kc * s->K.val[k];
+        if (s->damping) v += dc * s->D.val[k];
         v += 1.0 * s->M.val[k];
         s->J.val[k] = v;
     }
@@ -1956,7 +2022,7 @@ int be_step(orc_sim* s, double t0, double dt, double* u, double* v, int64_t* new
         s->su.assign(uu, uu + n);
         s->sv.assign(vv, vv + n);
         apply_bc_unknowns(s, s->su.data(), s->sv.data());
-        return solve_jacobian(s, alpha, rhs, sol);
+        return solve_jacobian(s, alpha, beta, rhs, sol);
     };
     int64_t evals = 0;
     double fnorm = 0.0, tol = 0.0;
@@ -1995,6 +2061,13 @@ int orc_sim_create(const orc_model* model, const orc_sim_desc* d, orc_sim** out)
     s->rep_force = norm2(tmp);
     s->K = model->pattern;
     s->J = model->pattern;
+    s->D = model->pattern;
+    std::fill(s->D.val.begin(), s->D.val.end(), 0.0);
+    if (!(d->mass_damping_coefficient >= 0.0) || !(d->stiffness_damping_coefficient >= 0.0)) {
+        delete s;
+        return fail(ORC_ERR_INVALID, "damping coefficients must be >= 0");
+    }
+    s->damping = d->mass_damping_coefficient > 0.0 || d->stiffness_damping_coefficient > 0.0;
     // compute_gravity_density (system_assembly.rs:30-75) then M * g
     std::vector<double> gd(n, 0.0);
     if (d->gravity_enabled)
@@ -2024,6 +2097,11 @@ double orc_sim_representative_force(const orc_sim* s) { return s->rep_force; }
 
 int orc_sim_mass_values(const orc_sim* s, double* values) {
     std::copy(s->M.val.begin(), s->M.val.end(), values);
+    return ORC_OK;
+}
+
+int orc_sim_damping_values(const orc_sim* s, double* values) {
+    std::copy(s->D.val.begin(), s->D.val.end(), values);
     return ORC_OK;
 }
 
@@ -2068,6 +2146,13 @@ int orc_sim_step(orc_sim* s, double t, double dt, double* u, double* v, orc_step
         int last = ORC_OK;
         for (int k = 0; k < num_substeps; ++k) {
             int64_t it = 0;
+            if (s->damping) {
+                const int dst = compute_damping(s, us.data());
+                if (dst != ORC_OK) {
+                    s->stats = nullptr;
+                    return dst;
+                }
+            }
             const int st = be_step(s, t + k * sdt, sdt, us.data(), vs.data(), &it);
             s->stats->newton_iterations += it;
             if (st != ORC_OK) {

@@ -128,6 +128,10 @@ int orc_model_pattern(const orc_model* m, int32_t* rowptr, int32_t* col);
 int orc_model_assemble_mass(const orc_model* m, double density, double* values);
 int orc_model_forces_add(const orc_model* m, double mu, double lambda, const double* u, double* f);
 int orc_model_stiffness_add(const orc_model* m, double mu, double lambda, const double* u, double* values);
+/* the same with every element matrix projected onto the positive semi-definite cone */
+int orc_model_stiffness_add_projected(const orc_model* m, double mu, double lambda, const double* u, double* values);
+/* DefaultSemidefiniteProjection of one symmetric dim x dim matrix (column-major) */
+int orc_psd_project(int dim, const double* a, double* out);
 
 /* paradis/src/coloring.rs:48-72 on an arbitrary NestedVec of node sets */
 int orc_greedy_coloring(int64_t nsets, const int64_t* offsets, const int32_t* nodes, int32_t* color_of_set,
@@ -191,6 +195,10 @@ typedef struct orc_sim_desc {
     double cg_tolerance;            /* 1e-5 */
     int64_t cg_max_iter;            /* 500 */
     int32_t num_threads;            /* 0: OpenMP default */
+    /* Material::mass_damping_coefficient / stiffness_damping_coefficient (0 = None):
+     * D = a_M M + a_K K+(u_n) (simulation_toolbox/src/fem/system_assembly.rs:145-194) */
+    double mass_damping_coefficient;
+    double stiffness_damping_coefficient;
 } orc_sim_desc;
 
 typedef struct orc_step_stats {
@@ -211,6 +219,8 @@ int orc_sim_mass_values(const orc_sim* s, double* values);
 int orc_sim_step(orc_sim* s, double t, double dt, double* u, double* v, orc_step_stats* stats);
 /* the Jacobian combination of the last Newton iteration (J = M + dt^2 K with BCs) */
 int orc_sim_jacobian_values(const orc_sim* s, double* values);
+/* the damping matrix of the last (sub)step (zeros without damping) */
+int orc_sim_damping_values(const orc_sim* s, double* values);
 int orc_sim_residual(orc_sim* s, double t, double dt, const double* u, const double* v, const double* w,
                      double* F);
 
