@@ -54,6 +54,9 @@ def parse():
     p.add_argument("--mixed-refined", action="store_true",
                    help="FP64 tangents and Jacobian, CG on FP32 copies inside FP64 iterative refinement")
     p.add_argument("--no-autotune", action="store_true", help="keep the heuristic SpMV lanes per node")
+    p.add_argument("--stiffness-damping", type=float, default=0.0,
+                   help="Material stiffness_damping_coefficient (the reference's cylinder shell scene uses 0.05)")
+    p.add_argument("--mass-damping", type=float, default=0.0, help="Material mass_damping_coefficient")
     return p.parse_args()
 
 
@@ -207,7 +210,8 @@ def main():
     t_setup = time.time()
     sc = scenes.config_scene(args.config)
     model = fem.ElasticityModel(sc)
-    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mode)
+    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mode,
+                                 mass_damping=args.mass_damping, stiffness_damping=args.stiffness_damping)
     L = _lib.load()
     lanes = None if args.no_autotune else sim.autotune()
     print(f"[bench] {sc.name}: {sc.num_elements} elements, {sc.ndof} dofs, nnz {model.nnz}, "
@@ -282,7 +286,8 @@ def main():
         rec = [args.cpu_steps, None]
         run(args.cpu_steps, record=rec)  # untimed GPU run for the error against the oracle
         om = O.Model(sc)
-        osim = O.Sim(om, newton_tolerance_factor=1e-6, num_threads=threads)
+        osim = O.Sim(om, newton_tolerance_factor=1e-6, num_threads=threads, mass_damping=args.mass_damping,
+                     stiffness_damping=args.stiffness_damping)
         uo = np.zeros(sc.ndof)
         vo = np.zeros(sc.ndof)
         c0 = time.perf_counter()
@@ -318,7 +323,9 @@ def main():
         "config": {"workload": sc.name, "element": scenes.ELEMENT_NAMES[sc.element_type], "cells": sc.num_elements,
                    "dofs": sc.ndof, "nnz": model.nnz, "dt": sc.dt, "newton_rel_tol": 1e-6, "cg_rel_tol": 1e-5,
                    "preconditioner": "schwarz", "precision": precision, "spmv_lanes_per_node": lanes,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   **({"mass_damping": args.mass_damping, "stiffness_damping": args.stiffness_damping}
+                      if args.mass_damping or args.stiffness_damping else {})},
         "newton_iterations_per_step": tot["newton_iterations"] / args.steps,
         "cg_iterations_per_step": tot["cg_iterations"] / args.steps,
         "error_vs_oracle": err,

@@ -169,6 +169,7 @@ struct Model {
     DevBuf<int> d_err;
     DevBuf<int32_t> d_elem_m;   // per element: nodes (n), for the batched element-matrix kernels
     DevBuf<int64_t> d_elem_off; // per element: offset of its (3n)^2 block in d_ke
+    DevBuf<int> d_elem_need;    // per element: its matrix needs the eigen path (projection)
     PinnedBuf<int> h_err{1};
     // kernel timing of the CG system-matrix kernel (HIP events on the model stream)
     bool prof = false;

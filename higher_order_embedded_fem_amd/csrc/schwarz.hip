@@ -206,6 +206,179 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
     if (t == 0) fallback[b] = 0;
 }
 
+// Positive semi-definiteness certificate of each symmetric block: Cholesky of B + delta I in LDS
+// with delta = 1e-11 |B|_F. When every pivot is positive, lambda_min(B) > -delta, so clamping the
+// eigenvalues at zero would move B by less than sqrt(dim) delta (the order of the eigensolver's
+// own rounding): need[b] = 0 and the block is kept. Otherwise need[b] = 1 and the block goes to
+// the eigen path.
+__global__ __launch_bounds__(EIG_THREADS) void k_psd_certify(const int32_t* __restrict__ msz,
+                                                             const int64_t* __restrict__ val_off,
+                                                             const double* __restrict__ blocks, int* __restrict__ need,
+                                                             int ldpad) {
+    extern __shared__ double lds[];
+    __shared__ double s_red[EIG_THREADS];
+    const int64_t b = blockIdx.x;
+    const int dim = 3 * msz[b];
+    const int ld = ldpad;
+    double* A = lds;
+    const double* blk = blocks + val_off[b];
+    const int t = threadIdx.x;
+    double loc = 0.0;
+    for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+        const double v = blk[idx];
+        A[(idx / dim) * ld + idx % dim] = v;
+        loc += v * v;
+    }
+    const double delta = 1e-11 * sqrt(block_sum(loc, s_red));  // block_sum synchronizes
+    for (int i = t; i < dim; i += EIG_THREADS) A[i * ld + i] += delta;
+    __syncthreads();
+    // right-looking Cholesky on the lower triangle
+    for (int k = 0; k < dim; ++k) {
+        const double piv = A[k * ld + k];
+        if (!(piv > 0.0)) {  // uniform: every thread reads the same LDS value
+            if (t == 0) need[b] = 1;
+            return;
+        }
+        const double d = sqrt(piv);
+        __syncthreads();
+        for (int i = k + 1 + t; i < dim; i += EIG_THREADS) A[i * ld + k] /= d;
+        __syncthreads();
+        const int m = dim - k - 1;
+        for (int idx = t; idx < m * m; idx += EIG_THREADS) {
+            const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+            if (j <= i) A[i * ld + j] -= A[i * ld + k] * A[j * ld + k];
+        }
+        __syncthreads();
+    }
+    if (t == 0) need[b] = 0;
+}
+
+// Positive semi-definite projection P = (B + B sign(B)) / 2 of each flagged symmetric block by the
+// Newton-Schulz iteration for the matrix sign, X_0 = B / |B|_F, X_{k+1} = X_k (3 I - X_k^2) / 2
+// (Higham, "Functions of Matrices", SIAM 2008, sec. 5.3). Every X_k is a polynomial in B, so the
+// eigenvectors are B's and each eigenvalue x follows the scalar map: x in (0, 1] goes to 1, x < 0
+// to -1, x = 0 stays. PSD_NS_ITERS steps settle every eigenvalue with |lambda| >= 1.5^-59 |B|_F
+// (~4e-11 |B|_F); smaller ones contribute at most |lambda| to the error. The block is dim x dim
+// in LDS padded to a multiple of 4; products in 4 x 4 register tiles, one or two per thread.
+constexpr int PSD_NS_ITERS = 64;
+
+__device__ inline void tile_gemm(const double* __restrict__ X, const double* __restrict__ Y, int dp, int ld, int ti,
+                                 int tj, double (&acc)[4][4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
+    for (int k = 0; k < dp; ++k) {
+        double x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) x[a] = X[(4 * ti + a) * ld + k];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) y[c] = Y[k * ld + 4 * tj + c];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[a][c] += x[a] * y[c];
+    }
+}
+
+__global__ __launch_bounds__(EIG_THREADS) void k_psd_newton_schulz(const int32_t* __restrict__ msz,
+                                                                   const int64_t* __restrict__ val_off,
+                                                                   double* __restrict__ blocks,
+                                                                   const int* __restrict__ need, int ldpad) {
+    if (need[blockIdx.x] == 0) return;
+    extern __shared__ double lds[];
+    __shared__ double s_red[EIG_THREADS];
+    const int64_t b = blockIdx.x;
+    const int dim = 3 * msz[b];
+    const int dp = (dim + 3) & ~3;
+    const int ld = ldpad;  // >= dp
+    const int nt = dp / 4, ntiles = nt * nt;
+    double* X = lds;
+    double* T = lds + (size_t)dp * ld;
+    double* blk = blocks + val_off[b];
+    const int t = threadIdx.x;
+    double loc = 0.0;
+    for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+        const double v = blk[idx];
+        loc += v * v;
+    }
+    const double nrm = sqrt(block_sum(loc, s_red));
+    if (nrm == 0.0) return;  // uniform
+    const double inv = 1.0 / nrm;
+    for (int idx = t; idx < dp * dp; idx += EIG_THREADS) {
+        const int r = idx / dp, c = idx % dp;
+        X[r * ld + c] = (r < dim && c < dim) ? 0.5 * (blk[r * dim + c] + blk[c * dim + r]) * inv : 0.0;
+    }
+    __syncthreads();
+    double acc[2][4][4];
+    for (int it = 0; it < PSD_NS_ITERS; ++it) {
+        // T = X X
+        for (int q = 0; q < 2; ++q) {
+            const int tile = t + q * EIG_THREADS;
+            if (tile < ntiles) tile_gemm(X, X, dp, ld, tile / nt, tile % nt, acc[q]);
+        }
+        for (int q = 0; q < 2; ++q) {
+            const int tile = t + q * EIG_THREADS;
+            if (tile < ntiles) {
+                const int ti = tile / nt, tj = tile % nt;
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) T[(4 * ti + a) * ld + 4 * tj + c] = acc[q][a][c];
+            }
+        }
+        __syncthreads();
+        // X <- 1.5 X - 0.5 X T
+        for (int q = 0; q < 2; ++q) {
+            const int tile = t + q * EIG_THREADS;
+            if (tile < ntiles) tile_gemm(X, T, dp, ld, tile / nt, tile % nt, acc[q]);
+        }
+        __syncthreads();
+        for (int q = 0; q < 2; ++q) {
+            const int tile = t + q * EIG_THREADS;
+            if (tile < ntiles) {
+                const int ti = tile / nt, tj = tile % nt;
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        double* x = &X[(4 * ti + a) * ld + 4 * tj + c];
+                        *x = 1.5 * *x - 0.5 * acc[q][a][c];
+                    }
+            }
+        }
+        __syncthreads();
+    }
+    // P = (B + B S) / 2 with B = nrm X_0: X_0 back into T, then the product
+    for (int idx = t; idx < dp * dp; idx += EIG_THREADS) {
+        const int r = idx / dp, c = idx % dp;
+        T[r * ld + c] = (r < dim && c < dim) ? 0.5 * (blk[r * dim + c] + blk[c * dim + r]) : 0.0;
+    }
+    __syncthreads();
+    for (int q = 0; q < 2; ++q) {
+        const int tile = t + q * EIG_THREADS;
+        if (tile < ntiles) tile_gemm(T, X, dp, ld, tile / nt, tile % nt, acc[q]);
+    }
+    __syncthreads();
+    for (int q = 0; q < 2; ++q) {
+        const int tile = t + q * EIG_THREADS;
+        if (tile < ntiles) {
+            const int ti = tile / nt, tj = tile % nt;
+            for (int a = 0; a < 4; ++a)
+                for (int c = 0; c < 4; ++c) {
+                    const int r = 4 * ti + a, cc = 4 * tj + c;
+                    if (r < dim && cc < dim) X[r * ld + cc] = 0.5 * (T[r * ld + cc] + acc[q][a][c]);
+                }
+        }
+    }
+    __syncthreads();
+    // symmetrize on the way out (B S is symmetric in exact arithmetic)
+    for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+        const int r = idx / dim, c = idx % dim;
+        blk[idx] = 0.5 * (X[r * ld + c] + X[c * ld + r]);
+    }
+}
+
 // Cyclic Jacobi eigen-decomposition of each symmetric block in LDS, then the block is replaced by
 // V f(lambda) V^T: project == 0, invert_element_block's clamped inverse (schwarz_precond.rs:329-342);
 // project == 1, the positive semi-definite projection max(lambda, 0) of DefaultSemidefiniteProjection
@@ -429,7 +602,8 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
 }
 
 // DefaultSemidefiniteProjection of every element matrix in m.d_ke (row-major (3n)^2 per element),
-// in place: one work-group per element runs the batched Jacobi kernel in projection mode.
+// in place: a Cholesky certificate keeps the blocks that are positive semi-definite to within
+// rounding, the Newton-Schulz sign iteration (one work-group per element) projects the rest.
 void launch_element_psd_projection(Model& m) {
     if (m.ne == 0) return;
     if (m.d_elem_m.n != (size_t)m.ne || m.d_elem_m.p == nullptr) {
@@ -446,9 +620,24 @@ void launch_element_psd_projection(Model& m) {
     HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   120 * 1024));
     if (lds > 120 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "element matrix too large for the projection");
-    hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)m.ne), dim3(EIG_THREADS), lds, m.stream, m.d_elem_m.get(),
-                       m.d_elem_off.get(), m.d_ke.get(), (const int*)nullptr, ld, 1);
-    check_launch("k_jacobi_inverse (projection)");
+    m.d_elem_need.ensure((size_t)m.ne);
+    const int ld_c = dim + 1;
+    const size_t lds_c = (size_t)dim * ld_c * sizeof(double);
+    HIP_CHECK(hipFuncSetAttribute((const void*)k_psd_certify, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  120 * 1024));
+    hipLaunchKernelGGL(k_psd_certify, dim3((unsigned)m.ne), dim3(EIG_THREADS), lds_c, m.stream, m.d_elem_m.get(),
+                       m.d_elem_off.get(), (const double*)m.d_ke.get(), m.d_elem_need.get(), ld_c);
+    check_launch("k_psd_certify");
+    const int dp = (dim + 3) & ~3;
+    const int ld_ns = dp + 1;
+    const size_t lds_ns = (size_t)2 * dp * ld_ns * sizeof(double);
+    if (lds_ns > 120 * 1024 || (dp / 4) * (dp / 4) > 2 * EIG_THREADS)
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "element matrix too large for the projection");
+    HIP_CHECK(hipFuncSetAttribute((const void*)k_psd_newton_schulz, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  120 * 1024));
+    hipLaunchKernelGGL(k_psd_newton_schulz, dim3((unsigned)m.ne), dim3(EIG_THREADS), lds_ns, m.stream,
+                       m.d_elem_m.get(), m.d_elem_off.get(), m.d_ke.get(), (const int*)m.d_elem_need.get(), ld_ns);
+    check_launch("k_psd_newton_schulz");
 }
 
 void schwarz_update(Schwarz& s, const double* J) { schwarz_update_t<double>(s, J); }
