@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--stiffness-damping", type=float, default=0.0,
                    help="Material stiffness_damping_coefficient (the reference's cylinder shell scene uses 0.05)")
     p.add_argument("--mass-damping", type=float, default=0.0, help="Material mass_damping_coefficient")
+    p.add_argument("--reorder", action="store_true",
+                   help="reorder the scene's mesh (reverse Cuthill-McKee, reorder_mesh_par) before building the model")
     return p.parse_args()
 
 
@@ -209,6 +211,8 @@ def main():
         del src, dst
     t_setup = time.time()
     sc = scenes.config_scene(args.config)
+    if args.reorder:
+        sc = scenes.reorder(sc)
     model = fem.ElasticityModel(sc)
     sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mode,
                                  mass_damping=args.mass_damping, stiffness_damping=args.stiffness_damping)

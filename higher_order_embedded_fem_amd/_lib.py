@@ -20,6 +20,7 @@ EXPORTED = [
     "hoefem_material_eval",
     "hoefem_material_eval_kind",
     "hoefem_rotation_svd",
+    "hoefem_mesh_reorder",
     "hoefem_model_create",
     "hoefem_model_destroy",
     "hoefem_model_set_stream",
@@ -198,6 +199,7 @@ def load(path: str = LIB_PATH):
     L.hoefem_material_eval.argtypes = [f64, f64, i64, vp, vp, vp]
     L.hoefem_material_eval_kind.argtypes = [i32, f64, f64, f64, i64, vp, vp, vp]
     L.hoefem_rotation_svd.argtypes = [i64, vp, vp, vp, vp]
+    L.hoefem_mesh_reorder.argtypes = [i32, i64, i64, vp, vp, vp]
     L.hoefem_model_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(vp)]
     L.hoefem_model_destroy.argtypes = [vp]
     L.hoefem_model_destroy.restype = None

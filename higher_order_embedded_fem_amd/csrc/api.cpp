@@ -594,6 +594,18 @@ hoefem_status hoefem_rotation_svd(int64_t nq, const double* F, double* U, double
     });
 }
 
+hoefem_status hoefem_mesh_reorder(int32_t nodes_per_element, int64_t num_vertices, int64_t num_elements,
+                                  const int32_t* connectivity, int64_t* vertex_permutation,
+                                  int64_t* element_permutation) {
+    return guarded([&] {
+        if (nodes_per_element <= 0 || num_vertices < 0 || num_elements < 0 || (num_elements > 0 && !connectivity) ||
+            (num_vertices > 0 && !vertex_permutation) || (num_elements > 0 && !element_permutation))
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
+        mesh_reorder(num_vertices, num_elements, nodes_per_element, connectivity, vertex_permutation,
+                     element_permutation);
+    });
+}
+
 hoefem_status hoefem_model_create(const hoefem_model_desc* desc, hoefem_model** out) {
     return guarded([&] {
         if (!out) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null output");

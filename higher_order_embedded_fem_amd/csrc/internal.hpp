@@ -116,6 +116,7 @@ struct NodePattern {
 
 // Build the pattern of the union of node sets (each set couples all its node pairs) plus
 // diagonal-only nodes.
+void mesh_reorder(int64_t nv, int64_t ne, int n, const int32_t* conn, int64_t* vperm, int64_t* eperm);
 void build_node_pattern(int64_t num_nodes, int64_t nsets, const int64_t* off, const int32_t* nodes,
                         const std::vector<int32_t>& diag_only_nodes, NodePattern& out);
 

@@ -137,6 +137,61 @@ void build_node_pattern(int64_t num_nodes, int64_t nsets, const int64_t* off, co
         }
 }
 
+// reorder_mesh_par (fenris/src/reorder.rs:52-88): a reverse Cuthill-McKee vertex permutation of
+// the node adjacency pattern (cuthill_mckee, :169-235: every connected component is started
+// from the first unvisited vertex of minimum degree and walked breadth first, the neighbours of
+// each vertex visited by increasing degree, ties in increasing vertex index; then reversed), and
+// the elements stably sorted by their smallest new vertex index. Permutations map target index
+// to source index: new_vertices[i] = vertices[vperm[i]].
+void mesh_reorder(int64_t nv, int64_t ne, int n, const int32_t* conn, int64_t* vperm, int64_t* eperm) {
+    std::vector<int64_t> eoff((size_t)ne + 1);
+    for (int64_t e = 0; e <= ne; ++e) eoff[e] = e * n;
+    NodePattern pat;
+    build_node_pattern(nv, ne, eoff.data(), conn, {}, pat);
+    std::vector<int32_t> deg((size_t)nv);
+    for (int64_t v = 0; v < nv; ++v) deg[v] = pat.nptr[v + 1] - pat.nptr[v];
+    std::vector<uint8_t> visited((size_t)nv, 0);
+    std::vector<int64_t> order;
+    order.reserve((size_t)nv);
+    std::vector<int32_t> adj;
+    int64_t scan = 0;  // every vertex before scan is visited
+    while ((int64_t)order.size() < nv) {
+        while (scan < nv && visited[scan]) ++scan;
+        int64_t start = scan;
+        for (int64_t v = scan; v < nv; ++v)
+            if (!visited[v] && deg[v] < deg[start]) start = v;
+        size_t head = order.size();
+        order.push_back(start);
+        visited[start] = 1;
+        while (head < order.size()) {
+            const int64_t v = order[head++];
+            adj.assign(pat.ncol.begin() + pat.nptr[v], pat.ncol.begin() + pat.nptr[v + 1]);
+            std::stable_sort(adj.begin(), adj.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+            for (int32_t a : adj)
+                if (!visited[a]) {
+                    visited[a] = 1;
+                    order.push_back(a);
+                }
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    std::vector<int64_t> inv((size_t)nv);
+    for (int64_t i = 0; i < nv; ++i) {
+        vperm[i] = order[i];
+        inv[order[i]] = i;
+    }
+    std::vector<int64_t> key((size_t)ne);
+    for (int64_t e = 0; e < ne; ++e) {
+        int64_t k = INT64_MAX;
+        for (int j = 0; j < n; ++j) k = std::min(k, inv[conn[e * n + j]]);
+        key[e] = k;
+    }
+    std::vector<int64_t> ep((size_t)ne);
+    std::iota(ep.begin(), ep.end(), 0);
+    std::stable_sort(ep.begin(), ep.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    std::copy(ep.begin(), ep.end(), eperm);
+}
+
 // First fit over colors kept as per-node bitmasks: bit c of node v is set when color c
 // already contains a set touching v, so the first zero bit of the OR over a set's nodes is
 // the first color whose node set is disjoint from it.

@@ -443,6 +443,63 @@ def cylinder_scene(cells: int = 24, element_type: int = HEX20, points: int = 8, 
     )
 
 
+def reorder(scene: Scene) -> Scene:
+    """The scene after reorder_mesh_par (fenris/src/reorder.rs:52-88), as the reference's scenes apply
+    it to their meshes (e.g. scene_runner/src/scenes/hollow_ball.rs:398): reverse Cuthill-McKee
+    vertices and elements sorted by their smallest new vertex index, computed by the library
+    (hoefem_mesh_reorder). Interior elements stay before interface elements, each group in the
+    permuted order (as the reference's embedding of a reordered background mesh lists them); the
+    per-interface-element quadrature rules, the Dirichlet nodes and the Schwarz node sets follow."""
+    import ctypes as C
+
+    from . import _lib
+
+    conn = np.ascontiguousarray(scene.connectivity, dtype=np.int32)
+    ne, n = conn.shape
+    nv = scene.num_vertices
+    vperm = np.zeros(nv, np.int64)
+    eperm = np.zeros(ne, np.int64)
+    st = _lib.load().hoefem_mesh_reorder(n, nv, ne, conn.ctypes.data, vperm.ctypes.data, eperm.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"hoefem_mesh_reorder failed ({st})")
+    inv = np.empty(nv, np.int64)
+    inv[vperm] = np.arange(nv)
+    nint = scene.num_interior
+    order = np.concatenate([eperm[eperm < nint], eperm[eperm >= nint]])
+    iface = order[nint:] - nint
+
+    def permute_rules(q: Quadrature) -> Quadrature:
+        if q.interface_offsets is None or iface.size == 0:
+            return q
+        off = np.asarray(q.interface_offsets, np.int64)
+        cnt = np.diff(off)[iface]
+        noff = np.zeros(iface.size + 1, np.int64)
+        noff[1:] = np.cumsum(cnt)
+        idx = np.concatenate([np.arange(off[k], off[k + 1]) for k in iface])
+        xi = np.asarray(q.interface_xi).reshape(-1, 3)
+        return Quadrature(q.interior_w, q.interior_xi, noff, np.asarray(q.interface_w)[idx], xi[idx])
+
+    so, sn = np.asarray(scene.subdomain_offsets, np.int64), np.asarray(scene.subdomain_nodes)
+    subs = [inv[sn[so[k]:so[k + 1]]] for k in range(len(so) - 1)]
+    sorder = sorted(range(len(subs)), key=lambda k: int(subs[k].min()) if subs[k].size else -1)
+    new_so = np.zeros(len(subs) + 1, np.int64)
+    new_so[1:] = np.cumsum([subs[k].size for k in sorder])
+    new_sn = (np.concatenate([subs[k] for k in sorder]) if subs else np.zeros(0)).astype(np.int32)
+    return dataclasses.replace(
+        scene,
+        vertices=np.ascontiguousarray(scene.vertices[vperm]),
+        connectivity=np.ascontiguousarray(inv[conn[order]].astype(np.int32)),
+        mass_quadrature=permute_rules(scene.mass_quadrature),
+        stiffness_quadrature=permute_rules(scene.stiffness_quadrature),
+        elliptic_quadrature=permute_rules(scene.elliptic_quadrature),
+        dirichlet_nodes=np.sort(inv[scene.dirichlet_nodes]).astype(np.int32),
+        subdomain_offsets=new_so,
+        subdomain_nodes=new_sn,
+        untreated_nodes=np.sort(inv[scene.untreated_nodes]).astype(np.int32),
+        name=scene.name + "_rcm",
+    )
+
+
 def config_scene(index: int, **kw) -> Scene:
     """The workload configurations: 1 = Tet10 cube 8^3, 2 = Hex20 cylinder 24^3 (q=8),
     3 = Hex27 cylinder 48^3 (q=27)."""

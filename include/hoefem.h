@@ -116,6 +116,16 @@ hoefem_status hoefem_material_eval_kind(int32_t kind, double inversion_threshold
  * column-major; S: [nq * 3]. Device pointers. */
 hoefem_status hoefem_rotation_svd(int64_t nq, const double* F, double* U, double* S, double* Vt);
 
+/* reorder_mesh_par (fenris/src/reorder.rs:52-88), the reorder the reference's scenes apply to their
+ * meshes before building models: a reverse Cuthill-McKee permutation of the vertices (neighbours
+ * visited by increasing degree, ties by increasing index) and the elements stably sorted by their
+ * smallest new vertex index. HOST arrays; connectivity [num_elements * nodes_per_element];
+ * outputs map target to source: new_vertices[i] = vertices[vertex_permutation[i]], likewise for
+ * elements. */
+hoefem_status hoefem_mesh_reorder(int32_t nodes_per_element, int64_t num_vertices, int64_t num_elements,
+                                  const int32_t* connectivity, int64_t* vertex_permutation,
+                                  int64_t* element_permutation);
+
 /* ElasticityModelParallel for NodalModel / EmbeddedModel (fenris/src/solid/mod.rs:124-172,
  * fenris/src/solid/impl_model.rs:111-164, fenris/src/embedding/mod.rs:506-639).
  * Creation builds the greedy element coloring (paradis/src/coloring.rs:48-72), the CSR

@@ -137,6 +137,8 @@ def lib():
         L.orc_model_stiffness_add.argtypes = [C.c_void_p, C.c_double, C.c_double, f64p, f64p]
         L.orc_model_stiffness_add_projected.argtypes = [C.c_void_p, C.c_double, C.c_double, f64p, f64p]
         L.orc_psd_project.argtypes = [C.c_int, f64p, f64p]
+        L.orc_cuthill_mckee.argtypes = [C.c_int64, i32p, i32p, i64p]
+        L.orc_mesh_reorder.argtypes = [C.c_int, C.c_int64, C.c_int64, i32p, i64p, i64p]
         L.orc_sim_damping_values.argtypes = [C.c_void_p, f64p]
         L.orc_greedy_coloring.argtypes = [C.c_int64, i64p, i32p, i32p, C.POINTER(C.c_int32)]
         L.orc_coo_pattern.argtypes = [C.c_int64, C.c_int64, i64p, i32p, C.c_int, C.POINTER(C.c_int64), C.c_void_p,
@@ -249,6 +251,24 @@ def _colmajor(F):
 
 def _rowmajor(buf, nq):
     return buf.reshape(nq, 3, 3).transpose(0, 2, 1).copy()
+
+
+def cuthill_mckee(rowptr, col):
+    """cuthill_mckee (fenris/src/reorder.rs:169-235): permutation, target index -> source index."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    perm = np.zeros(len(rowptr) - 1, np.int64)
+    check(lib().orc_cuthill_mckee(len(rowptr) - 1, rowptr, col, perm))
+    return perm
+
+
+def mesh_reorder(connectivity, num_vertices):
+    """reorder_mesh_par (fenris/src/reorder.rs:52-88): (vertex permutation, element permutation)."""
+    conn = np.ascontiguousarray(connectivity, dtype=np.int32)
+    vp = np.zeros(num_vertices, np.int64)
+    ep = np.zeros(conn.shape[0], np.int64)
+    check(lib().orc_mesh_reorder(conn.shape[1], num_vertices, conn.shape[0], conn.ravel(), vp, ep))
+    return vp, ep
 
 
 def psd_project(A):

@@ -1449,6 +1449,61 @@ int orc_model_stiffness_add_projected(const orc_model* m, double mu0, double lam
     return stiffness_add_impl(m, mu0, lambda0, u, values, true);
 }
 
+// cuthill_mckee (fenris/src/reorder.rs:169-235) on a square CSR pattern: per connected component,
+// start from the first unvisited vertex of minimum degree (lane length), breadth-first, the
+// adjacent vertices of each popped vertex visited by increasing degree. The reference sorts them
+// with sort_unstable_by_key; this restatement breaks ties by CSR position (increasing index).
+int orc_cuthill_mckee(int64_t n, const int32_t* rowptr, const int32_t* col, int64_t* perm) {
+    std::vector<char> visited(n, 0);
+    std::vector<int64_t> out;
+    out.reserve(n);
+    auto degree = [&](int64_t v) { return (int64_t)(rowptr[v + 1] - rowptr[v]); };
+    while ((int64_t)out.size() < n) {
+        int64_t start = -1;
+        for (int64_t v = 0; v < n; ++v)
+            if (!visited[v] && (start < 0 || degree(v) < degree(start))) start = v;
+        std::vector<int64_t> queue{start};
+        visited[start] = 1;
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const int64_t v = queue[h];
+            out.push_back(v);
+            std::vector<int64_t> adj(col + rowptr[v], col + rowptr[v + 1]);
+            std::stable_sort(adj.begin(), adj.end(), [&](int64_t a, int64_t b) { return degree(a) < degree(b); });
+            for (int64_t a : adj)
+                if (!visited[a]) {
+                    visited[a] = 1;
+                    queue.push_back(a);
+                }
+        }
+    }
+    std::copy(out.begin(), out.end(), perm);
+    return ORC_OK;
+}
+
+// reorder_mesh_par (fenris/src/reorder.rs:52-88): reverse Cuthill-McKee of the vertex pattern
+// (CsrParAssembler::assemble_pattern), elements stably sorted by their smallest new vertex index.
+int orc_mesh_reorder(int n, int64_t nv, int64_t ne, const int32_t* conn, int64_t* vperm, int64_t* eperm) {
+    std::vector<int64_t> off(ne + 1);
+    for (int64_t e = 0; e <= ne; ++e) off[e] = e * n;
+    Csr pat;
+    std::vector<int32_t> no_diag;
+    build_pattern(nv, ne, off.data(), conn, 1, no_diag, pat);
+    orc_cuthill_mckee(nv, pat.rowptr.data(), pat.col.data(), vperm);
+    std::reverse(vperm, vperm + nv);
+    std::vector<int64_t> inv(nv);
+    for (int64_t i = 0; i < nv; ++i) inv[vperm[i]] = i;
+    std::vector<int64_t> idx(ne);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto key = [&](int64_t e) {
+        int64_t k = inv[conn[e * n]];
+        for (int j = 1; j < n; ++j) k = std::min(k, inv[conn[e * n + j]]);
+        return k;
+    };
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return key(a) < key(b); });
+    std::copy(idx.begin(), idx.end(), eperm);
+    return ORC_OK;
+}
+
 int orc_psd_project(int dim, const double* a, double* out) {
     std::copy(a, a + (size_t)dim * dim, out);
     psd_project(dim, out);

@@ -130,6 +130,11 @@ int orc_model_forces_add(const orc_model* m, double mu, double lambda, const dou
 int orc_model_stiffness_add(const orc_model* m, double mu, double lambda, const double* u, double* values);
 /* the same with every element matrix projected onto the positive semi-definite cone */
 int orc_model_stiffness_add_projected(const orc_model* m, double mu, double lambda, const double* u, double* values);
+/* cuthill_mckee on a square CSR pattern (perm: target -> source) and reorder_mesh_par
+ * (fenris/src/reorder.rs:52-88, :169-243) on an element-major connectivity */
+int orc_cuthill_mckee(int64_t n, const int32_t* rowptr, const int32_t* col, int64_t* perm);
+int orc_mesh_reorder(int nodes_per_element, int64_t num_vertices, int64_t num_elements, const int32_t* conn,
+                     int64_t* vertex_perm, int64_t* element_perm);
 /* DefaultSemidefiniteProjection of one symmetric dim x dim matrix (column-major) */
 int orc_psd_project(int dim, const double* a, double* out);
 
