@@ -57,6 +57,9 @@ def parse():
     p.add_argument("--stiffness-damping", type=float, default=0.0,
                    help="Material stiffness_damping_coefficient (the reference's cylinder shell scene uses 0.05)")
     p.add_argument("--mass-damping", type=float, default=0.0, help="Material mass_damping_coefficient")
+    p.add_argument("--quadrature", choices=["sampled", "embedded"], default="sampled",
+                   help="config 2: cut-cell rules by rejection sampling (default) or built the reference's way "
+                        "(embedded tetrahedral mesh, clipping, moment fitting)")
     p.add_argument("--reorder", action="store_true",
                    help="reorder the scene's mesh (reverse Cuthill-McKee, reorder_mesh_par) before building the model")
     return p.parse_args()
@@ -210,7 +213,8 @@ def main():
         print("[bench] calibration copy: 3 x 268435456 bytes read + written", file=sys.stderr, flush=True)
         del src, dst
     t_setup = time.time()
-    sc = scenes.config_scene(args.config)
+    sc = (scenes.config_scene(args.config) if args.quadrature == "sampled"
+          else scenes.cylinder_scene(24, scenes.HEX20, 8, quadrature="embedded"))
     if args.reorder:
         sc = scenes.reorder(sc)
     model = fem.ElasticityModel(sc)

@@ -361,7 +361,7 @@ def _rejection_points(lo, hi, count, rng, r_in, r_out, half_h):
 def cylinder_scene(cells: int = 24, element_type: int = HEX20, points: int = 8, young: float = 5e5,
                    poisson: float = 0.45, density: float = 1000.0, dt: float = 1.0 / 200.0, steps: int = 20,
                    seed: int = 2024, r_in: float = 0.6, r_out: float = 0.8, height: float = 1.6,
-                   mass_points: int = 64, schwarz_threshold: float = 0.5) -> Scene:
+                   mass_points: int = 64, schwarz_threshold: float = 0.5, quadrature: str = "sampled") -> Scene:
     """Embedded hollow cylinder in a cells^3 background grid over [-1, 1]^3.
 
     Interior cells share a Gauss rule with ``points`` points; every cut cell gets
@@ -371,7 +371,16 @@ def cylinder_scene(cells: int = 24, element_type: int = HEX20, points: int = 8, 
     scene_runner/src/scenes/cylinder_shell.rs:749-762. Schwarz subdomains are the interface
     cells with volume fraction <= 0.5, all other nodes untreated
     (simulation_toolbox/src/fem/schwarz_precond.rs:65-127).
+
+    quadrature="embedded" builds the cut cells' rules the reference's way instead
+    (embedded_cylinder_scene).
     """
+    if quadrature == "embedded":
+        return embedded_cylinder_scene(cells, element_type, points, young=young, poisson=poisson, density=density,
+                                       dt=dt, steps=steps, r_in=r_in, r_out=r_out, height=height,
+                                       schwarz_threshold=schwarz_threshold)
+    if quadrature != "sampled":
+        raise ValueError(f"unknown quadrature {quadrature!r}")
     half_h = height / 2.0
     hv, hc = uniform_hex_grid(cells, 2.0, -1.0)
     h = 2.0 / cells
@@ -440,6 +449,100 @@ def cylinder_scene(cells: int = 24, element_type: int = HEX20, points: int = 8, 
         dt=dt,
         steps=steps,
         info={"interior_cells": int(nint), "interface_cells": int(nif), "schwarz_cells": int(sel.size)},
+    )
+
+
+def embedded_cylinder_scene(cells: int = 24, element_type: int = HEX20, points: int = 8, young: float = 5e5,
+                            poisson: float = 0.45, density: float = 1000.0, dt: float = 1.0 / 200.0,
+                            steps: int = 20, r_in: float = 0.6, r_out: float = 0.8, height: float = 1.6,
+                            schwarz_threshold: float = 0.5, facet: float = 1.0) -> Scene:
+    """The hollow cylinder embedded the reference's way (embedding.py): a Tet4 mesh of the cylinder
+    with facets of about facet * h, background cells classified by the intersected volume, and the
+    cut cells' rules built from tetrahedron rules on the intersection, mirroring
+    cylinder_shell_embedded_hex20 / _hex27 (scene_runner/src/scenes/cylinder_shell.rs:687-800):
+    stiffness from a strength 3 source reduced to strength 2 (Hex20, points = 8) or strength 5
+    reduced to strength 5 (Hex27, points = 27) by optimize_quadrature; mass from a strength 5 rule on
+    every piece (the reference uses strength 10 on its fine embedding mesh; here the pieces of the
+    clipped tetrahedra are small and strength 5 keeps the rule at ~5k points per cut cell) plus the
+    1e-8 stabilization with the strength 11 rule of the uncut cell. Interior cells use the k^3
+    Gauss rule (k^3 = points) for stiffness and 3^3 for mass."""
+    from . import embedding as E
+
+    half_h = height / 2.0
+    hv, hc = uniform_hex_grid(cells, 2.0, -1.0)
+    h = 2.0 / cells
+    nr = max(1, math.ceil((r_out - r_in) / (facet * h)))
+    nth = max(8, math.ceil(2 * math.pi * r_out / (facet * h)))
+    nz = max(1, math.ceil(height / (facet * h)))
+    tv, tt = E.hollow_cylinder_tet_mesh(r_in, r_out, half_h, nr, nth, nz)
+    lo_all, hi_all = hv[hc[:, 0]], hv[hc[:, 6]]
+    emb = E.embed_tet_mesh(lo_all, hi_all, tv, tt)
+    interior_cells, interface_cells = emb.interior_cells, emb.interface_cells
+    verts, conn = refine(hv, hc, element_type)
+    kept = np.concatenate([interior_cells, interface_cells])
+    verts, conn = keep_cells(verts, conn, kept)
+    k_int = round(points ** (1.0 / 3.0))
+    if k_int ** 3 != points:
+        raise ValueError("points must be a cube number for the interior Gauss rule")
+    src, red, mass_strength = (3, 2, 5) if element_type != HEX27 else (5, 5, 5)
+    iw, ixi = gauss_legendre_hex(k_int)
+    mw, mxi = gauss_legendre_hex(3)
+    sw, sxi = gauss_legendre_hex(6)  # strength 11
+    stiff_rule, mass_rule = E.tet_rule(src), E.tet_rule(mass_strength)
+    s_w, s_xi, s_cnt, m_w, m_xi, m_cnt = [], [], [], [], [], []
+    for k, c in enumerate(interface_cells):
+        lo, hi = lo_all[c], hi_all[c]
+        w0, p0 = E.element_embedded_quadrature(emb.interface_pieces[k], lo, hi, stiff_rule)
+        w, p = E.optimize_quadrature(w0, p0, red)
+        s_w.append(w)
+        s_xi.append(p)
+        s_cnt.append(w.size)
+        w, p = E.element_embedded_quadrature(emb.interface_pieces[k], lo, hi, mass_rule, stabilization=(1e-8, (sw, sxi)))
+        m_w.append(w)
+        m_xi.append(p)
+        m_cnt.append(w.size)
+    nif = interface_cells.size
+
+    def pack(ws, xs, cnt):
+        off = np.zeros(nif + 1, np.int64)
+        off[1:] = np.cumsum(cnt)
+        if nif == 0:
+            return off, np.zeros(0), np.zeros((0, 3))
+        return off, np.concatenate(ws), np.concatenate(xs, axis=0)
+
+    s_off, s_w, s_xi = pack(s_w, s_xi, s_cnt)
+    m_off, m_w, m_xi = pack(m_w, m_xi, m_cnt)
+    stiff = Quadrature(iw, ixi, s_off, s_w, s_xi)
+    mass = Quadrature(mw, mxi, m_off, m_w, m_xi)
+    nint = interior_cells.size
+    dir_nodes = np.nonzero(verts[:, 2] <= -half_h + 1e-12)[0].astype(np.int32)
+    sel = np.nonzero(emb.volume_fractions[interface_cells] <= schwarz_threshold)[0]
+    sub_conn = conn[nint + sel]
+    s_offsets = np.arange(sel.size + 1, dtype=np.int64) * conn.shape[1]
+    treated = np.zeros(verts.shape[0], dtype=bool)
+    treated[sub_conn.ravel()] = True
+    untreated = np.nonzero(~treated)[0].astype(np.int32)
+    return Scene(
+        name=f"cylinder{cells}_{ELEMENT_NAMES[element_type].lower()}_q{points}_emb",
+        element_type=element_type,
+        vertices=np.ascontiguousarray(verts, dtype=np.float64),
+        connectivity=np.ascontiguousarray(conn, dtype=np.int32),
+        num_interior=int(nint),
+        mass_quadrature=mass,
+        stiffness_quadrature=stiff,
+        elliptic_quadrature=stiff,
+        young=young,
+        poisson=poisson,
+        density=density,
+        dirichlet_nodes=dir_nodes,
+        subdomain_offsets=s_offsets,
+        subdomain_nodes=np.ascontiguousarray(sub_conn.ravel(), dtype=np.int32),
+        untreated_nodes=untreated,
+        dt=dt,
+        steps=steps,
+        info={"interior_cells": int(nint), "interface_cells": int(nif), "schwarz_cells": int(sel.size),
+              "embedded_tets": int(tt.shape[0]), "stiffness_points_interface": int(s_off[-1]),
+              "mass_points_interface": int(m_off[-1])},
     )
 
 

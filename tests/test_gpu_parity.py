@@ -38,6 +38,7 @@ SMALL_SCENES = {
     "cube2_tet20": lambda: scenes.cube_scene(2, scenes.TET20),
     "cyl8_hex20": lambda: scenes.cylinder_scene(8, scenes.HEX20, 8),
     "cyl6_hex27": lambda: scenes.cylinder_scene(6, scenes.HEX27, 27),
+    "cyl8_hex20_emb": lambda: scenes.cylinder_scene(8, scenes.HEX20, 8, quadrature="embedded"),
 }
 
 
@@ -189,7 +190,7 @@ def test_dirichlet_and_schwarz_match_oracle(fem, name):
     assert abs(it_g - it_o) <= max(2, it_o // 20)
 
 
-@pytest.mark.parametrize("name", ["cube3_tet10", "cyl8_hex20"])
+@pytest.mark.parametrize("name", ["cube3_tet10", "cyl8_hex20", "cyl8_hex20_emb"])
 def test_step_matches_oracle(fem, name):
     sc = SMALL_SCENES[name]()
     om = O.Model(sc)
