@@ -119,91 +119,146 @@ __device__ inline double block_sum(double v, double* s_red) {
     return r;
 }
 
-// Gauss-Jordan inversion of a symmetric block in LDS without pivoting. Each pivot is a
-// diagonal entry of a Schur complement, so all pivots are positive iff B is positive
-// definite. The block is accepted when they are and
-//     min(|B|_F |B^-1|_F, |B|_inf |B^-1|_inf) < SPD_COND_LIMIT,
-// both upper bounds of kappa_2(B) for symmetric B, which keeps kappa_2 below the 1e12 of the
-// eigenvalue clamp: then B^+ = B^-1 and the symmetrized inverse replaces the block. Otherwise
-// fallback[b] = 1 and the block is kept for the eigen path.
+// Gauss-Jordan inversion of a symmetric block without pivoting, held in registers: the 16 x 16
+// threads own the entries (i, j) with i = 16 r + tr, j = 16 c + tc (r, c < 6, so dim <= 96), and
+// each pivot step broadcasts row k and column k through LDS (double-buffered: one barrier per
+// pivot). Each pivot is a diagonal entry of a Schur complement, so all pivots are positive iff B
+// is positive definite. Results per block (fallback[b]):
+//   0  pivots positive and min(|B|_F |B^-1|_F, |B|_inf |B^-1|_inf) < SPD_COND_LIMIT (both upper
+//      bounds of kappa_2(B) for symmetric B): the clamp of the pseudo-inverse is inactive, so
+//      B^+ = B^-1, and the symmetrized inverse replaces the block;
+//   2  pivots positive, the cheap bound too loose: the inverse replaces the block and
+//      k_spd_power_certify decides with a sharper bound;
+//   1  a pivot not positive: B is kept for the eigen path.
+constexpr int GJ_R = 6;
+
 __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __restrict__ msz,
                                                              const int64_t* __restrict__ val_off,
-                                                             double* __restrict__ blocks, int* __restrict__ fallback,
-                                                             int ld) {
-    extern __shared__ double lds[];
+                                                             double* __restrict__ blocks, int* __restrict__ fallback) {
+    __shared__ double s_row[2][16 * GJ_R], s_col[2][16 * GJ_R];
+    __shared__ double s_x[GJ_R][EIG_THREADS];  // row-sum partials, then the transpose exchange
+    __shared__ double s_red[EIG_THREADS];
     const int64_t b = blockIdx.x;
     const int dim = 3 * msz[b];
-    double* A = lds;
-    double* rowk = lds + (size_t)dim * ld;
-    double* colk = rowk + dim;
-    __shared__ double s_red[EIG_THREADS];
+    const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
     double* blk = blocks + val_off[b];
-    const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
-    constexpr int NW = EIG_THREADS / 64;
-    double loc = 0.0;
-    for (int i = wid; i < dim; i += NW)
-        for (int j = lane; j < dim; j += 64) {
-            const double v = blk[(int64_t)i * dim + j];
-            A[i * ld + j] = v;
-            loc += v * v;
+    double a[GJ_R][GJ_R];
+#pragma unroll
+    for (int r = 0; r < GJ_R; ++r)
+#pragma unroll
+        for (int c = 0; c < GJ_R; ++c) {
+            const int i = 16 * r + tr, j = 16 * c + tc;
+            a[r][c] = (i < dim && j < dim) ? blk[(int64_t)i * dim + j] : 0.0;
         }
-    const double nb2 = block_sum(loc, s_red);
-    double rs = 0.0;  // max absolute row sum (infinity norm)
-    for (int i = t; i < dim; i += EIG_THREADS) {
-        double a = 0.0;
-        for (int j = 0; j < dim; ++j) a += fabs(A[i * ld + j]);
-        rs = fmax(rs, a);
-    }
-    const double nbinf = block_max(rs, s_red);
-    bool bad = false;
+    // |.|_F^2 and |.|_inf of the register-held matrix
+    auto norms = [&](double& f2, double& finf) {
+        double loc = 0.0, rs[GJ_R];
+#pragma unroll
+        for (int r = 0; r < GJ_R; ++r) {
+            rs[r] = 0.0;
+#pragma unroll
+            for (int c = 0; c < GJ_R; ++c) {
+                loc += a[r][c] * a[r][c];
+                rs[r] += fabs(a[r][c]);
+            }
+        }
+        f2 = block_sum(loc, s_red);  // synchronizes
+        double* part = &s_x[0][0];   // [16 (tc)][96 (row)]
+#pragma unroll
+        for (int r = 0; r < GJ_R; ++r) part[tc * (16 * GJ_R) + 16 * r + tr] = rs[r];
+        __syncthreads();
+        double row = 0.0;
+        if (t < 16 * GJ_R)
+            for (int q = 0; q < 16; ++q) row += part[q * (16 * GJ_R) + t];
+        finf = block_max(row, s_red);  // synchronizes
+    };
+    double nb2, nbinf;
+    norms(nb2, nbinf);
     for (int k = 0; k < dim; ++k) {
-        for (int i = t; i < dim; i += EIG_THREADS) {
-            rowk[i] = A[k * ld + i];
-            colk[i] = A[i * ld + k];
+        const int buf = k & 1, kr = k >> 4, kt = k & 15;
+        if (tr == kt) {
+#pragma unroll
+            for (int c = 0; c < GJ_R; ++c) {
+                double v = 0.0;
+#pragma unroll
+                for (int r = 0; r < GJ_R; ++r)
+                    if (r == kr) v = a[r][c];
+                s_row[buf][16 * c + tc] = v;
+            }
+        }
+        if (tc == kt) {
+#pragma unroll
+            for (int r = 0; r < GJ_R; ++r) {
+                double v = 0.0;
+#pragma unroll
+                for (int c = 0; c < GJ_R; ++c)
+                    if (c == kr) v = a[r][c];
+                s_col[buf][16 * r + tr] = v;
+            }
         }
         __syncthreads();
-        const double piv = rowk[k];
+        const double piv = s_row[buf][k];
         if (!(piv > 0.0) || !isfinite(piv)) {  // same value in every thread: uniform exit
-            bad = true;
-            break;
+            if (t == 0) fallback[b] = 1;
+            return;
         }
         const double ip = 1.0 / piv;
-        for (int i = wid; i < dim; i += NW)
-            for (int j = lane; j < dim; j += 64) {
-                double v;
-                if (i == k)
-                    v = j == k ? ip : rowk[j] * ip;
-                else if (j == k)
-                    v = -colk[i] * ip;
-                else
-                    v = A[i * ld + j] - colk[i] * rowk[j] * ip;
-                A[i * ld + j] = v;
-            }
+        double rk[GJ_R], ck[GJ_R];
+#pragma unroll
+        for (int c = 0; c < GJ_R; ++c) rk[c] = s_row[buf][16 * c + tc] * ip;
+#pragma unroll
+        for (int r = 0; r < GJ_R; ++r) ck[r] = s_col[buf][16 * r + tr];
+        // a_ij - a_ik a_kj / a_kk everywhere, then row k -> a_kj / a_kk, column k -> -a_ik / a_kk,
+        // the pivot -> 1 / a_kk
+#pragma unroll
+        for (int r = 0; r < GJ_R; ++r)
+#pragma unroll
+            for (int c = 0; c < GJ_R; ++c) a[r][c] -= ck[r] * rk[c];
+        if (tr == kt) {
+#pragma unroll
+            for (int r = 0; r < GJ_R; ++r)
+                if (r == kr) {
+#pragma unroll
+                    for (int c = 0; c < GJ_R; ++c) a[r][c] = rk[c];
+                }
+        }
+        if (tc == kt) {
+#pragma unroll
+            for (int c = 0; c < GJ_R; ++c)
+                if (c == kr) {
+#pragma unroll
+                    for (int r = 0; r < GJ_R; ++r) a[r][c] = -ck[r] * ip;
+                }
+        }
+        if (tr == kt && tc == kt) {
+#pragma unroll
+            for (int r = 0; r < GJ_R; ++r)
+                if (r == kr) {
+#pragma unroll
+                    for (int c = 0; c < GJ_R; ++c)
+                        if (c == kr) a[r][c] = ip;
+                }
+        }
+    }
+    double ni2, niinf;
+    norms(ni2, niinf);
+    const double cert = fmin(sqrt(nb2) * sqrt(ni2), nbinf * niinf);
+    // symmetrize: the transpose of entry (r, c) sits in thread (tc, tr) at (c, r); round r
+    // exchanges the entries (r, 0..5)
+    const int partner = tc * 16 + tr;
+#pragma unroll
+    for (int r = 0; r < GJ_R; ++r) {
+#pragma unroll
+        for (int x = 0; x < GJ_R; ++x) s_x[x][t] = a[x][r];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < GJ_R; ++c) {
+            const int i = 16 * r + tr, j = 16 * c + tc;
+            if (i < dim && j < dim) blk[(int64_t)i * dim + j] = 0.5 * (a[r][c] + s_x[c][partner]);
+        }
         __syncthreads();
     }
-    if (bad) {
-        if (t == 0) fallback[b] = 1;
-        return;
-    }
-    loc = 0.0;
-    for (int i = wid; i < dim; i += NW)
-        for (int j = lane; j < dim; j += 64) loc += A[i * ld + j] * A[i * ld + j];
-    const double ni2 = block_sum(loc, s_red);
-    rs = 0.0;
-    for (int i = t; i < dim; i += EIG_THREADS) {
-        double a = 0.0;
-        for (int j = 0; j < dim; ++j) a += fabs(A[i * ld + j]);
-        rs = fmax(rs, a);
-    }
-    const double niinf = block_max(rs, s_red);
-    const double cert = fmin(sqrt(nb2) * sqrt(ni2), nbinf * niinf);
-    if (!(cert < SPD_COND_LIMIT)) {
-        if (t == 0) fallback[b] = 1;
-        return;
-    }
-    for (int i = wid; i < dim; i += NW)
-        for (int j = lane; j < dim; j += 64) blk[(int64_t)i * dim + j] = 0.5 * (A[i * ld + j] + A[j * ld + i]);
-    if (t == 0) fallback[b] = 0;
+    if (t == 0) fallback[b] = cert < SPD_COND_LIMIT ? 0 : 2;
 }
 
 // Positive semi-definiteness certificate of each symmetric block: Cholesky of B + delta I in LDS
@@ -377,6 +432,91 @@ __global__ __launch_bounds__(EIG_THREADS) void k_psd_newton_schulz(const int32_t
         const int r = idx / dim, c = idx % dim;
         blk[idx] = 0.5 * (X[r * ld + c] + X[c * ld + r]);
     }
+}
+
+// Sharper certificate for the blocks k_spd_inverse marked 2. For symmetric X with eigenvalues
+// lambda, |X^8|_F = (sum lambda^16)^(1/2) >= max|lambda|^8, so
+//     |X|_2 <= s |(X / s)^8|_F^(1/8),  s = |X|_F,
+// an upper bound within a factor dim^(1/16) (<= 1.33) of |X|_2 instead of the factor up to dim of
+// the norms above. Three squarings in LDS for B (gathered again from J) and for the inverse now in
+// the block; kappa_2(B) <= |B|_2 |B^-1|_2 < SPD_COND_LIMIT accepts the inverse, otherwise B goes
+// back into the block for the eigen path.
+template <class VJ>
+__global__ __launch_bounds__(EIG_THREADS) void k_spd_power_certify(
+    const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off, const int64_t* __restrict__ slot_off,
+    const int64_t* __restrict__ noff, const int32_t* __restrict__ nodes, const int32_t* __restrict__ jslot,
+    const int32_t* __restrict__ jrowptr, const VJ* __restrict__ J, double* __restrict__ blocks,
+    int* __restrict__ fallback, int ldpad) {
+    if (fallback[blockIdx.x] != 2) return;
+    extern __shared__ double lds[];
+    __shared__ double s_red[EIG_THREADS];
+    const int64_t b = blockIdx.x;
+    const int mb = msz[b], dim = 3 * mb, dp = (dim + 3) & ~3, ld = ldpad;
+    double* X = lds;
+    double* Y = lds + (size_t)dp * ld;
+    double* blk = blocks + val_off[b];
+    const int32_t* g = nodes + noff[b];
+    const int t = threadIdx.x;
+    auto entry = [&](int r, int c) -> double {  // B(r, c) = J[3 g_a + i, 3 g_cb + j]
+        const int a = r / 3, i = r % 3, cb = c / 3, j = c % 3;
+        const int k = jslot[slot_off[b] + (int64_t)a * mb + cb];
+        return (double)J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
+    };
+    auto bound = [&]() -> double {  // spectral-norm bound of the symmetric matrix in X
+        double loc = 0.0;
+        for (int idx = t; idx < dp * dp; idx += EIG_THREADS) {
+            const double v = X[(idx / dp) * ld + idx % dp];
+            loc += v * v;
+        }
+        const double s = sqrt(block_sum(loc, s_red));  // synchronizes
+        if (!(s > 0.0)) return 0.0;                    // uniform
+        const double is = 1.0 / s;
+        for (int idx = t; idx < dp * dp; idx += EIG_THREADS) X[(idx / dp) * ld + idx % dp] *= is;
+        __syncthreads();
+        const int nt = dp / 4;
+        double* src = X;
+        double* dst = Y;
+        for (int sq = 0; sq < 3; ++sq) {
+            for (int tile = t; tile < nt * nt; tile += EIG_THREADS) {
+                const int ti = tile / nt, tj = tile % nt;
+                double acc[4][4];
+                tile_gemm(src, src, dp, ld, ti, tj, acc);
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) dst[(4 * ti + a) * ld + 4 * tj + c] = acc[a][c];
+            }
+            __syncthreads();
+            double* tmp = src;
+            src = dst;
+            dst = tmp;
+        }
+        loc = 0.0;
+        for (int idx = t; idx < dp * dp; idx += EIG_THREADS) {
+            const double v = src[(idx / dp) * ld + idx % dp];
+            loc += v * v;
+        }
+        const double n8 = sqrt(block_sum(loc, s_red));
+        return s * sqrt(sqrt(sqrt(n8)));
+    };
+    for (int idx = t; idx < dp * dp; idx += EIG_THREADS) {
+        const int r = idx / dp, c = idx % dp;
+        X[r * ld + c] = (r < dim && c < dim) ? entry(r, c) : 0.0;
+    }
+    __syncthreads();
+    const double nb = bound();
+    for (int idx = t; idx < dp * dp; idx += EIG_THREADS) {
+        const int r = idx / dp, c = idx % dp;
+        X[r * ld + c] = (r < dim && c < dim) ? blk[(int64_t)r * dim + c] : 0.0;
+    }
+    __syncthreads();
+    const double ni = bound();
+    if (nb * ni < SPD_COND_LIMIT) {
+        if (t == 0) fallback[b] = 0;
+        return;
+    }
+    for (int idx = t; idx < dim * dim; idx += EIG_THREADS) blk[idx] = entry(idx / dim, idx % dim);
+    if (t == 0) fallback[b] = 1;
 }
 
 // Cyclic Jacobi eigen-decomposition of each symmetric block in LDS, then the block is replaced by
@@ -565,16 +705,23 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         if (!attr_set) {
             HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           120 * 1024));
-            HIP_CHECK(hipFuncSetAttribute((const void*)k_spd_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          120 * 1024));
             attr_set = true;
         }
-        // Gauss-Jordan fast path, then the eigen path for the blocks it rejected
-        const int ld_gj = dim + 1;
-        const size_t lds_gj = ((size_t)dim * ld_gj + 2 * (size_t)dim) * sizeof(double);
-        hipLaunchKernelGGL(k_spd_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds_gj, st, s.scatter.d_m.get(),
-                           s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get(), ld_gj);
+        // Gauss-Jordan fast path (blocks up to 96 x 96, checked at setup), the sharper certificate
+        // for the blocks it could not accept, then the eigen path for the rest
+        hipLaunchKernelGGL(k_spd_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
+                           s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
         check_launch("k_spd_inverse");
+        const int dp = (dim + 3) & ~3;
+        const int ld_pc = dp + 1;
+        const size_t lds_pc = (size_t)2 * dp * ld_pc * sizeof(double);  // 149 KB at dim 96
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_spd_power_certify<VJ>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pc));
+        hipLaunchKernelGGL(k_spd_power_certify<VJ>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds_pc, st,
+                           s.scatter.d_m.get(), s.scatter.d_val_off.get(), s.scatter.d_slot_off.get(), s.d_off.get(),
+                           s.d_nodes.get(), s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get(),
+                           s.d_fallback.get(), ld_pc);
+        check_launch("k_spd_power_certify");
         const int nn = dim + (dim & 1);
         const int ld = nn + 1;
         const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
