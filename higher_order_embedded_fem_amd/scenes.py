@@ -180,7 +180,7 @@ def refine(verts: np.ndarray, conn: np.ndarray, target: int):
                 coords.append(sum(verts[conn[:, k]] for k in f) * 0.25)
             keys.append(key_of(conn[:, :8]))
             coords.append(sum(verts[conn[:, k]] for k in range(8)) * 0.125)
-    else:
+    elif target not in (TET4, HEX8):  # linear targets keep the corner nodes only
         raise ValueError("unsupported refinement target")
 
     nloc = len(keys)

@@ -38,6 +38,7 @@ SMALL_SCENES = {
     "cube2_tet20": lambda: scenes.cube_scene(2, scenes.TET20),
     "cyl8_hex20": lambda: scenes.cylinder_scene(8, scenes.HEX20, 8),
     "cyl6_hex27": lambda: scenes.cylinder_scene(6, scenes.HEX27, 27),
+    "cyl6_hex8": lambda: scenes.cylinder_scene(6, scenes.HEX8, 8),
     "cyl8_hex20_emb": lambda: scenes.cylinder_scene(8, scenes.HEX20, 8, quadrature="embedded"),
 }
 
