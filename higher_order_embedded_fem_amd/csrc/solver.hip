@@ -194,24 +194,40 @@ __device__ inline void entry_row_dot(const Csr3<VT>& A, const NodeEntry& e, int 
     r2 = double(a2);
 }
 
-// Grid-stride loops over the entries of a pattern: block nodes in groups of CG_THREADS / G
-// (G lanes each; block(e, lane, valid) is called the same number of times by every thread of a
-// work-group, as the G-lane shuffles need whole groups), the next pass's entry prefetched;
-// then diagonal-only nodes one per thread (diag(e)).
+// Group ranges of a work-group. Work-groups are dispatched to the 8 XCDs round robin (work-group b
+// on XCD b mod 8), and each XCD has its own L2. With the grid a multiple of 8, XCD x walks the
+// x-th contiguous eighth of the groups with its own work-groups in a grid-stride loop, so its L2
+// holds that slab's gathered vector entries (plus a halo) instead of a share of every node's, and
+// each XCD still streams through one compact window.
+struct GroupRange {
+    int64_t begin, end, step;
+};
+__device__ inline GroupRange group_range(int64_t groups) {
+    const int64_t n = gridDim.x, b = blockIdx.x;
+    if (n % 8) return {b, groups, n};
+    const int64_t x = b % 8;
+    return {x * groups / 8 + b / 8, (x + 1) * groups / 8, n / 8};
+}
+
+// Loops over the entries of a pattern: block nodes in groups of CG_THREADS / G (G lanes each;
+// block(e, lane, valid) is called the same number of times by every thread of a work-group, as
+// the G-lane shuffles need whole groups) over the work-group's group_range, the next group's
+// entry prefetched; then diagonal-only nodes one per thread (diag(e)).
 template <int G, class FB, class FD>
 __device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb, int64_t nd, FB block, FD diag) {
     constexpr int per = CG_THREADS / G;
     const int lane = (int)(threadIdx.x % G);
     const int64_t groups = (nb + per - 1) / per;
     const int64_t sub = threadIdx.x / G;
-    int64_t g = blockIdx.x;
+    const GroupRange R = group_range(groups);
+    int64_t g = R.begin;
     NodeEntry e{0, 0, 0, 0};
-    if (g < groups && g * per + sub < nb) e = ent[g * per + sub];
-    for (; g < groups; g += gridDim.x) {
+    if (g < R.end && g * per + sub < nb) e = ent[g * per + sub];
+    for (; g < R.end; g += R.step) {
         const NodeEntry cur = e;
         const bool valid = g * per + sub < nb;
-        const int64_t gn = g + gridDim.x;
-        if (gn < groups && gn * per + sub < nb) e = ent[gn * per + sub];
+        const int64_t gn = g + R.step;
+        if (gn < R.end && gn * per + sub < nb) e = ent[gn * per + sub];
         block(cur, lane, valid);
     }
     for (int64_t i = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; i < nd; i += (int64_t)gridDim.x * CG_THREADS)
@@ -292,9 +308,10 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
     const int64_t nb = A.n_block, nd = A.n_diag;
     const int64_t groups = (nb + per - 1) / per;
     const int64_t sub = threadIdx.x / G;
-    const int64_t stride = gridDim.x;
-    auto has = [&](int64_t g) { return g < groups && g * per + sub < nb; };
-    int64_t g = blockIdx.x;
+    const GroupRange R = group_range(groups);
+    const int64_t stride = R.step;
+    auto has = [&](int64_t g) { return g < R.end && g * per + sub < nb; };
+    int64_t g = R.begin;
     NodeEntry e{0, 0, 0, 0}, e1{0, 0, 0, 0};
     if (has(g)) e = A.ent[g * per + sub];
     pro_issue();
@@ -302,7 +319,7 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
     row_pf_issue(A, e, lane, has(g), pf);
     if (has(g + stride)) e1 = A.ent[(g + stride) * per + sub];
     if (!pro_finish()) return false;
-    for (; g < groups; g += stride) {
+    for (; g < R.end; g += stride) {
         const NodeEntry cur = e;
         const bool valid = has(g);
         RowPf<VT> pn;
@@ -319,7 +336,9 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
 
 inline unsigned cg_grid(int64_t n_block, int64_t n_diag, int G) {
     const int64_t groups = (n_block + CG_THREADS / G - 1) / (CG_THREADS / G) + (n_diag + CG_THREADS - 1) / CG_THREADS;
-    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(groups, CG_MAX_BLOCKS));
+    int64_t grid = std::max<int64_t>(1, std::min<int64_t>(groups, CG_MAX_BLOCKS));
+    if (grid >= 64) grid -= grid % 8;  // a multiple of the 8 XCDs (group_range)
+    return (unsigned)grid;
 }
 
 // The two vectors each SpMV gathers are stored interleaved per node, six doubles (48 B, three
