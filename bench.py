@@ -45,7 +45,9 @@ def parse():
     p.add_argument("--harvest-steps", type=int, default=10, help="config 4: plain steps that harvest snapshots")
     p.add_argument("--train-steps", type=int, default=600, help="config 4: Adam steps")
     p.add_argument("--scenes", type=int, default=64, help="config 5: number of scenes")
-    p.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline (0: skip)")
+    p.add_argument("--cpu-steps", type=int, default=10,
+                   help="oracle steps timed for the CPU baseline (0: skip); 10 steps of the default workload "
+                        "are 10-15 s of CPU work on 16 threads")
     p.add_argument("--no-profile", action="store_true", help="do not bracket the CG kernel with events")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run a plain 256 MiB device copy 3x first (reference kernel for HBM counters)")
