@@ -132,6 +132,34 @@ __device__ inline double block_sum(double v, double* s_red) {
 //   1  a pivot not positive: B is kept for the eigen path.
 constexpr int GJ_R = 6;
 
+// Pivot row k = 16 R + kt and column k of the register-held block into LDS (owners only).
+template <int R>
+__device__ inline void gj_publish(const double (&a)[GJ_R][GJ_R], double* srow, double* scol, int tr, int tc, int kt) {
+    if (tr == kt) {
+#pragma unroll
+        for (int c = 0; c < GJ_R; ++c) srow[16 * c + tc] = a[R][c];
+    }
+    if (tc == kt) {
+#pragma unroll
+        for (int r = 0; r < GJ_R; ++r) scol[16 * r + tr] = a[r][R];
+    }
+}
+
+// After the rank-1 update: row k -> a_kj / a_kk, column k -> -a_ik / a_kk, the pivot -> 1 / a_kk.
+template <int R>
+__device__ inline void gj_fix(double (&a)[GJ_R][GJ_R], const double (&rk)[GJ_R], const double (&ck)[GJ_R], double ip,
+                              int tr, int tc, int kt) {
+    if (tr == kt) {
+#pragma unroll
+        for (int c = 0; c < GJ_R; ++c) a[R][c] = rk[c];
+    }
+    if (tc == kt) {
+#pragma unroll
+        for (int r = 0; r < GJ_R; ++r) a[r][R] = -ck[r] * ip;
+    }
+    if (tr == kt && tc == kt) a[R][R] = ip;
+}
+
 __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __restrict__ msz,
                                                              const int64_t* __restrict__ val_off,
                                                              double* __restrict__ blocks, int* __restrict__ fallback) {
@@ -176,25 +204,14 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
     norms(nb2, nbinf);
     for (int k = 0; k < dim; ++k) {
         const int buf = k & 1, kr = k >> 4, kt = k & 15;
-        if (tr == kt) {
-#pragma unroll
-            for (int c = 0; c < GJ_R; ++c) {
-                double v = 0.0;
-#pragma unroll
-                for (int r = 0; r < GJ_R; ++r)
-                    if (r == kr) v = a[r][c];
-                s_row[buf][16 * c + tc] = v;
-            }
-        }
-        if (tc == kt) {
-#pragma unroll
-            for (int r = 0; r < GJ_R; ++r) {
-                double v = 0.0;
-#pragma unroll
-                for (int c = 0; c < GJ_R; ++c)
-                    if (c == kr) v = a[r][c];
-                s_col[buf][16 * r + tr] = v;
-            }
+        // kr is the same in every thread: the switch branches on it instead of selecting per entry
+        switch (kr) {
+            case 0: gj_publish<0>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 1: gj_publish<1>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 2: gj_publish<2>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 3: gj_publish<3>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 4: gj_publish<4>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            default: gj_publish<5>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
         }
         __syncthreads();
         const double piv = s_row[buf][k];
@@ -214,30 +231,13 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
         for (int r = 0; r < GJ_R; ++r)
 #pragma unroll
             for (int c = 0; c < GJ_R; ++c) a[r][c] -= ck[r] * rk[c];
-        if (tr == kt) {
-#pragma unroll
-            for (int r = 0; r < GJ_R; ++r)
-                if (r == kr) {
-#pragma unroll
-                    for (int c = 0; c < GJ_R; ++c) a[r][c] = rk[c];
-                }
-        }
-        if (tc == kt) {
-#pragma unroll
-            for (int c = 0; c < GJ_R; ++c)
-                if (c == kr) {
-#pragma unroll
-                    for (int r = 0; r < GJ_R; ++r) a[r][c] = -ck[r] * ip;
-                }
-        }
-        if (tr == kt && tc == kt) {
-#pragma unroll
-            for (int r = 0; r < GJ_R; ++r)
-                if (r == kr) {
-#pragma unroll
-                    for (int c = 0; c < GJ_R; ++c)
-                        if (c == kr) a[r][c] = ip;
-                }
+        switch (kr) {
+            case 0: gj_fix<0>(a, rk, ck, ip, tr, tc, kt); break;
+            case 1: gj_fix<1>(a, rk, ck, ip, tr, tc, kt); break;
+            case 2: gj_fix<2>(a, rk, ck, ip, tr, tc, kt); break;
+            case 3: gj_fix<3>(a, rk, ck, ip, tr, tc, kt); break;
+            case 4: gj_fix<4>(a, rk, ck, ip, tr, tc, kt); break;
+            default: gj_fix<5>(a, rk, ck, ip, tr, tc, kt); break;
         }
     }
     double ni2, niinf;
