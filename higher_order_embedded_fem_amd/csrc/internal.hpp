@@ -66,8 +66,8 @@ struct DevBuf {
 template <class T>
 struct PinnedBuf {
     T* p = nullptr;
-    explicit PinnedBuf(size_t count = 1) {
-        HIP_CHECK(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+    explicit PinnedBuf(size_t count = 1, unsigned flags = hipHostMallocDefault) {
+        HIP_CHECK(hipHostMalloc((void**)&p, count * sizeof(T), flags));
     }
     ~PinnedBuf() {
         if (p) (void)hipHostFree(p);
@@ -229,7 +229,8 @@ struct CgWork {
     hipGraphExec_t batch_exec = nullptr;
     std::vector<const void*> batch_key;
     bool use_graphs = true;
-    PinnedBuf<CgScalars> h_scalars{3};  // two polling slots + final readback
+    PinnedBuf<CgScalars> h_scalars{3};  // final readback (slot 2)
+    PinnedBuf<int> h_status{1, hipHostMallocCoherent};  // the solve's status, written by the kernel that ends it
     hipEvent_t ev[2] = {nullptr, nullptr};
     void ensure(int64_t n, int nblocks);
     ~CgWork();

@@ -372,6 +372,7 @@ struct CgPtrs {
     int nb_a;  // grid of the preamble-1 / A-kernels
     int nb_b;  // grid of the preamble-2 / B-kernels
     CgScalars* sc;
+    int* host_status;  // pinned host copy of sc->status, written when a kernel ends the solve
 };
 
 // preamble 1: r = b - A x -> ra_new.r ; partial |b|^2
@@ -478,6 +479,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
             sc->zTr = zTr;
             sc->rr = rr;
             sc->status = decided;
+            *P.host_status = decided;
         }
         return;
     }
@@ -555,6 +557,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
             if (lead) {
                 sc->pAp = pAp;
                 sc->status = decided;
+                *P.host_status = decided;
             }
             return false;
         }
@@ -775,6 +778,7 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     P.nb_a = (int)blkA;
     P.nb_b = FLEX ? lv->na : (int)blkB;
     P.sc = w.scalars.get();
+    P.host_status = w.h_status.p;
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
     HIP_CHECK(hipMemsetAsync(w.zp[0].get(), 0, sizeof(double) * 2 * w.n, s));  // p_{-1} = 0
     P.ra_new = w.ra[0].get();
@@ -861,14 +865,16 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
         HIP_CHECK(hipGraphLaunch(w.batch_exec, s));
         k_next += CG_BATCH;  // CG_BATCH is even: the buffer parity is back where it started
     };
-    CgScalars* slots = w.h_scalars.p;  // [0], [1]: polling slots
     CgScalars* hs = w.h_scalars.p + 2;
     // two batches in flight; poll the older one
     int64_t nbatches = 0;
     const int64_t max_batches = (max_iter >= 0 ? max_iter / CG_BATCH + 2 : INT64_MAX);
+    // the kernel that ends the solve also writes its status into pinned host memory, so polling
+    // needs no copy per batch: the status read after a batch's event is that batch's or a later one's
+    volatile int* hstat = w.h_status.p;
+    *hstat = CG_RUNNING;
     auto submit = [&](int slot) {
         enqueue_batch();
-        HIP_CHECK(hipMemcpyAsync(slots + slot, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipEventRecord(w.ev[slot], s));
         ++nbatches;
     };
@@ -877,7 +883,7 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     int poll = 0;
     while (true) {
         HIP_CHECK(hipEventSynchronize(w.ev[poll]));
-        if (slots[poll].status != CG_RUNNING || nbatches >= max_batches) break;
+        if (*hstat != CG_RUNNING || nbatches >= max_batches) break;
         submit(poll);
         poll ^= 1;
     }
