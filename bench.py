@@ -189,7 +189,7 @@ def main():
     args.mode = 2 if args.mixed_refined else (1 if args.mixed else 0)
     args.mixed = args.mode != 0
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no multi-GPU path.",
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the library has no multi-GPU path.",
               file=sys.stderr)
         sys.exit(2)
     if args.config in (4, 5):

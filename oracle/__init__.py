@@ -1,7 +1,7 @@
 """ctypes bindings of the CPU FP64 oracle (oracle/oracle.cpp).
 
 Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker of the HIP library.
+cpu_baseline leg of bench.py, always to verify the HIP library, never in its place.
 """
 
 from __future__ import annotations

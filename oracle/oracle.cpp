@@ -1,6 +1,6 @@
 // CPU FP64 oracle (test infrastructure only): a plain restatement of the reference's
 // implicit elastodynamics step. Every function cites the reference file:line it follows.
-// Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker;
+// Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to verify the HIP library;
 // the product library never links it.
 //
 // Parity status of each part (how it is pinned; see tests/test_oracle_*.py):

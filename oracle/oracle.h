@@ -7,8 +7,8 @@
  * conditions, the additive Schwarz preconditioner, preconditioned CG, Newton with
  * backtracking line search and Backward Euler with substepping.
  *
- * It is the CHECKER for the HIP library (tests, smoke test, CPU baseline of the
- * benchmark). Product code never links or calls it.
+ * It verifies the HIP library (tests, smoke test, CPU baseline of the benchmark).
+ * Product code never links or calls it.
  *
  * Conventions: 3x3 matrices are column-major double[9] (m[r + 3*c]); dP/dF is a
  * column-major 9x9 matrix in the vec() ordering vec(A)[3*c + r] = A(r, c), exactly
