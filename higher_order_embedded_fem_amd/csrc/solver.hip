@@ -162,6 +162,43 @@ __device__ inline void row_block_dot(const Csr3<VT>& A, int64_t v, int lane, boo
     r2 = double(a2);
 }
 
+// a_i += sum over the lane's blocks j = j0, j0 + G, ... < k of R_i[3j..3j+2] . x(nc[j]); two blocks
+// per step with all their loads issued before the products, so a lane with two blocks waits for one
+// index -> gather chain instead of two.
+template <int G, class VT, class XN>
+__device__ inline void row_blocks(const VT* __restrict__ R0, const VT* __restrict__ R1, const VT* __restrict__ R2,
+                                  const int32_t* __restrict__ nc, int j0, int k, XN xnode, VT& a0, VT& a1, VT& a2) {
+    int j = j0;
+    for (; j + G < k; j += 2 * G) {
+        const int32_t c0 = nc[j], c1 = nc[j + G];
+        VT u[9], w[9];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            u[c] = R0[3 * j + c];
+            u[3 + c] = R1[3 * j + c];
+            u[6 + c] = R2[3 * j + c];
+            w[c] = R0[3 * (j + G) + c];
+            w[3 + c] = R1[3 * (j + G) + c];
+            w[6 + c] = R2[3 * (j + G) + c];
+        }
+        double y0, y1, y2, z0, z1, z2;
+        xnode((int64_t)c0, y0, y1, y2);
+        xnode((int64_t)c1, z0, z1, z2);
+        const VT x0 = VT(y0), x1 = VT(y1), x2 = VT(y2), v0 = VT(z0), v1 = VT(z1), v2 = VT(z2);
+        a0 += u[0] * x0 + u[1] * x1 + u[2] * x2 + (w[0] * v0 + w[1] * v1 + w[2] * v2);
+        a1 += u[3] * x0 + u[4] * x1 + u[5] * x2 + (w[3] * v0 + w[4] * v1 + w[5] * v2);
+        a2 += u[6] * x0 + u[7] * x1 + u[8] * x2 + (w[6] * v0 + w[7] * v1 + w[8] * v2);
+    }
+    if (j < k) {
+        double y0, y1, y2;
+        xnode((int64_t)nc[j], y0, y1, y2);
+        const VT x0 = VT(y0), x1 = VT(y1), x2 = VT(y2);
+        a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
+        a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
+        a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
+    }
+}
+
 // acc_i = sum over the neighbours b of entry e of A(3v+i, 3b+c) x_c(b) (G lanes, reduced into
 // all lanes of the group). Products and accumulation in VT; the result is returned in FP64.
 template <int G, class VT, class XN>
@@ -174,14 +211,7 @@ __device__ inline void entry_row_dot(const Csr3<VT>& A, const NodeEntry& e, int 
         const VT* R1 = R0 + 3 * k;
         const VT* R2 = R1 + 3 * k;
         const int32_t* nc = A.ncol + e.first;
-        for (int j = lane; j < k; j += G) {
-            double y0, y1, y2;
-            xnode((int64_t)nc[j], y0, y1, y2);
-            const VT x0 = VT(y0), x1 = VT(y1), x2 = VT(y2);
-            a0 += R0[3 * j] * x0 + R0[3 * j + 1] * x1 + R0[3 * j + 2] * x2;
-            a1 += R1[3 * j] * x0 + R1[3 * j + 1] * x1 + R1[3 * j + 2] * x2;
-            a2 += R2[3 * j] * x0 + R2[3 * j + 1] * x1 + R2[3 * j + 2] * x2;
-        }
+        row_blocks<G>(R0, R1, R2, nc, lane, k, xnode, a0, a1, a2);
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) {
