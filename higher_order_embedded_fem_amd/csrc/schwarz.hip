@@ -701,12 +701,6 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
             s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get());
         check_launch("k_schwarz_gather");
         const int dim = 3 * s.scatter.max_m;
-        static bool attr_set = false;
-        if (!attr_set) {
-            HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          120 * 1024));
-            attr_set = true;
-        }
         // Gauss-Jordan fast path (blocks up to 96 x 96, checked at setup), the sharper certificate
         // for the blocks it could not accept, then the eigen path for the rest
         hipLaunchKernelGGL(k_spd_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
@@ -725,6 +719,9 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         const int nn = dim + (dim & 1);
         const int ld = nn + 1;
         const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
+        // the exact size, set on the current device before each launch (147 KB for 96 x 96 blocks)
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
         hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st, s.scatter.d_m.get(),
                            s.scatter.d_val_off.get(), s.d_blocks.get(), (const int*)s.d_fallback.get(), ld, 0);
         check_launch("k_jacobi_inverse");
@@ -761,12 +758,6 @@ void launch_element_psd_projection(Model& m) {
         m.d_elem_off.upload(off, m.stream);
     }
     const int dim = 3 * m.n;
-    const int nn = dim + (dim & 1);
-    const int ld = nn + 1;
-    const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
-    HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  120 * 1024));
-    if (lds > 120 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "element matrix too large for the projection");
     m.d_elem_need.ensure((size_t)m.ne);
     const int ld_c = dim + 1;
     const size_t lds_c = (size_t)dim * ld_c * sizeof(double);
