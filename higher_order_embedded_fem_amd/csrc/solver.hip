@@ -21,6 +21,9 @@ namespace hoefem {
 
 enum : int { CG_RUNNING = 0, CG_CONVERGED = 100, CG_ZERO_RHS = 101 };
 constexpr int CG_THREADS = 256;
+// the A-kernel runs 512-thread work-groups: half as many partial sums for the B-kernel to re-read
+// (measured: 18.5 vs 19.1 us per launch on the Hex20 cylinder; the B-kernel is faster at 256)
+constexpr int CG_THREADS_A = 512;
 constexpr int CG_BATCH = 8;
 
 __device__ inline double wave_sum(double v) {
@@ -31,14 +34,14 @@ __device__ inline double wave_sum(double v) {
 
 // tot[k] = sum over b < nb of partials[b * NV + k], the same value in every thread of every
 // block (fixed per-thread strides, fixed wave and cross-wave summation order).
-template <int NV>
+template <int NV, int BT = CG_THREADS>
 __device__ inline void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[NV]) {
-    __shared__ double s_w[CG_THREADS / 64][NV];
+    __shared__ double s_w[BT / 64][NV];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int b = t; b < nb; b += CG_THREADS)
+    for (int b = t; b < nb; b += BT)
 #pragma unroll
         for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
 #pragma unroll
@@ -50,7 +53,7 @@ __device__ inline void sum_partials(const double* __restrict__ partials, int nb,
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         double s = 0.0;
-        for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
+        for (int w = 0; w < BT / 64; ++w) s += s_w[w][k];
         tot[k] = s;
     }
 }
@@ -85,11 +88,13 @@ __device__ inline void partials_reduce(const double (&acc)[NV], double (&tot)[NV
     }
 }
 
-// partials[blockIdx.x * NV + k] = sum of v[k] over the block's threads.
+// partials[blockIdx.x * NV + k] = sum of v[k] over the block's threads (any block size up to
+// CG_THREADS, a multiple of 64).
 template <int NV>
 __device__ inline void block_partials(const double (&v)[NV], double* __restrict__ partials) {
     __shared__ double s_w[CG_THREADS / 64][NV];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int nw = (int)(blockDim.x >> 6);
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const double s = wave_sum(v[k]);
@@ -100,7 +105,7 @@ __device__ inline void block_partials(const double (&v)[NV], double* __restrict_
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = 0.0;
-            for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
+            for (int w = 0; w < nw; ++w) s += s_w[w][k];
             partials[(int64_t)blockIdx.x * NV + k] = s;
         }
     }
@@ -243,9 +248,9 @@ __device__ inline GroupRange group_range(int64_t groups) {
 // block(e, lane, valid) is called the same number of times by every thread of a work-group, as
 // the G-lane shuffles need whole groups) over the work-group's group_range, the next group's
 // entry prefetched; then diagonal-only nodes one per thread (diag(e)).
-template <int G, class FB, class FD>
+template <int G, int BT = CG_THREADS, class FB, class FD>
 __device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb, int64_t nd, FB block, FD diag) {
-    constexpr int per = CG_THREADS / G;
+    constexpr int per = BT / G;
     const int lane = (int)(threadIdx.x % G);
     const int64_t groups = (nb + per - 1) / per;
     const int64_t sub = threadIdx.x / G;
@@ -260,7 +265,7 @@ __device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb
         if (gn < R.end && gn * per + sub < nb) e = ent[gn * per + sub];
         block(cur, lane, valid);
     }
-    for (int64_t i = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; i < nd; i += (int64_t)gridDim.x * CG_THREADS)
+    for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < nd; i += (int64_t)gridDim.x * BT)
         diag(ent[nb + i]);
 }
 
@@ -364,9 +369,11 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
     return true;
 }
 
-inline unsigned cg_grid(int64_t n_block, int64_t n_diag, int G) {
-    const int64_t groups = (n_block + CG_THREADS / G - 1) / (CG_THREADS / G) + (n_diag + CG_THREADS - 1) / CG_THREADS;
-    int64_t grid = std::max<int64_t>(1, std::min<int64_t>(groups, CG_MAX_BLOCKS));
+inline unsigned cg_grid(int64_t n_block, int64_t n_diag, int G, int bt = CG_THREADS) {
+    const int64_t groups = (n_block + bt / G - 1) / (bt / G) + (n_diag + bt - 1) / bt;
+    // 4 waves per SIMD: 1024 work-groups of 256 threads or 512 of 512
+    const int64_t cap = (int64_t)CG_MAX_BLOCKS * CG_THREADS / bt;
+    int64_t grid = std::max<int64_t>(1, std::min<int64_t>(groups, cap));
     if (grid >= 64) grid -= grid % 8;  // a multiple of the 8 XCDs (group_range)
     return (unsigned)grid;
 }
@@ -487,13 +494,13 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
 // first entry is issued: issuing the entry early (as the B-kernel does) measured 0.25 us slower
 // per launch with 9 MB more HBM fetch on the Hex20 cylinder.
 template <int G, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
+__global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(Csr3<VT> A, CgPtrs P) {
     CgScalars* sc = P.sc;
     if (sc->status != CG_RUNNING) return;
     const int64_t k = sc->iters;  // written by the previous B-kernel (0 after the preamble)
     constexpr int NV = FLEX ? 3 : 2;
     double tot[NV];
-    sum_partials<NV>(P.part_b, P.nb_b, tot);
+    sum_partials<NV, CG_THREADS_A>(P.part_b, P.nb_b, tot);
     // z_k.r_k, |r_k|^2 and (flexible variant) z_k.r_{k-1}
     const double zTr = tot[0], zTr_cross = FLEX ? tot[1] : 0.0, rr = tot[NV - 1];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -541,7 +548,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_a(Csr3<VT> A, CgPtrs P) {
         P.ap_out[6 * v + 5] = a2;
         part[0] += p0 * a0 + p1 * a1 + p2 * a2;
     };
-    for_entries<G>(
+    for_entries<G, CG_THREADS_A>(
         A.ent, A.n_block, A.n_diag,
         [&](const NodeEntry& e, int lane, bool valid) {
             double a0, a1, a2;
@@ -796,7 +803,7 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     const hipStream_t s = w.cs;
     HIP_CHECK(hipEventRecord(w.join[0], m.stream));
     HIP_CHECK(hipStreamWaitEvent(s, w.join[0], 0));
-    const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA);
+    const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA, CG_THREADS_A);
     const unsigned blkB = JAC ? cg_grid(0, Pm.nn, GB) : cg_grid(Pm.n_block, Pm.n_diag, GB);
     CgPtrs P{};
     P.x = x;
@@ -848,11 +855,11 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
                     HIP_CHECK(hipEventCreate(&e));
                     m.prof_ev.push_back(e);
                 }
-                hipExtLaunchKernelGGL(k_cg_a<GA, FLEX, VT>, dim3(blkA), dim3(CG_THREADS), 0, s, m.prof_ev[nev],
+                hipExtLaunchKernelGGL(k_cg_a<GA, FLEX, VT>, dim3(blkA), dim3(CG_THREADS_A), 0, s, m.prof_ev[nev],
                                       m.prof_ev[nev + 1], 0, A, P);
                 nev += 2;
             } else {
-                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
+                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(A, P);
             }
             k_cg_b<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
             if (FLEX)
