@@ -265,8 +265,9 @@ __device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb
         if (gn < R.end && gn * per + sub < nb) e = ent[gn * per + sub];
         block(cur, lane, valid);
     }
-    for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < nd; i += (int64_t)gridDim.x * BT)
-        diag(ent[nb + i]);
+    // diagonal-only nodes from the last work-group backwards: those have the fewest block groups
+    const int64_t rb = (int64_t)gridDim.x - 1 - blockIdx.x;
+    for (int64_t i = rb * BT + threadIdx.x; i < nd; i += (int64_t)gridDim.x * BT) diag(ent[nb + i]);
 }
 
 // The lane's first 3x3 block of a row (j = lane) and its column, loaded a pass ahead.
@@ -364,7 +365,8 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
         block(cur, lane, valid, pf);
         pf = pn;
     }
-    for (int64_t i = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; i < nd; i += (int64_t)gridDim.x * CG_THREADS)
+    const int64_t rb = (int64_t)gridDim.x - 1 - blockIdx.x;  // as in for_entries
+    for (int64_t i = rb * CG_THREADS + threadIdx.x; i < nd; i += (int64_t)gridDim.x * CG_THREADS)
         diag(A.ent[nb + i]);
     return true;
 }
