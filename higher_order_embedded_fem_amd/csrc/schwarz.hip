@@ -130,62 +130,70 @@ __device__ inline double block_sum(double v, double* s_red) {
 //   2  pivots positive, the cheap bound too loose: the inverse replaces the block and
 //      k_spd_power_certify decides with a sharper bound;
 //   1  a pivot not positive: B is kept for the eigen path.
-constexpr int GJ_R = 6;
+// RR x RR register tiles per thread: RR = 6 covers blocks up to 96 x 96, RR = 4 up to 64 x 64 (with a
+// third of the instructions and registers).
 
 // Pivot row k = 16 R + kt and column k of the register-held block into LDS (owners only).
-template <int R>
-__device__ inline void gj_publish(const double (&a)[GJ_R][GJ_R], double* srow, double* scol, int tr, int tc, int kt) {
+template <int R, int RR>
+__device__ inline void gj_publish(const double (&a)[RR][RR], double* srow, double* scol, int tr, int tc, int kt) {
+    if constexpr (R >= RR) return;
+    else {
     if (tr == kt) {
 #pragma unroll
-        for (int c = 0; c < GJ_R; ++c) srow[16 * c + tc] = a[R][c];
+        for (int c = 0; c < RR; ++c) srow[16 * c + tc] = a[R][c];
     }
     if (tc == kt) {
 #pragma unroll
-        for (int r = 0; r < GJ_R; ++r) scol[16 * r + tr] = a[r][R];
+        for (int r = 0; r < RR; ++r) scol[16 * r + tr] = a[r][R];
+    }
     }
 }
 
 // After the rank-1 update: row k -> a_kj / a_kk, column k -> -a_ik / a_kk, the pivot -> 1 / a_kk.
-template <int R>
-__device__ inline void gj_fix(double (&a)[GJ_R][GJ_R], const double (&rk)[GJ_R], const double (&ck)[GJ_R], double ip,
-                              int tr, int tc, int kt) {
+template <int R, int RR>
+__device__ inline void gj_fix(double (&a)[RR][RR], const double (&rk)[RR], const double (&ck)[RR], double ip, int tr,
+                              int tc, int kt) {
+    if constexpr (R >= RR) return;
+    else {
     if (tr == kt) {
 #pragma unroll
-        for (int c = 0; c < GJ_R; ++c) a[R][c] = rk[c];
+        for (int c = 0; c < RR; ++c) a[R][c] = rk[c];
     }
     if (tc == kt) {
 #pragma unroll
-        for (int r = 0; r < GJ_R; ++r) a[r][R] = -ck[r] * ip;
+        for (int r = 0; r < RR; ++r) a[r][R] = -ck[r] * ip;
     }
     if (tr == kt && tc == kt) a[R][R] = ip;
+    }
 }
 
+template <int RR>
 __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __restrict__ msz,
                                                              const int64_t* __restrict__ val_off,
                                                              double* __restrict__ blocks, int* __restrict__ fallback) {
-    __shared__ double s_row[2][16 * GJ_R], s_col[2][16 * GJ_R];
-    __shared__ double s_x[GJ_R][EIG_THREADS];  // row-sum partials, then the transpose exchange
+    __shared__ double s_row[2][16 * RR], s_col[2][16 * RR];
+    __shared__ double s_x[RR][EIG_THREADS];  // row-sum partials, then the transpose exchange
     __shared__ double s_red[EIG_THREADS];
     const int64_t b = blockIdx.x;
     const int dim = 3 * msz[b];
     const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
     double* blk = blocks + val_off[b];
-    double a[GJ_R][GJ_R];
+    double a[RR][RR];
 #pragma unroll
-    for (int r = 0; r < GJ_R; ++r)
+    for (int r = 0; r < RR; ++r)
 #pragma unroll
-        for (int c = 0; c < GJ_R; ++c) {
+        for (int c = 0; c < RR; ++c) {
             const int i = 16 * r + tr, j = 16 * c + tc;
             a[r][c] = (i < dim && j < dim) ? blk[(int64_t)i * dim + j] : 0.0;
         }
     // |.|_F^2 and |.|_inf of the register-held matrix
     auto norms = [&](double& f2, double& finf) {
-        double loc = 0.0, rs[GJ_R];
+        double loc = 0.0, rs[RR];
 #pragma unroll
-        for (int r = 0; r < GJ_R; ++r) {
+        for (int r = 0; r < RR; ++r) {
             rs[r] = 0.0;
 #pragma unroll
-            for (int c = 0; c < GJ_R; ++c) {
+            for (int c = 0; c < RR; ++c) {
                 loc += a[r][c] * a[r][c];
                 rs[r] += fabs(a[r][c]);
             }
@@ -193,11 +201,11 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
         f2 = block_sum(loc, s_red);  // synchronizes
         double* part = &s_x[0][0];   // [16 (tc)][96 (row)]
 #pragma unroll
-        for (int r = 0; r < GJ_R; ++r) part[tc * (16 * GJ_R) + 16 * r + tr] = rs[r];
+        for (int r = 0; r < RR; ++r) part[tc * (16 * RR) + 16 * r + tr] = rs[r];
         __syncthreads();
         double row = 0.0;
-        if (t < 16 * GJ_R)
-            for (int q = 0; q < 16; ++q) row += part[q * (16 * GJ_R) + t];
+        if (t < 16 * RR)
+            for (int q = 0; q < 16; ++q) row += part[q * (16 * RR) + t];
         finf = block_max(row, s_red);  // synchronizes
     };
     double nb2, nbinf;
@@ -206,12 +214,12 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
         const int buf = k & 1, kr = k >> 4, kt = k & 15;
         // kr is the same in every thread: the switch branches on it instead of selecting per entry
         switch (kr) {
-            case 0: gj_publish<0>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
-            case 1: gj_publish<1>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
-            case 2: gj_publish<2>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
-            case 3: gj_publish<3>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
-            case 4: gj_publish<4>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
-            default: gj_publish<5>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 0: gj_publish<0, RR>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 1: gj_publish<1, RR>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 2: gj_publish<2, RR>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 3: gj_publish<3, RR>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            case 4: gj_publish<4, RR>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
+            default: gj_publish<5, RR>(a, s_row[buf], s_col[buf], tr, tc, kt); break;
         }
         __syncthreads();
         const double piv = s_row[buf][k];
@@ -220,24 +228,24 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
             return;
         }
         const double ip = 1.0 / piv;
-        double rk[GJ_R], ck[GJ_R];
+        double rk[RR], ck[RR];
 #pragma unroll
-        for (int c = 0; c < GJ_R; ++c) rk[c] = s_row[buf][16 * c + tc] * ip;
+        for (int c = 0; c < RR; ++c) rk[c] = s_row[buf][16 * c + tc] * ip;
 #pragma unroll
-        for (int r = 0; r < GJ_R; ++r) ck[r] = s_col[buf][16 * r + tr];
+        for (int r = 0; r < RR; ++r) ck[r] = s_col[buf][16 * r + tr];
         // a_ij - a_ik a_kj / a_kk everywhere, then row k -> a_kj / a_kk, column k -> -a_ik / a_kk,
         // the pivot -> 1 / a_kk
 #pragma unroll
-        for (int r = 0; r < GJ_R; ++r)
+        for (int r = 0; r < RR; ++r)
 #pragma unroll
-            for (int c = 0; c < GJ_R; ++c) a[r][c] -= ck[r] * rk[c];
+            for (int c = 0; c < RR; ++c) a[r][c] -= ck[r] * rk[c];
         switch (kr) {
-            case 0: gj_fix<0>(a, rk, ck, ip, tr, tc, kt); break;
-            case 1: gj_fix<1>(a, rk, ck, ip, tr, tc, kt); break;
-            case 2: gj_fix<2>(a, rk, ck, ip, tr, tc, kt); break;
-            case 3: gj_fix<3>(a, rk, ck, ip, tr, tc, kt); break;
-            case 4: gj_fix<4>(a, rk, ck, ip, tr, tc, kt); break;
-            default: gj_fix<5>(a, rk, ck, ip, tr, tc, kt); break;
+            case 0: gj_fix<0, RR>(a, rk, ck, ip, tr, tc, kt); break;
+            case 1: gj_fix<1, RR>(a, rk, ck, ip, tr, tc, kt); break;
+            case 2: gj_fix<2, RR>(a, rk, ck, ip, tr, tc, kt); break;
+            case 3: gj_fix<3, RR>(a, rk, ck, ip, tr, tc, kt); break;
+            case 4: gj_fix<4, RR>(a, rk, ck, ip, tr, tc, kt); break;
+            default: gj_fix<5, RR>(a, rk, ck, ip, tr, tc, kt); break;
         }
     }
     double ni2, niinf;
@@ -247,12 +255,12 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
     // exchanges the entries (r, 0..5)
     const int partner = tc * 16 + tr;
 #pragma unroll
-    for (int r = 0; r < GJ_R; ++r) {
+    for (int r = 0; r < RR; ++r) {
 #pragma unroll
-        for (int x = 0; x < GJ_R; ++x) s_x[x][t] = a[x][r];
+        for (int x = 0; x < RR; ++x) s_x[x][t] = a[x][r];
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < GJ_R; ++c) {
+        for (int c = 0; c < RR; ++c) {
             const int i = 16 * r + tr, j = 16 * c + tc;
             if (i < dim && j < dim) blk[(int64_t)i * dim + j] = 0.5 * (a[r][c] + s_x[c][partner]);
         }
@@ -703,8 +711,12 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         const int dim = 3 * s.scatter.max_m;
         // Gauss-Jordan fast path (blocks up to 96 x 96, checked at setup), the sharper certificate
         // for the blocks it could not accept, then the eigen path for the rest
-        hipLaunchKernelGGL(k_spd_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
-                           s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
+        if (dim <= 64)
+            hipLaunchKernelGGL(k_spd_inverse<4>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
+                               s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
+        else
+            hipLaunchKernelGGL(k_spd_inverse<6>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
+                               s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
         check_launch("k_spd_inverse");
         const int dp = (dim + 3) & ~3;
         const int ld_pc = dp + 1;
