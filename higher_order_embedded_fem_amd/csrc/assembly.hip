@@ -235,8 +235,9 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
 // keep their accumulators in registers across chunks. Lane layout of v_mfma_f64_16x16x4 (lane l):
 // A operand A(l % 16, l / 16), B operand B(l / 16, l % 16), result item v = D(4 v + l / 16, l % 16). Each entry of K_e is written exactly once
 // (diagonal pairs from their lower triangle), so K_e is exactly symmetric.
-constexpr int MF_QC = 8;            // quadrature points per chunk
+constexpr int MF_QC = 8;            // quadrature points per product chunk
 constexpr int MF_KK = 3 * MF_QC;    // rows of A^T / B per chunk
+constexpr int MF_QA = 16;           // quadrature points whose gradients and dP/dF are formed together
 
 typedef double mf_double4 __attribute__((ext_vector_type(4)));
 
@@ -254,8 +255,8 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness_mfma(
     __shared__ int s_node[N];
     __shared__ double s_u[3 * N];
     __shared__ double s_X[3 * GN];
-    __shared__ double s_C[MF_QC][81];
-    __shared__ double s_G[MF_KK * NP];
+    __shared__ double s_C[MF_QA][81];
+    __shared__ double s_G[3 * MF_QA * NP];
     __shared__ double s_U[6][MF_KK * NP];
     if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
     __syncthreads();
@@ -265,12 +266,14 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness_mfma(
 #pragma unroll
     for (int j = 0; j < JPW; ++j) acc[j] = mf_double4{0.0, 0.0, 0.0, 0.0};
     const int q0 = qoff[e], q1 = qoff[e + 1];
-    for (int qb = q0; qb < q1; qb += MF_QC) {
-        const int cnt = min(MF_QC, q1 - qb);
-        for (int k = t; k < MF_KK * NP; k += STIFF_THREADS) s_G[k] = 0.0;
+    // groups of up to MF_QA points: their gradients and dP/dF in parallel (one lane per point), then
+    // the products in chunks of MF_QC points
+    for (int qa = q0; qa < q1; qa += MF_QA) {
+        const int cntA = min(MF_QA, q1 - qa);
+        for (int k = t; k < 3 * MF_QA * NP; k += STIFF_THREADS) s_G[k] = 0.0;
         __syncthreads();
-        if (t < cnt) {
-            const int q = qb + t;
+        if (t < cntA) {
+            const int q = qa + t;
             const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
             double J[9], Jinv[9];
             ref_jacobian<ET>(s_X, xi, J);
@@ -305,38 +308,43 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness_mfma(
             }
         }
         __syncthreads();
-        // U^{ij} for the six pairs (i <= j); rows of quadrature points beyond cnt are zero
-        for (int idx = t; idx < 6 * MF_KK * NP; idx += STIFF_THREADS) {
-            const int p = idx / (MF_KK * NP), rem = idx % (MF_KK * NP);
-            const int kk = rem / NP, col = rem % NP;
-            const int qq = kk / 3, k = kk % 3;
-            const int i = p < 3 ? 0 : (p < 5 ? 1 : 2);
-            const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
-            double v = 0.0;
-            if (qq < cnt) {
-                const double* C = s_C[qq];
+        for (int qc = 0; qc < cntA; qc += MF_QC) {
+            const int cnt = min(MF_QC, cntA - qc);
+            const double* G0 = s_G + 3 * qc * NP;  // the chunk's rows
+            // U^{ij} for the six pairs (i <= j); rows of quadrature points beyond cnt are zero
+            for (int idx = t; idx < 6 * MF_KK * NP; idx += STIFF_THREADS) {
+                const int p = idx / (MF_KK * NP), rem = idx % (MF_KK * NP);
+                const int kk = rem / NP, col = rem % NP;
+                const int qq = kk / 3, k = kk % 3;
+                const int i = p < 3 ? 0 : (p < 5 ? 1 : 2);
+                const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
+                double v = 0.0;
+                if (qq < cnt) {
+                    const double* C = s_C[qc + qq];
 #pragma unroll
-                for (int m = 0; m < 3; ++m) v += C[(3 * k + i) + 9 * (3 * m + j)] * s_G[(3 * qq + m) * NP + col];
+                    for (int m = 0; m < 3; ++m) v += C[(3 * k + i) + 9 * (3 * m + j)] * G0[(3 * qq + m) * NP + col];
+                }
+                s_U[p][rem] = v;
             }
-            s_U[p][rem] = v;
-        }
-        __syncthreads();
-        const int ksteps = (3 * cnt + 3) / 4;
+            __syncthreads();
+            // rows kk >= 3 cnt of U are zero, so the A rows read there (zero or the next chunk's) add nothing
+            const int ksteps = (3 * cnt + 3) / 4;
 #pragma unroll
-        for (int jj = 0; jj < JPW; ++jj) {
-            const int jb = w + 4 * jj;
-            if (jb < JOBS) {
-                const int p = jb / (NT * NT), tile = jb % (NT * NT);
-                const int ti = tile / NT, tj = tile % NT;
-                for (int ks = 0; ks < ksteps; ++ks) {
-                    const int kk = 4 * ks + (l >> 4);
-                    const double a = s_G[kk * NP + 16 * ti + (l & 15)];
-                    const double b = s_U[p][kk * NP + 16 * tj + (l & 15)];
-                    acc[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[jj], 0, 0, 0);
+            for (int jj = 0; jj < JPW; ++jj) {
+                const int jb = w + 4 * jj;
+                if (jb < JOBS) {
+                    const int p = jb / (NT * NT), tile = jb % (NT * NT);
+                    const int ti = tile / NT, tj = tile % NT;
+                    for (int ks = 0; ks < ksteps; ++ks) {
+                        const int kk = 4 * ks + (l >> 4);
+                        const double a = G0[kk * NP + 16 * ti + (l & 15)];
+                        const double b = s_U[p][kk * NP + 16 * tj + (l & 15)];
+                        acc[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[jj], 0, 0, 0);
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     double* out = ke + e * (int64_t)M3 * M3;
 #pragma unroll
