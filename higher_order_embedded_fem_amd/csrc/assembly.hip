@@ -237,7 +237,7 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
 // (diagonal pairs from their lower triangle), so K_e is exactly symmetric.
 constexpr int MF_QC = 8;            // quadrature points per product chunk
 constexpr int MF_KK = 3 * MF_QC;    // rows of A^T / B per chunk
-constexpr int MF_QA = 16;           // quadrature points whose gradients and dP/dF are formed together
+constexpr int MF_QA = 28;           // quadrature points whose gradients and dP/dF are formed together
 
 typedef double mf_double4 __attribute__((ext_vector_type(4)));
 
