@@ -8,7 +8,8 @@
 * The reference's block preconditioner test (simulation_toolbox/src/fem/newton_cg.rs:827-876): the
   Schwarz operator with one subdomain per node is the 3 x 3 block-diagonal inverse of the known
   9 x 9 matrix.
-* CG status cases: zero right-hand side, iteration limit.
+* CG status cases: zero right-hand side, iteration limit, no iteration limit (the solve must end on
+  the converged status alone).
 * A model without elements, and interface elements with empty and single-point rules, against the
   oracle; a degenerate element reports the singular status, and the library stays usable after it.
 """
@@ -149,6 +150,20 @@ def test_cg_zero_rhs_and_iteration_limit(fem, tet1):
     x = torch.zeros(12, dtype=torch.float64, device="cuda")
     it, st = fem.conjugate_gradient(gm, A, dev(b), x, None, tol=1e-15, max_iter=1)
     assert st == HOEFEM_ERROR_CG_MAX_ITERATIONS and it == 1
+
+
+def test_cg_without_iteration_limit_stops_on_convergence(fem, tet1):
+    # max_iter < 0: no host-side batch limit, the solve ends only on the status the converging kernel
+    # writes back (ConjugateGradient without with_max_iter, fenris/src/cg.rs:377-482)
+    sc, gm, rp, col = tet1
+    D = np.eye(12)
+    D[:3, :3] = np.array([[21.0, -1.0, -5.0], [-1.0, 11.0, -4.0], [-5.0, -4.0, 26.0]])
+    x0 = np.zeros(12)
+    x0[:3] = [1.0, 3.0, 2.0]
+    x = torch.zeros(12, dtype=torch.float64, device="cuda")
+    it, st = fem.conjugate_gradient(gm, dev(_csr_values(rp, col, D)), dev(D @ x0), x, None, tol=1e-14, max_iter=-1)
+    assert st == 0 and 0 < it <= 3
+    assert np.abs(host(x) - x0).max() <= 1e-12
 
 
 def _assembly_matches_oracle(fem, sc, tol=1e-10):
