@@ -4,12 +4,14 @@
 //             beta] p <- z + beta p (on the fly, double-buffered), Ap = A p, partial pAp
 //   B-kernel: [sum pAp; indefiniteness status; alpha] x += alpha p, r <- r - alpha Ap
 //             (on the fly, double-buffered), z = P r, partial zTr and |r|^2
-// Every kernel runs a persistent grid of at most CG_MAX_BLOCKS blocks that writes one partial
-// sum per block; the next kernel's blocks each add those partials up again in the same fixed
-// order, so all blocks agree on the scalars bit for bit and the result is deterministic. No
-// atomics or device-scope fences are needed: the kernel boundary orders the partials.
-// The host enqueues batches of iterations and polls the status of the batch before the one
-// just enqueued. Also: the residual / state kernels of the Backward-Euler Newton driver.
+// Every kernel runs a persistent grid (at most 1024 work-groups of 256 threads, the A-kernel 512 of
+// 512) whose node ranges follow the XCDs, and writes one partial sum per work-group; the next
+// kernel's work-groups each add those partials up again in the same fixed order, so all of them
+// agree on the scalars bit for bit and the result is deterministic. No atomics or device-scope
+// fences are needed: the kernel boundary orders the partials. The host keeps two batches of
+// iterations (captured HIP graphs) in flight; the kernel that ends the solve writes its status into
+// pinned host memory, which the host reads after each batch's event. Also: the residual / state
+// kernels of the Backward-Euler Newton driver.
 #include <algorithm>
 #include <cmath>
 
