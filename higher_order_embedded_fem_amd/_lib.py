@@ -25,6 +25,7 @@ EXPORTED = [
     "hoefem_model_destroy",
     "hoefem_model_set_stream",
     "hoefem_model_set_material",
+    "hoefem_model_set_pointer_mode",
     "hoefem_model_ndof",
     "hoefem_model_nnz",
     "hoefem_model_pattern",
@@ -32,6 +33,7 @@ EXPORTED = [
     "hoefem_model_colors",
     "hoefem_model_forces_add",
     "hoefem_model_stiffness_add",
+    "hoefem_model_stiffness_add_prec",
     "hoefem_model_mass",
     "hoefem_model_dirichlet_csr",
     "hoefem_model_spmv",
@@ -56,6 +58,14 @@ EXPORTED = [
     "hoefem_sim_damping_values",
     "hoefem_sim_residual",
     "hoefem_sim_step",
+    "hoefem_sim_update_damping",
+    "hoefem_sim_eval_f",
+    "hoefem_sim_apply_mass_add",
+    "hoefem_sim_set_state",
+    "hoefem_sim_init_solve_jacobian_combination",
+    "hoefem_sim_solve_jacobian_combination",
+    "hoefem_sim_init_apply_jacobian_combination",
+    "hoefem_sim_apply_jacobian_combination",
     "hoefem_learned_param_count",
     "hoefem_learned_create",
     "hoefem_learned_destroy",
@@ -215,6 +225,8 @@ def load(path: str = LIB_PATH):
     L.hoefem_model_colors.argtypes = [vp, i32, vp, vp]
     L.hoefem_model_forces_add.argtypes = [vp, f64, f64, vp, vp]
     L.hoefem_model_stiffness_add.argtypes = [vp, f64, f64, vp, vp]
+    L.hoefem_model_stiffness_add_prec.argtypes = [vp, f64, f64, vp, vp, i32]
+    L.hoefem_model_set_pointer_mode.argtypes = [vp, i32]
     L.hoefem_model_mass.argtypes = [vp, f64, vp]
     L.hoefem_model_dirichlet_csr.argtypes = [vp, i64, vp, vp]
     L.hoefem_model_spmv.argtypes = [vp, vp, f64, vp, f64, vp]
@@ -243,6 +255,14 @@ def load(path: str = LIB_PATH):
     L.hoefem_sim_damping_values.argtypes = [vp, vp]
     L.hoefem_sim_residual.argtypes = [vp, f64, f64, vp, vp, vp, vp]
     L.hoefem_sim_step.argtypes = [vp, f64, f64, vp, vp, C.POINTER(StepStats)]
+    L.hoefem_sim_update_damping.argtypes = [vp, vp]
+    L.hoefem_sim_eval_f.argtypes = [vp, f64, vp, vp, vp]
+    L.hoefem_sim_apply_mass_add.argtypes = [vp, vp, vp]
+    L.hoefem_sim_set_state.argtypes = [vp, f64, vp, vp]
+    L.hoefem_sim_init_solve_jacobian_combination.argtypes = [vp, C.POINTER(f64), C.POINTER(f64)]
+    L.hoefem_sim_solve_jacobian_combination.argtypes = [vp, vp, vp, C.POINTER(i64)]
+    L.hoefem_sim_init_apply_jacobian_combination.argtypes = [vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64)]
+    L.hoefem_sim_apply_jacobian_combination.argtypes = [vp, vp, vp]
     L.hoefem_learned_param_count.argtypes = []
     L.hoefem_learned_param_count.restype = i64
     L.hoefem_learned_create.argtypes = [vp, C.POINTER(LearnedDesc), C.POINTER(vp)]

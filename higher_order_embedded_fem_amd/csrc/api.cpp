@@ -11,8 +11,12 @@ namespace hoefem {
 
 // solver.hip
 void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu);
-void launch_residual(Model& m, const double* Mvals, const double* gforce, const uint8_t* dirmask, double dt,
-                     const double* w, const double* dv, double* F, CgWork& cw, double* norm2);
+void launch_bc_copy(Model& m, const double* src, const uint8_t* dirmask, double* dst);
+void launch_node_force(Model& m, const double* gforce, const double* dv, const uint8_t* dirmask, double* f);
+void launch_scale(Model& m, const double* src, double sc, double* dst);
+void launch_mass_add(Model& m, const double* Mvals, const double* x, double* y, CgWork& cw, double* norm2);
+template <class T>
+void launch_comb(Model& m, int64_t n, T* out, double cd, const double* D, double cm, const double* M);
 void launch_state_v(Model& m, const double* vn, const double* w, const uint8_t* dirmask, double* vv);
 void launch_axpy_neg(Model& m, double* w, double delta, const double* sol);
 void launch_be_begin(Model& m, const double* u, const double* v, double dt, double* up, double* vn);
@@ -155,6 +159,17 @@ struct Sim {
     int last_err_flags = 0;
     bool damping = false;           // a damping matrix D is in use
     DevBuf<double> d_D, d_vv, d_dv;  // D (model pattern), v = v_n + w, D v
+    DevBuf<double> d_f;              // eval_f output of the Newton residual
+    // dynamic-system state of the hamilton2 entry points (NewtonCgDynamicSystem, newton_cg.rs:413-820)
+    DevBuf<double> d_su, d_sv;       // state of set_state, Dirichlet DOFs zeroed
+    bool state_set = false;
+    bool solve_outdated = true;      // J and the preconditioner need rebuilding for solve
+    bool has_alpha = false, has_beta = false;
+    double alpha = 0.0, beta = 0.0;  // init_solve_jacobian_combination
+    bool apply_outdated = true, apply_ready = false;
+    bool ap_has[3] = {false, false, false};
+    double ap_coef[3] = {0.0, 0.0, 0.0};  // init_apply_jacobian_combination (alpha, beta, gamma)
+    DevBuf<double> d_H;              // the combination applied by apply_jacobian_combination
 };
 
 static void build_bc(Model& m, const std::vector<int32_t>& dir, DevBuf<uint8_t>& dmask, DevBuf<int32_t>& dbc,
@@ -185,16 +200,17 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
     s->dir.assign(d->dirichlet_nodes, d->dirichlet_nodes + d->num_dirichlet);
     build_bc(m, s->dir, s->d_dirmask, s->d_bc_nodes, s->n_bc);
     const int64_t n = 3 * m.nv, nnz = m.pattern.nnz();
-    for (DevBuf<double>* b : {&s->d_gforce, &s->d_up, &s->d_vn, &s->d_w, &s->d_F, &s->d_sol, &s->d_uu, &s->d_us, &s->d_vs})
+    for (DevBuf<double>* b : {&s->d_gforce, &s->d_up, &s->d_vn, &s->d_w, &s->d_F, &s->d_sol, &s->d_uu, &s->d_us,
+                              &s->d_vs, &s->d_f, &s->d_su, &s->d_sv})
         b->ensure(n);
     if (d->mixed_precision < 0 || d->mixed_precision > 2) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "mixed_precision must be 0, 1 or 2");
     if (!(d->mass_damping_coefficient >= 0.0 && std::isfinite(d->mass_damping_coefficient)) ||
         !(d->stiffness_damping_coefficient >= 0.0 && std::isfinite(d->stiffness_damping_coefficient)))
         throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "damping coefficients must be finite and >= 0");
     s->damping = d->mass_damping_coefficient > 0.0 || d->stiffness_damping_coefficient > 0.0;
+    s->d_vv.ensure(n);
     if (s->damping) {
         s->d_D.ensure(nnz);
-        s->d_vv.ensure(n);
         s->d_dv.ensure(n);
         HIP_CHECK(hipMemsetAsync(s->d_D.get(), 0, sizeof(double) * nnz, m.stream));
     }
@@ -237,17 +253,25 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
     raise_kernel_errors(m);
 }
 
-// F(w) and |F|^2 for the current (u_p, v_n) held in the sim
+// f = M g + f_int(uu) - D vv with Dirichlet rows zeroed, uu and vv already carrying the Dirichlet
+// values (NewtonCgDynamicSystem::eval_f, newton_cg.rs:441-478)
+static void sim_eval_f(Sim& s, const double* uu, const double* vv, double* f) {
+    Model& m = *s.model;
+    launch_element_forces(m, s.mu_r, s.lambda_r, uu);
+    if (s.damping) launch_spmv(m.pattern, s.d_D.get(), 0.0, s.d_dv.get(), 1.0, vv, m.stream);
+    launch_node_force(m, s.d_gforce.get(), s.damping ? s.d_dv.get() : nullptr, s.d_dirmask.get(), f);
+}
+
+// F(w) = -dt f(u_p + dt w, v_n + w) + M w and |F|^2 for the current (u_p, v_n) held in the sim
+// (BackwardEulerImplicitFunction::eval_into, hamilton2/src/integrators/euler.rs:97-114), composed of
+// the same kernels as the dynamic-system entry points eval_f and apply_mass_add
 static double sim_eval(Sim& s, double dt, const double* w) {
     Model& m = *s.model;
     launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
-    launch_element_forces(m, s.mu_r, s.lambda_r, s.d_uu.get());
-    if (s.damping) {  // D v with v = v_n + w (Dirichlet velocities zero)
-        launch_state_v(m, s.d_vn.get(), w, s.d_dirmask.get(), s.d_vv.get());
-        launch_spmv(m.pattern, s.d_D.get(), 0.0, s.d_dv.get(), 1.0, s.d_vv.get(), m.stream);
-    }
-    launch_residual(m, s.d_M.get(), s.d_gforce.get(), s.d_dirmask.get(), dt, w, s.damping ? s.d_dv.get() : nullptr,
-                    s.d_F.get(), s.cg, s.d_norm2.get());
+    if (s.damping) launch_state_v(m, s.d_vn.get(), w, s.d_dirmask.get(), s.d_vv.get());
+    sim_eval_f(s, s.d_uu.get(), s.d_vv.get(), s.d_f.get());
+    launch_scale(m, s.d_f.get(), dt * -1.0, s.d_F.get());
+    launch_mass_add(m, s.d_M.get(), w, s.d_F.get(), s.cg, s.d_norm2.get());
     HIP_CHECK(hipMemcpyAsync(m.h_err.p, m.d_err.get(), sizeof(int), hipMemcpyDeviceToHost, m.stream));
     HIP_CHECK(hipMemcpyAsync(s.h_norm2.p, s.d_norm2.get(), sizeof(double), hipMemcpyDeviceToHost, m.stream));
     HIP_CHECK(hipStreamSynchronize(m.stream));
@@ -260,34 +284,37 @@ static double sim_eval(Sim& s, double dt, const double* w) {
     return *s.h_norm2.p;
 }
 
-// NewtonCgDynamicSystem::solve_jacobian_combination (newton_cg.rs:680-802): K(u) -> J = M + dt^2 K
-// with Dirichlet conditions -> preconditioner update -> PCG from a zero guess. CG's
-// MaxIterationsReached and IndefiniteOperator are tolerated (newton_cg.rs:776-797).
-static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double* rhs, double* sol) {
+// compute_jacobian_combination_into (simulation_toolbox/src/fem/system_assembly.rs:196-235) at the
+// state uu (Dirichlet values applied): J = M - alpha K(uu) - beta D, a term left out when its
+// coefficient is None, then the Dirichlet conditions and SchwarzPreconditioner::update
+// (newton_cg.rs:700-728). K is the unprojected stiffness. Mixed precision: mode 1 builds J in FP32
+// from FP32 element tangents; mode 2 builds it in FP64 and keeps an FP32 copy for the CG.
+static hoefem_status sim_assemble_jacobian(Sim& s, bool has_a, double a, bool has_b, double b, const double* uu) {
     Model& m = *s.model;
-    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
-    launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_uu.get(), s.mixed_mode == 1);
-    const double alpha = -dt * dt;
+    const int64_t nnz = m.pattern.nnz();
+    const double* D = (has_b && s.damping) ? s.d_D.get() : nullptr;
+    const double cd = has_b ? -b : 0.0;
+    if (has_a) launch_element_stiffness(m, s.mu_r, s.lambda_r, uu, s.mixed_mode == 1);
     if (s.mixed_mode == 1) {
-        launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -alpha, s.d_M.get(), true, m.stream,
-                            s.damping ? s.d_D.get() : nullptr, dt, 1.0);
+        if (has_a)
+            launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -a, s.d_M.get(), true,
+                                m.stream, D, cd, 1.0);
+        else
+            launch_comb(m, nnz, s.d_J32.get(), cd, D, 1.0, s.d_M.get());
         launch_dirichlet(m.pattern, s.d_J32.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
                          m.stream);
         if (s.schwarz) schwarz_update(*s.schwarz, s.d_J32.get());
-    } else if (s.mixed_mode == 2) {
-        // FP64 Jacobian (for the refinement residual and the Schwarz factorizations), FP32 copy
-        // of its values for the CG operator; P is stored in FP32
-        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream,
-                            s.damping ? s.d_D.get() : nullptr, dt, 1.0);
-        launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
-                         m.stream);
-        launch_f64_to_f32(m, s.d_J.get(), s.d_J32.get(), m.pattern.nnz());
-        if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
     } else {
-        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -alpha, s.d_M.get(), true, m.stream,
-                            s.damping ? s.d_D.get() : nullptr, dt, 1.0);
+        if (has_a)
+            launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -a, s.d_M.get(), true, m.stream,
+                                D, cd, 1.0);
+        else
+            launch_comb(m, nnz, s.d_J.get(), cd, D, 1.0, s.d_M.get());
         launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
                          m.stream);
+        // mode 2: the FP64 Jacobian (refinement residual, Schwarz factorizations) and an FP32 copy of
+        // its values for the CG operator; P is stored in FP32
+        if (s.mixed_mode == 2) launch_f64_to_f32(m, s.d_J.get(), s.d_J32.get(), nnz);
         if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
     }
     const int e = read_and_clear_err(m);
@@ -295,8 +322,17 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
         s.last_err_flags |= e;
         return HOEFEM_ERROR_SINGULAR;
     }
+    return HOEFEM_OK;
+}
+
+// The CG part of NewtonCgDynamicSystem::solve_jacobian_combination (newton_cg.rs:730-801): PCG from
+// a zero guess on the assembled J with the updated preconditioner. MaxIterationsReached and
+// IndefiniteOperator are tolerated (newton_cg.rs:776-797); any other CG status is returned.
+static hoefem_status sim_cg(Sim& s, const double* rhs, double* sol, int64_t* iterations) {
+    Model& m = *s.model;
     HIP_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * 3 * m.nv, m.stream));
     int64_t it = 0;
+    *iterations = 0;
     Learned* L = s.learned;
     if (L && L->enabled && L->trained && s.schwarz) {
         // learned flexible PCG. Its result is used under the plain path's rules (reaching the
@@ -314,6 +350,7 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
         L->iterations += it;
         L->last_status = (int)lst;
         s.cg_total += it;
+        *iterations += it;
         if (lst == HOEFEM_OK || lst == HOEFEM_ERROR_CG_MAX_ITERATIONS) return HOEFEM_OK;
         L->fallbacks += 1;
         HIP_CHECK(hipMemsetAsync(sol, 0, sizeof(double) * 3 * m.nv, m.stream));
@@ -367,8 +404,20 @@ static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double*
         st = pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter, &it);
     }
     s.cg_total += it;
+    *iterations += it;
     if (st == HOEFEM_ERROR_CG_MAX_ITERATIONS || st == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) st = HOEFEM_OK;
     return st;
+}
+
+// NewtonCgDynamicSystem::solve_jacobian_combination (newton_cg.rs:680-802) inside the Newton loop:
+// state u_p + dt w, (alpha, beta) = (-dt^2, -dt) (BackwardEulerImplicitFunction::solve_jacobian_system,
+// euler.rs:122-140), then the CG.
+static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double* rhs, double* sol) {
+    Model& m = *s.model;
+    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
+    if (sim_assemble_jacobian(s, true, -dt * dt, true, -dt, s.d_uu.get()) != HOEFEM_OK) return HOEFEM_ERROR_SINGULAR;
+    int64_t it = 0;
+    return sim_cg(s, rhs, sol, &it);
 }
 
 // backward_euler_step (hamilton2/src/integrators/euler.rs:155-199) with newton_line_search and
@@ -526,6 +575,55 @@ struct hoefem_learned {
     Sim* sim = nullptr;
 };
 
+// The caller's vectors of one call. In HOEFEM_POINTERS_HOST mode they live in host memory and are
+// staged through model-owned device buffers: inputs uploaded on the model stream, outputs copied back
+// by finish(), which also waits for the call's work. In device mode the pointers pass through.
+struct CallIo {
+    Model& m;
+    bool host;
+    int slot = 0;
+    struct Out {
+        void* user;
+        void* dev;
+        size_t bytes;
+    };
+    std::vector<Out> outs;
+    explicit CallIo(Model& mm) : m(mm), host(mm.pointer_mode == HOEFEM_POINTERS_HOST) {}
+    void* stage(size_t bytes) {
+        if (slot >= (int)(sizeof(m.io_stage) / sizeof(m.io_stage[0])))
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "too many host vectors in one call");
+        DevBuf<char>& b = m.io_stage[slot++];
+        b.ensure(std::max<size_t>(bytes, 1));
+        return b.get();
+    }
+    template <class T>
+    const T* in(const T* p, size_t n) {
+        if (!host || !p) return p;
+        void* d = stage(n * sizeof(T));
+        if (n) HIP_CHECK(hipMemcpyAsync(d, p, n * sizeof(T), hipMemcpyHostToDevice, m.stream));
+        return (const T*)d;
+    }
+    template <class T>
+    T* inout(T* p, size_t n) {
+        if (!host || !p) return p;
+        T* d = (T*)in<T>(p, n);
+        outs.push_back({p, d, n * sizeof(T)});
+        return d;
+    }
+    template <class T>
+    T* out(T* p, size_t n) {
+        if (!host || !p) return p;
+        T* d = (T*)stage(n * sizeof(T));
+        outs.push_back({p, d, n * sizeof(T)});
+        return d;
+    }
+    void finish() {
+        for (const Out& o : outs)
+            if (o.bytes) HIP_CHECK(hipMemcpyAsync(o.user, o.dev, o.bytes, hipMemcpyDeviceToHost, m.stream));
+        if (host) HIP_CHECK(hipStreamSynchronize(m.stream));
+    }
+};
+
 template <class F>
 static hoefem_status guarded(F&& f) {
     try {
@@ -661,41 +759,80 @@ hoefem_status hoefem_model_colors(const hoefem_model* h, int32_t pass, int32_t* 
     });
 }
 
+hoefem_status hoefem_model_set_pointer_mode(hoefem_model* model, int32_t mode) {
+    return guarded([&] {
+        Model& m = M_(model);
+        if (mode != HOEFEM_POINTERS_DEVICE && mode != HOEFEM_POINTERS_HOST)
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unknown pointer mode");
+        m.pointer_mode = mode;
+    });
+}
+
 hoefem_status hoefem_model_forces_add(hoefem_model* model, double mu, double lambda, const double* u, double* f) {
     return guarded([&] {
         Model& m = M_(model);
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* du = io.in(u, n);
+        double* df = io.inout(f, n);
         double mr, lr;
         dev::snh_reparam(mu, lambda, &mr, &lr);
-        launch_element_forces(m, mr, lr, u);
-        launch_force_gather(m, f);
+        launch_element_forces(m, mr, lr, du);
+        launch_force_gather(m, df);
         raise_kernel_errors(m);
+        io.finish();
+    });
+}
+
+hoefem_status hoefem_model_stiffness_add_prec(hoefem_model* model, double mu, double lambda, const double* u,
+                                              void* values, int32_t precision) {
+    return guarded([&] {
+        Model& m = M_(model);
+        if (precision != HOEFEM_PRECISION_F64 && precision != HOEFEM_PRECISION_F32)
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unknown precision");
+        CallIo io(m);
+        const size_t nnz = (size_t)m.pattern.nnz();
+        const double* du = io.in(u, 3 * (size_t)m.nv);
+        double mr, lr;
+        dev::snh_reparam(mu, lambda, &mr, &lr);
+        if (precision == HOEFEM_PRECISION_F64) {
+            double* dv = io.inout((double*)values, nnz);
+            launch_element_stiffness(m, mr, lr, du, false);
+            launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), dv, 0, 0.0, nullptr, false, m.stream);
+        } else {
+            // FP32 element tangents (FP32 arithmetic), added to the FP32 CSR values in color order
+            float* dv = io.inout((float*)values, nnz);
+            launch_element_stiffness(m, mr, lr, du, true);
+            launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), dv, 0, 0.0, nullptr, false, m.stream);
+        }
+        raise_kernel_errors(m);
+        io.finish();
     });
 }
 
 hoefem_status hoefem_model_stiffness_add(hoefem_model* model, double mu, double lambda, const double* u,
                                          double* values) {
-    return guarded([&] {
-        Model& m = M_(model);
-        double mr, lr;
-        dev::snh_reparam(mu, lambda, &mr, &lr);
-        launch_element_stiffness(m, mr, lr, u, false);
-        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), values, 0, 0.0, nullptr, false, m.stream);
-        raise_kernel_errors(m);
-    });
+    return hoefem_model_stiffness_add_prec(model, mu, lambda, u, values, HOEFEM_PRECISION_F64);
 }
 
 hoefem_status hoefem_model_mass(hoefem_model* model, double density, double* values) {
     return guarded([&] {
         Model& m = M_(model);
+        CallIo io(m);
+        double* dv = io.out(values, (size_t)m.pattern.nnz());
         launch_element_mass(m, density);
-        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), values, 0, 0.0, nullptr, true, m.stream);
+        launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), dv, 0, 0.0, nullptr, true, m.stream);
         raise_kernel_errors(m);
+        io.finish();
     });
 }
 
 hoefem_status hoefem_model_dirichlet_csr(hoefem_model* model, int64_t num_nodes, const int32_t* nodes, double* values) {
     return guarded([&] {
         Model& m = M_(model);
+        if (num_nodes < 0 || (num_nodes > 0 && !nodes)) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad node list");
+        CallIo io(m);
+        values = io.inout(values, (size_t)m.pattern.nnz());
         std::vector<int32_t> dir(nodes, nodes + num_nodes);
         DevBuf<uint8_t> mask;
         DevBuf<int32_t> bc;
@@ -704,6 +841,7 @@ hoefem_status hoefem_model_dirichlet_csr(hoefem_model* model, int64_t num_nodes,
         int64_t nbc = 0;
         build_bc(m, dir, mask, bc, nbc);
         launch_dirichlet(m.pattern, values, mask.get(), bc.get(), nbc, scale.get(), m.stream);
+        io.finish();
         HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }
@@ -712,7 +850,13 @@ hoefem_status hoefem_model_spmv(hoefem_model* model, const double* values, doubl
                                 const double* x) {
     return guarded([&] {
         Model& m = M_(model);
-        launch_spmv(m.pattern, values, beta, y, alpha, x, m.stream);
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* dA = io.in(values, (size_t)m.pattern.nnz());
+        const double* dx = io.in(x, n);
+        double* dy = io.inout(y, n);
+        launch_spmv(m.pattern, dA, beta, dy, alpha, dx, m.stream);
+        io.finish();
     });
 }
 
@@ -746,24 +890,36 @@ hoefem_status hoefem_schwarz_pattern(const hoefem_schwarz* s, int32_t* rowptr, i
 hoefem_status hoefem_schwarz_update(hoefem_schwarz* s, const double* J) {
     return guarded([&] {
         if (!s) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null preconditioner");
-        schwarz_update(s->s, J);
-        raise_kernel_errors(*s->s.model);
+        Model& m = *s->s.model;
+        CallIo io(m);
+        schwarz_update(s->s, io.in(J, (size_t)m.pattern.nnz()));
+        raise_kernel_errors(m);
     });
 }
 
 hoefem_status hoefem_schwarz_values(hoefem_schwarz* s, double* values) {
     return guarded([&] {
         if (!s) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null preconditioner");
-        HIP_CHECK(hipMemcpyAsync(values, s->s.d_vals.get(), sizeof(double) * s->s.P.nnz(), hipMemcpyDeviceToDevice,
-                                 s->s.model->stream));
-        HIP_CHECK(hipStreamSynchronize(s->s.model->stream));
+        Model& m = *s->s.model;
+        CallIo io(m);
+        double* dv = io.out(values, (size_t)s->s.P.nnz());
+        HIP_CHECK(hipMemcpyAsync(dv, s->s.d_vals.get(), sizeof(double) * s->s.P.nnz(), hipMemcpyDeviceToDevice,
+                                 m.stream));
+        io.finish();
+        HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }
 
 hoefem_status hoefem_schwarz_apply(hoefem_schwarz* s, const double* x, double* y) {
     return guarded([&] {
         if (!s) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null preconditioner");
-        schwarz_apply(s->s, x, y);
+        Model& m = *s->s.model;
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* dx = io.in(x, n);
+        double* dy = io.out(y, n);
+        schwarz_apply(s->s, dx, dy);
+        io.finish();
     });
 }
 
@@ -775,7 +931,13 @@ hoefem_status hoefem_pcg(hoefem_model* model, const double* A, hoefem_schwarz* s
         Model& m = M_(model);
         static thread_local CgWork* work = nullptr;
         if (!work) work = new CgWork();
-        st = pcg_solve(m, *work, A, schwarz ? &schwarz->s : nullptr, b, x, tolerance, max_iterations, &it);
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* dA = io.in(A, (size_t)m.pattern.nnz());
+        const double* db = io.in(b, n);
+        double* dx = io.inout(x, n);
+        st = pcg_solve(m, *work, dA, schwarz ? &schwarz->s : nullptr, db, dx, tolerance, max_iterations, &it);
+        io.finish();
     });
     if (iterations) *iterations = it;
     if (g != HOEFEM_OK) return g;
@@ -913,7 +1075,10 @@ hoefem_status hoefem_sim_mass_values(hoefem_sim* sim, double* values) {
     return guarded([&] {
         if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
         Model& m = *sim->s.model;
-        HIP_CHECK(hipMemcpyAsync(values, sim->s.d_M.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice, m.stream));
+        CallIo io(m);
+        double* dv = io.out(values, (size_t)m.pattern.nnz());
+        HIP_CHECK(hipMemcpyAsync(dv, sim->s.d_M.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice, m.stream));
+        io.finish();
         HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }
@@ -923,10 +1088,13 @@ hoefem_status hoefem_sim_damping_values(hoefem_sim* sim, double* values) {
         if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
         Model& m = *sim->s.model;
         const int64_t nnz = m.pattern.nnz();
+        CallIo io(m);
+        double* dv = io.out(values, (size_t)nnz);
         if (sim->s.damping)
-            HIP_CHECK(hipMemcpyAsync(values, sim->s.d_D.get(), sizeof(double) * nnz, hipMemcpyDeviceToDevice, m.stream));
+            HIP_CHECK(hipMemcpyAsync(dv, sim->s.d_D.get(), sizeof(double) * nnz, hipMemcpyDeviceToDevice, m.stream));
         else
-            HIP_CHECK(hipMemsetAsync(values, 0, sizeof(double) * nnz, m.stream));
+            HIP_CHECK(hipMemsetAsync(dv, 0, sizeof(double) * nnz, m.stream));
+        io.finish();
         HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }
@@ -935,11 +1103,14 @@ hoefem_status hoefem_sim_jacobian_values(hoefem_sim* sim, double* values) {
     return guarded([&] {
         if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
         Model& m = *sim->s.model;
-        if (sim->s.mixed)
-            launch_f32_to_f64(m, sim->s.d_J32.get(), values, m.pattern.nnz());
+        CallIo io(m);
+        double* dv = io.out(values, (size_t)m.pattern.nnz());
+        if (sim->s.mixed_mode == 1)
+            launch_f32_to_f64(m, sim->s.d_J32.get(), dv, m.pattern.nnz());
         else
-            HIP_CHECK(hipMemcpyAsync(values, sim->s.d_J.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice,
+            HIP_CHECK(hipMemcpyAsync(dv, sim->s.d_J.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice,
                                      m.stream));
+        io.finish();
         HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }
@@ -950,10 +1121,17 @@ hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const do
         if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
         (void)t;
         Model& m = *sim->s.model;
-        launch_be_begin(m, u, v, dt, sim->s.d_up.get(), sim->s.d_vn.get());
-        const double n2 = sim_eval(sim->s, dt, w);
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* du = io.in(u, n);
+        const double* dv = io.in(v, n);
+        const double* dw = io.in(w, n);
+        double* dF = io.out(F, n);
+        launch_be_begin(m, du, dv, dt, sim->s.d_up.get(), sim->s.d_vn.get());
+        const double n2 = sim_eval(sim->s, dt, dw);
         if (std::isnan(n2)) throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient");
-        HIP_CHECK(hipMemcpyAsync(F, sim->s.d_F.get(), sizeof(double) * 3 * m.nv, hipMemcpyDeviceToDevice, m.stream));
+        HIP_CHECK(hipMemcpyAsync(dF, sim->s.d_F.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+        io.finish();
         HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }
@@ -961,7 +1139,175 @@ hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const do
 hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, double* v, hoefem_step_stats* stats) {
     return guarded([&] {
         if (!sim || !u || !v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
-        sim_step(sim->s, t, dt, u, v, stats);
+        Model& m = *sim->s.model;
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        double* du = io.inout(u, n);
+        double* dv = io.inout(v, n);
+        sim_step(sim->s, t, dt, du, dv, stats);
+        io.finish();
+    });
+}
+
+
+// ------------------------------------------------------------------ dynamic system (hamilton2)
+static Sim& S_(hoefem_sim* sim) {
+    if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+    return sim->s;
+}
+
+hoefem_status hoefem_sim_update_damping(hoefem_sim* sim, const double* u) {
+    return guarded([&] {
+        Sim& s = S_(sim);
+        Model& m = *s.model;
+        if (!u) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        if (!s.damping) return;
+        CallIo io(m);
+        launch_bc_copy(m, io.in(u, 3 * (size_t)m.nv), s.d_dirmask.get(), s.d_uu.get());
+        if (sim_damping(s, s.d_uu.get()) != HOEFEM_OK)
+            throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient in the damping stiffness");
+        s.solve_outdated = s.apply_outdated = true;
+        io.finish();
+    });
+}
+
+hoefem_status hoefem_sim_eval_f(hoefem_sim* sim, double t, const double* u, const double* v, double* f) {
+    return guarded([&] {
+        (void)t;
+        Sim& s = S_(sim);
+        Model& m = *s.model;
+        if (!u || !v || !f) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* du = io.in(u, n);
+        const double* dv = io.in(v, n);
+        double* df = io.out(f, n);
+        // the reference applies the boundary conditions to copies of u and v (newton_cg.rs:449-455)
+        launch_bc_copy(m, du, s.d_dirmask.get(), s.d_uu.get());
+        launch_bc_copy(m, dv, s.d_dirmask.get(), s.d_vv.get());
+        sim_eval_f(s, s.d_uu.get(), s.d_vv.get(), df);
+        raise_kernel_errors(m);
+        io.finish();
+    });
+}
+
+hoefem_status hoefem_sim_apply_mass_add(hoefem_sim* sim, double* y, const double* x) {
+    return guarded([&] {
+        Sim& s = S_(sim);
+        Model& m = *s.model;
+        if (!y || !x) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* dx = io.in(x, n);
+        double* dy = io.inout(y, n);
+        launch_mass_add(m, s.d_M.get(), dx, dy, s.cg, nullptr);
+        io.finish();
+    });
+}
+
+hoefem_status hoefem_sim_set_state(hoefem_sim* sim, double t, const double* u, const double* v) {
+    return guarded([&] {
+        (void)t;
+        Sim& s = S_(sim);
+        Model& m = *s.model;
+        if (!u || !v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        launch_bc_copy(m, io.in(u, n), s.d_dirmask.get(), s.d_su.get());
+        launch_bc_copy(m, io.in(v, n), s.d_dirmask.get(), s.d_sv.get());
+        s.state_set = true;
+        s.has_alpha = s.has_beta = false;
+        s.solve_outdated = s.apply_outdated = true;
+        io.finish();
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+    });
+}
+
+hoefem_status hoefem_sim_init_solve_jacobian_combination(hoefem_sim* sim, const double* alpha, const double* beta) {
+    return guarded([&] {
+        Sim& s = S_(sim);
+        s.has_alpha = alpha != nullptr;
+        s.has_beta = beta != nullptr;
+        s.alpha = alpha ? *alpha : 0.0;
+        s.beta = beta ? *beta : 0.0;
+        s.solve_outdated = true;
+    });
+}
+
+hoefem_status hoefem_sim_solve_jacobian_combination(hoefem_sim* sim, const double* rhs, double* sol,
+                                                     int64_t* cg_iterations) {
+    int64_t it = 0;
+    const hoefem_status g = guarded([&] {
+        Sim& s = S_(sim);
+        Model& m = *s.model;
+        if (!rhs || !sol) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        if (!s.state_set) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "set_state must precede solve_jacobian_combination");
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const double* drhs = io.in(rhs, n);
+        double* dsol = io.out(sol, n);
+        if (s.solve_outdated) {
+            if (sim_assemble_jacobian(s, s.has_alpha, s.alpha, s.has_beta, s.beta, s.d_su.get()) != HOEFEM_OK)
+                throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient or element Jacobian");
+            s.solve_outdated = false;
+        }
+        const hoefem_status st = sim_cg(s, drhs, dsol, &it);
+        io.finish();
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+        if (st != HOEFEM_OK) throw Error(st, "CG stopped with status " + std::to_string((int)st));
+    });
+    if (cg_iterations) *cg_iterations = it;
+    return g;
+}
+
+hoefem_status hoefem_sim_init_apply_jacobian_combination(hoefem_sim* sim, const double* alpha, const double* beta,
+                                                         const double* gamma) {
+    return guarded([&] {
+        Sim& s = S_(sim);
+        const double* c[3] = {alpha, beta, gamma};
+        for (int k = 0; k < 3; ++k) {
+            s.ap_has[k] = c[k] != nullptr;
+            s.ap_coef[k] = c[k] ? *c[k] : 0.0;
+        }
+        s.apply_outdated = true;
+        s.apply_ready = true;
+    });
+}
+
+hoefem_status hoefem_sim_apply_jacobian_combination(hoefem_sim* sim, double* y, const double* x) {
+    return guarded([&] {
+        Sim& s = S_(sim);
+        Model& m = *s.model;
+        if (!y || !x) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        if (!s.state_set || !s.apply_ready)
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "set_state and init_apply_jacobian_combination must come first");
+        CallIo io(m);
+        const size_t n = 3 * (size_t)m.nv;
+        const int64_t nnz = m.pattern.nnz();
+        const double* dx = io.in(x, n);
+        double* dy = io.inout(y, n);
+        if (s.apply_outdated) {
+            // H = -alpha K - beta D + gamma M (df/du = -K, df/dv = -D), with the Dirichlet treatment of
+            // compute_jacobian_combination_into so that it is the operator solve_jacobian_combination inverts
+            s.d_H.ensure((size_t)nnz);
+            const double* D = (s.ap_has[1] && s.damping) ? s.d_D.get() : nullptr;
+            const double cd = s.ap_has[1] ? -s.ap_coef[1] : 0.0;
+            const double cm = s.ap_has[2] ? s.ap_coef[2] : 0.0;
+            if (s.ap_has[0]) {
+                launch_element_stiffness(m, s.mu_r, s.lambda_r, s.d_su.get(), false);
+                launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_H.get(), 1, -s.ap_coef[0],
+                                    s.ap_has[2] ? s.d_M.get() : nullptr, true, m.stream, D, cd, cm);
+            } else {
+                launch_comb(m, nnz, s.d_H.get(), cd, D, cm, s.d_M.get());
+            }
+            launch_dirichlet(m.pattern, s.d_H.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
+                             m.stream);
+            raise_kernel_errors(m);
+            s.apply_outdated = false;
+        }
+        launch_spmv(m.pattern, s.d_H.get(), 1.0, dy, 1.0, dx, m.stream);
+        io.finish();
+        HIP_CHECK(hipStreamSynchronize(m.stream));
     });
 }
 

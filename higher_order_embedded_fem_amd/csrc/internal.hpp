@@ -29,6 +29,11 @@ struct Error : std::runtime_error {
 
 void check_launch(const char* what);
 
+// Raises func's dynamic LDS limit on the current device to at least `bytes`. The attribute is
+// shared by every thread of the process, so it is only ever raised (under a lock): a launch
+// sized by one thread cannot find the limit lowered by another.
+void raise_lds_limit(const void* func, size_t bytes);
+
 // ------------------------------------------------------------------ buffers
 template <class T>
 struct DevBuf {
@@ -161,6 +166,8 @@ struct Model {
     int material = 0;                 // dev::MAT_* (elastic model wrapping Stable Neo-Hookean)
     double inversion_threshold = 0.0; // InvertibleMaterial threshold
     hipStream_t stream = nullptr;
+    int pointer_mode = HOEFEM_POINTERS_DEVICE;  // where the caller's vectors of the hot-path calls live
+    DevBuf<char> io_stage[8];                   // device staging of host-pointer calls
     DevBuf<double> d_vertices;
     DevBuf<int32_t> d_conn;  // SoA: [k * E + e]
     DeviceQuadrature q_mass, q_stiff, q_ell, q_interior_mass;

@@ -263,7 +263,7 @@ void learned_build_basis(Learned& L) {
     const int ld = L.max_dofs + 1;
     const size_t lds = (size_t)LC_IN * ld * sizeof(double);
     if (lds > 150 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "aggregate too large for the basis kernel");
-    HIP_CHECK(hipFuncSetAttribute((const void*)k_lc_basis, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+    raise_lds_limit((const void*)k_lc_basis, 150 * 1024);
     k_lc_basis<<<dim3((unsigned)L.na), dim3(256), lds, m.stream>>>(L.d_agg_ptr.get(), L.d_agg_dofs.get(),
                                                                    L.d_dofmask.get(), L.d_err.get(), d_slots.get(),
                                                                    LC_IN, 3 * m.nv, L.d_basis.get(), ld);

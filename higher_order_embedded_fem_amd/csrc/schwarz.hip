@@ -721,8 +721,7 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         const int dp = (dim + 3) & ~3;
         const int ld_pc = dp + 1;
         const size_t lds_pc = (size_t)2 * dp * ld_pc * sizeof(double);  // 149 KB at dim 96
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_spd_power_certify<VJ>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pc));
+        raise_lds_limit((const void*)k_spd_power_certify<VJ>, lds_pc);
         hipLaunchKernelGGL(k_spd_power_certify<VJ>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds_pc, st,
                            s.scatter.d_m.get(), s.scatter.d_val_off.get(), s.scatter.d_slot_off.get(), s.d_off.get(),
                            s.d_nodes.get(), s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get(),
@@ -731,9 +730,8 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         const int nn = dim + (dim & 1);
         const int ld = nn + 1;
         const size_t lds = (size_t)2 * nn * ld * sizeof(double) + (size_t)nn * sizeof(double) + (size_t)nn * sizeof(int);
-        // the exact size, set on the current device before each launch (147 KB for 96 x 96 blocks)
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
+        // at least the size of this launch (147 KB for 96 x 96 blocks), raised monotonically
+        raise_lds_limit((const void*)k_jacobi_inverse, lds);
         hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st, s.scatter.d_m.get(),
                            s.scatter.d_val_off.get(), s.d_blocks.get(), (const int*)s.d_fallback.get(), ld, 0);
         check_launch("k_jacobi_inverse");
@@ -773,8 +771,7 @@ void launch_element_psd_projection(Model& m) {
     m.d_elem_need.ensure((size_t)m.ne);
     const int ld_c = dim + 1;
     const size_t lds_c = (size_t)dim * ld_c * sizeof(double);
-    HIP_CHECK(hipFuncSetAttribute((const void*)k_psd_certify, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  120 * 1024));
+    raise_lds_limit((const void*)k_psd_certify, 120 * 1024);
     hipLaunchKernelGGL(k_psd_certify, dim3((unsigned)m.ne), dim3(EIG_THREADS), lds_c, m.stream, m.d_elem_m.get(),
                        m.d_elem_off.get(), (const double*)m.d_ke.get(), m.d_elem_need.get(), ld_c);
     check_launch("k_psd_certify");
@@ -783,8 +780,7 @@ void launch_element_psd_projection(Model& m) {
     const size_t lds_ns = (size_t)2 * dp * ld_ns * sizeof(double);
     if (lds_ns > 120 * 1024 || (dp / 4) * (dp / 4) > 2 * EIG_THREADS)
         throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "element matrix too large for the projection");
-    HIP_CHECK(hipFuncSetAttribute((const void*)k_psd_newton_schulz, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  120 * 1024));
+    raise_lds_limit((const void*)k_psd_newton_schulz, 120 * 1024);
     hipLaunchKernelGGL(k_psd_newton_schulz, dim3((unsigned)m.ne), dim3(EIG_THREADS), lds_ns, m.stream,
                        m.d_elem_m.get(), m.d_elem_off.get(), m.d_ke.get(), (const int*)m.d_elem_need.get(), ld_ns);
     check_launch("k_psd_newton_schulz");

@@ -1,6 +1,8 @@
 // Host-side setup of the hoefem library: node-level CSR patterns, first-fit coloring and
 // the color-ordered block incidence used by the atomic-free gather kernels.
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <numeric>
 
 #include "internal.hpp"
@@ -10,6 +12,18 @@ namespace hoefem {
 void check_launch(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error(HOEFEM_ERROR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void raise_lds_limit(const void* func, size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> limit;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& cur = limit[{dev, func}];
+    if (bytes <= cur) return;
+    HIP_CHECK(hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    cur = bytes;
 }
 
 int nodes_per_element(int t) {

@@ -90,13 +90,14 @@ __device__ inline void partials_reduce(const double (&acc)[NV], double (&tot)[NV
     }
 }
 
-// partials[blockIdx.x * NV + k] = sum of v[k] over the block's threads (any block size up to
-// CG_THREADS, a multiple of 64).
-template <int NV>
+// partials[blockIdx.x * NV + k] = sum of v[k] over the block's threads. BT is the block size
+// the kernel is launched with (its launch bound, a multiple of 64): one LDS slot per wave.
+template <int NV, int BT = CG_THREADS>
 __device__ inline void block_partials(const double (&v)[NV], double* __restrict__ partials) {
-    __shared__ double s_w[CG_THREADS / 64][NV];
+    static_assert(BT % 64 == 0 && BT <= 1024, "block size must be a multiple of 64, at most 1024");
+    __shared__ double s_w[BT / 64][NV];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    const int nw = (int)(blockDim.x >> 6);
+    constexpr int nw = BT / 64;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         const double s = wave_sum(v[k]);
@@ -570,7 +571,7 @@ __global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(Csr3<VT> A, CgPtrs P) {
             pvec(v, p0, p1, p2);
             finish(v, p0, p1, p2, double(d[0] * VT(p0)), double(d[1] * VT(p1)), double(d[2] * VT(p2)));
         });
-    block_partials<1>(part, P.part_a);
+    block_partials<1, CG_THREADS_A>(part, P.part_a);
 }
 
 // B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p_k ;
@@ -1040,72 +1041,96 @@ hoefem_status pcg_solve_learned(Model& m, CgWork& w, const float* A, Schwarz* P,
 }
 
 // ================================================================== Newton / BE kernels
+// The vector updates of the Backward-Euler Newton loop round every product and sum on its own
+// (no fused multiply-add), as the reference's nalgebra arithmetic does on the host: a caller that
+// composes the same step on the host from the dynamic-system entry points (eval_f, apply_mass,
+// solve_jacobian_combination) then gets bit-identical iterates.
+
 // uu = u_p + w * dt with Dirichlet DOFs zeroed (BackwardEulerImplicitFunction::eval_into,
 // hamilton2/src/integrators/euler.rs:97-114; apply_dirichlet_bcs_unknowns,
 // simulation_toolbox/src/fem/system_assembly.rs:78-104)
 __global__ void k_state(int64_t ndof, const double* __restrict__ up, const double* __restrict__ w, double dt,
                         const uint8_t* __restrict__ dirmask, double* __restrict__ uu) {
+#pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndof) return;
     const double val = up[i] + w[i] * dt;
     uu[i] = dirmask[i / 3] ? 0.0 : val;
 }
 
-// F = -dt * [ (M g + f_int(uu)) with Dirichlet rows zeroed ] + M w  and |F|^2
-// (NewtonCgDynamicSystem::eval_f, newton_cg.rs:441-478, then euler.rs:107-113)
+// dst = src with Dirichlet DOFs zeroed (the copies NewtonCgDynamicSystem::eval_f and set_state
+// make, newton_cg.rs:449-455, :662-671)
+__global__ void k_bc_copy(int64_t ndof, const double* __restrict__ src, const uint8_t* __restrict__ dirmask,
+                          double* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndof) return;
+    const double val = src[i];
+    dst[i] = dirmask[i / 3] ? 0.0 : val;
+}
+
+// f = M g + f_int (element forces in color order) - D v, Dirichlet rows zeroed
+// (NewtonCgDynamicSystem::eval_f, newton_cg.rs:441-478); one thread per node
+__global__ void k_node_force(int64_t nn, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk,
+                             const int32_t* __restrict__ inc_loc, int n, const double* __restrict__ fe,
+                             const double* __restrict__ gforce, const double* __restrict__ dv,
+                             const uint8_t* __restrict__ dirmask, double* __restrict__ f) {
+#pragma clang fp contract(off)
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    double f0 = gforce[3 * v], f1 = gforce[3 * v + 1], f2 = gforce[3 * v + 2];
+    for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
+        const double* src = fe + ((int64_t)inc_blk[it] * n + inc_loc[it]) * 3;
+        f0 += src[0];
+        f1 += src[1];
+        f2 += src[2];
+    }
+    if (dv) {  // f += -D v (spmv_csr with beta = 1, alpha = -1)
+        f0 = f0 - dv[3 * v];
+        f1 = f1 - dv[3 * v + 1];
+        f2 = f2 - dv[3 * v + 2];
+    }
+    if (dirmask[v]) f0 = f1 = f2 = 0.0;
+    f[3 * v] = f0;
+    f[3 * v + 1] = f1;
+    f[3 * v + 2] = f2;
+}
+
+// dst = src * sc  (F <- -dt f, euler.rs:110)
+__global__ void k_scale(int64_t n, const double* __restrict__ src, double sc, double* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] * sc;
+}
+
+// y <- y + M x (apply_mass_matrix, newton_cg.rs:428-431: spmv_csr(1, y, 1, M, x)); with
+// partials, also the per-block partial sums of |y|^2 (the Newton residual norm)
 template <int G>
-__global__ __launch_bounds__(CG_THREADS) void k_residual(Csr3<double> M, const int32_t* __restrict__ inc_ptr,
-                                                         const int32_t* __restrict__ inc_blk,
-                                                         const int32_t* __restrict__ inc_loc, int n,
-                                                         const double* __restrict__ fe,
-                                                         const double* __restrict__ gforce,
-                                                         const uint8_t* __restrict__ dirmask, double dt,
-                                                         const double* __restrict__ w,
-                                                         const double* __restrict__ dv, double* __restrict__ F,
-                                                         double* partials) {
+__global__ __launch_bounds__(CG_THREADS) void k_mass_add(Csr3<double> M, const double* __restrict__ x,
+                                                         double* __restrict__ y, double* __restrict__ partials) {
+#pragma clang fp contract(off)
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t v = gid / G;
     const int lane = (int)(gid % G);
     const bool valid = v < M.nn;
     double m0, m1, m2;
     row_block_dot<G>(M, v, lane, valid, [&](int64_t b, double& x0, double& x1, double& x2) {
-        x0 = w[3 * b];
-        x1 = w[3 * b + 1];
-        x2 = w[3 * b + 2];
+        x0 = x[3 * b];
+        x1 = x[3 * b + 1];
+        x2 = x[3 * b + 2];
     }, m0, m1, m2);
     double part[1] = {0.0};
     if (valid && lane == 0) {
-        double f0 = 1.0 * gforce[3 * v], f1 = 1.0 * gforce[3 * v + 1], f2 = 1.0 * gforce[3 * v + 2];
-        for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
-            const double* src = fe + ((int64_t)inc_blk[it] * n + inc_loc[it]) * 3;
-            f0 += src[0];
-            f1 += src[1];
-            f2 += src[2];
-        }
-        if (dv) {  // f += -D v (spmv_csr with beta = 1, alpha = -1)
-            f0 = 1.0 * f0 + -1.0 * dv[3 * v];
-            f1 = 1.0 * f1 + -1.0 * dv[3 * v + 1];
-            f2 = 1.0 * f2 + -1.0 * dv[3 * v + 2];
-        }
-        if (dirmask[v]) f0 = f1 = f2 = 0.0;
-        const double sc = dt * -1.0;
-        double F0 = f0 * sc, F1 = f1 * sc, F2 = f2 * sc;
-        F0 = 1.0 * F0;
-        F1 = 1.0 * F1;
-        F2 = 1.0 * F2;
-        F0 += 1.0 * m0;
-        F1 += 1.0 * m1;
-        F2 += 1.0 * m2;
-        F[3 * v] = F0;
-        F[3 * v + 1] = F1;
-        F[3 * v + 2] = F2;
-        part[0] = F0 * F0 + F1 * F1 + F2 * F2;
+        const double y0 = y[3 * v] + m0, y1 = y[3 * v + 1] + m1, y2 = y[3 * v + 2] + m2;
+        y[3 * v] = y0;
+        y[3 * v + 1] = y1;
+        y[3 * v + 2] = y2;
+        part[0] = y0 * y0 + y1 * y1 + y2 * y2;
     }
-    block_partials<1>(part, partials);
+    if (partials) block_partials<1>(part, partials);
 }
 
 // w <- w + delta * (-sol)  (line-search step along p = -sol, hamilton2/src/newton.rs:113-116, :219)
 __global__ void k_axpy_neg(int64_t n, double* __restrict__ w, double delta, const double* __restrict__ sol) {
+#pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     w[i] = w[i] + delta * (-sol[i]);
@@ -1114,6 +1139,7 @@ __global__ void k_axpy_neg(int64_t n, double* __restrict__ w, double delta, cons
 // v_n = v ; u_p = u + v * dt
 __global__ void k_be_begin(int64_t n, const double* __restrict__ u, const double* __restrict__ v, double dt,
                            double* __restrict__ up, double* __restrict__ vn) {
+#pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     vn[i] = v[i];
@@ -1123,6 +1149,7 @@ __global__ void k_be_begin(int64_t n, const double* __restrict__ u, const double
 // v += w ; u += v * dt ; then Dirichlet unknowns zeroed (euler.rs:195-196, newton_cg.rs:293)
 __global__ void k_be_end(int64_t n, double* __restrict__ u, double* __restrict__ v, const double* __restrict__ w,
                          double dt, const uint8_t* __restrict__ dirmask) {
+#pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double vv = v[i] + w[i];
@@ -1202,6 +1229,29 @@ void launch_f64_to_f32(Model& m, const double* a, float* b, int64_t n) {
     check_launch("k_f64_to_f32");
 }
 
+// out = (0 + cd D) + cm M, the damping and mass terms of compute_jacobian_combination_into
+// (simulation_toolbox/src/fem/system_assembly.rs:197-235) when the stiffness term is absent;
+// D may be null
+template <class T>
+__global__ void k_comb(int64_t n, T* __restrict__ out, double cd, const double* __restrict__ D, double cm,
+                       const double* __restrict__ M) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = 0.0;
+    if (D) v += cd * D[i];
+    v += cm * M[i];
+    out[i] = (T)v;
+}
+
+template <class T>
+void launch_comb(Model& m, int64_t n, T* out, double cd, const double* D, double cm, const double* M) {
+    if (n <= 0) return;
+    k_comb<T><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream>>>(n, out, cd, D, cm, M);
+    check_launch("k_comb");
+}
+template void launch_comb<double>(Model&, int64_t, double*, double, const double*, double, const double*);
+template void launch_comb<float>(Model&, int64_t, float*, double, const double*, double, const double*);
+
 void launch_fill(Model& m, double* a, int64_t n, double value) {
     if (n <= 0) return;
     k_fill<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream>>>(n, a, value);
@@ -1217,6 +1267,7 @@ void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n) {
 // v = v_n + w with the Dirichlet velocities applied (zero): the velocity eval_f sees
 __global__ void k_state_v(int64_t ndof, const double* __restrict__ vn, const double* __restrict__ w,
                           const uint8_t* __restrict__ dirmask, double* __restrict__ vv) {
+#pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndof) return;
     const double val = vn[i] + w[i];
@@ -1235,26 +1286,55 @@ void launch_state(Model& m, const double* up, const double* w, double dt, const 
     check_launch("k_state");
 }
 
-void launch_residual(Model& m, const double* Mvals, const double* gforce, const uint8_t* dirmask, double dt,
-                     const double* w, const double* dv, double* F, CgWork& cw, double* norm2) {
+void launch_bc_copy(Model& m, const double* src, const uint8_t* dirmask, double* dst) {
+    const int64_t n = 3 * m.nv;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bc_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, src, dirmask, dst);
+    check_launch("k_bc_copy");
+}
+
+void launch_node_force(Model& m, const double* gforce, const double* dv, const uint8_t* dirmask, double* f) {
+    const int64_t nn = m.nv;
+    if (nn <= 0) return;
+    hipLaunchKernelGGL(k_node_force, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, m.stream, nn,
+                       m.scatter.d_inc_ptr.get(), m.scatter.d_inc_blk.get(), m.scatter.d_inc_loc.get(), m.n,
+                       m.d_fe.get(), gforce, dv, dirmask, f);
+    check_launch("k_node_force");
+}
+
+void launch_scale(Model& m, const double* src, double sc, double* dst) {
+    const int64_t n = 3 * m.nv;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scale, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, src, sc, dst);
+    check_launch("k_scale");
+}
+
+// y += M x on the model pattern; with norm2 (device), also |y|^2 (deterministic partial sums)
+void launch_mass_add(Model& m, const double* Mvals, const double* x, double* y, CgWork& cw, double* norm2) {
     const NodePattern& pat = m.pattern;
+    if (pat.num_nodes == 0) return;
     const int G = spmv_group_size(pat);
     const Csr3<double> Mv = view(pat, Mvals);
     const unsigned blocks = (unsigned)((pat.num_nodes * G + CG_THREADS - 1) / CG_THREADS);
-    cw.ensure(pat.ndof(), (int)blocks + 1);
-#define RES_CASE(GG)                                                                                              \
-    hipLaunchKernelGGL(k_residual<GG>, dim3(blocks), dim3(CG_THREADS), 0, m.stream, Mv, m.scatter.d_inc_ptr.get(), \
-                       m.scatter.d_inc_blk.get(), m.scatter.d_inc_loc.get(), m.n, m.d_fe.get(), gforce, dirmask, dt,  \
-                       w, dv, F, cw.partials.get())
+    double* part = nullptr;
+    if (norm2) {
+        cw.ensure(pat.ndof(), (int)blocks + 1);
+        part = cw.partials.get();
+    }
+#define MASS_CASE(GG) \
+    hipLaunchKernelGGL(k_mass_add<GG>, dim3(blocks), dim3(CG_THREADS), 0, m.stream, Mv, x, y, part)
     if (G == 32)
-        RES_CASE(32);
+        MASS_CASE(32);
     else if (G == 16)
-        RES_CASE(16);
+        MASS_CASE(16);
     else
-        RES_CASE(8);
-#undef RES_CASE
-    k_sum_partials<<<dim3(1), dim3(CG_THREADS), 0, m.stream>>>(cw.partials.get(), (int)blocks, norm2);
-    check_launch("k_residual");
+        MASS_CASE(8);
+#undef MASS_CASE
+    check_launch("k_mass_add");
+    if (norm2) {
+        k_sum_partials<<<dim3(1), dim3(CG_THREADS), 0, m.stream>>>(part, (int)blocks, norm2);
+        check_launch("k_sum_partials");
+    }
 }
 
 void launch_axpy_neg(Model& m, double* w, double delta, const double* sol) {

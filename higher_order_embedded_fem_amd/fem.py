@@ -21,6 +21,20 @@ from ._lib import check
 from .scenes import Scene, lame_from_young_poisson
 
 
+def _vec(x, dtype=np.float64) -> int:
+    """Pointer of a caller vector: a device tensor (device pointer mode) or a contiguous numpy array
+    (host pointer mode, hoefem_model_set_pointer_mode)."""
+    if isinstance(x, np.ndarray):
+        if x.dtype != dtype or not x.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"expected a contiguous {np.dtype(dtype)} array")
+        return x.ctypes.data
+    return _dev(x, torch.float64 if dtype == np.float64 else torch.float32)
+
+
+def _opt(x):
+    return None if x is None else C.byref(C.c_double(float(x)))
+
+
 def _dev(t: torch.Tensor, dtype=torch.float64) -> int:
     if not t.is_cuda:
         raise ValueError("expected a device (cuda/hip) tensor")
@@ -156,6 +170,11 @@ class ElasticityModel:
     def zeros_matrix(self):
         return torch.zeros(self.nnz, dtype=torch.float64, device=self.device)
 
+    def set_pointer_mode(self, host: bool):
+        """Hot-path calls on this model (and on the simulations built on it) take host numpy arrays
+        (True) or device tensors (False, the default)."""
+        check(_lib.load().hoefem_model_set_pointer_mode(self.h, 1 if host else 0))
+
     def set_material(self, material: StableNeoHookeanMaterial):
         """The elastic model used by this model's assembly and by simulations built on it."""
         check(_lib.load().hoefem_model_set_material(self.h, material.kind, material.threshold))
@@ -167,9 +186,12 @@ class ElasticityModel:
         return f
 
     def assemble_stiffness_into(self, values: torch.Tensor, u: torch.Tensor, material: StableNeoHookeanMaterial):
-        """values += K(u) on the model pattern (assemble_transformed_stiffness_into_par)."""
+        """values += K(u) on the model pattern (assemble_transformed_stiffness_into_par); FP64 values,
+        or FP32 values (float32 tensor) from FP32 element tangents."""
         self.set_material(material)
-        check(_lib.load().hoefem_model_stiffness_add(self.h, material.mu, material.lam, _dev(u), _dev(values)))
+        prec = 1 if values.dtype == torch.float32 else 0
+        check(_lib.load().hoefem_model_stiffness_add_prec(self.h, material.mu, material.lam, _dev(u),
+                                                          _dev(values, values.dtype), prec))
         return values
 
     def assemble_mass(self, density: float) -> torch.Tensor:
@@ -324,10 +346,48 @@ class NewtonCgIntegrator:
         check(_lib.load().hoefem_sim_residual(self.h, t, dt, _dev(u), _dev(v), _dev(w), _dev(F)))
         return F
 
-    def step(self, t: float, dt: float, u: torch.Tensor, v: torch.Tensor) -> Dict:
+    def step(self, t: float, dt: float, u, v) -> Dict:
         st = _lib.StepStats()
-        check(_lib.load().hoefem_sim_step(self.h, t, dt, _dev(u), _dev(v), C.byref(st)))
+        check(_lib.load().hoefem_sim_step(self.h, t, dt, _vec(u), _vec(v), C.byref(st)))
         return {k: getattr(st, k) for k, _ in _lib.StepStats._fields_}
+
+    # ---- DynamicSystem / DifferentiableDynamicSystem (hamilton2/src/dynamic_system/mutable.rs:14-76) of
+    # NewtonCgDynamicSystem (simulation_toolbox/src/fem/newton_cg.rs:413-820). Vectors: device tensors,
+    # or numpy arrays when the model is in host pointer mode.
+    def update_damping(self, u):
+        """compute_damping_matrix_into at the step's initial u (integrate_model, newton_cg.rs:256-265)."""
+        check(_lib.load().hoefem_sim_update_damping(self.h, _vec(u)))
+
+    def eval_f(self, f, t: float, u, v):
+        """f = M g + f_int(u) - D v with Dirichlet rows zeroed (overwrites f)."""
+        check(_lib.load().hoefem_sim_eval_f(self.h, t, _vec(u), _vec(v), _vec(f)))
+        return f
+
+    def apply_mass_matrix(self, y, x):
+        """y += M x."""
+        check(_lib.load().hoefem_sim_apply_mass_add(self.h, _vec(y), _vec(x)))
+        return y
+
+    def set_state(self, t: float, u, v):
+        check(_lib.load().hoefem_sim_set_state(self.h, t, _vec(u), _vec(v)))
+
+    def init_solve_jacobian_combination(self, alpha: Optional[float], beta: Optional[float]):
+        check(_lib.load().hoefem_sim_init_solve_jacobian_combination(self.h, _opt(alpha), _opt(beta)))
+
+    def solve_jacobian_combination(self, sol, rhs) -> int:
+        """Solves (M + alpha df/du + beta df/dv) sol = rhs; returns the CG iteration count."""
+        it = C.c_int64()
+        check(_lib.load().hoefem_sim_solve_jacobian_combination(self.h, _vec(rhs), _vec(sol), C.byref(it)))
+        return it.value
+
+    def init_apply_jacobian_combination(self, alpha: Optional[float], beta: Optional[float],
+                                        gamma: Optional[float]):
+        check(_lib.load().hoefem_sim_init_apply_jacobian_combination(self.h, _opt(alpha), _opt(beta), _opt(gamma)))
+
+    def apply_jacobian_combination(self, y, x):
+        """y += (alpha df/du + beta df/dv + gamma M) x at the state of set_state."""
+        check(_lib.load().hoefem_sim_apply_jacobian_combination(self.h, _vec(y), _vec(x)))
+        return y
 
 
 class LearnedCoarseCorrection:

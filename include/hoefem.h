@@ -3,8 +3,10 @@
  * higher-order embedded FEM reference (InteractiveComputerGraphics/higher_order_embedded_fem).
  *
  * C ABI. Setup calls take HOST arrays (mesh, quadrature, boundary conditions); hot-path
- * calls take DEVICE pointers and are ordered on the handle's HIP stream (the null stream
- * unless hoefem_*_set_stream is called). Handles are not re-entrant; different handles
+ * calls take DEVICE pointers by default, or HOST pointers after
+ * hoefem_model_set_pointer_mode(model, HOEFEM_POINTERS_HOST) (every call that takes the model or a
+ * handle built on it), and are ordered on the handle's HIP stream (the null stream unless
+ * hoefem_*_set_stream is called). Handles are not re-entrant; different handles
  * may be used from different threads. Every function returns a hoefem_status; the
  * message of the last failure on the calling thread is hoefem_last_error().
  *
@@ -75,6 +77,20 @@ typedef struct hoefem_model_desc {
     double mass_regularization_factor;     /* EmbeddedModel::mass_regularization_factor, 0 = off */
 } hoefem_model_desc;
 
+/* Where the vectors and CSR values passed to a model's hot-path calls live (the pointer flag of the
+ * reference-side binding: a Rust caller hands over host DVector / CsrMatrix storage). */
+typedef enum hoefem_pointer_mode {
+    HOEFEM_POINTERS_DEVICE = 0,            /* device pointers, stream-ordered (the default) */
+    HOEFEM_POINTERS_HOST = 1               /* host pointers: staged through device buffers; calls return
+                                              with their outputs written */
+} hoefem_pointer_mode;
+
+/* Value type of a CSR matrix written by hoefem_model_stiffness_add_prec. */
+typedef enum hoefem_precision {
+    HOEFEM_PRECISION_F64 = 0,
+    HOEFEM_PRECISION_F32 = 1               /* FP32 element tangents added into FP32 values */
+} hoefem_precision;
+
 typedef struct hoefem_model hoefem_model;
 typedef struct hoefem_schwarz hoefem_schwarz;
 typedef struct hoefem_sim hoefem_sim;
@@ -133,6 +149,10 @@ hoefem_status hoefem_mesh_reorder(int32_t nodes_per_element, int64_t num_vertice
 hoefem_status hoefem_model_create(const hoefem_model_desc* desc, hoefem_model** out);
 void hoefem_model_destroy(hoefem_model* model);
 hoefem_status hoefem_model_set_stream(hoefem_model* model, void* hip_stream);
+/* hoefem_pointer_mode of every later call on this model and on the Schwarz operators and simulations
+ * built on it (default HOEFEM_POINTERS_DEVICE). hoefem_material_eval(_kind) and hoefem_rotation_svd
+ * have no model and always take device pointers. */
+hoefem_status hoefem_model_set_pointer_mode(hoefem_model* model, int32_t mode);
 /* The elastic model used by this model's forces and stiffness (and by every simulation built
  * on it): the ElasticMaterialModel of the reference's Material (fe_model.rs:35-45, :131-158).
  * kind is a hoefem_material_kind; inversion_threshold applies to the invertible kind only. */
@@ -150,6 +170,11 @@ hoefem_status hoefem_model_forces_add(hoefem_model* model, double mu, double lam
 /* assemble_transformed_stiffness_into_par with NoTransformation: values += K(u) (device, nnz). */
 hoefem_status hoefem_model_stiffness_add(hoefem_model* model, double mu, double lambda, const double* u,
                                          double* values);
+/* The same with the value type chosen by precision (hoefem_precision): values is double[nnz] for
+ * HOEFEM_PRECISION_F64 (identical to hoefem_model_stiffness_add) or float[nnz] for
+ * HOEFEM_PRECISION_F32 (element tangents evaluated in FP32, the mixed-precision operator). */
+hoefem_status hoefem_model_stiffness_add_prec(hoefem_model* model, double mu, double lambda, const double* u,
+                                              void* values, int32_t precision);
 /* ElasticityModel::assemble_mass + to_csr: values = M (device, nnz). */
 hoefem_status hoefem_model_mass(hoefem_model* model, double density, double* values);
 /* apply_homogeneous_dirichlet_bc_csr (fenris/src/assembly.rs:799-869) on a matrix with the
@@ -269,6 +294,45 @@ hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const do
                                   const double* w, double* F);
 /* one time step: u, v device vectors updated in place */
 hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, double* v, hoefem_step_stats* stats);
+
+/* ------------------------------------------------------------------ dynamic system
+ * The DynamicSystem / DifferentiableDynamicSystem surface of NewtonCgDynamicSystem
+ * (hamilton2/src/dynamic_system/mutable.rs:14-76, simulation_toolbox/src/fem/newton_cg.rs:413-820), so
+ * that hamilton2's own backward_euler_step (hamilton2/src/integrators/euler.rs:155-199) and
+ * newton_line_search can drive the simulation: hoefem_sim_step is that composition run inside the
+ * library. The vector arithmetic of hoefem_sim_step rounds every operation separately (no fused
+ * multiply-add), so the same step composed by the caller from these calls, with the caller's host
+ * IEEE arithmetic for the vector updates, reproduces hoefem_sim_step's iterates bit for bit. Option<T>
+ * arguments are pointers, NULL for None. */
+
+/* compute_damping_matrix_into (simulation_toolbox/src/fem/system_assembly.rs:145-194) at the step's
+ * initial displacement u, as integrate_model does before each backward_euler_step (newton_cg.rs:
+ * 256-265); nothing when the simulation has no damping. */
+hoefem_status hoefem_sim_update_damping(hoefem_sim* sim, const double* u);
+/* DynamicSystem::eval_f (newton_cg.rs:441-478): f = M g + f_int(u) - D v, Dirichlet rows zeroed, with
+ * the Dirichlet values applied to copies of u and v. Overwrites f. */
+hoefem_status hoefem_sim_eval_f(hoefem_sim* sim, double t, const double* u, const double* v, double* f);
+/* DynamicSystem::apply_mass_matrix (newton_cg.rs:428-431): y += M x (M with Dirichlet conditions). */
+hoefem_status hoefem_sim_apply_mass_add(hoefem_sim* sim, double* y, const double* x);
+/* DifferentiableDynamicSystem::set_state (newton_cg.rs:661-671): stores (u, v) with the Dirichlet values
+ * applied; clears the coefficients of init_solve_jacobian_combination. */
+hoefem_status hoefem_sim_set_state(hoefem_sim* sim, double t, const double* u, const double* v);
+/* init_solve_jacobian_combination (newton_cg.rs:673-679): H = M + alpha df/du + beta df/dv
+ * = M - alpha K - beta D at the stored state. */
+hoefem_status hoefem_sim_init_solve_jacobian_combination(hoefem_sim* sim, const double* alpha, const double* beta);
+/* solve_jacobian_combination (newton_cg.rs:680-802): assembles H with the Dirichlet conditions and
+ * updates the preconditioner when the state or the coefficients changed, then PCG from a zero guess
+ * (the simulation's CG settings; MaxIterationsReached and IndefiniteOperator are tolerated, any other
+ * CG status is returned). cg_iterations (optional) receives the iteration count. */
+hoefem_status hoefem_sim_solve_jacobian_combination(hoefem_sim* sim, const double* rhs, double* sol,
+                                                     int64_t* cg_iterations);
+/* init_apply_jacobian_combination / apply_jacobian_combination (mutable.rs:44-58; unimplemented in
+ * NewtonCgDynamicSystem, built here): y += (alpha df/du + beta df/dv + gamma M) x at the stored state,
+ * with the Dirichlet treatment of compute_jacobian_combination_into (so that with gamma = 1 it is the
+ * operator solve_jacobian_combination inverts). */
+hoefem_status hoefem_sim_init_apply_jacobian_combination(hoefem_sim* sim, const double* alpha, const double* beta,
+                                                         const double* gamma);
+hoefem_status hoefem_sim_apply_jacobian_combination(hoefem_sim* sim, double* y, const double* x);
 
 /* ------------------------------------------------------------------ learned coarse correction
  * No counterpart in the reference: an extra correction on top of its Schwarz preconditioner

@@ -162,6 +162,9 @@ def lib():
                                   C.c_double, C.c_int64, C.POINTER(C.c_int64)]
         L.orc_backward_euler_step.argtypes = [C.c_int64, f64p, f64p, C.c_double, C.c_double, C.c_void_p, DYN_EVAL_FN,
                                               DYN_MASS_FN, DYN_SOLVE_FN, C.c_double, C.c_int64, C.POINTER(C.c_int64)]
+        L.orc_integrate_with_substeps.argtypes = [C.c_int64, f64p, f64p, C.c_double, C.c_double, C.c_void_p,
+                                                  DYN_EVAL_FN, DYN_MASS_FN, DYN_SOLVE_FN, C.c_double, C.c_int64,
+                                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.orc_newton.argtypes = [C.c_int64, f64p, C.c_void_p, EVAL_FN, SOLVE_FN, C.c_double, C.c_int64, C.c_int,
                                  C.POINTER(C.c_int64)]
         L.orc_sim_create.argtypes = [C.c_void_p, C.POINTER(OrcSimDesc), C.POINTER(C.c_void_p)]
@@ -511,6 +514,41 @@ def backward_euler_step(u, v, t0, dt, eval_f, apply_mass, solve_jac, tol, max_ne
     it = C.c_int64()
     st = lib().orc_backward_euler_step(n, u, v, t0, dt, None, e, m, s, tol, max_newton_iter, C.byref(it))
     return it.value, st
+
+
+def _dyn_callbacks(n, eval_f, apply_mass, solve_jac):
+    """ctypes callbacks of a Python dynamic system; a solve_jac that raises reports a Jacobian error."""
+
+    def _ev(ctx, t, up, vp, fp):
+        fv = np.ctypeslib.as_array(fp, shape=(n,))
+        fv[:] = eval_f(t, np.ctypeslib.as_array(up, shape=(n,)).copy(), np.ctypeslib.as_array(vp, shape=(n,)).copy())
+        return 0
+
+    def _ms(ctx, yp, xp):
+        yv = np.ctypeslib.as_array(yp, shape=(n,))
+        yv += apply_mass(np.ctypeslib.as_array(xp, shape=(n,)).copy())
+
+    def _so(ctx, t, up, vp, a, b, rp, sp):
+        sv = np.ctypeslib.as_array(sp, shape=(n,))
+        try:
+            sv[:] = solve_jac(t, np.ctypeslib.as_array(up, shape=(n,)).copy(),
+                              np.ctypeslib.as_array(vp, shape=(n,)).copy(), a, b,
+                              np.ctypeslib.as_array(rp, shape=(n,)).copy())
+        except Exception:
+            return 12
+        return 0
+
+    return DYN_EVAL_FN(_ev), DYN_MASS_FN(_ms), DYN_SOLVE_FN(_so)
+
+
+def integrate_with_substeps(u, v, t0, dt, eval_f, apply_mass, solve_jac, tol, max_newton_iter=100):
+    """orc_integrate_with_substeps driven by Python callbacks (u, v updated in place on success);
+    returns (substeps, newton_iterations, status)."""
+    e, m, s = _dyn_callbacks(len(u), eval_f, apply_mass, solve_jac)
+    ns, it = C.c_int64(), C.c_int64()
+    st = lib().orc_integrate_with_substeps(len(u), u, v, t0, dt, None, e, m, s, tol, max_newton_iter, C.byref(ns),
+                                           C.byref(it))
+    return ns.value, it.value, st
 
 
 class Schwarz:

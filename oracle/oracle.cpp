@@ -1898,9 +1898,59 @@ int be_step_generic(int64_t n, double t0, double dt, double* u, double* v, EvalF
     return ORC_OK;
 }
 
+// integrate_model_with_substeps (simulation_toolbox/src/fem/newton_cg.rs:146-228): the step is
+// retried from the saved initial state with 5 times as many substeps of dt / substeps after any
+// failure, and abandoned once the count would reach 1000. step(t, dt, u, v) advances the state by
+// one (sub)step and returns a status. On success u and v hold the result and *substeps the count.
+template <class Step>
+int substep_loop(int64_t n, double t, double dt, double* u, double* v, Step& step, int* substeps, int* last_error) {
+    const int max_substeps = 1000, factor = 5;
+    int num = 1;
+    std::vector<double> us(u, u + n), vs(v, v + n);
+    while (true) {
+        const double sdt = dt / (double)num;
+        int failed = ORC_OK;
+        for (int k = 0; k < num && failed == ORC_OK; ++k) failed = step(t + k * sdt, sdt, us.data(), vs.data());
+        if (failed == ORC_OK) break;
+        *last_error = failed;
+        num *= factor;
+        if (num >= max_substeps) return fail(ORC_ERR_SUBSTEPPING, "Substepping failed; last error " + std::to_string(failed));
+        std::copy(u, u + n, us.begin());
+        std::copy(v, v + n, vs.begin());
+    }
+    *substeps = num;
+    std::copy(us.begin(), us.end(), u);
+    std::copy(vs.begin(), vs.end(), v);
+    return ORC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int orc_integrate_with_substeps(int64_t n, double* u, double* v, double t0, double dt, void* ctx,
+                                orc_dyn_eval_fn eval_f, orc_dyn_mass_fn apply_mass, orc_dyn_solve_fn solve_jac,
+                                double tolerance, int64_t max_newton_iter, int64_t* substeps,
+                                int64_t* newton_iterations) {
+    auto ef = [&](double t, const double* uu, const double* vv, double* f) { return eval_f(ctx, t, uu, vv, f); };
+    auto mf = [&](double* y, const double* x) { apply_mass(ctx, y, x); };
+    auto sf = [&](double t, const double* uu, const double* vv, double a, double b, const double* rhs, double* sol) {
+        return solve_jac(ctx, t, uu, vv, a, b, rhs, sol);
+    };
+    int64_t total = 0;
+    auto step = [&](double t, double sdt, double* us, double* vs) -> int {
+        int64_t it = 0, evals = 0;
+        double fn = 0.0, tol = 0.0;
+        const int st = be_step_generic(n, t, sdt, us, vs, ef, mf, sf, tolerance, max_newton_iter, &it, &evals, &fn, &tol);
+        total += it;
+        return st;
+    };
+    int num = 0, last = ORC_OK;
+    const int st = substep_loop(n, t0, dt, u, v, step, &num, &last);
+    if (substeps) *substeps = num;
+    if (newton_iterations) *newton_iterations = total;
+    return st;
+}
 
 int orc_pcg_csr(int64_t n, const int32_t* rowptr, const int32_t* col, const double* vals, const int32_t* prowptr,
                 const int32_t* pcol, const double* pvals, const double* b, double* x, double tol, int64_t max_iter,
@@ -2192,40 +2242,26 @@ int orc_sim_step(orc_sim* s, double t, double dt, double* u, double* v, orc_step
     *s->stats = orc_step_stats{};
     const int64_t n = s->M.n;
     apply_bc_unknowns(s, u, v);
-    const int max_substeps = 1000, factor = 5;
-    int num_substeps = 1;
-    std::vector<double> us(u, u + n), vs(v, v + n);
-    while (true) {
-        const double sdt = dt / (double)num_substeps;
-        bool failed = false;
-        int last = ORC_OK;
-        for (int k = 0; k < num_substeps; ++k) {
-            int64_t it = 0;
-            if (s->damping) {
-                const int dst = compute_damping(s, us.data());
-                if (dst != ORC_OK) {
-                    s->stats = nullptr;
-                    return dst;
-                }
-            }
-            const int st = be_step(s, t + k * sdt, sdt, us.data(), vs.data(), &it);
-            s->stats->newton_iterations += it;
-            if (st != ORC_OK) {
-                failed = true;
-                last = st;
-                break;
-            }
-            apply_bc_unknowns(s, us.data(), vs.data());
+    auto step = [&](double ts, double sdt, double* us, double* vs) -> int {
+        int64_t it = 0;
+        if (s->damping) {
+            const int dst = compute_damping(s, us);
+            if (dst != ORC_OK) return dst;
         }
-        if (!failed) break;
-        num_substeps *= factor;
-        if (num_substeps >= max_substeps) {
-            s->stats = nullptr;
-            return fail(ORC_ERR_SUBSTEPPING, "Substepping failed; last error " + std::to_string(last));
-        }
-        std::copy(u, u + n, us.begin());
-        std::copy(v, v + n, vs.begin());
+        const int st = be_step(s, ts, sdt, us, vs, &it);
+        s->stats->newton_iterations += it;
+        if (st != ORC_OK) return st;
+        apply_bc_unknowns(s, us, vs);
+        return ORC_OK;
+    };
+    int num_substeps = 0, last = ORC_OK;
+    std::vector<double> u0(u, u + n), v0(v, v + n);
+    const int sst = substep_loop(n, t, dt, u0.data(), v0.data(), step, &num_substeps, &last);
+    if (sst != ORC_OK) {
+        s->stats = nullptr;
+        return sst;
     }
+    std::vector<double> us(u0), vs(v0);
     s->stats->substeps = num_substeps;
     for (int64_t i = 0; i < n; ++i)
         if (!std::isfinite(us[i]) || !std::isfinite(vs[i])) {

@@ -182,6 +182,14 @@ int orc_backward_euler_step(int64_t n, double* u, double* v, double t0, double d
                             orc_dyn_mass_fn apply_mass, orc_dyn_solve_fn solve_jac, double tolerance,
                             int64_t max_newton_iter, int64_t* iterations);
 
+/* integrate_model_with_substeps (simulation_toolbox/src/fem/newton_cg.rs:146-228) over
+ * orc_backward_euler_step's dynamic system: on failure the step is retried from the initial state
+ * with 5x as many substeps, up to (excluding) 1000. *substeps receives the count that succeeded. */
+int orc_integrate_with_substeps(int64_t n, double* u, double* v, double t0, double dt, void* ctx,
+                                orc_dyn_eval_fn eval_f, orc_dyn_mass_fn apply_mass, orc_dyn_solve_fn solve_jac,
+                                double tolerance, int64_t max_newton_iter, int64_t* substeps,
+                                int64_t* newton_iterations);
+
 /* ---------------- the FE dynamic system (simulation_toolbox/src/fem/newton_cg.rs) ---------- */
 typedef struct orc_sim_desc {
     double mu, lambda, density;
