@@ -23,15 +23,23 @@ workload, with the displacement error of the GPU run against it.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import re
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+COPY_BYTES = 32 * 1024 * 1024 * 8  # the calibration copy: 256 MiB read and 256 MiB written
 
 
 def parse():
@@ -64,6 +72,11 @@ def parse():
                         "(embedded tetrahedral mesh, clipping, moment fitting)")
     p.add_argument("--reorder", action="store_true",
                    help="reorder the scene's mesh (reverse Cuthill-McKee, reorder_mesh_par) before building the model")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) that measure HBM traffic")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="internal: the short run profiled by the counter runs (autotune trials, calibration copy, "
+                        "one step)")
     return p.parse_args()
 
 
@@ -75,6 +88,181 @@ def cg_kernel_bytes(model, value_bytes: int = 8) -> int:
     nnz, ndof = model.nnz, model.ndof
     nv = ndof // 3
     return nnz * value_bytes + (nnz // 9) * 4 + (ndof + 1) * 4 + (nv + 1) * 4 + 4 * ndof * 8
+
+
+def stiffness_bytes(sc, value_bytes: int = 8) -> int:
+    """Algorithmic bytes of one stiffness assembly (per element: every entry of K_e into the CSR
+    values, read-modify-write, with a 4-byte slot index; u gathered; geometry corners; connectivity;
+    32 B per point of per-element rules)."""
+    n = sc.nodes_per_element
+    g = 4 if sc.element_type <= 2 else 8
+    q = per_element_points(sc.stiffness_quadrature, sc.num_interior, sc.num_elements)
+    per = (3 * n) ** 2 * (2 * value_bytes + 4) + 24 * n + 24 * g + 4 * n
+    return int(sc.num_elements * per + 32 * q[sc.num_interior:].sum())
+
+
+def force_bytes(sc) -> int:
+    """Algorithmic bytes of one force assembly (per element: u gathered, geometry corners,
+    connectivity, f read-modify-write; 32 B per point of per-element rules)."""
+    n = sc.nodes_per_element
+    g = 4 if sc.element_type <= 2 else 8
+    q = per_element_points(sc.elliptic_quadrature, sc.num_interior, sc.num_elements)
+    return int(sc.num_elements * (24 * n + 24 * g + 4 * n + 48 * n) + 32 * q[sc.num_interior:].sum())
+
+
+def per_element_points(quad, num_interior, num_elements):
+    import numpy as np
+
+    q = np.full(num_elements, len(quad.interior_w), dtype=np.int64)
+    if quad.interface_offsets is not None and num_elements > num_interior:
+        q[num_interior:] = np.diff(np.asarray(quad.interface_offsets, dtype=np.int64))
+    return q
+
+
+def tangent_flops(sc):
+    """(algorithmic, issued) flops of one set of element tangents. Algorithmic: per point
+    n(n+1)/2 * 162 for the contraction plus ~1000 for dP/dF and the gradients. Issued: the
+    v_mfma_f64_16x16x4 products of the tangent kernel (six component pairs, nodes padded to 16 or
+    32, ceil(3 c / 4) k-steps per chunk of c <= 8 points, 2048 flops each)."""
+    import numpy as np
+
+    n = sc.nodes_per_element
+    q = per_element_points(sc.stiffness_quadrature, sc.num_interior, sc.num_elements)
+    alg = float(q.sum()) * (n * (n + 1) / 2 * 162 + 1000)
+    nt = 2 if n > 16 else 1
+    full, rest = q // 8, q % 8
+    ksteps = full * 6 + (3 * rest + 3) // 4
+    issued = float(ksteps.sum()) * 6 * nt * nt * 2048
+    return alg, issued
+
+
+# ------------------------------------------------------------------ HBM traffic (rocprofv3 counters)
+def _probe_command(args, counter, outdir):
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__), "--traffic-probe", "--config", str(args.config)]
+    for flag, on in (("--mixed", args.mode == 1), ("--mixed-refined", args.mode == 2), ("--reorder", args.reorder),
+                     ("--no-autotune", args.no_autotune)):
+        if on:
+            cmd.append(flag)
+    if args.quadrature != "sampled":
+        cmd += ["--quadrature", args.quadrature]
+    return cmd
+
+
+def _read_counters(outdir):
+    """rows (dispatch id, kernel name, counter value) of a counter run, in dispatch order"""
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                try:
+                    rows.append((int(r.get("Dispatch_Id", 0)), r["Kernel_Name"], float(r["Counter_Value"])))
+                except (KeyError, ValueError):
+                    continue
+    rows.sort()
+    return rows
+
+
+def _median(xs):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    m = len(xs) // 2
+    return xs[m] if len(xs) % 2 else 0.5 * (xs[m - 1] + xs[m])
+
+
+def _kernel_kib(rows):
+    """median counter value (KiB) per kernel kind: k_cg_a by lanes per node (launches that found the
+    solve finished, which move almost nothing, left out), the element tangent, the Jacobian gather (the
+    block gather that directly follows a tangent launch), element forces, force gather, and the
+    calibration copy"""
+    out = {"cg_a": {}, "tangent": [], "jgather": [], "forces": [], "fgather": [], "copy": []}
+    cga = {}
+    prev = ""
+    for _, name, v in rows:
+        m = re.search(r"k_cg_a<(\d+), false, (double|float)>", name)
+        if m:
+            cga.setdefault(int(m.group(1)), []).append(v)
+        elif "k_element_stiffness_tw" in name or "k_element_stiffness<" in name:
+            out["tangent"].append(v)
+        elif "k_block_gather" in name and ("k_element_stiffness" in prev):
+            out["jgather"].append(v)
+        elif "k_element_forces" in name:
+            out["forces"].append(v)
+        elif "k_node_force" in name:
+            out["fgather"].append(v)
+        elif "copyBuffer" in name or "copy_kernel" in name:
+            out["copy"].append(v)
+        prev = name
+    for g, vs in cga.items():
+        top = max(vs)
+        out["cg_a"][g] = _median([v for v in vs if v > 0.5 * top])
+    for k in ("tangent", "jgather", "forces", "fgather"):
+        out[k] = _median(out[k])
+    out["copy"] = max(out["copy"]) if out["copy"] else None  # the 256 MiB calibration copy
+    return out
+
+
+def measure_traffic(args):
+    """Two short counter runs of this workload under rocprofv3 (FETCH_SIZE, then WRITE_SIZE), started
+    before this process touches the GPU. Each counter is scaled by what it reads for a 256 MiB device
+    copy in the same run. Returns per-kernel HBM bytes per launch, or a reason string."""
+    if shutil.which("rocprofv3") is None:
+        return "rocprofv3 not found"
+    res = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        outdir = tempfile.mkdtemp(prefix=f"hoefem_{counter.lower()}_", dir="/tmp")
+        t0 = time.time()
+        try:
+            r = subprocess.run(_probe_command(args, counter, outdir), cwd="/tmp", env=env, capture_output=True,
+                               text=True, timeout=180)
+            rc = r.returncode
+        except subprocess.TimeoutExpired:
+            rc = "timeout"
+        rows = _read_counters(outdir)
+        shutil.rmtree(outdir, ignore_errors=True)
+        print(f"[bench] counter run {counter}: rc {rc}, {len(rows)} dispatches, {time.time() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+        if not rows:
+            return f"{counter} run produced no counters (rc {rc})"
+        kib = _kernel_kib(rows)
+        if not kib["copy"]:
+            return f"{counter} run has no calibration copy"
+        scale = COPY_BYTES / 1024.0 / kib["copy"]  # true KiB per counted KiB
+        res[counter] = {"scale": scale, "kib": kib}
+    return res
+
+
+def traffic_bytes(tr, kind, lanes=None):
+    """measured HBM bytes (read + write) per launch of one kernel kind, or None"""
+    if not isinstance(tr, dict):
+        return None
+    tot = 0.0
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        kib = tr[counter]["kib"][kind]
+        if kind == "cg_a":
+            kib = kib.get(lanes) if kib else None
+        if kib is None:
+            return None
+        tot += kib * 1024.0 * tr[counter]["scale"]
+    return int(round(tot))
+
+
+def cpu_info():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
 
 
 def common_line(args, metric, value, ms, dtype, config, **extra):
@@ -184,10 +372,44 @@ def bench_batched(args):
     print(json.dumps(line))
 
 
+def build_scene(args):
+    from higher_order_embedded_fem_amd import scenes
+
+    sc = (scenes.config_scene(args.config) if args.quadrature == "sampled"
+          else scenes.cylinder_scene(24, scenes.HEX20, 8, quadrature="embedded"))
+    return scenes.reorder(sc) if args.reorder else sc
+
+
+def run_probe(args):
+    """The run profiled by measure_traffic: the autotuner's trials (every lanes-per-node variant of
+    the CG kernels), one 256 MiB calibration copy, and one step from rest."""
+    import torch
+
+    from higher_order_embedded_fem_amd import fem
+
+    sc = build_scene(args)
+    model = fem.ElasticityModel(sc)
+    sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mode)
+    if not args.no_autotune:
+        sim.autotune()
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    u = torch.zeros(sc.ndof, dtype=torch.float64, device="cuda")
+    v = torch.zeros_like(u)
+    sim.step(0.0, sc.dt, u, v)
+    torch.cuda.synchronize()
+
+
 def main():
     args = parse()
     args.mode = 2 if args.mixed_refined else (1 if args.mixed else 0)
     args.mixed = args.mode != 0
+    if args.traffic_probe:
+        run_probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the library has no multi-GPU path.",
               file=sys.stderr)
@@ -198,6 +420,8 @@ def main():
         assert torch.cuda.is_available(), "bench.py needs a HIP device"
         (bench_learned if args.config == 4 else bench_batched)(args)
         return
+    # the counter runs go first: this process must not have touched the GPU when it starts them
+    traffic = "skipped (--no-traffic)" if args.no_traffic else measure_traffic(args)
     import ctypes as C
 
     import numpy as np
@@ -215,10 +439,7 @@ def main():
         print("[bench] calibration copy: 3 x 268435456 bytes read + written", file=sys.stderr, flush=True)
         del src, dst
     t_setup = time.time()
-    sc = (scenes.config_scene(args.config) if args.quadrature == "sampled"
-          else scenes.cylinder_scene(24, scenes.HEX20, 8, quadrature="embedded"))
-    if args.reorder:
-        sc = scenes.reorder(sc)
+    sc = build_scene(args)
     model = fem.ElasticityModel(sc)
     sim = fem.NewtonCgIntegrator(model, sc, newton_tolerance_factor=1e-6, mixed_precision=args.mode,
                                  mass_damping=args.mass_damping, stiffness_damping=args.stiffness_damping)
@@ -255,6 +476,7 @@ def main():
     # events (per-kernel launches instead of the replayed batch graphs)
     full_n, full_ms, all_n, all_ms = C.c_int64(), C.c_double(), C.c_int64(), C.c_double()
     prof_elapsed = None
+    kern = {}
     if not args.no_profile:
         L.hoefem_profile_enable(model.h, 1)
         torch.cuda.synchronize()
@@ -263,6 +485,10 @@ def main():
         torch.cuda.synchronize()
         prof_elapsed = time.perf_counter() - p0
         L.hoefem_profile_read(model.h, C.byref(full_n), C.byref(full_ms), C.byref(all_n), C.byref(all_ms))
+        for name, kind in (("tangent", 0), ("jgather", 1), ("forces", 2), ("fgather", 3)):
+            n_, ms_ = C.c_int64(), C.c_double()
+            L.hoefem_profile_kernel(model.h, kind, C.byref(n_), C.byref(ms_))
+            kern[name] = (n_.value, ms_.value)
         L.hoefem_profile_enable(model.h, 0)
     value = args.steps / elapsed
     last_res = round(tot["res_over_tol"], 4)
@@ -270,22 +496,64 @@ def main():
           f"cg {tot['cg_iterations']}", file=sys.stderr, flush=True)
 
     roofline = None
+    traffic_note = traffic if isinstance(traffic, str) else (
+        "rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload started by this bench (one step from rest "
+        "after the autotuner's trials), each scaled by its reading of a 256 MiB device copy in the same run "
+        f"(scales {traffic['FETCH_SIZE']['scale']:.4f}, {traffic['WRITE_SIZE']['scale']:.4f})")
     if full_n.value > 0:
         avg_ms = full_ms.value / full_n.value
         nbytes = cg_kernel_bytes(model, 4 if args.mixed else 8)
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", f"traffic_cg_matrix_kernel_{sc.name}_{precision}.json")
-        if os.path.exists(tpath):
-            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        ga = lanes[0] if lanes is not None else None
+        if ga is None:
+            nnz_per_node = model.nnz / 9 / max(sc.num_vertices, 1)
+            ga = 32 if nnz_per_node >= 40 else (16 if nnz_per_node >= 12 else 8)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(traffic, "cg_a", ga),
+                    "traffic_measured_in": traffic_note,
                     "kernel": "k_cg_a (CG system-matrix kernel: p = z + beta p, Ap = J p, p.Ap)",
                     "bytes_per_launch": nbytes, "avg_launch_us": round(avg_ms * 1e3, 3),
                     "launches": full_n.value, "all_launches_avg_us": round(all_ms.value / max(all_n.value, 1) * 1e3, 3),
                     "all_launches": all_n.value,
                     "measured_in": f"a second pass over the same {args.steps} steps with per-kernel timing "
                                    f"({1e3 * prof_elapsed / args.steps:.3f} ms/step)"}
+
+    assembly = None
+    if kern.get("tangent", (0, 0))[0] > 0 and kern.get("jgather", (0, 0))[0] > 0:
+        # element tangents + their gather into J, per assembly (one per Newton iteration)
+        n_asm = kern["tangent"][0]
+        t_ms = (kern["tangent"][1] + kern["jgather"][1]) / n_asm
+        sb = stiffness_bytes(sc, 4 if args.mode == 1 else 8)
+        alg_fl, mfma_fl = tangent_flops(sc)
+        tt = traffic_bytes(traffic, "tangent")
+        tg = traffic_bytes(traffic, "jgather")
+        fb = force_bytes(sc)
+        f_ms = ((kern["forces"][1] + kern["fgather"][1]) / kern["forces"][0]) if kern["forces"][0] else None
+        ft, fg = traffic_bytes(traffic, "forces"), traffic_bytes(traffic, "fgather")
+        assembly = {
+            "stiffness": {
+                "bound": "hbm", "achieved": round(sb / (t_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(sb / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": (tt + tg) if (tt is not None and tg is not None) else None,
+                "bytes_per_assembly": sb, "avg_assembly_us": round(t_ms * 1e3, 3),
+                "tangent_us": round(kern["tangent"][1] / n_asm * 1e3, 3),
+                "gather_us": round(kern["jgather"][1] / kern["jgather"][0] * 1e3, 3),
+                "assemblies": n_asm,
+                "kernels": "element tangents (k_element_stiffness_tw, v_mfma_f64_16x16x4) + color-ordered gather "
+                           "into J = M + dt^2 K (k_block_gather)",
+                "mfma": {"algorithmic_gflop": round(alg_fl / 1e9, 3), "issued_gflop": round(mfma_fl / 1e9, 3),
+                         "tflops_algorithmic": round(alg_fl / (kern["tangent"][1] / n_asm * 1e-3) / 1e12, 2),
+                         "tflops_issued": round(mfma_fl / (kern["tangent"][1] / n_asm * 1e-3) / 1e12, 2),
+                         "peak_tflops": FP64_PEAK_TFLOPS,
+                         "frac_issued": round(mfma_fl / (kern["tangent"][1] / n_asm * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
+                if args.mode != 1 else None},
+            "forces": None if f_ms is None else {
+                "bound": "hbm", "achieved": round(fb / (f_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(fb / (f_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": (ft + fg) if (ft is not None and fg is not None) else None,
+                "bytes_per_assembly": fb, "avg_assembly_us": round(f_ms * 1e3, 3), "assemblies": kern["forces"][0],
+                "kernels": "element forces (k_element_forces) + gather into f = M g + f_int (k_node_force)"},
+        }
 
     cpu = None
     err = None
@@ -300,13 +568,20 @@ def main():
                      stiffness_damping=args.stiffness_damping)
         uo = np.zeros(sc.ndof)
         vo = np.zeros(sc.ndof)
+        phases = {"forces": 0.0, "stiffness": 0.0, "jacobian": 0.0, "schwarz": 0.0, "cg": 0.0}
         c0 = time.perf_counter()
         for k in range(args.cpu_steps):
-            osim.step(k * sc.dt, sc.dt, uo, vo)
+            st = osim.step(k * sc.dt, sc.dt, uo, vo)
+            for key, field in (("forces", "t_forces"), ("stiffness", "t_stiffness"), ("jacobian", "t_jacobian"),
+                               ("schwarz", "t_precond"), ("cg", "t_cg")):
+                phases[key] += st[field]
         c1 = time.perf_counter()
+        nproc, model_name = cpu_info()
         cpu = {"value": round(args.cpu_steps / (c1 - c0), 4), "unit": "timesteps/s", "cores": threads, "kind": "port",
                "sample": f"first {args.cpu_steps} of the {sc.steps} steps of {sc.name} (same settings), "
-                         f"C++ FP64 oracle with OpenMP over element colors"}
+                         f"C++ FP64 oracle with OpenMP over element colors",
+               "phase_ms_per_step": {k: round(1e3 * v / args.cpu_steps, 2) for k, v in phases.items()},
+               "nproc": nproc, "cpu_model": model_name}
         rp, col = om.pattern()
         Mv = om.mass(sc.density)
         d = rec[1] - uo
@@ -341,6 +616,7 @@ def main():
         "error_vs_oracle": err,
         "final_residual_over_tolerance": last_res,
         "roofline": roofline,
+        "assembly": assembly,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

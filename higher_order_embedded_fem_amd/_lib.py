@@ -50,6 +50,7 @@ EXPORTED = [
     "hoefem_sim_autotune",
     "hoefem_profile_enable",
     "hoefem_profile_read",
+    "hoefem_profile_kernel",
     "hoefem_sim_create",
     "hoefem_sim_destroy",
     "hoefem_sim_representative_force",
@@ -245,6 +246,7 @@ def load(path: str = LIB_PATH):
     L.hoefem_sim_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), vp]
     L.hoefem_profile_enable.argtypes = [vp, i32]
     L.hoefem_profile_read.argtypes = [vp, C.POINTER(i64), C.POINTER(f64), C.POINTER(i64), C.POINTER(f64)]
+    L.hoefem_profile_kernel.argtypes = [vp, i32, C.POINTER(i64), C.POINTER(f64)]
     L.hoefem_sim_create.argtypes = [vp, C.POINTER(SimDesc), C.POINTER(vp)]
     L.hoefem_sim_destroy.argtypes = [vp]
     L.hoefem_sim_destroy.restype = None

@@ -130,6 +130,9 @@ static void raise_kernel_errors(Model& m) {
     if (e & KERR_SINGULAR) throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient or element Jacobian");
     if (e & KERR_ZERO_DIAG) throw Error(HOEFEM_ERROR_SINGULAR, "zero diagonal entry");
     if (e & KERR_NONFINITE) throw Error(HOEFEM_ERROR_NOT_FINITE, "non-finite values");
+    if (e & KERR_SCHWARZ)
+        throw Error(HOEFEM_ERROR_SINGULAR, "a Schwarz block above 96 x 96 is not certified positive definite "
+                                           "(|B|_F |B^-1|_F >= 5e11 or a non-positive pivot)");
     if (e) throw Error(HOEFEM_ERROR_DEVICE, "kernel error flag set");
 }
 
@@ -257,9 +260,13 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
 // values (NewtonCgDynamicSystem::eval_f, newton_cg.rs:441-478)
 static void sim_eval_f(Sim& s, const double* uu, const double* vv, double* f) {
     Model& m = *s.model;
+    m.prof_tag = PROF_ELEMENT_FORCES;
     launch_element_forces(m, s.mu_r, s.lambda_r, uu);
+    m.prof_tag = -1;
     if (s.damping) launch_spmv(m.pattern, s.d_D.get(), 0.0, s.d_dv.get(), 1.0, vv, m.stream);
+    m.prof_tag = PROF_FORCE_GATHER;
     launch_node_force(m, s.d_gforce.get(), s.damping ? s.d_dv.get() : nullptr, s.d_dirmask.get(), f);
+    m.prof_tag = -1;
 }
 
 // F(w) = -dt f(u_p + dt w, v_n + w) + M w and |F|^2 for the current (u_p, v_n) held in the sim
@@ -294,11 +301,16 @@ static hoefem_status sim_assemble_jacobian(Sim& s, bool has_a, double a, bool ha
     const int64_t nnz = m.pattern.nnz();
     const double* D = (has_b && s.damping) ? s.d_D.get() : nullptr;
     const double cd = has_b ? -b : 0.0;
-    if (has_a) launch_element_stiffness(m, s.mu_r, s.lambda_r, uu, s.mixed_mode == 1);
+    KernelProf* gprof = m.prof ? &m.kprof[PROF_JACOBIAN_GATHER] : nullptr;
+    if (has_a) {
+        m.prof_tag = PROF_TANGENT;
+        launch_element_stiffness(m, s.mu_r, s.lambda_r, uu, s.mixed_mode == 1);
+        m.prof_tag = -1;
+    }
     if (s.mixed_mode == 1) {
         if (has_a)
             launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -a, s.d_M.get(), true,
-                                m.stream, D, cd, 1.0);
+                                m.stream, D, cd, 1.0, gprof);
         else
             launch_comb(m, nnz, s.d_J32.get(), cd, D, 1.0, s.d_M.get());
         launch_dirichlet(m.pattern, s.d_J32.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
@@ -307,7 +319,7 @@ static hoefem_status sim_assemble_jacobian(Sim& s, bool has_a, double a, bool ha
     } else {
         if (has_a)
             launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J.get(), 1, -a, s.d_M.get(), true, m.stream,
-                                D, cd, 1.0);
+                                D, cd, 1.0, gprof);
         else
             launch_comb(m, nnz, s.d_J.get(), cd, D, 1.0, s.d_M.get());
         launch_dirichlet(m.pattern, s.d_J.get(), s.d_dirmask.get(), s.d_bc_nodes.get(), s.n_bc, s.d_scale.get(),
@@ -1036,9 +1048,23 @@ hoefem_status hoefem_sim_autotune(hoefem_sim* sim, int32_t* lanes_matrix, int32_
 hoefem_status hoefem_profile_enable(hoefem_model* model, int32_t enable) {
     return guarded([&] {
         Model& m = M_(model);
+        HIP_CHECK(hipStreamSynchronize(m.stream));
         m.prof = enable != 0;
         m.prof_full_launches = m.prof_all_launches = 0;
         m.prof_full_ms = m.prof_all_ms = 0.0;
+        for (KernelProf& k : m.kprof) k.reset();
+    });
+}
+
+hoefem_status hoefem_profile_kernel(hoefem_model* model, int32_t kernel, int64_t* launches, double* total_ms) {
+    return guarded([&] {
+        Model& m = M_(model);
+        if (kernel < 0 || kernel >= PROF_KINDS) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unknown profiled kernel");
+        HIP_CHECK(hipStreamSynchronize(m.stream));
+        KernelProf& k = m.kprof[kernel];
+        k.resolve();
+        if (launches) *launches = k.launches;
+        if (total_ms) *total_ms = k.ms;
     });
 }
 

@@ -367,6 +367,170 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness_mfma(
     }
 }
 
+// One element per 128-lane work-group (two wavefronts; many work-groups per CU, so the SIMDs
+// interleave the phases of different elements). Quadrature points go in chunks of TW_QC:
+//  - per-point phase with TW_SUB lanes per point: the Jacobian (each lane), the node gradients split
+//    over the point's lanes (scaled by sqrt(w |det J|) into s_G), u_grad reduced over the lanes by
+//    shuffles, then the 81 entries of dP/dF split over the lanes (snh_dpdf_entry; the material
+//    variants evaluate dP/dF on the point's first lane);
+//  - product phase: K^{ij} += G~^T U^{ij} on v_mfma_f64_16x16x4, wavefront w taking the pairs
+//    (i, j) = (w, w..2) for w = 0 and (1, 1), (1, 2), (2, 2) for w = 1, with the B operand
+//    U^{ij}(kk, col) = sum_m C_q(3k + i, 3m + j) G~(3q + m, col) formed in registers at each k-step
+//    (no U staging in LDS).
+// The lane layouts and the write-out (each K_e entry written once) are those of
+// k_element_stiffness_mfma.
+constexpr int TW_QC = 8, TW_THREADS = 128, TW_SUB = TW_THREADS / TW_QC;
+
+template <int ET, int MAT>
+__global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_element_stiffness_tw(
+    int64_t ne, const int32_t* __restrict__ conn, const double* __restrict__ verts, const int32_t* __restrict__ qoff,
+    const double* __restrict__ qw, const double* __restrict__ qxi, double thr, double mu, double lambda,
+    const double* __restrict__ u, double* __restrict__ ke, int* err) {
+    constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN, M3 = 3 * N;
+    constexpr int NP = N > 16 ? 32 : 16, NT = NP / 16, JW = 3 * NT * NT;  // jobs per wavefront
+    constexpr int KS = (3 * TW_QC) / 4;                                    // k-steps of a full chunk
+    static_assert(N <= 32 && (3 * TW_QC) % 4 == 0, "tile layout");
+    const int64_t e = blockIdx.x;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    __shared__ int s_node[N];
+    __shared__ double s_u[3 * N];
+    __shared__ double s_X[3 * GN];
+    __shared__ double s_C[TW_QC][81];
+    __shared__ double s_G[3 * TW_QC * NP];
+    if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
+    __syncthreads();
+    for (int k = t; k < 3 * N; k += TW_THREADS) s_u[k] = u[3 * (int64_t)s_node[k / 3] + k % 3];
+    for (int k = t; k < 3 * GN; k += TW_THREADS) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
+    mf_double4 acc[JW];
+#pragma unroll
+    for (int j = 0; j < JW; ++j) acc[j] = mf_double4{0.0, 0.0, 0.0, 0.0};
+    const int pq = t / TW_SUB, sub = t % TW_SUB;
+    const int q0 = qoff[e], q1 = qoff[e + 1];
+    for (int qc = q0; qc < q1; qc += TW_QC) {
+        const int cnt = min(TW_QC, q1 - qc);
+        for (int k = t; k < 3 * TW_QC * NP; k += TW_THREADS) s_G[k] = 0.0;
+        __syncthreads();
+        // ---- per-point phase: TW_SUB lanes per quadrature point
+        const bool act = pq < cnt;
+        bool ok = act;
+        double ug[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // u_grad(a, b) = sum_I G(a, I) u(b, I)
+        if (act) {
+            const int q = qc + pq;
+            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
+            double J[9], Jinv[9], psi[4];
+            ref_jacobian<ET>(s_X, xi, J);
+            const double detJ = det3(J);
+            ok = inv3(J, Jinv);
+            tet4_psi(xi, psi);
+            if (ok) {
+                const double sq = sqrt(qw[q] * fabs(detJ));
+                for (int I = sub; I < N; I += TW_SUB) {
+                    double Gr[3], G[3];
+                    grad_node<ET>(I, xi, psi, Gr);
+                    for (int a = 0; a < 3; ++a) G[a] = Jinv[0 + 3 * a] * Gr[0] + Jinv[1 + 3 * a] * Gr[1] + Jinv[2 + 3 * a] * Gr[2];
+                    for (int b = 0; b < 3; ++b)
+                        for (int a = 0; a < 3; ++a) ug[a + 3 * b] += G[a] * s_u[3 * I + b];
+                    for (int a = 0; a < 3; ++a) s_G[(3 * pq + a) * NP + I] = G[a] * sq;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+#pragma unroll
+            for (int off = 1; off < TW_SUB; off <<= 1) ug[k] += __shfl_xor(ug[k], off, 64);
+        double* C = s_C[pq];
+        if (ok) {
+            double F[9];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) F[b + 3 * a] = ug[a + 3 * b];
+            F[0] += 1.0;
+            F[4] += 1.0;
+            F[8] += 1.0;
+            if constexpr (MAT == MAT_SNH) {
+                const double alpha = 1.0 + mu / lambda * 0.75;
+                double I_C = 0.0;
+                for (int k = 0; k < 9; ++k) I_C += F[k] * F[k];
+                const double Jf = det3(F);
+                if (Jf == 0.0) {
+                    ok = false;
+                } else {
+                    double g[9];
+                    cof3(F, g);
+                    const double beta = I_C + 1.0;
+                    const double diag = mu * (1.0 - 1.0 / beta), cf = mu * 2.0 / (beta * beta);
+                    const double scale = lambda * (Jf - alpha);
+                    for (int idx = sub; idx < 81; idx += TW_SUB)
+                        C[idx] = snh_dpdf_entry(idx % 9, idx / 9, F, g, diag, cf, lambda, scale);
+                }
+            } else {
+                int okl = 1;
+                if (sub == 0) okl = material_dpdf(MAT, thr, mu, lambda, F, C) ? 1 : 0;
+                okl = __shfl(okl, l - (l % TW_SUB), 64);  // a point's lanes lie in one wavefront
+                ok = okl != 0;
+            }
+        }
+        if (act && !ok) {  // singular element Jacobian or deformation gradient: the point adds nothing
+            if (sub == 0) atomicOr(err, KERR_SINGULAR);
+            for (int idx = sub; idx < 81; idx += TW_SUB) C[idx] = 0.0;
+            for (int k = sub; k < 3 * NP; k += TW_SUB) s_G[3 * pq * NP + k] = 0.0;
+        }
+        __syncthreads();
+        // ---- product phase: rows kk of the chunk (3 per point); rows >= 3 cnt are zero in s_G
+        const int ksteps = (3 * cnt + 3) / 4;
+        double a[NT][KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int ti = 0; ti < NT; ++ti) a[ti][ks] = s_G[(4 * ks + (l >> 4)) * NP + 16 * ti + (l & 15)];
+#pragma unroll
+        for (int pp = 0; pp < 3; ++pp) {
+            const int p = 3 * w + pp;  // pairs (0,0) (0,1) (0,2) | (1,1) (1,2) (2,2)
+            const int i = p < 3 ? 0 : (p < 5 ? 1 : 2);
+            const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                if (ks < ksteps) {
+                    const int kk = 4 * ks + (l >> 4), qq = kk / 3, k = kk % 3;
+                    const bool live = qq < cnt;
+                    const double* Cq = s_C[live ? qq : 0];
+                    const double c0 = Cq[(3 * k + i) + 9 * j], c1 = Cq[(3 * k + i) + 9 * (3 + j)],
+                                 c2 = Cq[(3 * k + i) + 9 * (6 + j)];
+#pragma unroll
+                    for (int tj = 0; tj < NT; ++tj) {
+                        const int col = 16 * tj + (l & 15);
+                        double b = 0.0;
+                        if (live)
+                            b = c0 * s_G[(3 * qq) * NP + col] + c1 * s_G[(3 * qq + 1) * NP + col] +
+                                c2 * s_G[(3 * qq + 2) * NP + col];
+#pragma unroll
+                        for (int ti = 0; ti < NT; ++ti)
+                            acc[(pp * NT + ti) * NT + tj] =
+                                __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti][ks], b, acc[(pp * NT + ti) * NT + tj], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* out = ke + e * (int64_t)M3 * M3;
+#pragma unroll
+    for (int jb = 0; jb < JW; ++jb) {
+        const int pp = jb / (NT * NT), tile = jb % (NT * NT);
+        const int ti = tile / NT, tj = tile % NT;
+        const int p = 3 * w + pp;
+        const int i = p < 3 ? 0 : (p < 5 ? 1 : 2);
+        const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int I = 16 * ti + (4 * v + (l >> 4)), Jn = 16 * tj + (l & 15);
+            if (I < N && Jn < N && (i != j || I >= Jn)) {
+                out[(3 * I + i) * M3 + 3 * Jn + j] = acc[jb][v];
+                out[(3 * Jn + j) * M3 + 3 * I + i] = acc[jb][v];
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ element mass
 // assemble_generalized_element_mass (fenris/src/assembly.rs:624-671): the scalar matrix
 // rho * sum_q |det J| w phi_i phi_j duplicated on the diagonal of each 3x3 block. With a
@@ -659,12 +823,14 @@ __global__ void k_rotation_svd(int64_t nq, const double* __restrict__ F, double*
 void launch_element_forces(Model& m, double mu_r, double lambda_r, const double* u) {
     m.d_fe.ensure((size_t)m.ne * 3 * m.n);
     if (m.ne == 0) return;
+    KernelProf* prof = prof_slot(m);
     HOEFEM_MATERIAL_DISPATCH(
         m.material,
-        HOEFEM_DISPATCH(m.type, k_element_forces<ET, MK><<<dim3((unsigned)m.ne), dim3(64), 0, m.stream>>>(
-                                    m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_ell.off.get(), m.q_ell.w.get(),
-                                    m.q_ell.xi.get(), m.inversion_threshold, mu_r, lambda_r, u, m.d_fe.get(),
-                                    m.d_err.get())));
+        HOEFEM_DISPATCH(m.type, launch_timed(prof, k_element_forces<ET, MK>, dim3((unsigned)m.ne), dim3(64), 0, m.stream,
+                                             m.ne, (const int32_t*)m.d_conn.get(), (const double*)m.d_vertices.get(),
+                                             (const int32_t*)m.q_ell.off.get(), (const double*)m.q_ell.w.get(),
+                                             (const double*)m.q_ell.xi.get(), m.inversion_threshold, mu_r, lambda_r,
+                                             u, m.d_fe.get(), m.d_err.get())));
     check_launch("k_element_forces");
 }
 
@@ -684,10 +850,11 @@ void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const doub
         HOEFEM_MATERIAL_DISPATCH(
             m.material,
             HOEFEM_DISPATCH(m.type,
-                            k_element_stiffness_mfma<ET, MK><<<dim3((unsigned)m.ne), dim3(STIFF_THREADS), 0, m.stream>>>(
-                                m.ne, m.d_conn.get(), m.d_vertices.get(), m.q_stiff.off.get(), m.q_stiff.w.get(),
-                                m.q_stiff.xi.get(), m.inversion_threshold, mu_r, lambda_r, u, m.d_ke.get(),
-                                m.d_err.get())));
+                            launch_timed(prof_slot(m), k_element_stiffness_tw<ET, MK>, dim3((unsigned)m.ne),
+                                         dim3(TW_THREADS), 0, m.stream, m.ne, (const int32_t*)m.d_conn.get(),
+                                         (const double*)m.d_vertices.get(), (const int32_t*)m.q_stiff.off.get(),
+                                         (const double*)m.q_stiff.w.get(), (const double*)m.q_stiff.xi.get(),
+                                         m.inversion_threshold, mu_r, lambda_r, u, m.d_ke.get(), m.d_err.get())));
     }
     check_launch("k_element_stiffness");
 }
@@ -706,28 +873,30 @@ void launch_element_mass(Model& m, double density) {
 template <class TB, class TV>
 void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const TB* blocks, TV* values, int mode,
                          double c, const double* M, bool zero_first, hipStream_t s, const double* D2, double cd,
-                         double cm) {
+                         double cm, KernelProf* prof) {
     if (pat.num_nodes == 0) return;
     const size_t lds = (size_t)9 * std::max(pat.max_nbr, 1) * sizeof(double);
     if (lds > 64 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "node row block exceeds 64 KiB of LDS");
-    k_block_gather<TB, TV><<<dim3((unsigned)pat.num_nodes), dim3(64), lds, s>>>(
-        pat.d_nptr.get(), pat.d_rowptr.get(), pat.d_diag_only.get(), sc.d_inc_ptr.get(), sc.d_inc_blk.get(),
-        sc.d_inc_loc.get(), sc.d_m.get(), sc.d_val_off.get(), sc.d_slot_off.get(), sc.d_slot.get(), blocks, values, mode,
-        c, M, zero_first ? 1 : 0, D2, cd, cm);
+    launch_timed(prof, k_block_gather<TB, TV>, dim3((unsigned)pat.num_nodes), dim3(64), (uint32_t)lds, s,
+                 (const int32_t*)pat.d_nptr.get(), (const int32_t*)pat.d_rowptr.get(),
+                 (const uint8_t*)pat.d_diag_only.get(), (const int32_t*)sc.d_inc_ptr.get(),
+                 (const int32_t*)sc.d_inc_blk.get(), (const int32_t*)sc.d_inc_loc.get(), (const int32_t*)sc.d_m.get(),
+                 (const int64_t*)sc.d_val_off.get(), (const int64_t*)sc.d_slot_off.get(),
+                 (const int32_t*)sc.d_slot.get(), blocks, values, mode, c, M, zero_first ? 1 : 0, D2, cd, cm);
     check_launch("k_block_gather");
 }
 template void launch_block_gather<double, double>(const NodePattern&, const BlockScatter&, const double*,
                                                   double*, int, double, const double*, bool, hipStream_t,
-                                                  const double*, double, double);
+                                                  const double*, double, double, KernelProf*);
 template void launch_block_gather<float, float>(const NodePattern&, const BlockScatter&, const float*,
                                                   float*, int, double, const double*, bool, hipStream_t,
-                                                  const double*, double, double);
+                                                  const double*, double, double, KernelProf*);
 template void launch_block_gather<double, float>(const NodePattern&, const BlockScatter&, const double*,
                                                   float*, int, double, const double*, bool, hipStream_t,
-                                                  const double*, double, double);
+                                                  const double*, double, double, KernelProf*);
 template void launch_block_gather<float, double>(const NodePattern&, const BlockScatter&, const float*,
                                                   double*, int, double, const double*, bool, hipStream_t,
-                                                  const double*, double, double);
+                                                  const double*, double, double, KernelProf*);
 
 void launch_force_gather(Model& m, double* f) {
     if (m.nv == 0) return;

@@ -295,6 +295,26 @@ __device__ inline bool snh_dpdf(T mu, T lambda, const T* F, T* dp) {
 }
 
 
+// Entry (R, C) of snh_dpdf's 9x9 matrix from the quantities every entry shares: F, g = vec(J F^-T),
+// diag = mu (1 - 1 / beta), cf = 2 mu / beta^2 and scale = lambda (J - alpha). The cross-product
+// blocks of add_volume_hessian_3d (fenris/src/solid/materials.rs:763-776) contribute +-F(m, k) scale
+// to the entries of off-diagonal 3x3 blocks with r != c (m the third index, k the third block).
+template <class T>
+__device__ inline T snh_dpdf_entry(int R, int C, const T* F, const T* g, T diag, T cf, T lambda, T scale) {
+    T v = (R == C) ? diag : T(0);
+    v += cf * F[R] * F[C];
+    v += lambda * g[R] * g[C];
+    const int br = R / 3, bc = C / 3, r = R % 3, c = C % 3;
+    if (br != bc && r != c) {
+        const int m = 3 - r - c, k = 3 - br - bc;
+        const bool cyclic = (r + 1) % 3 == c;  // (r, c, m) an even permutation
+        const bool plus_block = (br == 1 && bc == 0) || (br == 0 && bc == 2) || (br == 2 && bc == 1);
+        const T x = F[3 * k + m] * scale;
+        v += (plus_block != cyclic) ? x : -x;
+    }
+    return v;
+}
+
 // ------------------------------------------------------------------ material variants
 // The elastic models of simulation_toolbox/src/fem/fe_model.rs:35-45 that wrap Stable
 // Neo-Hookean: plain, InvertibleMaterial<StableNeoHookean> (F projected before the material,

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <memory>
@@ -87,7 +88,7 @@ int nodes_per_element(int t);
 int geometry_nodes(int t);
 
 // error flags raised by kernels (bitwise OR into a device int)
-enum : int { KERR_SINGULAR = 1, KERR_PATTERN = 2, KERR_NONFINITE = 4, KERR_ZERO_DIAG = 8 };
+enum : int { KERR_SINGULAR = 1, KERR_PATTERN = 2, KERR_NONFINITE = 4, KERR_ZERO_DIAG = 8, KERR_SCHWARZ = 16 };
 
 // ------------------------------------------------------------------ node-blocked CSR
 // A CSR pattern whose rows come in xyz triples sharing one node-level column set (element
@@ -146,6 +147,24 @@ struct BlockScatter {
     void upload(hipStream_t s);
 };
 
+// ------------------------------------------------------------------ kernel timing
+// Kernel-timestamp event pairs of one kernel kind (profiling mode only): launches go through
+// launch_timed, which brackets the kernel with its own start / end timestamps.
+enum : int { PROF_TANGENT = 0, PROF_JACOBIAN_GATHER = 1, PROF_ELEMENT_FORCES = 2, PROF_FORCE_GATHER = 3, PROF_KINDS = 4 };
+struct KernelProf {
+    std::vector<hipEvent_t> ev;  // start / stop pairs
+    size_t pending = 0;          // pairs recorded since the last resolve
+    int64_t launches = 0;
+    double ms = 0.0;
+    void next(hipEvent_t& a, hipEvent_t& b);
+    void resolve();  // the stream that recorded the events must be synchronized
+    void reset() { pending = 0, launches = 0, ms = 0.0; }
+    KernelProf() = default;
+    KernelProf(const KernelProf&) = delete;
+    KernelProf& operator=(const KernelProf&) = delete;
+    ~KernelProf();
+};
+
 // ------------------------------------------------------------------ model
 struct DeviceQuadrature {
     DevBuf<int32_t> off;  // [E + 1]
@@ -181,6 +200,8 @@ struct Model {
     PinnedBuf<int> h_err{1};
     // kernel timing of the CG system-matrix kernel (HIP events on the model stream)
     bool prof = false;
+    int prof_tag = -1;           // PROF_* kind of the next launches of the tagged launchers, -1: none
+    KernelProf kprof[PROF_KINDS];
     std::vector<hipEvent_t> prof_ev;
     int64_t prof_full_launches = 0, prof_all_launches = 0;
     double prof_full_ms = 0.0, prof_all_ms = 0.0;
@@ -206,7 +227,14 @@ struct Schwarz {
     DevBuf<float> d_vals32;        // P values (FP32)
     DevBuf<double> d_blocks;       // gathered / inverted blocks (always FP64)
     DevBuf<int> d_fallback;        // per block: 1 if the Gauss-Jordan path rejected it (eigen path)
+    // blocks above SCHWARZ_SMALL_DIM: blocked Gauss-Jordan in global memory (32 x 32 MFMA tiles)
+    int small_max_dim = 0, large_max_dim = 0;
+    std::vector<int32_t> large;    // their block indices
+    DevBuf<int32_t> d_large;
+    DevBuf<int64_t> d_colp_off;    // per large block: offset of its column-panel copy (dim x 32)
+    DevBuf<double> d_piv, d_colp, d_lnorm;  // pivot tile inverses, column panels, |B|_F^2 and |B^-1|_F^2
 };
+constexpr int SCHWARZ_SMALL_DIM = 96;  // blocks up to this size: register Gauss-Jordan / LDS eigen paths
 
 // ------------------------------------------------------------------ CG workspace
 struct CgScalars {
@@ -247,12 +275,27 @@ struct CgWork {
 void launch_element_forces(Model& m, double mu_r, double lambda_r, const double* u);  // -> m.d_fe
 void launch_element_stiffness(Model& m, double mu_r, double lambda_r, const double* u, bool fp32);  // -> d_ke / d_ke32
 void launch_element_mass(Model& m, double density);  // -> m.d_ke (includes regularization)
+// hipLaunchKernelGGL on stream s, or, with prof set, hipExtLaunchKernelGGL with a start / stop
+// event pair of *prof around the kernel's own execution
+template <class K, class... Args>
+inline void launch_timed(KernelProf* prof, K kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s,
+                         Args... args) {
+    if (prof) {
+        hipEvent_t a, b;
+        prof->next(a, b);
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+    }
+}
+inline KernelProf* prof_slot(Model& m) { return (m.prof && m.prof_tag >= 0) ? &m.kprof[m.prof_tag] : nullptr; }
+
 // values (+)= gathered blocks; mode 0: values += sum; mode 1: values = (0 + c*sum) + 1*M
 template <class TB, class TV>
 // mode 0: values = sum of the blocks; mode 1: values = c * sum + cd * D2 + cm * M (D2, M optional)
 void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const TB* blocks, TV* values, int mode,
                          double c, const double* M, bool zero_first, hipStream_t s, const double* D2 = nullptr,
-                         double cd = 0.0, double cm = 1.0);
+                         double cd = 0.0, double cm = 1.0, KernelProf* prof = nullptr);
 void launch_force_gather(Model& m, double* f);
 template <class VT>
 void launch_spmv(const NodePattern& pat, const VT* vals, double beta, double* y, double alpha, const double* x,

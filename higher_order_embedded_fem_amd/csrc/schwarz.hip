@@ -30,21 +30,30 @@ void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const
     s.colors = greedy_coloring(nsub, s.off.data(), s.nodes.data());
     build_node_pattern(m.nv, nsub, s.off.data(), s.nodes.data(), s.untreated, s.P);
     s.scatter.build(s.P, nsub, s.off.data(), s.nodes.data(), &s.colors, 1);
-    if (3 * s.scatter.max_m > 96)
-        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "Schwarz subdomain too large for the batched eigensolver (max 32 nodes)");
-    // slots of every block entry in J's (model) pattern: SchwarzElementAssembler gathers with
-    // csr_row.get(col).expect(..) (schwarz_precond.rs:304-320)
+    // slots of every block entry in J's (model) pattern. SchwarzElementAssembler gathers with
+    // csr_row.get(col).expect(..) (schwarz_precond.rs:304-320), which panics on a node pair that J
+    // does not couple; here such entries are the zeros they are in J (-1 slot), so subdomains may be
+    // any node sets, e.g. patches of many elements.
     s.jslot.resize(s.scatter.slot.size());
+    s.small_max_dim = s.large_max_dim = 0;
+    s.large.clear();
+    std::vector<int64_t> colp_off;
+    int64_t colp_total = 0;
     for (int64_t b = 0; b < nsub; ++b) {
         const int32_t* g = s.nodes.data() + s.off[b];
         const int mb = (int)(s.off[b + 1] - s.off[b]);
         for (int a = 0; a < mb; ++a)
-            for (int c = 0; c < mb; ++c) {
-                const int32_t k = m.pattern.find(g[a], g[c]);
-                if (k < 0) throw Error(HOEFEM_ERROR_PATTERN, "Schwarz block entry missing from the system matrix pattern");
-                s.jslot[s.scatter.slot_off[b] + (int64_t)a * mb + c] = k;
-            }
+            for (int c = 0; c < mb; ++c) s.jslot[s.scatter.slot_off[b] + (int64_t)a * mb + c] = m.pattern.find(g[a], g[c]);
+        if (3 * mb > SCHWARZ_SMALL_DIM) {
+            s.large.push_back((int32_t)b);
+            s.large_max_dim = std::max(s.large_max_dim, 3 * mb);
+            colp_off.push_back(colp_total);
+            colp_total += (int64_t)3 * mb * 32;
+        } else {
+            s.small_max_dim = std::max(s.small_max_dim, 3 * mb);
+        }
     }
+    if (s.large_max_dim > 4096) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "Schwarz subdomain above 4096 DOFs");
     for (int32_t v : s.untreated)
         if (v < 0 || v >= m.nv || !s.P.diag_only[v])
             throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "untreated node must not belong to a subdomain");
@@ -63,6 +72,13 @@ void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const
     }
     s.d_blocks.ensure(std::max<int64_t>(s.scatter.val_off[nsub], 1));
     s.d_fallback.ensure(std::max<int64_t>(nsub, 1));
+    if (!s.large.empty()) {
+        s.d_large.upload(s.large, m.stream);
+        s.d_colp_off.upload(colp_off, m.stream);
+        s.d_piv.ensure(s.large.size() * 32 * 32);
+        s.d_colp.ensure((size_t)colp_total);
+        s.d_lnorm.ensure(2 * s.large.size());
+    }
 }
 
 // B_s(r, c) = J[3 g_a + i, 3 g_c + j], row-major (3m)^2, always FP64
@@ -80,7 +96,7 @@ __global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t*
         const int r = idx / dim, c = idx % dim;
         const int a = r / 3, i = r % 3, cb = c / 3, j = c % 3;
         const int k = jslot[slot_off[b] + (int64_t)a * mb + cb];
-        out[idx] = (double)J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
+        out[idx] = k < 0 ? 0.0 : (double)J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
     }
 }
 
@@ -176,6 +192,10 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
     __shared__ double s_red[EIG_THREADS];
     const int64_t b = blockIdx.x;
     const int dim = 3 * msz[b];
+    if (dim > SCHWARZ_SMALL_DIM) {  // handled by the blocked path
+        if (threadIdx.x == 0) fallback[b] = 4;
+        return;
+    }
     const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
     double* blk = blocks + val_off[b];
     double a[RR][RR];
@@ -460,6 +480,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_power_certify(
     __shared__ double s_red[EIG_THREADS];
     const int64_t b = blockIdx.x;
     const int mb = msz[b], dim = 3 * mb, dp = (dim + 3) & ~3, ld = ldpad;
+    if (dim > SCHWARZ_SMALL_DIM) return;
     double* X = lds;
     double* Y = lds + (size_t)dp * ld;
     double* blk = blocks + val_off[b];
@@ -468,7 +489,7 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_power_certify(
     auto entry = [&](int r, int c) -> double {  // B(r, c) = J[3 g_a + i, 3 g_cb + j]
         const int a = r / 3, i = r % 3, cb = c / 3, j = c % 3;
         const int k = jslot[slot_off[b] + (int64_t)a * mb + cb];
-        return (double)J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
+        return k < 0 ? 0.0 : (double)J[(int64_t)jrowptr[3 * (int64_t)g[a] + i] + 3 * k + j];
     };
     auto bound = [&]() -> double {  // spectral-norm bound of the symmetric matrix in X
         double loc = 0.0;
@@ -536,10 +557,11 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_inverse(const int32_t* _
                                                                 double* __restrict__ blocks,
                                                                 const int* __restrict__ fallback, int ldpad,
                                                                 int project) {
-    if (fallback && fallback[blockIdx.x] == 0) return;
+    if (fallback && (fallback[blockIdx.x] == 0 || fallback[blockIdx.x] >= 4)) return;
     extern __shared__ double lds[];
     const int64_t b = blockIdx.x;
     const int dim = 3 * msz[b];
+    if (fallback && dim > SCHWARZ_SMALL_DIM) return;  // the LDS is sized for the small blocks
     const int nn = dim + (dim & 1);  // even number of players (one idle index when dim is odd)
     const int ld = ldpad;            // row stride (>= nn + 1)
     double* A = lds;
@@ -683,6 +705,246 @@ __global__ __launch_bounds__(EIG_THREADS) void k_jacobi_inverse(const int32_t* _
     }
 }
 
+// ------------------------------------------------------------------ blocks above 96 x 96
+// Blocked Gauss-Jordan inversion in global memory, 32 x 32 tiles, all large blocks of an update
+// together. For panel k (rows / columns 32k ..): (1) the pivot tile A_kk is inverted in LDS
+// (P = A_kk^-1, unpivoted: every pivot is a diagonal entry of a Schur complement, positive iff the
+// block is positive definite); (2) the row panel A_kj <- P A_kj (j != k) and a copy C of the old
+// column panel A_ik (i != k); (3) A_ij -= C_i A_kj (i, j != k), A_ik <- -C_i P, A_kk <- P. Steps 2
+// and 3 multiply 32 x 32 tiles on v_mfma_f64_16x16x4 (each of four wavefronts a 16 x 16 quarter).
+// The inverse is accepted when |B|_F |B^-1|_F < SPD_COND_LIMIT (an upper bound of kappa_2): the
+// eigenvalue clamp is then inactive and B^+ = B^-1; it is symmetrized as in k_spd_inverse. A block
+// without that certificate (indefinite or too ill-conditioned) raises KERR_SCHWARZ: large blocks have
+// no eigen fallback.
+constexpr int BT = 32, BTP = 33;  // tile size, LDS row stride
+
+typedef double bgj_double4 __attribute__((ext_vector_type(4)));
+
+// dst(32 x 32 tile in LDS) = src rows r0.., cols c0.. of a row-major dim x dim block, zero outside
+__device__ inline void bgj_load(double* dst, const double* src, int dim, int r0, int c0, double sign) {
+    for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) {
+        const int r = idx / BT, c = idx % BT;
+        dst[r * BTP + c] = (r0 + r < dim && c0 + c < dim) ? sign * src[(int64_t)(r0 + r) * dim + c0 + c] : 0.0;
+    }
+}
+
+// acc (this wavefront's 16 x 16 quarter (si, sj) of a 32 x 32 tile) += A B, A and B 32 x 32 in LDS
+__device__ inline void bgj_mfma(bgj_double4& acc, const double* A, const double* B, int si, int sj) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int ks = 0; ks < BT / 4; ++ks) {
+        const double a = A[(16 * si + (l & 15)) * BTP + 4 * ks + (l >> 4)];
+        const double b = B[(4 * ks + (l >> 4)) * BTP + 16 * sj + (l & 15)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+}
+
+__global__ __launch_bounds__(EIG_THREADS) void k_bgj_norm(const int32_t* __restrict__ large,
+                                                          const int32_t* __restrict__ msz,
+                                                          const int64_t* __restrict__ val_off,
+                                                          const double* __restrict__ blocks, double* __restrict__ out,
+                                                          int which) {
+    __shared__ double s_red[EIG_THREADS];
+    const int b = large[blockIdx.x];
+    const int dim = 3 * msz[b];
+    const double* blk = blocks + val_off[b];
+    double loc = 0.0;
+    for (int64_t idx = threadIdx.x; idx < (int64_t)dim * dim; idx += EIG_THREADS) loc += blk[idx] * blk[idx];
+    const double tot = block_sum(loc, s_red);
+    if (threadIdx.x == 0) out[2 * blockIdx.x + which] = tot;
+}
+
+// (1) P = A_kk^-1 for every large block with a panel k
+__global__ __launch_bounds__(EIG_THREADS) void k_bgj_pivot(const int32_t* __restrict__ large,
+                                                           const int32_t* __restrict__ msz,
+                                                           const int64_t* __restrict__ val_off,
+                                                           const double* __restrict__ blocks, double* __restrict__ piv,
+                                                           int* __restrict__ fallback, int* err, int k) {
+    __shared__ double T[BT * BTP];
+    __shared__ double s_row[BT], s_col[BT];
+    const int lb = blockIdx.x, b = large[lb];
+    const int dim = 3 * msz[b];
+    if (BT * k >= dim || fallback[b] != 4) return;
+    const int bk = min(BT, dim - BT * k);
+    const double* blk = blocks + val_off[b];
+    for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) {
+        const int r = idx / BT, c = idx % BT;
+        T[r * BTP + c] = (r < bk && c < bk) ? blk[(int64_t)(BT * k + r) * dim + BT * k + c] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int p = 0; p < bk; ++p) {
+        if (threadIdx.x < BT) {
+            s_row[threadIdx.x] = T[p * BTP + threadIdx.x];
+            s_col[threadIdx.x] = T[threadIdx.x * BTP + p];
+        }
+        __syncthreads();
+        const double pv = s_row[p];
+        if (!(pv > 0.0) || !isfinite(pv)) {  // uniform: the block is not positive definite
+            if (threadIdx.x == 0) {
+                fallback[b] = 5;
+                atomicOr(err, KERR_SCHWARZ);
+            }
+            return;
+        }
+        const double ip = 1.0 / pv;
+        for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) {
+            const int r = idx / BT, c = idx % BT;
+            double v = T[r * BTP + c];
+            if (r == p && c == p)
+                v = ip;
+            else if (r == p)
+                v = s_row[c] * ip;
+            else if (c == p)
+                v = -s_col[r] * ip;
+            else
+                v -= s_col[r] * s_row[c] * ip;
+            T[r * BTP + c] = v;
+        }
+        __syncthreads();
+    }
+    double* out = piv + (int64_t)lb * BT * BT;
+    for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) out[idx] = T[(idx / BT) * BTP + idx % BT];
+}
+
+// (2) row panel A_kj <- P A_kj and the copy C_j = A_jk of the column panel, tile j != k
+__global__ __launch_bounds__(EIG_THREADS) void k_bgj_panel(const int32_t* __restrict__ large,
+                                                           const int32_t* __restrict__ msz,
+                                                           const int64_t* __restrict__ val_off,
+                                                           double* __restrict__ blocks, const double* __restrict__ piv,
+                                                           double* __restrict__ colp,
+                                                           const int64_t* __restrict__ colp_off,
+                                                           const int* __restrict__ fallback, int k) {
+    __shared__ double sP[BT * BTP], sR[BT * BTP];
+    const int lb = blockIdx.x, b = large[lb], t = blockIdx.y;
+    const int dim = 3 * msz[b];
+    const int nt = (dim + BT - 1) / BT;
+    if (BT * k >= dim || t >= nt || t == k || fallback[b] != 4) return;
+    double* blk = blocks + val_off[b];
+    // the column panel's tile (rows 32 t.., columns 32 k..), before step 3 overwrites it
+    double* C = colp + colp_off[lb];
+    for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) {
+        const int r = idx / BT, c = idx % BT;
+        if (BT * t + r < dim && BT * k + c < dim)
+            C[(int64_t)(BT * t + r) * BT + c] = blk[(int64_t)(BT * t + r) * dim + BT * k + c];
+    }
+    const double* P = piv + (int64_t)lb * BT * BT;
+    for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) sP[(idx / BT) * BTP + idx % BT] = P[idx];
+    bgj_load(sR, blk, dim, BT * k, BT * t, 1.0);
+    __syncthreads();
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, si = w >> 1, sj = w & 1;
+    bgj_double4 acc = {0.0, 0.0, 0.0, 0.0};
+    bgj_mfma(acc, sP, sR, si, sj);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = BT * k + 16 * si + 4 * v + (l >> 4), c = BT * t + 16 * sj + (l & 15);
+        if (r < dim && c < dim) blk[(int64_t)r * dim + c] = acc[v];
+    }
+}
+
+// (3) A_ij -= C_i A_kj (i, j != k), A_ik = -C_i P, A_kk = P
+__global__ __launch_bounds__(EIG_THREADS) void k_bgj_update(const int32_t* __restrict__ large,
+                                                            const int32_t* __restrict__ msz,
+                                                            const int64_t* __restrict__ val_off,
+                                                            double* __restrict__ blocks, const double* __restrict__ piv,
+                                                            const double* __restrict__ colp,
+                                                            const int64_t* __restrict__ colp_off,
+                                                            const int* __restrict__ fallback, int k, int ntm) {
+    __shared__ double sA[BT * BTP], sB[BT * BTP];
+    const int lb = blockIdx.x, b = large[lb];
+    const int ti = blockIdx.y / ntm, tj = blockIdx.y % ntm;
+    const int dim = 3 * msz[b];
+    const int nt = (dim + BT - 1) / BT;
+    if (BT * k >= dim || ti >= nt || tj >= nt || (ti == k && tj != k) || fallback[b] != 4) return;
+    double* blk = blocks + val_off[b];
+    const double* P = piv + (int64_t)lb * BT * BT;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, si = w >> 1, sj = w & 1;
+    bgj_double4 acc = {0.0, 0.0, 0.0, 0.0};
+    if (ti == k) {  // tj == k: the pivot tile becomes P
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = P[(16 * si + 4 * v + (l >> 4)) * BT + 16 * sj + (l & 15)];
+    } else {
+        // -C_i: the column panel copy, rows 32 ti..
+        const double* C = colp + colp_off[lb];
+        for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) {
+            const int r = idx / BT, c = idx % BT;
+            sA[r * BTP + c] = (BT * ti + r < dim && BT * k + c < dim) ? -C[(int64_t)(BT * ti + r) * BT + c] : 0.0;
+        }
+        if (tj == k) {
+            for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) sB[(idx / BT) * BTP + idx % BT] = P[idx];
+        } else {
+            bgj_load(sB, blk, dim, BT * k, BT * tj, 1.0);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = BT * ti + 16 * si + 4 * v + (l >> 4), c = BT * tj + 16 * sj + (l & 15);
+                acc[v] = (r < dim && c < dim) ? blk[(int64_t)r * dim + c] : 0.0;
+            }
+        }
+        __syncthreads();
+        bgj_mfma(acc, sA, sB, si, sj);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = BT * ti + 16 * si + 4 * v + (l >> 4), c = BT * tj + 16 * sj + (l & 15);
+        if (r < dim && c < dim) blk[(int64_t)r * dim + c] = acc[v];
+    }
+}
+
+// certificate and symmetrization: tile pairs (ti <= tj) of every large block
+__global__ __launch_bounds__(EIG_THREADS) void k_bgj_finish(const int32_t* __restrict__ large,
+                                                            const int32_t* __restrict__ msz,
+                                                            const int64_t* __restrict__ val_off,
+                                                            double* __restrict__ blocks, const double* __restrict__ lnorm,
+                                                            int* __restrict__ fallback, int* err, int ntm) {
+    __shared__ double sA[BT * BTP], sB[BT * BTP];
+    const int lb = blockIdx.x, b = large[lb];
+    const int ti = blockIdx.y / ntm, tj = blockIdx.y % ntm;
+    const int dim = 3 * msz[b];
+    const int nt = (dim + BT - 1) / BT;
+    if (ti >= nt || tj >= nt || ti > tj || fallback[b] != 4) return;
+    const double cert = sqrt(lnorm[2 * lb]) * sqrt(lnorm[2 * lb + 1]);
+    if (!(cert < SPD_COND_LIMIT)) {  // uniform over the block's work-groups
+        if (threadIdx.x == 0 && ti == 0 && tj == 0) atomicOr(err, KERR_SCHWARZ);
+        return;
+    }
+    double* blk = blocks + val_off[b];
+    bgj_load(sA, blk, dim, BT * ti, BT * tj, 1.0);
+    bgj_load(sB, blk, dim, BT * tj, BT * ti, 1.0);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < BT * BT; idx += EIG_THREADS) {
+        const int r = idx / BT, c = idx % BT;
+        const double v = 0.5 * (sA[r * BTP + c] + sB[c * BTP + r]);
+        if (BT * ti + r < dim && BT * tj + c < dim) blk[(int64_t)(BT * ti + r) * dim + BT * tj + c] = v;
+        if (BT * tj + c < dim && BT * ti + r < dim) blk[(int64_t)(BT * tj + c) * dim + BT * ti + r] = v;
+    }
+}
+
+static void schwarz_update_large(Schwarz& s) {
+    Model& m = *s.model;
+    const hipStream_t st = m.stream;
+    const unsigned nl = (unsigned)s.large.size();
+    const int ntm = (s.large_max_dim + BT - 1) / BT;
+    const int32_t* lg = s.d_large.get();
+    const int32_t* msz = s.scatter.d_m.get();
+    const int64_t* vo = s.scatter.d_val_off.get();
+    hipLaunchKernelGGL(k_bgj_norm, dim3(nl), dim3(EIG_THREADS), 0, st, lg, msz, vo, (const double*)s.d_blocks.get(),
+                       s.d_lnorm.get(), 0);
+    for (int k = 0; k < ntm; ++k) {
+        hipLaunchKernelGGL(k_bgj_pivot, dim3(nl), dim3(EIG_THREADS), 0, st, lg, msz, vo,
+                           (const double*)s.d_blocks.get(), s.d_piv.get(), s.d_fallback.get(), m.d_err.get(), k);
+        hipLaunchKernelGGL(k_bgj_panel, dim3(nl, (unsigned)ntm), dim3(EIG_THREADS), 0, st, lg, msz, vo,
+                           s.d_blocks.get(), (const double*)s.d_piv.get(), s.d_colp.get(),
+                           (const int64_t*)s.d_colp_off.get(), (const int*)s.d_fallback.get(), k);
+        hipLaunchKernelGGL(k_bgj_update, dim3(nl, (unsigned)(ntm * ntm)), dim3(EIG_THREADS), 0, st, lg, msz, vo,
+                           s.d_blocks.get(), (const double*)s.d_piv.get(), (const double*)s.d_colp.get(),
+                           (const int64_t*)s.d_colp_off.get(), (const int*)s.d_fallback.get(), k, ntm);
+    }
+    hipLaunchKernelGGL(k_bgj_norm, dim3(nl), dim3(EIG_THREADS), 0, st, lg, msz, vo, (const double*)s.d_blocks.get(),
+                       s.d_lnorm.get(), 1);
+    hipLaunchKernelGGL(k_bgj_finish, dim3(nl, (unsigned)(ntm * ntm)), dim3(EIG_THREADS), 0, st, lg, msz, vo,
+                       s.d_blocks.get(), (const double*)s.d_lnorm.get(), s.d_fallback.get(), m.d_err.get(), ntm);
+    check_launch("blocked Gauss-Jordan");
+}
+
 // untreated nodes: P_ii = 1 / J_ii (schwarz_precond.rs:195-213)
 template <class VJ, class VP>
 __global__ void k_schwarz_untreated(int64_t nunt, const int32_t* __restrict__ unt, const int32_t* __restrict__ prowptr,
@@ -708,9 +970,10 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
             s.scatter.d_m.get(), s.scatter.d_val_off.get(), s.scatter.d_slot_off.get(), s.d_off.get(), s.d_nodes.get(),
             s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get());
         check_launch("k_schwarz_gather");
-        const int dim = 3 * s.scatter.max_m;
-        // Gauss-Jordan fast path (blocks up to 96 x 96, checked at setup), the sharper certificate
-        // for the blocks it could not accept, then the eigen path for the rest
+        const int dim = std::max(3, s.small_max_dim);
+        // Gauss-Jordan fast path (blocks up to 96 x 96; it marks the larger ones 4), the sharper
+        // certificate for the blocks it could not accept, then the eigen path for the rest; blocks
+        // above 96 x 96 go through the blocked path
         if (dim <= 64)
             hipLaunchKernelGGL(k_spd_inverse<4>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
                                s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
@@ -735,6 +998,7 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         hipLaunchKernelGGL(k_jacobi_inverse, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st, s.scatter.d_m.get(),
                            s.scatter.d_val_off.get(), s.d_blocks.get(), (const int*)s.d_fallback.get(), ld, 0);
         check_launch("k_jacobi_inverse");
+        if (!s.large.empty()) schwarz_update_large(s);
         if (s.fp32)
             launch_block_gather(s.P, s.scatter, s.d_blocks.get(), s.d_vals32.get(), 0, 0.0, nullptr, true, st);
         else

@@ -14,6 +14,31 @@ void check_launch(const char* what) {
     if (e != hipSuccess) throw Error(HOEFEM_ERROR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void KernelProf::next(hipEvent_t& a, hipEvent_t& b) {
+    while (ev.size() < 2 * (pending + 1)) {
+        hipEvent_t x;
+        HIP_CHECK(hipEventCreate(&x));
+        ev.push_back(x);
+    }
+    a = ev[2 * pending];
+    b = ev[2 * pending + 1];
+    ++pending;
+}
+
+void KernelProf::resolve() {
+    for (size_t k = 0; k < pending; ++k) {
+        float t = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+        ms += t;
+        launches += 1;
+    }
+    pending = 0;
+}
+
+KernelProf::~KernelProf() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
+
 void raise_lds_limit(const void* func, size_t bytes) {
     static std::mutex mu;
     static std::map<std::pair<int, const void*>, size_t> limit;

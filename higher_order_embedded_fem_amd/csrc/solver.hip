@@ -1296,9 +1296,9 @@ void launch_bc_copy(Model& m, const double* src, const uint8_t* dirmask, double*
 void launch_node_force(Model& m, const double* gforce, const double* dv, const uint8_t* dirmask, double* f) {
     const int64_t nn = m.nv;
     if (nn <= 0) return;
-    hipLaunchKernelGGL(k_node_force, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, m.stream, nn,
-                       m.scatter.d_inc_ptr.get(), m.scatter.d_inc_blk.get(), m.scatter.d_inc_loc.get(), m.n,
-                       m.d_fe.get(), gforce, dv, dirmask, f);
+    launch_timed(prof_slot(m), k_node_force, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, m.stream, nn,
+                 (const int32_t*)m.scatter.d_inc_ptr.get(), (const int32_t*)m.scatter.d_inc_blk.get(),
+                 (const int32_t*)m.scatter.d_inc_loc.get(), m.n, (const double*)m.d_fe.get(), gforce, dv, dirmask, f);
     check_launch("k_node_force");
 }
 

@@ -278,11 +278,61 @@ def _per_element_subdomains(conn: np.ndarray):
     return (np.arange(ne + 1, dtype=np.int64) * n), conn.astype(np.int32).ravel().copy(), np.zeros(0, np.int32)
 
 
+def patch_of_elements(conn: np.ndarray, patch_size: int = 64) -> np.ndarray:
+    """Patch index of every element: seeds in element order, each patch grown breadth first over
+    element adjacency (elements sharing a node, neighbours in increasing element index) until it
+    holds patch_size elements or its front is exhausted."""
+    return _grow_patches(np.asarray(conn), patch_size)[0]
+
+
+def greedy_patches(conn: np.ndarray, patch_size: int = 64):
+    """Schwarz subdomains of greedy patches of about patch_size elements (patch_of_elements):
+    subdomain k is the sorted union of its elements' nodes (a NestedVec of node sets, as
+    SchwarzPreconditionerComponent takes them, simulation_toolbox/src/fem/schwarz_precond.rs:26-42).
+    Returns (offsets, nodes, untreated) with no untreated nodes."""
+    return _grow_patches(np.asarray(conn), patch_size)[1]
+
+
+def _grow_patches(conn: np.ndarray, patch_size: int):
+    from collections import deque
+
+    ne = conn.shape[0]
+    order = np.argsort(conn.ravel(), kind="stable")
+    node_of = conn.ravel()[order]
+    elem_of = order // conn.shape[1]
+    starts = np.searchsorted(node_of, np.arange(int(conn.max()) + 2))
+    patch = np.full(ne, -1, dtype=np.int64)
+    offsets, nodes = [0], []
+    for seed in range(ne):
+        if patch[seed] >= 0:
+            continue
+        pid = len(offsets) - 1
+        members = [seed]
+        patch[seed] = pid
+        front = deque([seed])
+        while front and len(members) < patch_size:
+            x = front.popleft()
+            nb = np.unique(np.concatenate([elem_of[starts[v]:starts[v + 1]] for v in conn[x]]))
+            for y in nb:
+                if len(members) >= patch_size:
+                    break
+                if patch[y] < 0:
+                    patch[y] = pid
+                    members.append(int(y))
+                    front.append(int(y))
+        union = np.unique(conn[members].ravel())
+        nodes.append(union)
+        offsets.append(offsets[-1] + union.size)
+    return patch, (np.asarray(offsets, dtype=np.int64), np.concatenate(nodes).astype(np.int32), np.zeros(0, np.int32))
+
+
 def cube_scene(cells: int = 8, element_type: int = TET10, young: float = 1e6, poisson: float = 0.4,
-               density: float = 1000.0, dt: float = 0.01, steps: int = 10) -> Scene:
+               density: float = 1000.0, dt: float = 0.01, steps: int = 10, schwarz: str = "element") -> Scene:
     """Unit cube of cells^3 hexes, 6 tets per hex, refined to Tet10 (or Tet4/Tet20).
 
     Tet quadrature strength 2 for mass, stiffness and elliptic terms; nodes on z = 0 fixed.
+    Schwarz subdomains: "element" (every element, SchwarzPreconditionerComponent::from_space) or
+    "patches" (greedy 64-element patches, greedy_patches).
     """
     hv, hc = uniform_hex_grid(cells, 1.0)
     tets = hex_to_tets(hv, hc)
@@ -294,9 +344,14 @@ def cube_scene(cells: int = 8, element_type: int = TET10, young: float = 1e6, po
     w, xi = tet_quadrature_strength_2()
     q = Quadrature(w, xi)
     dir_nodes = np.nonzero(np.abs(verts[:, 2]) <= 1e-12)[0].astype(np.int32)
-    so, sn, su = _per_element_subdomains(conn)
+    if schwarz == "patches":
+        so, sn, su = greedy_patches(conn, 64)
+    elif schwarz == "element":
+        so, sn, su = _per_element_subdomains(conn)
+    else:
+        raise ValueError(f"unknown Schwarz subdomains {schwarz!r}")
     return Scene(
-        name=f"cube{cells}_{ELEMENT_NAMES[element_type].lower()}",
+        name=f"cube{cells}_{ELEMENT_NAMES[element_type].lower()}" + ("_patches" if schwarz == "patches" else ""),
         element_type=element_type,
         vertices=np.ascontiguousarray(verts, dtype=np.float64),
         connectivity=np.ascontiguousarray(conn, dtype=np.int32),
@@ -604,9 +659,10 @@ def reorder(scene: Scene) -> Scene:
 
 
 def config_scene(index: int, **kw) -> Scene:
-    """The workload configurations: 1 = Tet10 cube 8^3, 2 = Hex20 cylinder 24^3 (q=8),
-    3 = Hex27 cylinder 48^3 (q=27)."""
+    """The workload configurations: 1 = Tet10 cube 8^3 with Schwarz subdomains of greedy
+    64-element patches, 2 = Hex20 cylinder 24^3 (q=8), 3 = Hex27 cylinder 48^3 (q=27)."""
     if index == 1:
+        kw.setdefault("schwarz", "patches")
         return cube_scene(8, TET10, **kw)
     if index == 2:
         return cylinder_scene(24, HEX20, 8, **kw)

@@ -232,6 +232,17 @@ hoefem_status hoefem_sim_autotune(hoefem_sim* sim, int32_t* lanes_matrix, int32_
 hoefem_status hoefem_profile_enable(hoefem_model* model, int32_t enable);
 hoefem_status hoefem_profile_read(hoefem_model* model, int64_t* full_launches, double* full_ms,
                                   int64_t* all_launches, double* all_ms);
+/* Kernels of the simulation's assembly timed the same way while profiling is enabled (the
+ * reference's coarse_prof scopes "assemble elastic forces" and the stiffness assembly inside
+ * "solve jacobian combination", newton_cg.rs:460, :685-764). */
+typedef enum hoefem_profiled_kernel {
+    HOEFEM_PROF_ELEMENT_TANGENT = 0,       /* element stiffness matrices of the Newton Jacobian */
+    HOEFEM_PROF_JACOBIAN_GATHER = 1,       /* their color-ordered gather into J = M + dt^2 K (+ dt D) */
+    HOEFEM_PROF_ELEMENT_FORCES = 2,        /* element force vectors of eval_f */
+    HOEFEM_PROF_FORCE_GATHER = 3           /* their gather into f = M g + f_int - D v */
+} hoefem_profiled_kernel;
+/* launches and total kernel time (ms) of one hoefem_profiled_kernel since profiling was enabled */
+hoefem_status hoefem_profile_kernel(hoefem_model* model, int32_t kernel, int64_t* launches, double* total_ms);
 
 /* NewtonCgIntegrator3d (simulation_toolbox/src/fem/newton_cg.rs:43-295) over
  * backward_euler_step (hamilton2/src/integrators/euler.rs:155-199) and
