@@ -45,6 +45,7 @@ EXPORTED = [
     "hoefem_schwarz_values",
     "hoefem_schwarz_apply",
     "hoefem_pcg",
+    "hoefem_pcg_noalloc",
     "hoefem_model_autotune",
     "hoefem_schwarz_autotune",
     "hoefem_sim_autotune",
@@ -241,6 +242,7 @@ def load(path: str = LIB_PATH):
     L.hoefem_schwarz_values.argtypes = [vp, vp]
     L.hoefem_schwarz_apply.argtypes = [vp, vp, vp]
     L.hoefem_pcg.argtypes = [vp, vp, vp, vp, vp, f64, i64, C.POINTER(i64)]
+    L.hoefem_pcg_noalloc.argtypes = [vp, vp, vp, vp, vp, f64, i64, C.POINTER(i64)]
     L.hoefem_model_autotune.argtypes = [vp, C.POINTER(i32), vp]
     L.hoefem_schwarz_autotune.argtypes = [vp, C.POINTER(i32), vp]
     L.hoefem_sim_autotune.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), vp]

@@ -957,6 +957,23 @@ hoefem_status hoefem_pcg(hoefem_model* model, const double* A, hoefem_schwarz* s
     return st;
 }
 
+hoefem_status hoefem_pcg_noalloc(hoefem_model* model, const double* A, hoefem_schwarz* schwarz, const double* b,
+                                 double* x, double tolerance, int64_t max_iterations, int64_t* iterations) {
+    hoefem_status st = HOEFEM_OK;
+    int64_t it = 0;
+    const hoefem_status g = guarded([&] {
+        Model& m = M_(model);
+        if (m.pointer_mode != HOEFEM_POINTERS_DEVICE)
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "hoefem_pcg_noalloc takes device pointers only");
+        if (!A || !b || !x) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        st = pcg_solve_noalloc(m, A, schwarz ? &schwarz->s : nullptr, b, x, tolerance, max_iterations, &it);
+    });
+    if (iterations) *iterations = it;
+    if (g != HOEFEM_OK) return g;
+    if (st != HOEFEM_OK) g_last_error = "CG stopped with status " + std::to_string((int)st);
+    return st;
+}
+
 hoefem_status hoefem_model_autotune(hoefem_model* model, int32_t* lanes_per_node, double* candidate_us) {
     return guarded([&] {
         Model& m = M_(model);

@@ -325,6 +325,10 @@ void schwarz_apply(Schwarz& s, const double* x, double* y);
 // Solves A x = b. A has the model pattern; preconditioner = Schwarz or Jacobi(diag(A)).
 hoefem_status pcg_solve(Model& m, CgWork& w, const double* A, Schwarz* P, const double* b, double* x, double tol,
                         int64_t max_iter, int64_t* iterations);
+// pcg_solve (FP64) with every work buffer in a fixed per-device scratch array: no allocation;
+// models of at most 262144 DOFs; calls overlapping in time on one GPU race
+hoefem_status pcg_solve_noalloc(Model& m, const double* A, Schwarz* P, const double* b, double* x, double tol,
+                                int64_t max_iter, int64_t* iterations);
 // mixed precision: FP32 system matrix and FP32 Schwarz operator (P->fp32 must be set), FP64 vectors / dots
 hoefem_status pcg_solve(Model& m, CgWork& w, const float* A, Schwarz* P, const double* b, double* x, double tol,
                         int64_t max_iter, int64_t* iterations);

@@ -14,6 +14,7 @@
 // kernels of the Backward-Euler Newton driver.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include <hip/hip_ext.h>
 
@@ -1001,6 +1002,143 @@ static hoefem_status pcg_solve_t(Model& m, CgWork& w, const VT* Avals, const VT*
     CG_CASE(16, 8, false, true) CG_CASE(16, 16, false, true) CG_CASE(16, 32, false, true)
     CG_CASE(32, 8, false, true) CG_CASE(32, 16, false, true) CG_CASE(32, 32, false, true)
 #undef CG_CASE
+    throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unsupported CG configuration");
+}
+
+__global__ void k_fill(int64_t n, double* __restrict__ a, double value);
+
+// ================================================================== allocation-free PCG
+// hoefem_pcg_noalloc runs the same kernels as pcg_solve with every work buffer (the interleaved CG
+// vectors, the Jacobi diagonal, the per-block partial sums and the iteration state) in `scratch`,
+// a fixed array in device memory that exists once per GPU for the life of the process. The host
+// side allocates nothing: it launches on the model stream and polls the status with a small copy
+// after each batch of iterations. Calls on one GPU share the array, so calls that overlap in time
+// on one GPU race (different models included).
+constexpr int64_t NOALLOC_MAX_DOFS = 262144;
+constexpr int64_t NOALLOC_VEC = 2 * NOALLOC_MAX_DOFS;  // one interleaved [z | p] or [r | Ap] buffer
+// layout in doubles: iteration state, the kernels' status word, partial sums, four CG vectors,
+// the Jacobi diagonal (every region starts on a 256-byte boundary)
+constexpr int64_t NA_SCALARS = 0, NA_STATUS = 32, NA_PART_PRE = 64;
+constexpr int64_t NA_PART_A = NA_PART_PRE + CG_MAX_BLOCKS, NA_PART_B = NA_PART_A + CG_MAX_BLOCKS;
+constexpr int64_t NA_VEC0 = NA_PART_B + 3 * CG_MAX_BLOCKS;
+constexpr int64_t NA_DIAG = NA_VEC0 + 4 * NOALLOC_VEC;
+constexpr int64_t NA_TOTAL = NA_DIAG + NOALLOC_MAX_DOFS;  // 9 * 262144 + 5184 doubles = 18.9 MB
+static_assert(sizeof(CgScalars) <= 32 * sizeof(double), "iteration state does not fit its region");
+static_assert(NA_PART_PRE % 32 == 0 && NA_VEC0 % 32 == 0 && NOALLOC_VEC % 32 == 0, "region alignment");
+
+static __device__ __align__(256) double scratch[NA_TOTAL];
+
+// Device address of `scratch` on the current device, looked up once per device (each GPU holds
+// its own copy of the array). The array's name is not a device address in host code.
+static double* scratch_address() {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static double* addr[MAX_DEVICES] = {};
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) throw Error(HOEFEM_ERROR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(HOEFEM_ERROR_DEVICE, "scratch array has no device address");
+        addr[dev] = (double*)p;
+    }
+    return addr[dev];
+}
+
+template <int GA, int GB, bool JAC>
+static hoefem_status cg_launch_noalloc(Model& m, double* base, const Csr3<double>& A, const Csr3<double>& Pm,
+                                       const double* b, double* x, double tol, int64_t max_iter,
+                                       int64_t* iterations) {
+    const hipStream_t s = m.stream;
+    const int64_t n = 3 * m.nv;
+    const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA, CG_THREADS_A);
+    const unsigned blkB = JAC ? cg_grid(0, Pm.nn, GB) : cg_grid(Pm.n_block, Pm.n_diag, GB);
+    double* zp[2] = {base + NA_VEC0, base + NA_VEC0 + NOALLOC_VEC};
+    double* ra[2] = {base + NA_VEC0 + 2 * NOALLOC_VEC, base + NA_VEC0 + 3 * NOALLOC_VEC};
+    CgScalars* sc = (CgScalars*)(base + NA_SCALARS);
+    CgPtrs P{};
+    P.x = x;
+    P.b = b;
+    P.dinv = base + NA_DIAG;
+    P.part_pre = base + NA_PART_PRE;
+    P.part_a = base + NA_PART_A;
+    P.part_b = base + NA_PART_B;
+    P.nb_a = (int)blkA;
+    P.nb_b = (int)blkB;
+    P.sc = sc;
+    P.host_status = (int*)(base + NA_STATUS);  // a device word here: the status is polled from sc
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, sc, tol, max_iter);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((2 * n + 255) / 256)), dim3(256), 0, s, 2 * n, zp[0], 0.0);  // p_{-1}
+    P.ra_new = ra[0];
+    k_cg_pre1<GA, double><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
+    P.ra_old = ra[0];
+    P.zp_new = zp[0];
+    k_cg_pre2<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+    check_launch("cg preamble");
+    int cur = 0, status = CG_RUNNING;
+    int64_t launched = 0;
+    const int64_t max_batches = max_iter >= 0 ? max_iter / CG_BATCH + 2 : INT64_MAX;
+    for (int64_t batch = 0; batch < max_batches && status == CG_RUNNING; ++batch) {
+        for (int q = 0; q < CG_BATCH; ++q) {
+            P.zp_old = zp[cur];
+            P.zp_new = zp[1 - cur];
+            P.ap_out = ra[cur];
+            P.ra_old = ra[cur];
+            P.ra_new = ra[1 - cur];
+            k_cg_a<GA, false, double><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(A, P);
+            k_cg_b<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+            cur ^= 1;
+        }
+        launched += CG_BATCH;
+        check_launch("cg iteration");
+        HIP_CHECK(hipMemcpyAsync(&status, &sc->status, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    CgScalars hs;
+    HIP_CHECK(hipMemcpyAsync(&hs, sc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    *iterations = hs.iters;
+    switch (hs.status) {
+        case CG_CONVERGED: return HOEFEM_OK;
+        case CG_ZERO_RHS:
+            hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, 0.0);
+            check_launch("k_fill");
+            return HOEFEM_OK;
+        case CG_RUNNING: return HOEFEM_ERROR_CG_MAX_ITERATIONS;
+        default: return (hoefem_status)hs.status;
+    }
+}
+
+hoefem_status pcg_solve_noalloc(Model& m, const double* Avals, Schwarz* Pre, const double* b, double* x, double tol,
+                                int64_t max_iter, int64_t* iterations) {
+    const NodePattern& pat = m.pattern;
+    *iterations = 0;
+    if (pat.ndof() > NOALLOC_MAX_DOFS)
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "hoefem_pcg_noalloc supports at most " +
+                                                       std::to_string(NOALLOC_MAX_DOFS) + " DOFs (model has " +
+                                                       std::to_string(pat.ndof()) + ")");
+    if (Pre && Pre->fp32) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "FP32 preconditioner with an FP64 operator");
+    if (pat.num_nodes == 0) return HOEFEM_OK;
+    double* base = scratch_address();
+    const int GA = spmv_group_size(pat);
+    const int GB = Pre ? spmv_group_size(Pre->P) : 8;
+    const Csr3<double> A = view(pat, Avals);
+    const Csr3<double> Pm = Pre ? view(Pre->P, (const double*)Pre->d_vals.get()) : A;
+    if (!Pre) {
+        k_jacobi_diag<double><<<dim3((unsigned)((pat.ndof() + 255) / 256)), dim3(256), 0, m.stream>>>(
+            pat.ndof(), pat.d_diag_pos.get(), Avals, base + NA_DIAG, m.d_err.get());
+        check_launch("k_jacobi_diag");
+        if (read_and_clear_err(m) & KERR_ZERO_DIAG) throw Error(HOEFEM_ERROR_SINGULAR, "Diagonal matrix is not invertible.");
+    }
+#define NA_CASE(ga, gb, j) \
+    if (GA == ga && GB == gb && (Pre == nullptr) == j) return cg_launch_noalloc<ga, gb, j>(m, base, A, Pm, b, x, tol, max_iter, iterations);
+    NA_CASE(8, 8, true) NA_CASE(16, 8, true) NA_CASE(32, 8, true)
+    NA_CASE(8, 8, false) NA_CASE(8, 16, false) NA_CASE(8, 32, false)
+    NA_CASE(16, 8, false) NA_CASE(16, 16, false) NA_CASE(16, 32, false)
+    NA_CASE(32, 8, false) NA_CASE(32, 16, false) NA_CASE(32, 32, false)
+#undef NA_CASE
     throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unsupported CG configuration");
 }
 

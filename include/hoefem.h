@@ -210,6 +210,17 @@ hoefem_status hoefem_schwarz_apply(hoefem_schwarz* s, const double* x, double* y
 hoefem_status hoefem_pcg(hoefem_model* model, const double* A_values, hoefem_schwarz* schwarz, const double* b,
                          double* x, double tolerance, int64_t max_iterations, int64_t* iterations);
 
+/* hoefem_pcg without allocation. No counterpart in the reference: for callers that cannot
+ * allocate device or pinned memory during a solve (serving loops, harnesses that forbid it, code
+ * that runs under stream capture nearby). Same kernels, arguments and results as hoefem_pcg; the
+ * work buffers live in a fixed per-GPU array of 18.9 MB held until the process ends, so the model
+ * may have at most 262144 DOFs (larger models: HOEFEM_ERROR_INVALID_ARGUMENT) and the model must use
+ * device pointers. Runs on the model's stream and synchronizes it. Calls that overlap in time on
+ * one GPU share that array and race; serialize them. FP64 operators only. */
+hoefem_status hoefem_pcg_noalloc(hoefem_model* model, const double* A_values, hoefem_schwarz* schwarz,
+                                 const double* b, double* x, double tolerance, int64_t max_iterations,
+                                 int64_t* iterations);
+
 /* SpMV tile autotuner (no counterpart in the reference, whose SpMV is MKL's inspector-executor
  * with set_mv_hint/optimize, newton_cg.rs:731-743): times the node-row-block kernel with 8, 16
  * and 32 lanes per node on this pattern and keeps the fastest for every later SpMV and CG
