@@ -342,8 +342,8 @@ def bench_learned(args):
 
 
 def bench_batched(args):
-    """Config 5: independent scenes of uneven size stepped together (one stream and one
-    captured CG batch graph per scene, host threads overlap the scenes)."""
+    """Config 5: independent scenes of uneven size stepped together by hoefem_batch_step (lockstep
+    Newton loops, one batched PCG per round with captured iteration batches)."""
     import numpy as np
     import torch
 
@@ -367,6 +367,7 @@ def bench_batched(args):
          "total_dofs": r["total_dofs"], "precision": "mixed" if args.mixed else "fp64", "parallelism": "single GPU"},
         batched={"p50_step_latency_ms": round(r["p50_step_latency_ms"], 3),
                  "p99_step_latency_ms": round(r["p99_step_latency_ms"], 3),
+                 "batch_step_ms": round(r["batch_step_ms"], 3),
                  "newton_iterations": r["newton_iterations"], "cg_iterations": r["cg_iterations"]},
         roofline=None, cpu_baseline=None)
     print(json.dumps(line))

@@ -60,6 +60,10 @@ EXPORTED = [
     "hoefem_sim_damping_values",
     "hoefem_sim_residual",
     "hoefem_sim_step",
+    "hoefem_sim_set_dirichlet_values",
+    "hoefem_batch_create",
+    "hoefem_batch_destroy",
+    "hoefem_batch_step",
     "hoefem_sim_update_damping",
     "hoefem_sim_eval_f",
     "hoefem_sim_apply_mass_add",
@@ -259,6 +263,12 @@ def load(path: str = LIB_PATH):
     L.hoefem_sim_damping_values.argtypes = [vp, vp]
     L.hoefem_sim_residual.argtypes = [vp, f64, f64, vp, vp, vp, vp]
     L.hoefem_sim_step.argtypes = [vp, f64, f64, vp, vp, C.POINTER(StepStats)]
+    L.hoefem_sim_set_dirichlet_values.argtypes = [vp, vp, vp]
+    L.hoefem_batch_create.argtypes = [C.POINTER(vp), i32, C.POINTER(vp)]
+    L.hoefem_batch_destroy.argtypes = [vp]
+    L.hoefem_batch_destroy.restype = None
+    L.hoefem_batch_step.argtypes = [vp, C.POINTER(f64), C.POINTER(vp), C.POINTER(vp), C.POINTER(StepStats),
+                                    C.POINTER(f64), C.POINTER(i32)]
     L.hoefem_sim_update_damping.argtypes = [vp, vp]
     L.hoefem_sim_eval_f.argtypes = [vp, f64, vp, vp, vp]
     L.hoefem_sim_apply_mass_add.argtypes = [vp, vp, vp]
@@ -282,7 +292,7 @@ def load(path: str = LIB_PATH):
     for name in EXPORTED:
         if name not in ("hoefem_last_error", "hoefem_nodes_per_element", "hoefem_lame_from_young_poisson",
                         "hoefem_model_destroy", "hoefem_model_ndof", "hoefem_model_nnz", "hoefem_model_num_colors",
-                        "hoefem_schwarz_destroy", "hoefem_schwarz_nnz", "hoefem_sim_destroy",
+                        "hoefem_schwarz_destroy", "hoefem_schwarz_nnz", "hoefem_sim_destroy", "hoefem_batch_destroy",
                         "hoefem_sim_representative_force", "hoefem_learned_param_count"):
             getattr(L, name).restype = C.c_int
     _lib = L

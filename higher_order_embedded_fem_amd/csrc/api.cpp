@@ -1,6 +1,7 @@
 // C ABI of the hoefem library: model setup, assembly entry points, Schwarz, PCG and the
 // Backward-Euler Newton driver (all device work is enqueued on the model's stream).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -10,18 +11,22 @@
 namespace hoefem {
 
 // solver.hip
-void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu);
-void launch_bc_copy(Model& m, const double* src, const uint8_t* dirmask, double* dst);
+void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, const double* bcu,
+                  double* uu);
+void launch_bc_copy(Model& m, const double* src, const uint8_t* dirmask, const double* bc, double* dst);
 void launch_node_force(Model& m, const double* gforce, const double* dv, const uint8_t* dirmask, double* f);
 void launch_scale(Model& m, const double* src, double sc, double* dst);
 void launch_mass_add(Model& m, const double* Mvals, const double* x, double* y, CgWork& cw, double* norm2);
 template <class T>
 void launch_comb(Model& m, int64_t n, T* out, double cd, const double* D, double cm, const double* M);
-void launch_state_v(Model& m, const double* vn, const double* w, const uint8_t* dirmask, double* vv);
+void launch_state_v(Model& m, const double* vn, const double* w, const uint8_t* dirmask, const double* bcv,
+                    double* vv);
 void launch_axpy_neg(Model& m, double* w, double delta, const double* sol);
 void launch_be_begin(Model& m, const double* u, const double* v, double dt, double* up, double* vn);
-void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask);
-void launch_zero_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask);
+void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask,
+                   const double* bcu, const double* bcv);
+void launch_set_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask, const double* bca,
+                          const double* bcb);
 void launch_finite_check(Model& m, const double* a, const double* b);
 void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n);
 void launch_f64_to_f32(Model& m, const double* a, float* b, int64_t n);
@@ -173,6 +178,11 @@ struct Sim {
     bool ap_has[3] = {false, false, false};
     double ap_coef[3] = {0.0, 0.0, 0.0};  // init_apply_jacobian_combination (alpha, beta, gamma)
     DevBuf<double> d_H;              // the combination applied by apply_jacobian_combination
+    // Dirichlet values (DOF vectors, zero off the Dirichlet DOFs); empty: homogeneous conditions
+    DevBuf<double> d_bcu, d_bcv;
+    bool has_bcu = false, has_bcv = false;
+    const double* bcu() const { return has_bcu ? d_bcu.get() : nullptr; }
+    const double* bcv() const { return has_bcv ? d_bcv.get() : nullptr; }
 };
 
 static void build_bc(Model& m, const std::vector<int32_t>& dir, DevBuf<uint8_t>& dmask, DevBuf<int32_t>& dbc,
@@ -272,16 +282,21 @@ static void sim_eval_f(Sim& s, const double* uu, const double* vv, double* f) {
 // F(w) = -dt f(u_p + dt w, v_n + w) + M w and |F|^2 for the current (u_p, v_n) held in the sim
 // (BackwardEulerImplicitFunction::eval_into, hamilton2/src/integrators/euler.rs:97-114), composed of
 // the same kernels as the dynamic-system entry points eval_f and apply_mass_add
-static double sim_eval(Sim& s, double dt, const double* w) {
+// (sim_eval_launch enqueues it with the readback of |F|^2 and the error flags; sim_eval_finish reads
+// them once the model stream has been synchronized)
+static void sim_eval_launch(Sim& s, double dt, const double* w) {
     Model& m = *s.model;
-    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
-    if (s.damping) launch_state_v(m, s.d_vn.get(), w, s.d_dirmask.get(), s.d_vv.get());
+    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.bcu(), s.d_uu.get());
+    if (s.damping) launch_state_v(m, s.d_vn.get(), w, s.d_dirmask.get(), s.bcv(), s.d_vv.get());
     sim_eval_f(s, s.d_uu.get(), s.d_vv.get(), s.d_f.get());
     launch_scale(m, s.d_f.get(), dt * -1.0, s.d_F.get());
     launch_mass_add(m, s.d_M.get(), w, s.d_F.get(), s.cg, s.d_norm2.get());
     HIP_CHECK(hipMemcpyAsync(m.h_err.p, m.d_err.get(), sizeof(int), hipMemcpyDeviceToHost, m.stream));
     HIP_CHECK(hipMemcpyAsync(s.h_norm2.p, s.d_norm2.get(), sizeof(double), hipMemcpyDeviceToHost, m.stream));
-    HIP_CHECK(hipStreamSynchronize(m.stream));
+}
+
+static double sim_eval_finish(Sim& s) {
+    Model& m = *s.model;
     s.evals++;
     if (*m.h_err.p) {
         s.last_err_flags |= *m.h_err.p;
@@ -291,12 +306,20 @@ static double sim_eval(Sim& s, double dt, const double* w) {
     return *s.h_norm2.p;
 }
 
+static double sim_eval(Sim& s, double dt, const double* w) {
+    sim_eval_launch(s, dt, w);
+    HIP_CHECK(hipStreamSynchronize(s.model->stream));
+    return sim_eval_finish(s);
+}
+
 // compute_jacobian_combination_into (simulation_toolbox/src/fem/system_assembly.rs:196-235) at the
 // state uu (Dirichlet values applied): J = M - alpha K(uu) - beta D, a term left out when its
 // coefficient is None, then the Dirichlet conditions and SchwarzPreconditioner::update
 // (newton_cg.rs:700-728). K is the unprojected stiffness. Mixed precision: mode 1 builds J in FP32
 // from FP32 element tangents; mode 2 builds it in FP64 and keeps an FP32 copy for the CG.
-static hoefem_status sim_assemble_jacobian(Sim& s, bool has_a, double a, bool has_b, double b, const double* uu) {
+// (sim_assemble_jacobian_launch enqueues it with the readback of the error flags, and
+// sim_assemble_jacobian_finish reads them once the model stream has been synchronized)
+static void sim_assemble_jacobian_launch(Sim& s, bool has_a, double a, bool has_b, double b, const double* uu) {
     Model& m = *s.model;
     const int64_t nnz = m.pattern.nnz();
     const double* D = (has_b && s.damping) ? s.d_D.get() : nullptr;
@@ -329,12 +352,24 @@ static hoefem_status sim_assemble_jacobian(Sim& s, bool has_a, double a, bool ha
         if (s.mixed_mode == 2) launch_f64_to_f32(m, s.d_J.get(), s.d_J32.get(), nnz);
         if (s.schwarz) schwarz_update(*s.schwarz, s.d_J.get());
     }
-    const int e = read_and_clear_err(m);
+    HIP_CHECK(hipMemcpyAsync(m.h_err.p, m.d_err.get(), sizeof(int), hipMemcpyDeviceToHost, m.stream));
+}
+
+static hoefem_status sim_assemble_jacobian_finish(Sim& s) {
+    Model& m = *s.model;
+    const int e = *m.h_err.p;
     if (e) {
+        HIP_CHECK(hipMemsetAsync(m.d_err.get(), 0, sizeof(int), m.stream));
         s.last_err_flags |= e;
         return HOEFEM_ERROR_SINGULAR;
     }
     return HOEFEM_OK;
+}
+
+static hoefem_status sim_assemble_jacobian(Sim& s, bool has_a, double a, bool has_b, double b, const double* uu) {
+    sim_assemble_jacobian_launch(s, has_a, a, has_b, b, uu);
+    HIP_CHECK(hipStreamSynchronize(s.model->stream));
+    return sim_assemble_jacobian_finish(s);
 }
 
 // The CG part of NewtonCgDynamicSystem::solve_jacobian_combination (newton_cg.rs:730-801): PCG from
@@ -426,7 +461,7 @@ static hoefem_status sim_cg(Sim& s, const double* rhs, double* sol, int64_t* ite
 // euler.rs:122-140), then the CG.
 static hoefem_status sim_solve(Sim& s, double dt, const double* w, const double* rhs, double* sol) {
     Model& m = *s.model;
-    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.d_uu.get());
+    launch_state(m, s.d_up.get(), w, dt, s.d_dirmask.get(), s.bcu(), s.d_uu.get());
     if (sim_assemble_jacobian(s, true, -dt * dt, true, -dt, s.d_uu.get()) != HOEFEM_OK) return HOEFEM_ERROR_SINGULAR;
     int64_t it = 0;
     return sim_cg(s, rhs, sol, &it);
@@ -513,59 +548,332 @@ static hoefem_status sim_be_step(Sim& s, double dt, double* u, double* v, int64_
         ++it;
     }
     *newton_iters = it;
-    launch_be_end(m, u, v, s.d_w.get(), dt, s.d_dirmask.get());
+    launch_be_end(m, u, v, s.d_w.get(), dt, s.d_dirmask.get(), s.bcu(), s.bcv());
     return HOEFEM_OK;
 }
 
-// integrate_model_with_substeps (simulation_toolbox/src/fem/newton_cg.rs:146-228)
-static hoefem_status sim_step(Sim& s, double t, double dt, double* u, double* v, hoefem_step_stats* stats) {
+// integrate_model_with_substeps (simulation_toolbox/src/fem/newton_cg.rs:146-228), in parts: the
+// step's start (counters, u and v with the Dirichlet DOFs zeroed copied to the working state), the
+// attempts from `num_substeps` substeps on, and the end (finite check, working state back to u, v).
+static void sim_step_begin(Sim& s, double* u, double* v, hoefem_step_stats& st) {
     Model& m = *s.model;
     const int64_t n = 3 * m.nv;
-    hoefem_step_stats local{};
-    hoefem_step_stats& st = stats ? *stats : local;
     st = hoefem_step_stats{};
     s.cg_total = 0;
     s.evals = 0;
     s.last_err_flags = 0;
-    launch_zero_dirichlet(m, u, v, s.d_dirmask.get());
+    launch_set_dirichlet(m, u, v, s.d_dirmask.get(), s.bcu(), s.bcv());
     HIP_CHECK(hipMemcpyAsync(s.d_us.get(), u, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
     HIP_CHECK(hipMemcpyAsync(s.d_vs.get(), v, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+}
+
+// the substep count after an attempt with num_substeps failed (last: its error); throws when it
+// reaches the limit, else restores the working state for the next attempt
+static int sim_next_substeps(Sim& s, const double* u, const double* v, hoefem_step_stats& st, int num_substeps,
+                             hoefem_status last) {
+    Model& m = *s.model;
+    const int64_t n = 3 * m.nv;
     const int max_substeps = 1000, factor = 5;
-    int num_substeps = 1;
-    hoefem_status last = HOEFEM_OK;
+    num_substeps *= factor;
+    if (num_substeps >= max_substeps) {
+        st.cg_iterations = s.cg_total;
+        st.residual_evaluations = s.evals;
+        throw Error(HOEFEM_ERROR_SUBSTEPPING, "Substepping failed; last error " + std::to_string((int)last));
+    }
+    HIP_CHECK(hipMemcpyAsync(s.d_us.get(), u, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+    HIP_CHECK(hipMemcpyAsync(s.d_vs.get(), v, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+    return num_substeps;
+}
+
+static void sim_substep_loop(Sim& s, double dt, const double* u, const double* v, hoefem_step_stats& st,
+                             int num_substeps) {
     while (true) {
         const double sdt = dt / (double)num_substeps;
-        bool failed = false;
-        for (int k = 0; k < num_substeps; ++k) {
-            (void)t;
+        hoefem_status last = HOEFEM_OK;
+        for (int k = 0; k < num_substeps && last == HOEFEM_OK; ++k) {
             int64_t it = 0;
             last = sim_be_step(s, sdt, s.d_us.get(), s.d_vs.get(), &it, &st.final_residual, &st.tolerance);
             st.newton_iterations += it;
-            if (last != HOEFEM_OK) {
-                failed = true;
-                break;
-            }
         }
-        if (!failed) break;
-        num_substeps *= factor;
-        if (num_substeps >= max_substeps) {
-            st.cg_iterations = s.cg_total;
-            st.residual_evaluations = s.evals;
-            throw Error(HOEFEM_ERROR_SUBSTEPPING, "Substepping failed; last error " + std::to_string((int)last));
-        }
-        HIP_CHECK(hipMemcpyAsync(s.d_us.get(), u, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
-        HIP_CHECK(hipMemcpyAsync(s.d_vs.get(), v, sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+        if (last == HOEFEM_OK) break;
+        num_substeps = sim_next_substeps(s, u, v, st, num_substeps, last);
     }
     st.substeps = num_substeps;
+}
+
+static void sim_step_end_launch(Sim& s, hoefem_step_stats& st) {
+    Model& m = *s.model;
     st.cg_iterations = s.cg_total;
     st.residual_evaluations = s.evals;
     launch_finite_check(m, s.d_us.get(), s.d_vs.get());
-    const int e = read_and_clear_err(m);
+    HIP_CHECK(hipMemcpyAsync(m.h_err.p, m.d_err.get(), sizeof(int), hipMemcpyDeviceToHost, m.stream));
+}
+
+// after the model stream has been synchronized: the finite check's verdict, then the copy back
+static void sim_step_end_finish(Sim& s, double* u, double* v) {
+    Model& m = *s.model;
+    const int64_t n = 3 * m.nv;
+    const int e = *m.h_err.p;
+    if (e) HIP_CHECK(hipMemsetAsync(m.d_err.get(), 0, sizeof(int), m.stream));
     if (e & KERR_NONFINITE) throw Error(HOEFEM_ERROR_NOT_FINITE, "non-finite state after step");
     HIP_CHECK(hipMemcpyAsync(u, s.d_us.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
     HIP_CHECK(hipMemcpyAsync(v, s.d_vs.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, m.stream));
+}
+
+static hoefem_status sim_step(Sim& s, double t, double dt, double* u, double* v, hoefem_step_stats* stats) {
+    Model& m = *s.model;
+    hoefem_step_stats local{};
+    hoefem_step_stats& st = stats ? *stats : local;
+    (void)t;
+    sim_step_begin(s, u, v, st);
+    sim_substep_loop(s, dt, u, v, st, 1);
+    sim_step_end_launch(s, st);
+    HIP_CHECK(hipStreamSynchronize(m.stream));
+    sim_step_end_finish(s, u, v);
     HIP_CHECK(hipStreamSynchronize(m.stream));
     return HOEFEM_OK;
+}
+
+// ------------------------------------------------------------------ batched scenes
+// Independent scenes advanced by one time step together. Every scene runs the kernels of sim_step
+// in the same order with the same arguments; the batch changes only how they are issued: one host
+// thread walks all scenes' Newton loops in lockstep (residual evaluations, Jacobian assemblies and
+// line-search trials of all scenes are enqueued before one synchronization), and the linear solves
+// of one Newton round run as one batched PCG (pcg_solve_batch), in which each scene stops on the
+// device when it converges. Each scene's u and v are therefore the ones sim_step gives it, bit for
+// bit. A scene whose single-step attempt fails continues alone with the substepping of sim_step.
+struct Batch {
+    std::vector<Sim*> sims;
+    CgBatch cg;
+    std::vector<hipStream_t> streams;  // the distinct model streams
+};
+
+namespace {
+enum BatchPhase { BP_NEWTON, BP_SOLVE, BP_LINESEARCH, BP_DONE, BP_FAILED, BP_ERROR };
+struct BatchScene {
+    Sim* s = nullptr;
+    double dt = 0.0;
+    double* u = nullptr;
+    double* v = nullptr;
+    hoefem_step_stats st{};
+    BatchPhase phase = BP_NEWTON;
+    hoefem_status status = HOEFEM_OK;  // BP_FAILED: the attempt's error; BP_ERROR: the step's
+    std::string message;
+    double n2 = 0.0, tol = 0.0, g_initial = 0.0, alpha = 1.0, alpha_prev = 0.0, tail = 0.25;
+    int idx = 0;
+    int64_t it = 0;
+    double latency_ms = 0.0;
+};
+}  // namespace
+
+static void batch_sync(Batch& B) {
+    for (hipStream_t st : B.streams) HIP_CHECK(hipStreamSynchronize(st));
+}
+
+static void batch_step(Batch& B, const double* dt, double* const* u, double* const* v, hoefem_step_stats* stats,
+                       double* latency_ms, int32_t* status) {
+    const size_t ns = B.sims.size();
+    B.streams.clear();
+    for (Sim* s : B.sims) {
+        if (s->learned) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "a batched scene has a learned correction attached");
+        if (std::find(B.streams.begin(), B.streams.end(), s->model->stream) == B.streams.end())
+            B.streams.push_back(s->model->stream);
+    }
+    std::vector<BatchScene> q(ns);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed_ms = [&]() {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    auto fail = [&](BatchScene& b, hoefem_status e) {
+        b.status = e;
+        b.phase = BP_FAILED;
+        b.st.newton_iterations += b.it;
+    };
+    auto error = [&](BatchScene& b, const Error& e) {
+        b.status = e.code;
+        b.message = e.what();
+        b.phase = BP_ERROR;
+        b.latency_ms = elapsed_ms();
+    };
+    // start of sim_step and of sim_be_step for one substep of dt
+    for (size_t i = 0; i < ns; ++i) {
+        BatchScene& b = q[i];
+        b.s = B.sims[i];
+        b.dt = dt[i];
+        b.u = u[i];
+        b.v = v[i];
+        Sim& s = *b.s;
+        Model& m = *s.model;
+        if (!b.u || !b.v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null state vector");
+        sim_step_begin(s, b.u, b.v, b.st);
+        if (s.damping) {
+            const hoefem_status ds = sim_damping(s, s.d_us.get());
+            if (ds != HOEFEM_OK) {
+                fail(b, ds);
+                continue;
+            }
+        }
+        launch_be_begin(m, s.d_us.get(), s.d_vs.get(), b.dt, s.d_up.get(), s.d_vn.get());
+        HIP_CHECK(hipMemsetAsync(s.d_w.get(), 0, sizeof(double) * 3 * m.nv, m.stream));
+        b.tol = b.dt * (s.d.newton_tolerance_factor * s.rep_force);
+        b.st.tolerance = b.tol;
+        sim_eval_launch(s, b.dt, s.d_w.get());
+    }
+    batch_sync(B);
+    for (BatchScene& b : q)
+        if (b.phase == BP_NEWTON) b.n2 = sim_eval_finish(*b.s);
+    std::vector<CgBatchSystem> sys;
+    std::vector<BatchScene*> solving;
+    const bool fp32 = B.sims.empty() ? false : B.sims[0]->mixed_mode == 1;
+    while (true) {
+        // Newton tests of every scene whose last step was accepted (newton.rs:76-131)
+        bool any = false;
+        for (BatchScene& b : q) {
+            if (b.phase != BP_NEWTON) continue;
+            Sim& s = *b.s;
+            const double fnorm = std::sqrt(b.n2);
+            b.st.final_residual = fnorm;
+            if (!(fnorm > b.tol)) {
+                b.phase = BP_DONE;
+                b.st.newton_iterations += b.it;
+                b.st.substeps = 1;
+                launch_be_end(*s.model, s.d_us.get(), s.d_vs.get(), s.d_w.get(), b.dt, s.d_dirmask.get(), s.bcu(),
+                              s.bcv());
+                b.latency_ms = elapsed_ms();
+                continue;
+            }
+            if (s.d.max_newton_iter >= 0 && b.it == s.d.max_newton_iter) {
+                fail(b, HOEFEM_ERROR_NEWTON_MAX_ITERATIONS);
+                continue;
+            }
+            b.phase = BP_SOLVE;
+            any = true;
+        }
+        if (!any) break;
+        // J (-dx) = F: the Jacobians, then one batched solve (sim_solve)
+        for (BatchScene& b : q) {
+            if (b.phase != BP_SOLVE) continue;
+            Sim& s = *b.s;
+            launch_state(*s.model, s.d_up.get(), s.d_w.get(), b.dt, s.d_dirmask.get(), s.bcu(), s.d_uu.get());
+            sim_assemble_jacobian_launch(s, true, -b.dt * b.dt, true, -b.dt, s.d_uu.get());
+        }
+        batch_sync(B);
+        sys.clear();
+        solving.clear();
+        for (BatchScene& b : q) {
+            if (b.phase != BP_SOLVE) continue;
+            Sim& s = *b.s;
+            if (sim_assemble_jacobian_finish(s) != HOEFEM_OK) {
+                fail(b, HOEFEM_ERROR_JACOBIAN);
+                continue;
+            }
+            CgBatchSystem c;
+            c.m = s.model;
+            c.w = &s.cg;
+            c.A = fp32 ? (const void*)s.d_J32.get() : (const void*)s.d_J.get();
+            c.P = s.schwarz.get();
+            c.b = s.d_F.get();
+            c.x = s.d_sol.get();
+            c.tol = s.d.cg_tolerance;
+            c.max_iter = s.d.cg_max_iter;
+            sys.push_back(c);
+            solving.push_back(&b);
+        }
+        pcg_solve_batch(B.cg, sys, fp32);
+        for (size_t k = 0; k < sys.size(); ++k) {
+            BatchScene& b = *solving[k];
+            b.s->cg_total += sys[k].iterations;
+            hoefem_status cs = sys[k].status;
+            if (cs == HOEFEM_ERROR_SINGULAR) {  // zero Jacobi diagonal: pcg_solve throws this
+                error(b, Error(HOEFEM_ERROR_SINGULAR, "Diagonal matrix is not invertible."));
+                continue;
+            }
+            if (cs == HOEFEM_ERROR_CG_MAX_ITERATIONS || cs == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) cs = HOEFEM_OK;
+            if (cs != HOEFEM_OK) {
+                fail(b, HOEFEM_ERROR_JACOBIAN);
+                continue;
+            }
+            // backtracking on g = 0.5 |F|^2 along p = -sol (BacktrackingLineSearch, newton.rs:171-246)
+            b.phase = BP_LINESEARCH;
+            b.g_initial = 0.5 * b.n2;
+            b.alpha_prev = 0.0;
+            b.alpha = 1.0;
+            b.tail = 0.25;
+            b.idx = 0;
+        }
+        while (true) {
+            bool trial = false;
+            for (BatchScene& b : q) {
+                if (b.phase != BP_LINESEARCH) continue;
+                Sim& s = *b.s;
+                launch_axpy_neg(*s.model, s.d_w.get(), b.alpha - b.alpha_prev, s.d_sol.get());
+                sim_eval_launch(s, b.dt, s.d_w.get());
+                trial = true;
+            }
+            if (!trial) break;
+            batch_sync(B);
+            const double c = 1e-4, alpha_min = 1e-6;
+            const double initial[3] = {1.0, 0.75, 0.5};
+            for (BatchScene& b : q) {
+                if (b.phase != BP_LINESEARCH) continue;
+                b.n2 = sim_eval_finish(*b.s);
+                const double g = 0.5 * b.n2;
+                if (g <= (1.0 - c * b.alpha) * b.g_initial) {
+                    ++b.it;
+                    b.phase = BP_NEWTON;
+                    continue;
+                }
+                if (b.alpha < alpha_min) {
+                    fail(b, HOEFEM_ERROR_LINE_SEARCH);
+                    continue;
+                }
+                b.alpha_prev = b.alpha;
+                ++b.idx;
+                if (b.idx < 3) {
+                    b.alpha = initial[b.idx];
+                } else {
+                    b.alpha = b.tail;
+                    b.tail *= 0.25;
+                }
+            }
+        }
+    }
+    // scenes whose attempt failed: the rest of sim_step's substepping, one scene at a time
+    for (BatchScene& b : q) {
+        if (b.phase != BP_FAILED) continue;
+        Sim& s = *b.s;
+        try {
+            const int next = sim_next_substeps(s, b.u, b.v, b.st, 1, b.status);
+            sim_substep_loop(s, b.dt, b.u, b.v, b.st, next);
+            b.phase = BP_DONE;
+            b.latency_ms = elapsed_ms();
+        } catch (const Error& e) {
+            error(b, e);
+        }
+    }
+    for (BatchScene& b : q)
+        if (b.phase == BP_DONE) sim_step_end_launch(*b.s, b.st);
+    batch_sync(B);
+    for (BatchScene& b : q) {
+        if (b.phase != BP_DONE) continue;
+        try {
+            sim_step_end_finish(*b.s, b.u, b.v);
+        } catch (const Error& e) {
+            error(b, e);
+        }
+    }
+    batch_sync(B);
+    const BatchScene* first_error = nullptr;
+    for (size_t i = 0; i < ns; ++i) {
+        BatchScene& b = q[i];
+        if (b.phase == BP_DONE) b.status = HOEFEM_OK;
+        if (stats) stats[i] = b.st;
+        if (latency_ms) latency_ms[i] = b.latency_ms;
+        if (status) status[i] = (int32_t)b.status;
+        if (b.phase == BP_ERROR && !first_error) first_error = &b;
+    }
+    if (first_error)
+        throw Error(first_error->status, "scene " + std::to_string(first_error - q.data()) + ": " + first_error->message);
 }
 
 }  // namespace hoefem
@@ -1179,6 +1487,64 @@ hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const do
     });
 }
 
+hoefem_status hoefem_sim_set_dirichlet_values(hoefem_sim* sim, const double* u_values, const double* v_values) {
+    return guarded([&] {
+        if (!sim) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+        Sim& s = sim->s;
+        Model& m = *s.model;
+        const int64_t n = 3 * m.nv;
+        auto upload = [&](const double* vals, DevBuf<double>& buf, bool& has) {
+            has = vals != nullptr;
+            if (!has) return;
+            std::vector<double> full((size_t)n, 0.0);
+            for (size_t k = 0; k < s.dir.size(); ++k)
+                for (int i = 0; i < 3; ++i) {
+                    const double x = vals[3 * k + i];
+                    if (!std::isfinite(x)) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "non-finite Dirichlet value");
+                    full[3 * (size_t)s.dir[k] + i] = x;
+                }
+            buf.upload(full, m.stream);
+        };
+        upload(u_values, s.d_bcu, s.has_bcu);
+        upload(v_values, s.d_bcv, s.has_bcv);
+    });
+}
+
+struct hoefem_batch {
+    Batch b;
+};
+
+hoefem_status hoefem_batch_create(hoefem_sim* const* sims, int32_t count, hoefem_batch** out) {
+    return guarded([&] {
+        if (!sims || !out || count < 1) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "a batch needs at least one sim");
+        std::unique_ptr<hoefem_batch> h(new hoefem_batch());
+        for (int32_t i = 0; i < count; ++i) {
+            if (!sims[i]) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null sim");
+            Sim* s = &sims[i]->s;
+            if (std::find(h->b.sims.begin(), h->b.sims.end(), s) != h->b.sims.end())
+                throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "a sim appears twice in the batch");
+            if (s->model->pointer_mode != HOEFEM_POINTERS_DEVICE)
+                throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "batched scenes take device pointers");
+            if (s->mixed_mode == 2)
+                throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "batched scenes run the FP64 or the FP32-operator step");
+            if (i > 0 && s->mixed_mode != h->b.sims[0]->mixed_mode)
+                throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "batched scenes share one precision mode");
+            h->b.sims.push_back(s);
+        }
+        *out = h.release();
+    });
+}
+
+void hoefem_batch_destroy(hoefem_batch* batch) { delete batch; }
+
+hoefem_status hoefem_batch_step(hoefem_batch* batch, const double* dt, double* const* u, double* const* v,
+                                hoefem_step_stats* stats, double* latency_ms, int32_t* status) {
+    return guarded([&] {
+        if (!batch || !dt || !u || !v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
+        batch_step(batch->b, dt, u, v, stats, latency_ms, status);
+    });
+}
+
 hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, double* v, hoefem_step_stats* stats) {
     return guarded([&] {
         if (!sim || !u || !v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
@@ -1206,7 +1572,7 @@ hoefem_status hoefem_sim_update_damping(hoefem_sim* sim, const double* u) {
         if (!u) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
         if (!s.damping) return;
         CallIo io(m);
-        launch_bc_copy(m, io.in(u, 3 * (size_t)m.nv), s.d_dirmask.get(), s.d_uu.get());
+        launch_bc_copy(m, io.in(u, 3 * (size_t)m.nv), s.d_dirmask.get(), s.bcu(), s.d_uu.get());
         if (sim_damping(s, s.d_uu.get()) != HOEFEM_OK)
             throw Error(HOEFEM_ERROR_SINGULAR, "singular deformation gradient in the damping stiffness");
         s.solve_outdated = s.apply_outdated = true;
@@ -1226,8 +1592,8 @@ hoefem_status hoefem_sim_eval_f(hoefem_sim* sim, double t, const double* u, cons
         const double* dv = io.in(v, n);
         double* df = io.out(f, n);
         // the reference applies the boundary conditions to copies of u and v (newton_cg.rs:449-455)
-        launch_bc_copy(m, du, s.d_dirmask.get(), s.d_uu.get());
-        launch_bc_copy(m, dv, s.d_dirmask.get(), s.d_vv.get());
+        launch_bc_copy(m, du, s.d_dirmask.get(), s.bcu(), s.d_uu.get());
+        launch_bc_copy(m, dv, s.d_dirmask.get(), s.bcv(), s.d_vv.get());
         sim_eval_f(s, s.d_uu.get(), s.d_vv.get(), df);
         raise_kernel_errors(m);
         io.finish();
@@ -1256,8 +1622,8 @@ hoefem_status hoefem_sim_set_state(hoefem_sim* sim, double t, const double* u, c
         if (!u || !v) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "null argument");
         CallIo io(m);
         const size_t n = 3 * (size_t)m.nv;
-        launch_bc_copy(m, io.in(u, n), s.d_dirmask.get(), s.d_su.get());
-        launch_bc_copy(m, io.in(v, n), s.d_dirmask.get(), s.d_sv.get());
+        launch_bc_copy(m, io.in(u, n), s.d_dirmask.get(), s.bcu(), s.d_su.get());
+        launch_bc_copy(m, io.in(v, n), s.d_dirmask.get(), s.bcv(), s.d_sv.get());
         s.state_set = true;
         s.has_alpha = s.has_beta = false;
         s.solve_outdated = s.apply_outdated = true;

@@ -582,13 +582,18 @@ __global__ __launch_bounds__(256) void k_element_mass(int64_t ne, const int32_t*
 // fenris/src/assembly.rs:1042-1075, without the linear search: slots are precomputed).
 // mode 0: values = (values or 0) + sum; mode 1: values = (0 + c * sum) + 1 * M (J combination,
 // simulation_toolbox/src/fem/system_assembly.rs:196-235).
+// A row block wider than `chunk` neighbour nodes (large Schwarz subdomains) is done in column
+// chunks of `chunk` nodes, each a pass over the incident blocks; every entry still receives its
+// terms in color order, so the sums do not depend on the chunking.
+constexpr int GATHER_CHUNK = 896;  // 9 * 896 doubles = 63 KiB of LDS
 template <class TB, class TV>
 __global__ __launch_bounds__(64) void k_block_gather(
     const int32_t* __restrict__ nptr, const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ diag_only,
     const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk, const int32_t* __restrict__ inc_loc,
     const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off, const int64_t* __restrict__ slot_off,
     const int32_t* __restrict__ slot, const TB* __restrict__ blocks, TV* __restrict__ values, int mode,
-    double c, const double* __restrict__ M, int zero_first, const double* __restrict__ D2, double cd, double cm) {
+    double c, const double* __restrict__ M, int zero_first, const double* __restrict__ D2, double cd, double cm,
+    int chunk) {
     extern __shared__ double s_rb[];
     const int64_t v = blockIdx.x;
     const int t = threadIdx.x;
@@ -596,37 +601,48 @@ __global__ __launch_bounds__(64) void k_block_gather(
     const int k = nptr[v + 1] - nptr[v];
     if (k == 0) return;
     const int rw = 3 * k;
-    const int R = 9 * k;
     const int64_t r0 = rowptr[3 * v];
     // rows 3v, 3v+1, 3v+2 are contiguous: row i starts at r0 + i * rw
-    for (int idx = t; idx < R; idx += 64) s_rb[idx] = (zero_first || mode == 1) ? 0.0 : (double)values[r0 + idx];
-    __syncthreads();
-    for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
-        const int b = inc_blk[it], a = inc_loc[it];
-        const int mb = msz[b];
-        const int bw = 3 * mb;
-        const TB* base = blocks + val_off[b] + (int64_t)(3 * a) * bw;
-        const int32_t* sl = slot + slot_off[b] + (int64_t)a * mb;
-        for (int idx = t; idx < 3 * bw; idx += 64) {
-            const int i = idx / bw, cc = idx % bw;
-            const int bb = cc / 3, jj = cc % 3;
-            s_rb[i * rw + 3 * sl[bb] + jj] += (double)base[idx];
+    for (int c0 = 0; c0 < k; c0 += chunk) {
+        const int cw = min(chunk, k - c0);
+        const int cr = 3 * cw;  // a row's width within the chunk
+        const int R = 3 * cr;
+        for (int idx = t; idx < R; idx += 64) {
+            const int i = idx / cr, cc = idx % cr;
+            s_rb[idx] = (zero_first || mode == 1) ? 0.0 : (double)values[r0 + i * rw + 3 * c0 + cc];
         }
         __syncthreads();
-    }
-    for (int idx = t; idx < R; idx += 64) {
-        const double s = s_rb[idx];
-        if (mode == 0)
-            values[r0 + idx] = (TV)s;
-        else {
-            // compute_jacobian_combination_into (system_assembly.rs:197-235): the terms in the order
-            // stiffness, damping, mass
-            double v2 = 0.0;
-            v2 += c * s;
-            if (D2) v2 += cd * D2[r0 + idx];
-            if (M) v2 += cm * M[r0 + idx];
-            values[r0 + idx] = (TV)v2;
+        for (int it = inc_ptr[v]; it < inc_ptr[v + 1]; ++it) {
+            const int b = inc_blk[it], a = inc_loc[it];
+            const int mb = msz[b];
+            const int bw = 3 * mb;
+            const TB* base = blocks + val_off[b] + (int64_t)(3 * a) * bw;
+            const int32_t* sl = slot + slot_off[b] + (int64_t)a * mb;
+            for (int idx = t; idx < 3 * bw; idx += 64) {
+                const int i = idx / bw, cc = idx % bw;
+                const int bb = cc / 3, jj = cc % 3;
+                const int col = sl[bb] - c0;
+                if (col >= 0 && col < cw) s_rb[i * cr + 3 * col + jj] += (double)base[idx];
+            }
+            __syncthreads();
         }
+        for (int idx = t; idx < R; idx += 64) {
+            const int i = idx / cr, cc = idx % cr;
+            const int64_t o = r0 + i * rw + 3 * c0 + cc;
+            const double s = s_rb[idx];
+            if (mode == 0)
+                values[o] = (TV)s;
+            else {
+                // compute_jacobian_combination_into (system_assembly.rs:197-235): the terms in the order
+                // stiffness, damping, mass
+                double v2 = 0.0;
+                v2 += c * s;
+                if (D2) v2 += cd * D2[o];
+                if (M) v2 += cm * M[o];
+                values[o] = (TV)v2;
+            }
+        }
+        __syncthreads();  // the next chunk reuses s_rb
     }
 }
 
@@ -875,14 +891,14 @@ void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const T
                          double c, const double* M, bool zero_first, hipStream_t s, const double* D2, double cd,
                          double cm, KernelProf* prof) {
     if (pat.num_nodes == 0) return;
-    const size_t lds = (size_t)9 * std::max(pat.max_nbr, 1) * sizeof(double);
-    if (lds > 64 * 1024) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "node row block exceeds 64 KiB of LDS");
+    const int chunk = std::min(std::max(pat.max_nbr, 1), GATHER_CHUNK);
+    const size_t lds = (size_t)9 * chunk * sizeof(double);
     launch_timed(prof, k_block_gather<TB, TV>, dim3((unsigned)pat.num_nodes), dim3(64), (uint32_t)lds, s,
                  (const int32_t*)pat.d_nptr.get(), (const int32_t*)pat.d_rowptr.get(),
                  (const uint8_t*)pat.d_diag_only.get(), (const int32_t*)sc.d_inc_ptr.get(),
                  (const int32_t*)sc.d_inc_blk.get(), (const int32_t*)sc.d_inc_loc.get(), (const int32_t*)sc.d_m.get(),
                  (const int64_t*)sc.d_val_off.get(), (const int64_t*)sc.d_slot_off.get(),
-                 (const int32_t*)sc.d_slot.get(), blocks, values, mode, c, M, zero_first ? 1 : 0, D2, cd, cm);
+                 (const int32_t*)sc.d_slot.get(), blocks, values, mode, c, M, zero_first ? 1 : 0, D2, cd, cm, chunk);
     check_launch("k_block_gather");
 }
 template void launch_block_gather<double, double>(const NodePattern&, const BlockScatter&, const double*,

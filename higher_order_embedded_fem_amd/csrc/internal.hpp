@@ -265,6 +265,10 @@ struct CgWork {
     std::vector<const void*> batch_key;
     bool use_graphs = true;
     PinnedBuf<CgScalars> h_scalars{3};  // final readback (slot 2)
+    // the solve's segment descriptor (CgSeg, solver.hip): pinned staging and its device copy
+    static constexpr size_t SEG_BYTES = 512;
+    PinnedBuf<char> h_seg{SEG_BYTES};
+    DevBuf<char> d_seg;
     PinnedBuf<int> h_status{1, hipHostMallocCoherent};  // the solve's status, written by the kernel that ends it
     hipEvent_t ev[2] = {nullptr, nullptr};
     void ensure(int64_t n, int nblocks);
@@ -332,6 +336,51 @@ hoefem_status pcg_solve_noalloc(Model& m, const double* A, Schwarz* P, const dou
 // mixed precision: FP32 system matrix and FP32 Schwarz operator (P->fp32 must be set), FP64 vectors / dots
 hoefem_status pcg_solve(Model& m, CgWork& w, const float* A, Schwarz* P, const double* b, double* x, double tol,
                         int64_t max_iter, int64_t* iterations);
+
+// One system of a batched solve: the arguments of pcg_solve (A of value type double, or float
+// with an FP32 Schwarz operator) and its results.
+struct CgBatchSystem {
+    Model* m = nullptr;
+    CgWork* w = nullptr;
+    const void* A = nullptr;
+    Schwarz* P = nullptr;
+    const double* b = nullptr;
+    double* x = nullptr;
+    double tol = 0.0;
+    int64_t max_iter = -1;
+    int64_t iterations = 0;            // out
+    hoefem_status status = HOEFEM_OK;  // out: as pcg_solve returns it
+};
+
+// State of the batched solver: its stream, the segment descriptors and work-group maps (pinned
+// staging and device copies) and the captured iteration batches, keyed by their launch shapes.
+struct CgBatch {
+    hipStream_t cs = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t done = nullptr;
+    std::vector<hipEvent_t> joins;  // one per distinct caller stream
+    std::unique_ptr<PinnedBuf<char>> h_seg;
+    std::unique_ptr<PinnedBuf<int>> h_map;
+    size_t seg_cap = 0, map_cap = 0;
+    DevBuf<char> d_seg;
+    DevBuf<int> d_map;
+    struct Graph {
+        std::vector<int64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    std::vector<Graph> graphs;
+    uint64_t clock = 0;
+    bool use_graphs = true;
+    void ensure(size_t seg_bytes, size_t map_ints, size_t streams);
+    ~CgBatch();
+};
+
+// Solves every system of `sys` (all of one value type: fp32 selects float) together: the same
+// kernels as pcg_solve, each system's work-groups running exactly as they would alone, so every
+// x, iteration count and status equals what pcg_solve gives that system. One launch per kernel
+// and tile shape covers all systems; each system stops on the device when it converges.
+void pcg_solve_batch(CgBatch& B, std::vector<CgBatchSystem>& sys, bool fp32);
 
 // ------------------------------------------------------------------ learned coarse correction (learned.hip)
 constexpr int LC_MAX_AGG_NODES = 64;   // nodes per aggregate (LDS staging of the apply kernels)

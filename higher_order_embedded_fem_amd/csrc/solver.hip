@@ -91,10 +91,10 @@ __device__ inline void partials_reduce(const double (&acc)[NV], double (&tot)[NV
     }
 }
 
-// partials[blockIdx.x * NV + k] = sum of v[k] over the block's threads. BT is the block size
-// the kernel is launched with (its launch bound, a multiple of 64): one LDS slot per wave.
+// partials[bid * NV + k] = sum of v[k] over the block's threads. BT is the block size the kernel is
+// launched with (its launch bound, a multiple of 64): one LDS slot per wave.
 template <int NV, int BT = CG_THREADS>
-__device__ inline void block_partials(const double (&v)[NV], double* __restrict__ partials) {
+__device__ inline void block_partials(const double (&v)[NV], double* __restrict__ partials, int64_t bid) {
     static_assert(BT % 64 == 0 && BT <= 1024, "block size must be a multiple of 64, at most 1024");
     __shared__ double s_w[BT / 64][NV];
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -110,7 +110,7 @@ __device__ inline void block_partials(const double (&v)[NV], double* __restrict_
         for (int k = 0; k < NV; ++k) {
             double s = 0.0;
             for (int w = 0; w < nw; ++w) s += s_w[w][k];
-            partials[(int64_t)blockIdx.x * NV + k] = s;
+            partials[bid * NV + k] = s;
         }
     }
 }
@@ -241,8 +241,7 @@ __device__ inline void entry_row_dot(const Csr3<VT>& A, const NodeEntry& e, int 
 struct GroupRange {
     int64_t begin, end, step;
 };
-__device__ inline GroupRange group_range(int64_t groups) {
-    const int64_t n = gridDim.x, b = blockIdx.x;
+__device__ inline GroupRange group_range(int64_t groups, int64_t b, int64_t n) {
     if (n % 8) return {b, groups, n};
     const int64_t x = b % 8;
     return {x * groups / 8 + b / 8, (x + 1) * groups / 8, n / 8};
@@ -253,12 +252,13 @@ __device__ inline GroupRange group_range(int64_t groups) {
 // the G-lane shuffles need whole groups) over the work-group's group_range, the next group's
 // entry prefetched; then diagonal-only nodes one per thread (diag(e)).
 template <int G, int BT = CG_THREADS, class FB, class FD>
-__device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb, int64_t nd, FB block, FD diag) {
+__device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb, int64_t nd, FB block, FD diag,
+                                   int64_t bid, int64_t nblk) {
     constexpr int per = BT / G;
     const int lane = (int)(threadIdx.x % G);
     const int64_t groups = (nb + per - 1) / per;
     const int64_t sub = threadIdx.x / G;
-    const GroupRange R = group_range(groups);
+    const GroupRange R = group_range(groups, bid, nblk);
     int64_t g = R.begin;
     NodeEntry e{0, 0, 0, 0};
     if (g < R.end && g * per + sub < nb) e = ent[g * per + sub];
@@ -270,8 +270,8 @@ __device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb
         block(cur, lane, valid);
     }
     // diagonal-only nodes from the last work-group backwards: those have the fewest block groups
-    const int64_t rb = (int64_t)gridDim.x - 1 - blockIdx.x;
-    for (int64_t i = rb * BT + threadIdx.x; i < nd; i += (int64_t)gridDim.x * BT) diag(ent[nb + i]);
+    const int64_t rb = nblk - 1 - bid;
+    for (int64_t i = rb * BT + threadIdx.x; i < nd; i += nblk * BT) diag(ent[nb + i]);
 }
 
 // The lane's first 3x3 block of a row (j = lane) and its column, loaded a pass ahead.
@@ -342,13 +342,14 @@ __device__ inline void entry_row_dot_pf(const Csr3<VT>& A, const NodeEntry& e, i
 // returns false to stop, uniformly over the grid) runs while those are in flight. In the loop
 // the row data of pass n + 1 and the entry of pass n + 2 are loaded before pass n is computed.
 template <int G, class VT, class PI, class PF, class FB, class FD>
-__device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_finish, FB block, FD diag) {
+__device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_finish, FB block, FD diag, int64_t bid,
+                                      int64_t nblk) {
     constexpr int per = CG_THREADS / G;
     const int lane = (int)(threadIdx.x % G);
     const int64_t nb = A.n_block, nd = A.n_diag;
     const int64_t groups = (nb + per - 1) / per;
     const int64_t sub = threadIdx.x / G;
-    const GroupRange R = group_range(groups);
+    const GroupRange R = group_range(groups, bid, nblk);
     const int64_t stride = R.step;
     auto has = [&](int64_t g) { return g < R.end && g * per + sub < nb; };
     int64_t g = R.begin;
@@ -369,8 +370,8 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
         block(cur, lane, valid, pf);
         pf = pn;
     }
-    const int64_t rb = (int64_t)gridDim.x - 1 - blockIdx.x;  // as in for_entries
-    for (int64_t i = rb * CG_THREADS + threadIdx.x; i < nd; i += (int64_t)gridDim.x * CG_THREADS)
+    const int64_t rb = nblk - 1 - bid;  // as in for_entries
+    for (int64_t i = rb * CG_THREADS + threadIdx.x; i < nd; i += nblk * CG_THREADS)
         diag(A.ent[nb + i]);
     return true;
 }
@@ -400,6 +401,32 @@ __device__ inline void load6(const double* __restrict__ base, int64_t b, double 
     q[5] = u2.y;
 }
 
+// One linear system of a (possibly batched) CG launch, in device memory: its matrices, vectors,
+// partial sums and iteration state, and where its work-groups sit in the launches. A single solve
+// is a batch of one segment; a batch of independent systems (independent scenes) runs every
+// iteration of all of them in one A and one B launch, each work-group working on its own segment
+// exactly as it would alone (same local index, same local grid), so the results are bit-identical.
+template <class VT>
+struct CgSeg {
+    Csr3<VT> A, Pm;      // system matrix; preconditioner (the Jacobi variant reads dinv instead)
+    double* x;
+    const double* b;
+    const double* dinv;
+    double* zp[2];       // interleaved [z | p], double-buffered
+    double* ra[2];       // interleaved [r | Ap], double-buffered
+    double* part_pre;
+    double* part_a;
+    double* part_b;
+    CgScalars* sc;
+    int* host_status;    // pinned host copy of sc->status (or any device word)
+    int nb_a, nb_b;      // work-groups of the segment in the A-side (preamble 1, A) and B-side launches
+    int base_a, base_b;  // its first work-group in those launches
+    int nparts_b;        // partial sums the B side writes (nb_b, or the aggregates of the learned apply)
+    int pad;
+    double tol;          // relative tolerance and iteration limit (read by the batched start kernel)
+    int64_t max_iter;
+};
+
 struct CgPtrs {
     double* x;
     const double* zp_old;  // A_k: zp[c]                       (pre2: unused)
@@ -418,9 +445,60 @@ struct CgPtrs {
     int* host_status;  // pinned host copy of sc->status, written when a kernel ends the solve
 };
 
+// The kernel view of segment S: stage 0 preamble 1, 1 preamble 2, 2 the A and B kernels of an
+// iteration whose buffer parity is cur.
+template <class VT>
+__device__ inline CgPtrs seg_ptrs(const CgSeg<VT>& S, int stage, int cur) {
+    CgPtrs P{};
+    P.x = S.x;
+    P.b = S.b;
+    P.dinv = S.dinv;
+    P.part_pre = S.part_pre;
+    P.part_a = S.part_a;
+    P.part_b = S.part_b;
+    P.nb_a = S.nb_a;
+    P.nb_b = S.nparts_b;
+    P.sc = S.sc;
+    P.host_status = S.host_status;
+    if (stage == 0) {
+        P.ra_new = S.ra[0];
+    } else if (stage == 1) {
+        P.ra_old = S.ra[0];
+        P.zp_new = S.zp[0];
+    } else {
+        P.zp_old = S.zp[cur];
+        P.zp_new = S.zp[1 - cur];
+        P.ap_out = S.ra[cur];
+        P.ra_old = S.ra[cur];
+        P.ra_new = S.ra[1 - cur];
+    }
+    return P;
+}
+
+// the segment of this work-group and its index / count within the segment's A-side (a) or B-side launch
+struct SegIdx {
+    int s;
+    int64_t bid, nblk;
+};
+// (s = -1: a padding work-group between two segments of a batched launch, which returns at once)
+template <class VT>
+__device__ inline SegIdx seg_index(const CgSeg<VT>* segs, const int* seg_of, bool a_side) {
+    const int s = seg_of ? seg_of[blockIdx.x] : 0;
+    if (s < 0) return SegIdx{-1, 0, 0};
+    const CgSeg<VT>& S = segs[s];
+    return a_side ? SegIdx{s, (int64_t)blockIdx.x - S.base_a, S.nb_a}
+                  : SegIdx{s, (int64_t)blockIdx.x - S.base_b, S.nb_b};
+}
+
 // preamble 1: r = b - A x -> ra_new.r ; partial |b|^2
 template <int G, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3<VT> A, CgPtrs P) {
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of) {
+    const SegIdx I = seg_index(segs, seg_of, true);
+    if (I.s < 0) return;
+    const CgSeg<VT>& S = segs[I.s];
+    if (S.sc->status != CG_RUNNING) return;  // a segment without a solve in this batch
+    const Csr3<VT> A = S.A;
+    const CgPtrs P = seg_ptrs(S, 0, 0);
     const double* x = P.x;
     double part[1] = {0.0};
     auto finish = [&](int64_t v, double a0, double a1, double a2) {
@@ -445,16 +523,23 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(Csr3<VT> A, CgPtrs P) {
             const int64_t v = e.node;
             const VT* d = A.vals + e.rowoff;
             finish(v, double(d[0] * VT(x[3 * v])), double(d[1] * VT(x[3 * v + 1])), double(d[2] * VT(x[3 * v + 2])));
-        });
-    block_partials<1>(part, P.part_pre);
+        },
+        I.bid, I.nblk);
+    block_partials<1>(part, P.part_pre, I.bid);
 }
 
 // preamble 2: |b| ; z = P r (r from ra_old) -> zp_new.z ; partial zTr, |r|^2
 template <int G, bool JAC, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of) {
+    const SegIdx I = seg_index(segs, seg_of, false);
+    if (I.s < 0) return;
+    const CgSeg<VT>& S = segs[I.s];
+    if (S.sc->status != CG_RUNNING) return;
+    const Csr3<VT> Pm = S.Pm;
+    const CgPtrs P = seg_ptrs(S, 1, 0);
     double bb[1];
     sum_partials<1>(P.part_pre, P.nb_a, bb);
-    if (blockIdx.x == 0 && threadIdx.x == 0) P.sc->bnorm = sqrt(bb[0]);
+    if (I.bid == 0 && threadIdx.x == 0) P.sc->bnorm = sqrt(bb[0]);
     const double* ra = P.ra_old;
     double part[2] = {0.0, 0.0};
     auto finish = [&](int64_t v, double z0, double z1, double z2) {
@@ -471,8 +556,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
         part[1] += r0 * r0 + r1 * r1 + r2 * r2;
     };
     if (JAC) {
-        for (int64_t v = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; v < Pm.nn; v += (int64_t)gridDim.x * CG_THREADS)
-            finish(v, 0.0, 0.0, 0.0);
+        for (int64_t v = I.bid * CG_THREADS + threadIdx.x; v < S.A.nn; v += I.nblk * CG_THREADS) finish(v, 0.0, 0.0, 0.0);
     } else {
         for_entries<G>(
             Pm.ent, Pm.n_block, Pm.n_diag,
@@ -490,9 +574,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
                 const VT* d = Pm.vals + e.rowoff;
                 finish(v, double(d[0] * VT(ra[6 * v])), double(d[1] * VT(ra[6 * v + 1])),
                        double(d[2] * VT(ra[6 * v + 2])));
-            });
+            },
+            I.bid, I.nblk);
     }
-    if (!FLEX) block_partials<2>(part, P.part_b);
+    if (!FLEX) block_partials<2>(part, P.part_b, I.bid);
 }
 
 // A-kernel of iteration k: stopping tests on r_k, beta_k = zTr_k / zTr_{k-1},
@@ -500,16 +585,22 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(Csr3<VT> Pm, CgPtrs P) {
 // first entry is issued: issuing the entry early (as the B-kernel does) measured 0.25 us slower
 // per launch with 9 MB more HBM fetch on the Hex20 cylinder.
 template <int G, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(Csr3<VT> A, CgPtrs P) {
-    CgScalars* sc = P.sc;
+__global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of,
+                                                       int cur) {
+    const SegIdx I = seg_index(segs, seg_of, true);
+    if (I.s < 0) return;
+    const CgSeg<VT>& S = segs[I.s];
+    CgScalars* sc = S.sc;
     if (sc->status != CG_RUNNING) return;
+    const Csr3<VT> A = S.A;
+    const CgPtrs P = seg_ptrs(S, 2, cur);
     const int64_t k = sc->iters;  // written by the previous B-kernel (0 after the preamble)
     constexpr int NV = FLEX ? 3 : 2;
     double tot[NV];
     sum_partials<NV, CG_THREADS_A>(P.part_b, P.nb_b, tot);
     // z_k.r_k, |r_k|^2 and (flexible variant) z_k.r_{k-1}
     const double zTr = tot[0], zTr_cross = FLEX ? tot[1] : 0.0, rr = tot[NV - 1];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = I.bid == 0 && threadIdx.x == 0;
     int decided = CG_RUNNING;
     if (sc->bnorm == 0.0)
         decided = CG_ZERO_RHS;
@@ -571,18 +662,25 @@ __global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(Csr3<VT> A, CgPtrs P) {
             double p0, p1, p2;
             pvec(v, p0, p1, p2);
             finish(v, p0, p1, p2, double(d[0] * VT(p0)), double(d[1] * VT(p1)), double(d[2] * VT(p2)));
-        });
-    block_partials<1, CG_THREADS_A>(part, P.part_a);
+        },
+        I.bid, I.nblk);
+    block_partials<1, CG_THREADS_A>(part, P.part_a, I.bid);
 }
 
 // B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p_k ;
 // r_{k+1} = r_k - alpha Ap_k ; z_{k+1} = P r_{k+1} ; partial zTr, |r|^2
 template <int G, bool JAC, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
+__global__ __launch_bounds__(CG_THREADS) void k_cg_b(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of,
+                                                     int cur) {
+    const SegIdx I = seg_index(segs, seg_of, false);
+    if (I.s < 0) return;
+    const CgSeg<VT>& S = segs[I.s];
+    const Csr3<VT> Pm = S.Pm;
+    const CgPtrs P = seg_ptrs(S, 2, cur);
     CgScalars* sc = P.sc;
     double acc[1];
     double alpha = 0.0;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = I.bid == 0 && threadIdx.x == 0;
     auto pro_issue = [&] { partials_load<1>(P.part_a, P.nb_a, acc); };
     auto pro_finish = [&]() -> bool {
         if (sc->status != CG_RUNNING) return false;
@@ -643,7 +741,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
     if (JAC) {
         pro_issue();
         if (!pro_finish()) return;
-        for (int64_t v = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; v < Pm.nn; v += (int64_t)gridDim.x * CG_THREADS) {
+        for (int64_t v = I.bid * CG_THREADS + threadIdx.x; v < S.A.nn; v += I.nblk * CG_THREADS) {
             double r0, r1, r2;
             rvec(v, r0, r1, r2);
             finish(v, r0, r1, r2, 0.0, 0.0, 0.0);
@@ -666,10 +764,11 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(Csr3<VT> Pm, CgPtrs P) {
                 double r0, r1, r2;
                 rvec(v, r0, r1, r2);
                 finish(v, r0, r1, r2, double(d[0] * VT(r0)), double(d[1] * VT(r1)), double(d[2] * VT(r2)));
-            });
+            },
+            I.bid, I.nblk);
         if (!ran) return;
     }
-    if (!FLEX) block_partials<2>(part, P.part_b);
+    if (!FLEX) block_partials<2>(part, P.part_b, I.bid);
 }
 
 // Learned coarse correction, one work-group per aggregate: g = Phi_a^T r_new,
@@ -713,7 +812,7 @@ __global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* 
         part[1] += z * ro;
         part[2] += rn * rn;
     }
-    block_partials<3>(part, partials);
+    block_partials<3>(part, partials, blockIdx.x);
 }
 
 __global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
@@ -769,6 +868,7 @@ void CgWork::ensure(int64_t nn_dof, int nblocks) {
         scalars.alloc(1);
         HIP_CHECK(hipMemset(scalars.get(), 0, sizeof(CgScalars)));
     }
+    if (!d_seg.p) d_seg.alloc(SEG_BYTES);
     if (!ev[0]) {
         HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -811,24 +911,34 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     HIP_CHECK(hipStreamWaitEvent(s, w.join[0], 0));
     const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA, CG_THREADS_A);
     const unsigned blkB = JAC ? cg_grid(0, Pm.nn, GB) : cg_grid(Pm.n_block, Pm.n_diag, GB);
-    CgPtrs P{};
-    P.x = x;
-    P.b = b;
-    P.dinv = w.diag_inv.get();
-    P.part_pre = w.part_pre.get();
-    P.part_a = w.part_a.get();
-    P.part_b = w.part_b.get();
-    P.nb_a = (int)blkA;
-    P.nb_b = FLEX ? lv->na : (int)blkB;
-    P.sc = w.scalars.get();
-    P.host_status = w.h_status.p;
+    // the solve as a batch of one segment; its descriptor goes to the device through pinned memory
+    // (the previous solve synchronized the stream, so the staging copy is free)
+    CgSeg<VT>* hs_seg = reinterpret_cast<CgSeg<VT>*>(w.h_seg.p);
+    *hs_seg = CgSeg<VT>{};
+    hs_seg->A = A;
+    hs_seg->Pm = Pm;
+    hs_seg->x = x;
+    hs_seg->b = b;
+    hs_seg->dinv = w.diag_inv.get();
+    for (int k = 0; k < 2; ++k) {
+        hs_seg->zp[k] = w.zp[k].get();
+        hs_seg->ra[k] = w.ra[k].get();
+    }
+    hs_seg->part_pre = w.part_pre.get();
+    hs_seg->part_a = w.part_a.get();
+    hs_seg->part_b = w.part_b.get();
+    hs_seg->sc = w.scalars.get();
+    hs_seg->host_status = w.h_status.p;
+    hs_seg->nb_a = (int)blkA;
+    hs_seg->nb_b = (int)blkB;
+    hs_seg->nparts_b = FLEX ? lv->na : (int)blkB;
+    static_assert(sizeof(CgSeg<VT>) <= CgWork::SEG_BYTES, "segment descriptor staging too small");
+    const CgSeg<VT>* segs = reinterpret_cast<const CgSeg<VT>*>(w.d_seg.get());
+    HIP_CHECK(hipMemcpyAsync(w.d_seg.get(), w.h_seg.p, sizeof(CgSeg<VT>), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
     HIP_CHECK(hipMemsetAsync(w.zp[0].get(), 0, sizeof(double) * 2 * w.n, s));  // p_{-1} = 0
-    P.ra_new = w.ra[0].get();
-    k_cg_pre1<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
-    P.ra_old = w.ra[0].get();
-    P.zp_new = w.zp[0].get();
-    k_cg_pre2<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+    k_cg_pre1<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
+    k_cg_pre2<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
     if (FLEX)
         k_lc_apply<<<dim3((unsigned)lv->na), dim3(256), 0, s>>>(*lv, w.ra[0].get(), nullptr, w.zp[0].get(),
                                                                w.part_b.get());
@@ -841,18 +951,16 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     // lives on the device and CG_BATCH is even, so the buffer parity repeats); per-iteration
     // hooks (kernel timing, snapshots) need the direct launches
     const bool graph = w.use_graphs && !m.prof && !h;
-    const std::vector<const void*> key = {A.vals, Pm.vals, x, A.ent, Pm.ent, w.zp[0].get(), w.ra[0].get(),
+    // the batch's kernels take the segment descriptor, so the captured graph does not depend on the
+    // matrices or vectors of a solve, only on the launch shapes and the learned correction's arguments
+    const std::vector<const void*> key = {(const void*)segs, (const void*)(intptr_t)blkA, (const void*)(intptr_t)blkB,
                                           (const void*)(intptr_t)GA, (const void*)(intptr_t)GB,
                                           (const void*)(intptr_t)JAC, (const void*)(intptr_t)FLEX,
+                                          (const void*)(intptr_t)(sizeof(VT)), w.zp[0].get(), w.ra[0].get(),
                                           lv ? lv->basis : nullptr, lv ? lv->gain : nullptr,
-                                          (const void*)(intptr_t)(lv ? lv->na : 0), P.dinv, P.sc};
+                                          (const void*)(intptr_t)(lv ? lv->na : 0)};
     auto launch_batch = [&]() {
         for (int q = 0; q < CG_BATCH; ++q) {
-            P.zp_old = w.zp[cur].get();
-            P.zp_new = w.zp[1 - cur].get();
-            P.ap_out = w.ra[cur].get();
-            P.ra_old = w.ra[cur].get();
-            P.ra_new = w.ra[1 - cur].get();
             if (m.prof) {
                 // the events take the kernel's own start / end timestamps (not a marker
                 // before and after the launch), so dispatch gaps are not counted
@@ -862,12 +970,12 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
                     m.prof_ev.push_back(e);
                 }
                 hipExtLaunchKernelGGL(k_cg_a<GA, FLEX, VT>, dim3(blkA), dim3(CG_THREADS_A), 0, s, m.prof_ev[nev],
-                                      m.prof_ev[nev + 1], 0, A, P);
+                                      m.prof_ev[nev + 1], 0, segs, (const int*)nullptr, cur);
                 nev += 2;
             } else {
-                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(A, P);
+                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(segs, nullptr, cur);
             }
-            k_cg_b<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+            k_cg_b<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr, cur);
             if (FLEX)
                 k_lc_apply<<<dim3((unsigned)lv->na), dim3(256), 0, s>>>(*lv, w.ra[1 - cur].get(), w.ra[cur].get(),
                                                                        w.zp[1 - cur].get(), w.part_b.get());
@@ -1016,14 +1124,15 @@ __global__ void k_fill(int64_t n, double* __restrict__ a, double value);
 // on one GPU race (different models included).
 constexpr int64_t NOALLOC_MAX_DOFS = 262144;
 constexpr int64_t NOALLOC_VEC = 2 * NOALLOC_MAX_DOFS;  // one interleaved [z | p] or [r | Ap] buffer
-// layout in doubles: iteration state, the kernels' status word, partial sums, four CG vectors,
-// the Jacobi diagonal (every region starts on a 256-byte boundary)
-constexpr int64_t NA_SCALARS = 0, NA_STATUS = 32, NA_PART_PRE = 64;
+// layout in doubles: iteration state, the kernels' status word, the segment descriptor, partial
+// sums, four CG vectors, the Jacobi diagonal (every region starts on a 256-byte boundary)
+constexpr int64_t NA_SCALARS = 0, NA_STATUS = 32, NA_SEG = 64, NA_PART_PRE = 128;
 constexpr int64_t NA_PART_A = NA_PART_PRE + CG_MAX_BLOCKS, NA_PART_B = NA_PART_A + CG_MAX_BLOCKS;
 constexpr int64_t NA_VEC0 = NA_PART_B + 3 * CG_MAX_BLOCKS;
 constexpr int64_t NA_DIAG = NA_VEC0 + 4 * NOALLOC_VEC;
-constexpr int64_t NA_TOTAL = NA_DIAG + NOALLOC_MAX_DOFS;  // 9 * 262144 + 5184 doubles = 18.9 MB
+constexpr int64_t NA_TOTAL = NA_DIAG + NOALLOC_MAX_DOFS;  // 9 * 262144 + 5248 doubles = 18.9 MB
 static_assert(sizeof(CgScalars) <= 32 * sizeof(double), "iteration state does not fit its region");
+static_assert(sizeof(CgSeg<double>) <= 64 * sizeof(double), "segment descriptor does not fit its region");
 static_assert(NA_PART_PRE % 32 == 0 && NA_VEC0 % 32 == 0 && NOALLOC_VEC % 32 == 0, "region alignment");
 
 static __device__ __align__(256) double scratch[NA_TOTAL];
@@ -1058,37 +1167,37 @@ static hoefem_status cg_launch_noalloc(Model& m, double* base, const Csr3<double
     double* zp[2] = {base + NA_VEC0, base + NA_VEC0 + NOALLOC_VEC};
     double* ra[2] = {base + NA_VEC0 + 2 * NOALLOC_VEC, base + NA_VEC0 + 3 * NOALLOC_VEC};
     CgScalars* sc = (CgScalars*)(base + NA_SCALARS);
-    CgPtrs P{};
-    P.x = x;
-    P.b = b;
-    P.dinv = base + NA_DIAG;
-    P.part_pre = base + NA_PART_PRE;
-    P.part_a = base + NA_PART_A;
-    P.part_b = base + NA_PART_B;
-    P.nb_a = (int)blkA;
-    P.nb_b = (int)blkB;
-    P.sc = sc;
-    P.host_status = (int*)(base + NA_STATUS);  // a device word here: the status is polled from sc
+    CgSeg<double> seg{};  // copied into the scratch array below (pageable source: a synchronous copy)
+    seg.A = A;
+    seg.Pm = Pm;
+    seg.x = x;
+    seg.b = b;
+    seg.dinv = base + NA_DIAG;
+    for (int k = 0; k < 2; ++k) {
+        seg.zp[k] = zp[k];
+        seg.ra[k] = ra[k];
+    }
+    seg.part_pre = base + NA_PART_PRE;
+    seg.part_a = base + NA_PART_A;
+    seg.part_b = base + NA_PART_B;
+    seg.sc = sc;
+    seg.host_status = (int*)(base + NA_STATUS);  // a device word here: the status is polled from sc
+    seg.nb_a = (int)blkA;
+    seg.nb_b = seg.nparts_b = (int)blkB;
+    const CgSeg<double>* segs = (const CgSeg<double>*)(base + NA_SEG);
+    HIP_CHECK(hipMemcpyAsync(base + NA_SEG, &seg, sizeof(seg), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, sc, tol, max_iter);
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((2 * n + 255) / 256)), dim3(256), 0, s, 2 * n, zp[0], 0.0);  // p_{-1}
-    P.ra_new = ra[0];
-    k_cg_pre1<GA, double><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(A, P);
-    P.ra_old = ra[0];
-    P.zp_new = zp[0];
-    k_cg_pre2<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+    k_cg_pre1<GA, double><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
+    k_cg_pre2<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
     check_launch("cg preamble");
     int cur = 0, status = CG_RUNNING;
     int64_t launched = 0;
     const int64_t max_batches = max_iter >= 0 ? max_iter / CG_BATCH + 2 : INT64_MAX;
     for (int64_t batch = 0; batch < max_batches && status == CG_RUNNING; ++batch) {
         for (int q = 0; q < CG_BATCH; ++q) {
-            P.zp_old = zp[cur];
-            P.zp_new = zp[1 - cur];
-            P.ap_out = ra[cur];
-            P.ra_old = ra[cur];
-            P.ra_new = ra[1 - cur];
-            k_cg_a<GA, false, double><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(A, P);
-            k_cg_b<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(Pm, P);
+            k_cg_a<GA, false, double><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(segs, nullptr, cur);
+            k_cg_b<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr, cur);
             cur ^= 1;
         }
         launched += CG_BATCH;
@@ -1178,6 +1287,342 @@ hoefem_status pcg_solve_learned(Model& m, CgWork& w, const float* A, Schwarz* P,
     return pcg_solve_t<float>(m, w, A, P->d_vals32.get(), P, b, x, tol, max_iter, iterations, nullptr, &lv);
 }
 
+// ================================================================== batched PCG
+// Independent systems solved together. Systems are grouped by tile shape (lanes per node row of A
+// and of P, Jacobi or Schwarz); each group has one launch per kernel, whose work-groups are its
+// systems' grids laid end to end (the segments), every segment starting on a multiple of 8
+// work-groups, so a work-group sits on the XCD (b mod 8) it has when its system runs alone; the
+// work-groups in between map to segment -1 and return. The work-group -> segment maps and the
+// descriptors are uploaded before each solve, so a captured batch of iterations depends only on the
+// launch shapes and is replayed for any systems of the same shapes.
+
+// x = 0, [z | p] = 0 (p_{-1}) and the iteration state of every segment (grid.y = segment)
+template <class VT>
+__global__ __launch_bounds__(256) void k_cg_begin(const CgSeg<VT>* __restrict__ segs) {
+    const CgSeg<VT>& S = segs[blockIdx.y];
+    const int64_t n = 3 * S.A.nn;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += (int64_t)gridDim.x * blockDim.x) {
+        S.zp[0][i] = 0.0;
+        if (i < n) S.x[i] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        CgScalars* sc = S.sc;
+        sc->tol = S.tol;
+        sc->max_iter = S.max_iter;
+        sc->status = CG_RUNNING;
+        sc->iters = 0;
+        sc->kcur = 0;
+        sc->beta = 0.0;
+        sc->zrec[0] = sc->zrec[1] = 0.0;
+    }
+}
+
+template <class VT>
+struct SegOps {
+    void (*pre)(const CgSeg<VT>*, const int*, unsigned, const int*, unsigned, hipStream_t);
+    void (*a)(const CgSeg<VT>*, const int*, unsigned, int, hipStream_t);
+    void (*b)(const CgSeg<VT>*, const int*, unsigned, int, hipStream_t);
+};
+
+template <int GA, int GB, bool JAC, class VT>
+struct SegLaunch {
+    static void pre(const CgSeg<VT>* s, const int* ma, unsigned na, const int* mb, unsigned nb, hipStream_t st) {
+        k_cg_pre1<GA, VT><<<dim3(na), dim3(CG_THREADS), 0, st>>>(s, ma);
+        k_cg_pre2<GB, JAC, false, VT><<<dim3(nb), dim3(CG_THREADS), 0, st>>>(s, mb);
+    }
+    static void a(const CgSeg<VT>* s, const int* ma, unsigned na, int cur, hipStream_t st) {
+        k_cg_a<GA, false, VT><<<dim3(na), dim3(CG_THREADS_A), 0, st>>>(s, ma, cur);
+    }
+    static void b(const CgSeg<VT>* s, const int* mb, unsigned nb, int cur, hipStream_t st) {
+        k_cg_b<GB, JAC, false, VT><<<dim3(nb), dim3(CG_THREADS), 0, st>>>(s, mb, cur);
+    }
+};
+
+template <class VT>
+static SegOps<VT> seg_ops(int GA, int GB, bool JAC) {
+#define SEG_CASE(ga, gb, j) \
+    if (GA == ga && GB == gb && JAC == j) \
+        return SegOps<VT>{&SegLaunch<ga, gb, j, VT>::pre, &SegLaunch<ga, gb, j, VT>::a, &SegLaunch<ga, gb, j, VT>::b};
+    SEG_CASE(8, 8, true) SEG_CASE(16, 8, true) SEG_CASE(32, 8, true)
+    SEG_CASE(8, 8, false) SEG_CASE(8, 16, false) SEG_CASE(8, 32, false)
+    SEG_CASE(16, 8, false) SEG_CASE(16, 16, false) SEG_CASE(16, 32, false)
+    SEG_CASE(32, 8, false) SEG_CASE(32, 16, false) SEG_CASE(32, 32, false)
+#undef SEG_CASE
+    throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unsupported CG configuration");
+}
+
+void CgBatch::ensure(size_t seg_bytes, size_t map_ints, size_t streams) {
+    if (!cs) {
+        HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    }
+    while (joins.size() < streams) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        joins.push_back(e);
+    }
+    if (seg_bytes > seg_cap) {
+        seg_cap = std::max(seg_bytes, 2 * seg_cap);
+        h_seg.reset(new PinnedBuf<char>(seg_cap));
+        d_seg.alloc(seg_cap);
+    }
+    if (map_ints > map_cap) {
+        map_cap = std::max(map_ints, 2 * map_cap);
+        h_map.reset(new PinnedBuf<int>(map_cap));
+        d_map.alloc(map_cap);
+    }
+}
+
+CgBatch::~CgBatch() {
+    for (Graph& g : graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
+    if (cs) (void)hipStreamDestroy(cs);
+}
+
+template <class VT>
+static void pcg_batch_t(CgBatch& B, std::vector<CgBatchSystem>& sys) {
+    struct Item {
+        int sys, ga, gb;
+        bool jac;
+        unsigned blkA, blkB;
+        Csr3<VT> A, Pm;
+    };
+    struct Group {
+        int ga, gb;
+        bool jac;
+        SegOps<VT> ops;
+        std::vector<int> items;
+        int seg0 = 0;
+        int64_t mapA = 0, nA = 0, mapB = 0, nB = 0;
+    };
+    std::vector<Item> items;
+    std::vector<Group> groups;
+    std::vector<hipStream_t> streams;
+    for (size_t i = 0; i < sys.size(); ++i) {
+        CgBatchSystem& q = sys[i];
+        q.iterations = 0;
+        q.status = HOEFEM_OK;
+        Model& m = *q.m;
+        const NodePattern& pat = m.pattern;
+        if (q.P && q.P->fp32 != (sizeof(VT) == sizeof(float)))
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "preconditioner precision differs from the operator's");
+        const int64_t blocks = (pat.num_nodes * 32 + CG_THREADS - 1) / CG_THREADS + 1;
+        q.w->ensure(pat.ndof(), (int)blocks);
+        Item it{};
+        it.sys = (int)i;
+        it.jac = q.P == nullptr;
+        it.ga = spmv_group_size(pat);
+        it.gb = q.P ? spmv_group_size(q.P->P) : 8;
+        it.A = view(pat, (const VT*)q.A);
+        if (q.P) {
+            const VT* pv;
+            if constexpr (sizeof(VT) == sizeof(double))
+                pv = (const VT*)q.P->d_vals.get();
+            else
+                pv = (const VT*)q.P->d_vals32.get();
+            it.Pm = view(q.P->P, pv);
+        } else {
+            it.Pm = it.A;
+            k_jacobi_diag<VT><<<dim3((unsigned)((pat.ndof() + 255) / 256)), dim3(256), 0, m.stream>>>(
+                pat.ndof(), pat.d_diag_pos.get(), (const VT*)q.A, q.w->diag_inv.get(), m.d_err.get());
+            check_launch("k_jacobi_diag");
+            if (read_and_clear_err(m) & KERR_ZERO_DIAG) {  // what pcg_solve reports by an exception
+                q.status = HOEFEM_ERROR_SINGULAR;
+                continue;
+            }
+        }
+        it.blkA = cg_grid(it.A.n_block, it.A.n_diag, it.ga, CG_THREADS_A);
+        it.blkB = it.jac ? cg_grid(0, it.Pm.nn, it.gb) : cg_grid(it.Pm.n_block, it.Pm.n_diag, it.gb);
+        size_t g = 0;
+        while (g < groups.size() && !(groups[g].ga == it.ga && groups[g].gb == it.gb && groups[g].jac == it.jac)) ++g;
+        if (g == groups.size()) {
+            Group ng;
+            ng.ga = it.ga;
+            ng.gb = it.gb;
+            ng.jac = it.jac;
+            ng.ops = seg_ops<VT>(it.ga, it.gb, it.jac);
+            groups.push_back(ng);
+        }
+        groups[g].items.push_back((int)items.size());
+        items.push_back(it);
+        if (std::find(streams.begin(), streams.end(), m.stream) == streams.end()) streams.push_back(m.stream);
+    }
+    if (items.empty()) return;
+    // layout: segments in group order; per group the A-side map, then the B-side map
+    auto round8 = [](int64_t v) { return (v + 7) / 8 * 8; };
+    int64_t map_total = 0;
+    int nseg = 0;
+    for (Group& g : groups) {
+        g.seg0 = nseg;
+        nseg += (int)g.items.size();
+        int64_t a = 0, b = 0;
+        for (int k : g.items) {
+            a = round8(a) + items[k].blkA;
+            b = round8(b) + items[k].blkB;
+        }
+        g.mapA = map_total;
+        g.nA = a;
+        g.mapB = map_total + a;
+        g.nB = b;
+        map_total += a + b;
+    }
+    if (map_total > INT32_MAX) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "batched grid too large");
+    B.ensure(sizeof(CgSeg<VT>) * nseg, (size_t)map_total, streams.size());
+    CgSeg<VT>* hs = reinterpret_cast<CgSeg<VT>*>(B.h_seg->p);
+    int* hm = B.h_map->p;
+    int64_t max_batches = 0, max_n = 0;
+    for (Group& g : groups) {
+        int64_t a = 0, b = 0;
+        for (size_t j = 0; j < g.items.size(); ++j) {
+            const Item& it = items[g.items[j]];
+            CgBatchSystem& q = sys[it.sys];
+            CgWork& w = *q.w;
+            const int64_t ba = round8(a), bb = round8(b);
+            for (int64_t t = a; t < ba; ++t) hm[g.mapA + t] = -1;
+            for (int64_t t = b; t < bb; ++t) hm[g.mapB + t] = -1;
+            for (int64_t t = 0; t < it.blkA; ++t) hm[g.mapA + ba + t] = (int)j;
+            for (int64_t t = 0; t < it.blkB; ++t) hm[g.mapB + bb + t] = (int)j;
+            a = ba + it.blkA;
+            b = bb + it.blkB;
+            CgSeg<VT>& S = hs[g.seg0 + j];
+            S = CgSeg<VT>{};
+            S.A = it.A;
+            S.Pm = it.Pm;
+            S.x = q.x;
+            S.b = q.b;
+            S.dinv = w.diag_inv.get();
+            for (int k = 0; k < 2; ++k) {
+                S.zp[k] = w.zp[k].get();
+                S.ra[k] = w.ra[k].get();
+            }
+            S.part_pre = w.part_pre.get();
+            S.part_a = w.part_a.get();
+            S.part_b = w.part_b.get();
+            S.sc = w.scalars.get();
+            S.host_status = w.h_status.p;
+            S.nb_a = (int)it.blkA;
+            S.nb_b = S.nparts_b = (int)it.blkB;
+            S.base_a = (int)ba;
+            S.base_b = (int)bb;
+            S.tol = q.tol;
+            S.max_iter = q.max_iter;
+            *(volatile int*)w.h_status.p = CG_RUNNING;
+            max_batches = std::max(max_batches, q.max_iter >= 0 ? q.max_iter / CG_BATCH + 2 : INT64_MAX);
+            max_n = std::max(max_n, it.A.nn);
+        }
+    }
+    const hipStream_t cs = B.cs;
+    for (size_t k = 0; k < streams.size(); ++k) {
+        HIP_CHECK(hipEventRecord(B.joins[k], streams[k]));
+        HIP_CHECK(hipStreamWaitEvent(cs, B.joins[k], 0));
+    }
+    const CgSeg<VT>* segs = reinterpret_cast<const CgSeg<VT>*>(B.d_seg.get());
+    const int* dmap = B.d_map.get();
+    HIP_CHECK(hipMemcpyAsync(B.d_seg.get(), hs, sizeof(CgSeg<VT>) * nseg, hipMemcpyHostToDevice, cs));
+    HIP_CHECK(hipMemcpyAsync(B.d_map.get(), hm, sizeof(int) * map_total, hipMemcpyHostToDevice, cs));
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((6 * max_n + 255) / 256, 512));
+    k_cg_begin<VT><<<dim3(gx, (unsigned)nseg), dim3(256), 0, cs>>>(segs);
+    for (Group& g : groups)
+        g.ops.pre(segs + g.seg0, dmap + g.mapA, (unsigned)g.nA, dmap + g.mapB, (unsigned)g.nB, cs);
+    check_launch("batched cg preamble");
+    auto launch_batch = [&]() {
+        int cur = 0;
+        for (int q = 0; q < CG_BATCH; ++q) {
+            for (Group& g : groups) g.ops.a(segs + g.seg0, dmap + g.mapA, (unsigned)g.nA, cur, cs);
+            for (Group& g : groups) g.ops.b(segs + g.seg0, dmap + g.mapB, (unsigned)g.nB, cur, cs);
+            cur ^= 1;  // CG_BATCH is even: every batch starts at parity 0
+        }
+        check_launch("batched cg iteration");
+    };
+    std::vector<int64_t> key = {(int64_t)sizeof(VT), (int64_t)(intptr_t)segs, (int64_t)(intptr_t)dmap};
+    for (const Group& g : groups)
+        for (int64_t v : {(int64_t)g.ga, (int64_t)g.gb, (int64_t)g.jac, (int64_t)g.seg0, g.mapA, g.nA, g.mapB, g.nB})
+            key.push_back(v);
+    hipGraphExec_t exec = nullptr;
+    if (B.use_graphs) {
+        for (CgBatch::Graph& gr : B.graphs)
+            if (gr.key == key) {
+                exec = gr.exec;
+                gr.used = ++B.clock;
+            }
+        if (!exec) {
+            constexpr size_t MAX_GRAPHS = 16;
+            if (B.graphs.size() >= MAX_GRAPHS) {
+                size_t old = 0;
+                for (size_t k = 1; k < B.graphs.size(); ++k)
+                    if (B.graphs[k].used < B.graphs[old].used) old = k;
+                HIP_CHECK(hipGraphExecDestroy(B.graphs[old].exec));
+                B.graphs.erase(B.graphs.begin() + (std::ptrdiff_t)old);
+            }
+            hipGraph_t gph = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            launch_batch();
+            HIP_CHECK(hipStreamEndCapture(cs, &gph));
+            CgBatch::Graph gr;
+            gr.key = key;
+            HIP_CHECK(hipGraphInstantiate(&gr.exec, gph, nullptr, nullptr, 0));
+            HIP_CHECK(hipGraphDestroy(gph));
+            gr.used = ++B.clock;
+            exec = gr.exec;
+            B.graphs.push_back(std::move(gr));
+        }
+    }
+    auto all_stopped = [&]() {
+        for (const Item& it : items)
+            if (*(volatile int*)sys[it.sys].w->h_status.p == CG_RUNNING) return false;
+        return true;
+    };
+    int64_t nbatches = 0;
+    auto submit = [&](int slot) {
+        if (exec)
+            HIP_CHECK(hipGraphLaunch(exec, cs));
+        else
+            launch_batch();
+        HIP_CHECK(hipEventRecord(B.ev[slot], cs));
+        ++nbatches;
+    };
+    submit(0);
+    if (nbatches < max_batches) submit(1);
+    int poll = 0;
+    while (true) {
+        HIP_CHECK(hipEventSynchronize(B.ev[poll]));
+        if (all_stopped() || nbatches >= max_batches) break;
+        submit(poll);
+        poll ^= 1;
+    }
+    for (const Item& it : items) {
+        CgWork& w = *sys[it.sys].w;
+        HIP_CHECK(hipMemcpyAsync(w.h_scalars.p + 2, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, cs));
+    }
+    HIP_CHECK(hipStreamSynchronize(cs));
+    HIP_CHECK(hipEventRecord(B.done, cs));
+    for (hipStream_t st : streams) HIP_CHECK(hipStreamWaitEvent(st, B.done, 0));
+    for (const Item& it : items) {
+        CgBatchSystem& q = sys[it.sys];
+        const CgScalars& r = q.w->h_scalars.p[2];
+        q.iterations = r.iters;
+        switch (r.status) {
+            case CG_CONVERGED:
+            case CG_ZERO_RHS: q.status = HOEFEM_OK; break;  // x was set to zero by k_cg_begin
+            case CG_RUNNING: q.status = HOEFEM_ERROR_CG_MAX_ITERATIONS; break;
+            default: q.status = (hoefem_status)r.status; break;
+        }
+    }
+}
+
+void pcg_solve_batch(CgBatch& B, std::vector<CgBatchSystem>& sys, bool fp32) {
+    if (fp32)
+        pcg_batch_t<float>(B, sys);
+    else
+        pcg_batch_t<double>(B, sys);
+}
+
 // ================================================================== Newton / BE kernels
 // The vector updates of the Backward-Euler Newton loop round every product and sum on its own
 // (no fused multiply-add), as the reference's nalgebra arithmetic does on the host: a caller that
@@ -1188,22 +1633,23 @@ hoefem_status pcg_solve_learned(Model& m, CgWork& w, const float* A, Schwarz* P,
 // hamilton2/src/integrators/euler.rs:97-114; apply_dirichlet_bcs_unknowns,
 // simulation_toolbox/src/fem/system_assembly.rs:78-104)
 __global__ void k_state(int64_t ndof, const double* __restrict__ up, const double* __restrict__ w, double dt,
-                        const uint8_t* __restrict__ dirmask, double* __restrict__ uu) {
+                        const uint8_t* __restrict__ dirmask, const double* __restrict__ bcu, double* __restrict__ uu) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndof) return;
     const double val = up[i] + w[i] * dt;
-    uu[i] = dirmask[i / 3] ? 0.0 : val;
+    uu[i] = dirmask[i / 3] ? (bcu ? bcu[i] : 0.0) : val;
 }
 
-// dst = src with Dirichlet DOFs zeroed (the copies NewtonCgDynamicSystem::eval_f and set_state
-// make, newton_cg.rs:449-455, :662-671)
+// dst = src with the Dirichlet values on the Dirichlet DOFs (bc: a DOF vector of prescribed values,
+// nullptr for zeros); the copies NewtonCgDynamicSystem::eval_f and set_state make,
+// newton_cg.rs:449-455, :662-671
 __global__ void k_bc_copy(int64_t ndof, const double* __restrict__ src, const uint8_t* __restrict__ dirmask,
-                          double* __restrict__ dst) {
+                          const double* __restrict__ bc, double* __restrict__ dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndof) return;
     const double val = src[i];
-    dst[i] = dirmask[i / 3] ? 0.0 : val;
+    dst[i] = dirmask[i / 3] ? (bc ? bc[i] : 0.0) : val;
 }
 
 // f = M g + f_int (element forces in color order) - D v, Dirichlet rows zeroed
@@ -1263,7 +1709,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_mass_add(Csr3<double> M, const d
         y[3 * v + 2] = y2;
         part[0] = y0 * y0 + y1 * y1 + y2 * y2;
     }
-    if (partials) block_partials<1>(part, partials);
+    if (partials) block_partials<1>(part, partials, blockIdx.x);
 }
 
 // w <- w + delta * (-sol)  (line-search step along p = -sol, hamilton2/src/newton.rs:113-116, :219)
@@ -1284,26 +1730,31 @@ __global__ void k_be_begin(int64_t n, const double* __restrict__ u, const double
     up[i] = u[i] + v[i] * dt;
 }
 
-// v += w ; u += v * dt ; then Dirichlet unknowns zeroed (euler.rs:195-196, newton_cg.rs:293)
+// v += w ; u += v * dt ; then the Dirichlet values on the Dirichlet unknowns (euler.rs:195-196,
+// newton_cg.rs:293)
 __global__ void k_be_end(int64_t n, double* __restrict__ u, double* __restrict__ v, const double* __restrict__ w,
-                         double dt, const uint8_t* __restrict__ dirmask) {
+                         double dt, const uint8_t* __restrict__ dirmask, const double* __restrict__ bcu,
+                         const double* __restrict__ bcv) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double vv = v[i] + w[i];
     const double uu = u[i] + vv * dt;
     const bool d = dirmask[i / 3] != 0;
-    v[i] = d ? 0.0 : vv;
-    u[i] = d ? 0.0 : uu;
+    v[i] = d ? (bcv ? bcv[i] : 0.0) : vv;
+    u[i] = d ? (bcu ? bcu[i] : 0.0) : uu;
 }
 
-__global__ void k_zero_dirichlet(int64_t n, double* __restrict__ a, double* __restrict__ b,
-                                 const uint8_t* __restrict__ dirmask) {
+// the Dirichlet values on the Dirichlet DOFs of a and b (apply_dirichlet_bcs_unknowns,
+// simulation_toolbox/src/fem/system_assembly.rs:78-104)
+__global__ void k_set_dirichlet(int64_t n, double* __restrict__ a, double* __restrict__ b,
+                                const uint8_t* __restrict__ dirmask, const double* __restrict__ bca,
+                                const double* __restrict__ bcb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (dirmask[i / 3]) {
-        a[i] = 0.0;
-        if (b) b[i] = 0.0;
+        a[i] = bca ? bca[i] : 0.0;
+        if (b) b[i] = bcb ? bcb[i] : 0.0;
     }
 }
 
@@ -1334,7 +1785,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_dot_partials(int64_t n, const do
     double part[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * CG_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * CG_THREADS)
         part[0] += a[i] * b[i];
-    block_partials<1>(part, partials);
+    block_partials<1>(part, partials, blockIdx.x);
 }
 
 // a.b on the model stream (deterministic), returned to the host
@@ -1404,30 +1855,34 @@ void launch_f32_to_f64(Model& m, const float* a, double* b, int64_t n) {
 
 // v = v_n + w with the Dirichlet velocities applied (zero): the velocity eval_f sees
 __global__ void k_state_v(int64_t ndof, const double* __restrict__ vn, const double* __restrict__ w,
-                          const uint8_t* __restrict__ dirmask, double* __restrict__ vv) {
+                          const uint8_t* __restrict__ dirmask, const double* __restrict__ bcv, double* __restrict__ vv) {
 #pragma clang fp contract(off)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndof) return;
     const double val = vn[i] + w[i];
-    vv[i] = dirmask[i / 3] ? 0.0 : val;
+    vv[i] = dirmask[i / 3] ? (bcv ? bcv[i] : 0.0) : val;
 }
 
-void launch_state_v(Model& m, const double* vn, const double* w, const uint8_t* dirmask, double* vv) {
+void launch_state_v(Model& m, const double* vn, const double* w, const uint8_t* dirmask, const double* bcv,
+                    double* vv) {
     const int64_t n = 3 * m.nv;
-    hipLaunchKernelGGL(k_state_v, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, vn, w, dirmask, vv);
+    hipLaunchKernelGGL(k_state_v, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, vn, w, dirmask, bcv,
+                       vv);
     check_launch("k_state_v");
 }
 
-void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, double* uu) {
+void launch_state(Model& m, const double* up, const double* w, double dt, const uint8_t* dirmask, const double* bcu,
+                  double* uu) {
     const int64_t n = 3 * m.nv;
-    hipLaunchKernelGGL(k_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, up, w, dt, dirmask, uu);
+    hipLaunchKernelGGL(k_state, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, up, w, dt, dirmask, bcu,
+                       uu);
     check_launch("k_state");
 }
 
-void launch_bc_copy(Model& m, const double* src, const uint8_t* dirmask, double* dst) {
+void launch_bc_copy(Model& m, const double* src, const uint8_t* dirmask, const double* bc, double* dst) {
     const int64_t n = 3 * m.nv;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_bc_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, src, dirmask, dst);
+    hipLaunchKernelGGL(k_bc_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, src, dirmask, bc, dst);
     check_launch("k_bc_copy");
 }
 
@@ -1487,16 +1942,20 @@ void launch_be_begin(Model& m, const double* u, const double* v, double dt, doub
     check_launch("k_be_begin");
 }
 
-void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask) {
+void launch_be_end(Model& m, double* u, double* v, const double* w, double dt, const uint8_t* dirmask,
+                   const double* bcu, const double* bcv) {
     const int64_t n = 3 * m.nv;
-    hipLaunchKernelGGL(k_be_end, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, u, v, w, dt, dirmask);
+    hipLaunchKernelGGL(k_be_end, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, u, v, w, dt, dirmask,
+                       bcu, bcv);
     check_launch("k_be_end");
 }
 
-void launch_zero_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask) {
+void launch_set_dirichlet(Model& m, double* a, double* b, const uint8_t* dirmask, const double* bca,
+                          const double* bcb) {
     const int64_t n = 3 * m.nv;
-    hipLaunchKernelGGL(k_zero_dirichlet, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, a, b, dirmask);
-    check_launch("k_zero_dirichlet");
+    hipLaunchKernelGGL(k_set_dirichlet, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m.stream, n, a, b, dirmask,
+                       bca, bcb);
+    check_launch("k_set_dirichlet");
 }
 
 void launch_finite_check(Model& m, const double* a, const double* b) {

@@ -346,6 +346,18 @@ class NewtonCgIntegrator:
         check(_lib.load().hoefem_sim_residual(self.h, t, dt, _dev(u), _dev(v), _dev(w), _dev(F)))
         return F
 
+    def set_dirichlet_values(self, u_values=None, v_values=None):
+        """Prescribed values of the Dirichlet nodes (DirichletBoundaryConditions::apply_displacement_bcs /
+        apply_velocity_bcs, simulation_toolbox/src/fem/bcs.rs:13-27): host arrays of three values per
+        Dirichlet node in the scene's order, None for zeros (the Homogeneous condition)."""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (u_values, v_values)]
+        m = 3 * len(self.scene.dirichlet_nodes)
+        for a in keep:
+            if a is not None and a.size != m:
+                raise ValueError(f"expected {m} Dirichlet values, got {a.size}")
+        check(_lib.load().hoefem_sim_set_dirichlet_values(self.h, *[None if a is None else a.ctypes.data
+                                                                   for a in keep]))
+
     def step(self, t: float, dt: float, u, v) -> Dict:
         st = _lib.StepStats()
         check(_lib.load().hoefem_sim_step(self.h, t, dt, _vec(u), _vec(v), C.byref(st)))

@@ -317,6 +317,34 @@ hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const do
 /* one time step: u, v device vectors updated in place */
 hoefem_status hoefem_sim_step(hoefem_sim* sim, double t, double dt, double* u, double* v, hoefem_step_stats* stats);
 
+/* Prescribed Dirichlet values: what DirichletBoundaryConditions::apply_displacement_bcs and
+ * apply_velocity_bcs (simulation_toolbox/src/fem/bcs.rs:13-27) return for the sim's Dirichlet nodes,
+ * three per node in the order of hoefem_sim_desc.dirichlet_nodes (host arrays; NULL: zeros, the
+ * Homogeneous condition). Every step and dynamic-system call then sets the Dirichlet unknowns of u and v
+ * to them (apply_dirichlet_bcs_unknowns, simulation_toolbox/src/fem/system_assembly.rs:78-104).
+ * Time-independent conditions (ConstantDisplacement, ConstantUniformDisplacement) are set once;
+ * a time-dependent one is set before each step with its values at the step's end time. */
+hoefem_status hoefem_sim_set_dirichlet_values(hoefem_sim* sim, const double* u_values, const double* v_values);
+
+/* ------------------------------------------------------------------ batched scenes
+ * No counterpart in the reference, which steps one scene per process (integrate_model_with_substeps,
+ * simulation_toolbox/src/fem/newton_cg.rs:146-228). A batch steps independent sims (each its own
+ * model, material and boundary conditions) together: their Newton loops run in lockstep, their
+ * residual evaluations, assemblies and line-search trials are issued before one synchronization per
+ * round, and each round's linear solves run as one batched PCG whose scenes stop on the device as
+ * they converge. Every scene's u and v equal what hoefem_sim_step gives it, bit for bit (a scene
+ * whose single-step attempt fails continues alone with hoefem_sim_step's substepping). The sims
+ * must outlive the batch, take device pointers, have no learned correction attached and share one
+ * precision mode (FP64, or mixed_precision 1); their models may use different streams. */
+typedef struct hoefem_batch hoefem_batch;
+hoefem_status hoefem_batch_create(hoefem_sim* const* sims, int32_t count, hoefem_batch** out);
+void hoefem_batch_destroy(hoefem_batch* batch);
+/* One time step of every scene: dt[i], u[i], v[i] (device vectors, updated in place) of sim i.
+ * Optional outputs per scene: stats; latency_ms, the time from the call's start until the scene's
+ * Newton loop ended; status, the scene's hoefem_sim_step status. Returns the first scene error. */
+hoefem_status hoefem_batch_step(hoefem_batch* batch, const double* dt, double* const* u, double* const* v,
+                                hoefem_step_stats* stats, double* latency_ms, int32_t* status);
+
 /* ------------------------------------------------------------------ dynamic system
  * The DynamicSystem / DifferentiableDynamicSystem surface of NewtonCgDynamicSystem
  * (hamilton2/src/dynamic_system/mutable.rs:14-76, simulation_toolbox/src/fem/newton_cg.rs:413-820), so

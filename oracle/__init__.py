@@ -171,6 +171,7 @@ def lib():
         L.orc_sim_destroy.argtypes = [C.c_void_p]
         L.orc_sim_representative_force.argtypes = [C.c_void_p]
         L.orc_sim_representative_force.restype = C.c_double
+        L.orc_sim_set_dirichlet_values.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_sim_mass_values.argtypes = [C.c_void_p, f64p]
         L.orc_sim_jacobian_values.argtypes = [C.c_void_p, f64p]
         L.orc_sim_step.argtypes = [C.c_void_p, C.c_double, C.c_double, f64p, f64p, C.POINTER(OrcStepStats)]
@@ -716,6 +717,16 @@ class Sim:
         out = np.zeros(self.model.nnz)
         check(lib().orc_sim_jacobian_values(self.h, out))
         return out
+
+    def set_dirichlet_values(self, u_values=None, v_values=None):
+        """values of the Dirichlet condition for the sim's Dirichlet nodes ([num_dirichlet, 3] or flat;
+        None: zeros)"""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (u_values, v_values)]
+        m = 3 * len(self.model.scene.dirichlet_nodes)
+        for a in keep:
+            if a is not None and a.size != m:
+                raise ValueError(f"expected {m} Dirichlet values, got {a.size}")
+        check(lib().orc_sim_set_dirichlet_values(self.h, *[None if a is None else a.ctypes.data for a in keep]))
 
     def step(self, t, dt, u, v):
         st = OrcStepStats()

@@ -2108,15 +2108,21 @@ struct orc_sim {
     // state for set_state
     std::vector<double> su, sv;
     orc_step_stats* stats = nullptr;
+    // prescribed Dirichlet values, three per Dirichlet node (zeros: the Homogeneous condition)
+    std::vector<double> bc_u, bc_v;
 };
 
 namespace {
+// apply_dirichlet_bcs_unknowns (simulation_toolbox/src/fem/system_assembly.rs:78-104): the
+// condition's displacement and velocity values on the Dirichlet unknowns
 void apply_bc_unknowns(const orc_sim* s, double* u, double* v) {
-    for (int32_t node : s->dir)
+    for (size_t k = 0; k < s->dir.size(); ++k) {
+        const int64_t node = s->dir[k];
         for (int i = 0; i < 3; ++i) {
-            u[3 * (int64_t)node + i] = 0.0;
-            v[3 * (int64_t)node + i] = 0.0;
+            u[3 * node + i] = s->bc_u[3 * k + i];
+            v[3 * node + i] = s->bc_v[3 * k + i];
         }
+    }
 }
 
 // NewtonCgDynamicSystem::eval_f (simulation_toolbox/src/fem/newton_cg.rs:441-478)
@@ -2253,6 +2259,8 @@ int orc_sim_create(const orc_model* model, const orc_sim_desc* d, orc_sim** out)
     s->model = model;
     s->d = *d;
     s->dir.assign(d->dirichlet_nodes, d->dirichlet_nodes + d->num_dirichlet);
+    s->bc_u.assign(3 * s->dir.size(), 0.0);
+    s->bc_v.assign(3 * s->dir.size(), 0.0);
     s->M = model->pattern;
     int st = orc_model_assemble_mass(model, d->density, s->M.val.data());
     if (st != ORC_OK) {
@@ -2300,6 +2308,17 @@ void orc_sim_destroy(orc_sim* s) {
 }
 
 double orc_sim_representative_force(const orc_sim* s) { return s->rep_force; }
+
+// DirichletBoundaryConditions::apply_displacement_bcs / apply_velocity_bcs values of the sim's
+// Dirichlet nodes (simulation_toolbox/src/fem/bcs.rs:13-27), three per node; NULL: zeros
+int orc_sim_set_dirichlet_values(orc_sim* s, const double* u_values, const double* v_values) {
+    const size_t m = 3 * s->dir.size();
+    for (size_t k = 0; k < m; ++k) {
+        s->bc_u[k] = u_values ? u_values[k] : 0.0;
+        s->bc_v[k] = v_values ? v_values[k] : 0.0;
+    }
+    return ORC_OK;
+}
 
 int orc_sim_mass_values(const orc_sim* s, double* values) {
     std::copy(s->M.val.begin(), s->M.val.end(), values);

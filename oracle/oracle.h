@@ -228,6 +228,7 @@ typedef struct orc_sim orc_sim;
 int orc_sim_create(const orc_model* model, const orc_sim_desc* desc, orc_sim** out);
 void orc_sim_destroy(orc_sim* s);
 double orc_sim_representative_force(const orc_sim* s);
+int orc_sim_set_dirichlet_values(orc_sim* s, const double* u_values, const double* v_values);
 int orc_sim_mass_values(const orc_sim* s, double* values);
 int orc_sim_step(orc_sim* s, double t, double dt, double* u, double* v, orc_step_stats* stats);
 /* the Jacobian combination of the last Newton iteration (J = M + dt^2 K with BCs) */

@@ -45,6 +45,12 @@ def test_host_only_entry_points():
     assert L.hoefem_model_create(None, ctypes.byref(h)) == 1
     assert b"null" in L.hoefem_last_error()
     assert L.hoefem_model_ndof(None) == 0
+    # batches: no sims, a null sim, a null batch
+    assert L.hoefem_batch_create(None, 0, ctypes.byref(h)) == 1
+    assert L.hoefem_batch_create((ctypes.c_void_p * 1)(None), 1, ctypes.byref(h)) == 1
+    assert b"null sim" in L.hoefem_last_error()
+    assert L.hoefem_batch_step(None, None, None, None, None, None, None) == 1
+    L.hoefem_batch_destroy(None)
 
 
 def test_missing_library_fails_loudly(tmp_path):
