@@ -636,53 +636,68 @@ static hoefem_status sim_step(Sim& s, double t, double dt, double* u, double* v,
 
 // ------------------------------------------------------------------ batched scenes
 // Independent scenes advanced by one time step together. Every scene runs the kernels of sim_step
-// in the same order with the same arguments; the batch changes only how they are issued: one host
-// thread walks all scenes' Newton loops in lockstep (residual evaluations, Jacobian assemblies and
-// line-search trials of all scenes are enqueued before one synchronization), and the linear solves
-// of one Newton round run as one batched PCG (pcg_solve_batch), in which each scene stops on the
-// device when it converges. Each scene's u and v are therefore the ones sim_step gives it, bit for
-// bit. A scene whose single-step attempt fails continues alone with the substepping of sim_step.
+// in the same order with the same arguments, on its own model stream; the batch changes only how
+// they are issued. One host thread drives every scene's Newton loop as a state machine: it enqueues a
+// scene's next piece of work (residual evaluation, Jacobian assembly, line-search trial), records an
+// event and moves on, and takes the scene's next decision (Newton test, line-search acceptance, the
+// same ones sim_be_step takes) when that event has completed. A scene whose Jacobian is ready joins
+// the CG pool, whose batched launches carry every running solve and which reports each solve as it
+// ends on the device. Scenes therefore never wait for each other, and each scene's u and v are the
+// ones sim_step gives it, bit for bit. A scene whose single-step attempt fails continues alone with
+// the substepping of sim_step once the others are done.
 struct Batch {
     std::vector<Sim*> sims;
     CgBatch cg;
-    std::vector<hipStream_t> streams;  // the distinct model streams
+    std::vector<hipEvent_t> ev;  // one per scene
+    ~Batch() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
 };
 
 namespace {
-enum BatchPhase { BP_NEWTON, BP_SOLVE, BP_LINESEARCH, BP_DONE, BP_FAILED, BP_ERROR };
+enum BatchPhase { BP_EVAL, BP_JACOBIAN, BP_CG, BP_TRIAL, BP_DONE, BP_FAILED, BP_ERROR };
 struct BatchScene {
     Sim* s = nullptr;
     double dt = 0.0;
     double* u = nullptr;
     double* v = nullptr;
     hoefem_step_stats st{};
-    BatchPhase phase = BP_NEWTON;
+    BatchPhase phase = BP_EVAL;
+    bool waiting = false;  // its event is pending
     hoefem_status status = HOEFEM_OK;  // BP_FAILED: the attempt's error; BP_ERROR: the step's
     std::string message;
     double n2 = 0.0, tol = 0.0, g_initial = 0.0, alpha = 1.0, alpha_prev = 0.0, tail = 0.25;
     int idx = 0;
     int64_t it = 0;
+    int cg_id = -1;
     double latency_ms = 0.0;
 };
 }  // namespace
 
-static void batch_sync(Batch& B) {
-    for (hipStream_t st : B.streams) HIP_CHECK(hipStreamSynchronize(st));
-}
-
 static void batch_step(Batch& B, const double* dt, double* const* u, double* const* v, hoefem_step_stats* stats,
                        double* latency_ms, int32_t* status) {
     const size_t ns = B.sims.size();
-    B.streams.clear();
+    std::vector<hipStream_t> streams;
     for (Sim* s : B.sims) {
         if (s->learned) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "a batched scene has a learned correction attached");
-        if (std::find(B.streams.begin(), B.streams.end(), s->model->stream) == B.streams.end())
-            B.streams.push_back(s->model->stream);
+        if (std::find(streams.begin(), streams.end(), s->model->stream) == streams.end())
+            streams.push_back(s->model->stream);
     }
+    while (B.ev.size() < ns) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        B.ev.push_back(e);
+    }
+    const bool fp32 = B.sims[0]->mixed_mode == 1;
+    std::unique_ptr<CgPool> pool = make_cg_pool(B.cg, fp32, (int)ns);
     std::vector<BatchScene> q(ns);
     const auto t0 = std::chrono::steady_clock::now();
     auto elapsed_ms = [&]() {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    auto mark = [&](size_t i) {
+        HIP_CHECK(hipEventRecord(B.ev[i], q[i].s->model->stream));
+        q[i].waiting = true;
     };
     auto fail = [&](BatchScene& b, hoefem_status e) {
         b.status = e;
@@ -694,6 +709,45 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
         b.message = e.what();
         b.phase = BP_ERROR;
         b.latency_ms = elapsed_ms();
+    };
+    // the Newton test of a scene whose last step was accepted (newton.rs:76-131), then its next work
+    auto newton = [&](size_t i) {
+        BatchScene& b = q[i];
+        Sim& s = *b.s;
+        const double fnorm = std::sqrt(b.n2);
+        b.st.final_residual = fnorm;
+        if (!(fnorm > b.tol)) {
+            b.phase = BP_DONE;
+            b.st.newton_iterations += b.it;
+            b.st.substeps = 1;
+            launch_be_end(*s.model, s.d_us.get(), s.d_vs.get(), s.d_w.get(), b.dt, s.d_dirmask.get(), s.bcu(),
+                          s.bcv());
+            b.latency_ms = elapsed_ms();
+            return;
+        }
+        if (s.d.max_newton_iter >= 0 && b.it == s.d.max_newton_iter) {
+            fail(b, HOEFEM_ERROR_NEWTON_MAX_ITERATIONS);
+            return;
+        }
+        // J (-dx) = F (sim_solve): the Jacobian (and the Jacobi diagonal), then the pool's CG
+        launch_state(*s.model, s.d_up.get(), s.d_w.get(), b.dt, s.d_dirmask.get(), s.bcu(), s.d_uu.get());
+        sim_assemble_jacobian_launch(s, true, -b.dt * b.dt, true, -b.dt, s.d_uu.get());
+        if (!s.schwarz) {
+            launch_jacobi_diag(*s.model, s.cg, fp32 ? (const void*)s.d_J32.get() : (const void*)s.d_J.get(), fp32);
+            HIP_CHECK(hipMemcpyAsync(s.model->h_err.p, s.model->d_err.get(), sizeof(int), hipMemcpyDeviceToHost,
+                                     s.model->stream));
+        }
+        b.phase = BP_JACOBIAN;
+        mark(i);
+    };
+    // the next line-search trial of a scene (BacktrackingLineSearch, newton.rs:171-246)
+    auto trial = [&](size_t i) {
+        BatchScene& b = q[i];
+        Sim& s = *b.s;
+        launch_axpy_neg(*s.model, s.d_w.get(), b.alpha - b.alpha_prev, s.d_sol.get());
+        sim_eval_launch(s, b.dt, s.d_w.get());
+        b.phase = BP_TRIAL;
+        mark(i);
     };
     // start of sim_step and of sim_be_step for one substep of dt
     for (size_t i = 0; i < ns; ++i) {
@@ -718,125 +772,103 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
         b.tol = b.dt * (s.d.newton_tolerance_factor * s.rep_force);
         b.st.tolerance = b.tol;
         sim_eval_launch(s, b.dt, s.d_w.get());
+        b.phase = BP_EVAL;
+        mark(i);
     }
-    batch_sync(B);
-    for (BatchScene& b : q)
-        if (b.phase == BP_NEWTON) b.n2 = sim_eval_finish(*b.s);
-    std::vector<CgBatchSystem> sys;
-    std::vector<BatchScene*> solving;
-    const bool fp32 = B.sims.empty() ? false : B.sims[0]->mixed_mode == 1;
+    std::vector<int> ended;
+    std::vector<int> owner;  // scene of every pool system
+    const double c = 1e-4, alpha_min = 1e-6;
+    const double initial[3] = {1.0, 0.75, 0.5};
     while (true) {
-        // Newton tests of every scene whose last step was accepted (newton.rs:76-131)
-        bool any = false;
-        for (BatchScene& b : q) {
-            if (b.phase != BP_NEWTON) continue;
+        for (size_t i = 0; i < ns; ++i) {
+            BatchScene& b = q[i];
+            if (!b.waiting) continue;
+            const hipError_t qe = hipEventQuery(B.ev[i]);
+            if (qe == hipErrorNotReady) continue;
+            HIP_CHECK(qe);
+            b.waiting = false;
             Sim& s = *b.s;
-            const double fnorm = std::sqrt(b.n2);
-            b.st.final_residual = fnorm;
-            if (!(fnorm > b.tol)) {
-                b.phase = BP_DONE;
-                b.st.newton_iterations += b.it;
-                b.st.substeps = 1;
-                launch_be_end(*s.model, s.d_us.get(), s.d_vs.get(), s.d_w.get(), b.dt, s.d_dirmask.get(), s.bcu(),
-                              s.bcv());
-                b.latency_ms = elapsed_ms();
-                continue;
+            if (b.phase == BP_EVAL) {
+                b.n2 = sim_eval_finish(s);
+                newton(i);
+            } else if (b.phase == BP_JACOBIAN) {
+                const int e = *s.model->h_err.p;
+                if (e) {
+                    HIP_CHECK(hipMemsetAsync(s.model->d_err.get(), 0, sizeof(int), s.model->stream));
+                    s.last_err_flags |= e;
+                }
+                // without Schwarz, a zero diagonal comes from the Jacobi setup, which pcg_solve reports
+                // by an exception; any other flag (a Schwarz zero diagonal included) fails the assembly
+                const int jac_err = s.schwarz ? 0 : (e & KERR_ZERO_DIAG);
+                if (e & ~jac_err) {
+                    fail(b, HOEFEM_ERROR_JACOBIAN);
+                } else if (jac_err) {
+                    error(b, Error(HOEFEM_ERROR_SINGULAR, "Diagonal matrix is not invertible."));
+                } else {
+                    CgBatchSystem sys;
+                    sys.m = s.model;
+                    sys.w = &s.cg;
+                    sys.A = fp32 ? (const void*)s.d_J32.get() : (const void*)s.d_J.get();
+                    sys.P = s.schwarz.get();
+                    sys.b = s.d_F.get();
+                    sys.x = s.d_sol.get();
+                    sys.tol = s.d.cg_tolerance;
+                    sys.max_iter = s.d.cg_max_iter;
+                    b.cg_id = pool->join(sys);
+                    if ((int)owner.size() <= b.cg_id) owner.resize(b.cg_id + 1, -1);
+                    owner[b.cg_id] = (int)i;
+                    b.phase = BP_CG;
+                }
+            } else if (b.phase == BP_TRIAL) {
+                b.n2 = sim_eval_finish(s);
+                const double g = 0.5 * b.n2;
+                if (g <= (1.0 - c * b.alpha) * b.g_initial) {
+                    ++b.it;
+                    newton(i);
+                } else if (b.alpha < alpha_min) {
+                    fail(b, HOEFEM_ERROR_LINE_SEARCH);
+                } else {
+                    b.alpha_prev = b.alpha;
+                    ++b.idx;
+                    if (b.idx < 3) {
+                        b.alpha = initial[b.idx];
+                    } else {
+                        b.alpha = b.tail;
+                        b.tail *= 0.25;
+                    }
+                    trial(i);
+                }
             }
-            if (s.d.max_newton_iter >= 0 && b.it == s.d.max_newton_iter) {
-                fail(b, HOEFEM_ERROR_NEWTON_MAX_ITERATIONS);
-                continue;
-            }
-            b.phase = BP_SOLVE;
-            any = true;
         }
-        if (!any) break;
-        // J (-dx) = F: the Jacobians, then one batched solve (sim_solve)
-        for (BatchScene& b : q) {
-            if (b.phase != BP_SOLVE) continue;
-            Sim& s = *b.s;
-            launch_state(*s.model, s.d_up.get(), s.d_w.get(), b.dt, s.d_dirmask.get(), s.bcu(), s.d_uu.get());
-            sim_assemble_jacobian_launch(s, true, -b.dt * b.dt, true, -b.dt, s.d_uu.get());
-        }
-        batch_sync(B);
-        sys.clear();
-        solving.clear();
-        for (BatchScene& b : q) {
-            if (b.phase != BP_SOLVE) continue;
-            Sim& s = *b.s;
-            if (sim_assemble_jacobian_finish(s) != HOEFEM_OK) {
-                fail(b, HOEFEM_ERROR_JACOBIAN);
-                continue;
-            }
-            CgBatchSystem c;
-            c.m = s.model;
-            c.w = &s.cg;
-            c.A = fp32 ? (const void*)s.d_J32.get() : (const void*)s.d_J.get();
-            c.P = s.schwarz.get();
-            c.b = s.d_F.get();
-            c.x = s.d_sol.get();
-            c.tol = s.d.cg_tolerance;
-            c.max_iter = s.d.cg_max_iter;
-            sys.push_back(c);
-            solving.push_back(&b);
-        }
-        pcg_solve_batch(B.cg, sys, fp32);
-        for (size_t k = 0; k < sys.size(); ++k) {
-            BatchScene& b = *solving[k];
-            b.s->cg_total += sys[k].iterations;
-            hoefem_status cs = sys[k].status;
-            if (cs == HOEFEM_ERROR_SINGULAR) {  // zero Jacobi diagonal: pcg_solve throws this
-                error(b, Error(HOEFEM_ERROR_SINGULAR, "Diagonal matrix is not invertible."));
-                continue;
-            }
+        ended.clear();
+        pool->progress(ended);
+        for (int id : ended) {
+            BatchScene& b = q[owner[id]];
+            const CgBatchSystem& r = pool->system(id);
+            b.s->cg_total += r.iterations;
+            hoefem_status cs = r.status;
             if (cs == HOEFEM_ERROR_CG_MAX_ITERATIONS || cs == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) cs = HOEFEM_OK;
             if (cs != HOEFEM_OK) {
                 fail(b, HOEFEM_ERROR_JACOBIAN);
                 continue;
             }
-            // backtracking on g = 0.5 |F|^2 along p = -sol (BacktrackingLineSearch, newton.rs:171-246)
-            b.phase = BP_LINESEARCH;
             b.g_initial = 0.5 * b.n2;
             b.alpha_prev = 0.0;
             b.alpha = 1.0;
             b.tail = 0.25;
             b.idx = 0;
+            trial((size_t)owner[id]);
         }
-        while (true) {
-            bool trial = false;
-            for (BatchScene& b : q) {
-                if (b.phase != BP_LINESEARCH) continue;
-                Sim& s = *b.s;
-                launch_axpy_neg(*s.model, s.d_w.get(), b.alpha - b.alpha_prev, s.d_sol.get());
-                sim_eval_launch(s, b.dt, s.d_w.get());
-                trial = true;
-            }
-            if (!trial) break;
-            batch_sync(B);
-            const double c = 1e-4, alpha_min = 1e-6;
-            const double initial[3] = {1.0, 0.75, 0.5};
-            for (BatchScene& b : q) {
-                if (b.phase != BP_LINESEARCH) continue;
-                b.n2 = sim_eval_finish(*b.s);
-                const double g = 0.5 * b.n2;
-                if (g <= (1.0 - c * b.alpha) * b.g_initial) {
-                    ++b.it;
-                    b.phase = BP_NEWTON;
-                    continue;
-                }
-                if (b.alpha < alpha_min) {
-                    fail(b, HOEFEM_ERROR_LINE_SEARCH);
-                    continue;
-                }
-                b.alpha_prev = b.alpha;
-                ++b.idx;
-                if (b.idx < 3) {
-                    b.alpha = initial[b.idx];
-                } else {
-                    b.alpha = b.tail;
-                    b.tail *= 0.25;
-                }
-            }
+        // the step goes on while any scene is still in its Newton loop (evaluating, assembling, in the
+        // pool or trying a step length)
+        bool active = false, host_pending = false;
+        for (const BatchScene& b : q) {
+            active |= b.phase == BP_EVAL || b.phase == BP_JACOBIAN || b.phase == BP_CG || b.phase == BP_TRIAL;
+            host_pending |= b.waiting;
         }
+        if (!active) break;
+        // nothing for the host to do until the GPU moves on: wait for the pool's oldest work
+        if (ended.empty() && !host_pending && !pool->idle()) pool->wait();
     }
     // scenes whose attempt failed: the rest of sim_step's substepping, one scene at a time
     for (BatchScene& b : q) {
@@ -853,7 +885,7 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
     }
     for (BatchScene& b : q)
         if (b.phase == BP_DONE) sim_step_end_launch(*b.s, b.st);
-    batch_sync(B);
+    for (hipStream_t st : streams) HIP_CHECK(hipStreamSynchronize(st));
     for (BatchScene& b : q) {
         if (b.phase != BP_DONE) continue;
         try {
@@ -862,7 +894,7 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
             error(b, e);
         }
     }
-    batch_sync(B);
+    for (hipStream_t st : streams) HIP_CHECK(hipStreamSynchronize(st));
     const BatchScene* first_error = nullptr;
     for (size_t i = 0; i < ns; ++i) {
         BatchScene& b = q[i];

@@ -229,143 +229,12 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness(
 //     K[(I, i), (J, j)] = sum_q sum_{k, l} G~_q(k, I) C_q(3k + i, 3l + j) G~_q(l, J)
 // is, for each component pair (i, j), the product K^{ij} = A B with A(I, kk) = G~_q(k, I) and
 // B(kk, J) = U^{ij}_q(k, J) = sum_l C_q(3k + i, 3l + j) G~_q(l, J) over kk = 3q + k. The six pairs
-// i <= j suffice (C has the major symmetry, so K^{ji} = (K^{ij})^T). Quadrature points go in
-// chunks of 8 (kk = 24, six k-steps of v_mfma_f64_16x16x4); nodes are padded to 16 or 32, so a
-// pair is one or four 16 x 16 tiles; the 4 waves share the 6 or 24 (pair, tile) products and
-// keep their accumulators in registers across chunks. Lane layout of v_mfma_f64_16x16x4 (lane l):
-// A operand A(l % 16, l / 16), B operand B(l / 16, l % 16), result item v = D(4 v + l / 16, l % 16). Each entry of K_e is written exactly once
-// (diagonal pairs from their lower triangle), so K_e is exactly symmetric.
-constexpr int MF_QC = 8;            // quadrature points per product chunk
-constexpr int MF_KK = 3 * MF_QC;    // rows of A^T / B per chunk
-constexpr int MF_QA = 28;           // quadrature points whose gradients and dP/dF are formed together
-
+// i <= j suffice (C has the major symmetry, so K^{ji} = (K^{ij})^T). Nodes are padded to 16 or 32,
+// so a pair is one or four 16 x 16 tiles, accumulated in registers across the quadrature chunks.
+// Lane layout of v_mfma_f64_16x16x4 (lane l): A operand A(l % 16, l / 16), B operand
+// B(l / 16, l % 16), result item v = D(4 v + l / 16, l % 16). Each entry of K_e is written exactly
+// once (diagonal pairs from their lower triangle), so K_e is exactly symmetric.
 typedef double mf_double4 __attribute__((ext_vector_type(4)));
-
-template <int ET, int MAT>
-__global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness_mfma(
-    int64_t ne, const int32_t* __restrict__ conn, const double* __restrict__ verts, const int32_t* __restrict__ qoff,
-    const double* __restrict__ qw, const double* __restrict__ qxi, double thr, double mu, double lambda,
-    const double* __restrict__ u, double* __restrict__ ke, int* err) {
-    constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN, M3 = 3 * N;
-    constexpr int NP = N > 16 ? 32 : 16, NT = NP / 16;
-    constexpr int JOBS = 6 * NT * NT, JPW = (JOBS + 3) / 4;
-    static_assert(N <= 32, "element too large for the tile layout");
-    const int64_t e = blockIdx.x;
-    const int t = threadIdx.x, w = t >> 6, l = t & 63;
-    __shared__ int s_node[N];
-    __shared__ double s_u[3 * N];
-    __shared__ double s_X[3 * GN];
-    __shared__ double s_C[MF_QA][81];
-    __shared__ double s_G[3 * MF_QA * NP];
-    __shared__ double s_U[6][MF_KK * NP];
-    if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
-    __syncthreads();
-    for (int k = t; k < 3 * N; k += STIFF_THREADS) s_u[k] = u[3 * (int64_t)s_node[k / 3] + k % 3];
-    for (int k = t; k < 3 * GN; k += STIFF_THREADS) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
-    mf_double4 acc[JPW];
-#pragma unroll
-    for (int j = 0; j < JPW; ++j) acc[j] = mf_double4{0.0, 0.0, 0.0, 0.0};
-    const int q0 = qoff[e], q1 = qoff[e + 1];
-    // groups of up to MF_QA points: their gradients and dP/dF in parallel (one lane per point), then
-    // the products in chunks of MF_QC points
-    for (int qa = q0; qa < q1; qa += MF_QA) {
-        const int cntA = min(MF_QA, q1 - qa);
-        for (int k = t; k < 3 * MF_QA * NP; k += STIFF_THREADS) s_G[k] = 0.0;
-        __syncthreads();
-        if (t < cntA) {
-            const int q = qa + t;
-            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
-            double J[9], Jinv[9];
-            ref_jacobian<ET>(s_X, xi, J);
-            const double detJ = det3(J);
-            bool ok = inv3(J, Jinv);
-            double psi[4];
-            tet4_psi(xi, psi);
-            double ug[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // u_grad(a, b) = sum_I G(a, I) u(b, I)
-            if (ok) {
-                const double sq = sqrt(qw[q] * fabs(detJ));
-                for (int I = 0; I < N; ++I) {
-                    double Gr[3], G[3];
-                    grad_node<ET>(I, xi, psi, Gr);
-                    for (int a = 0; a < 3; ++a) G[a] = Jinv[0 + 3 * a] * Gr[0] + Jinv[1 + 3 * a] * Gr[1] + Jinv[2 + 3 * a] * Gr[2];
-                    for (int b = 0; b < 3; ++b)
-                        for (int a = 0; a < 3; ++a) ug[a + 3 * b] += G[a] * s_u[3 * I + b];
-                    for (int a = 0; a < 3; ++a) s_G[(3 * t + a) * NP + I] = G[a] * sq;
-                }
-                double F[9];
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) F[b + 3 * a] = ug[a + 3 * b];
-                F[0] += 1.0;
-                F[4] += 1.0;
-                F[8] += 1.0;
-                ok = material_dpdf(MAT, thr, mu, lambda, F, s_C[t]);
-            }
-            if (!ok) {
-                atomicOr(err, KERR_SINGULAR);
-                for (int k = 0; k < 81; ++k) s_C[t][k] = 0.0;
-                for (int a = 0; a < 3; ++a)
-                    for (int I = 0; I < N; ++I) s_G[(3 * t + a) * NP + I] = 0.0;
-            }
-        }
-        __syncthreads();
-        for (int qc = 0; qc < cntA; qc += MF_QC) {
-            const int cnt = min(MF_QC, cntA - qc);
-            const double* G0 = s_G + 3 * qc * NP;  // the chunk's rows
-            // U^{ij} for the six pairs (i <= j); rows of quadrature points beyond cnt are zero
-            for (int idx = t; idx < 6 * MF_KK * NP; idx += STIFF_THREADS) {
-                const int p = idx / (MF_KK * NP), rem = idx % (MF_KK * NP);
-                const int kk = rem / NP, col = rem % NP;
-                const int qq = kk / 3, k = kk % 3;
-                const int i = p < 3 ? 0 : (p < 5 ? 1 : 2);
-                const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
-                double v = 0.0;
-                if (qq < cnt) {
-                    const double* C = s_C[qc + qq];
-#pragma unroll
-                    for (int m = 0; m < 3; ++m) v += C[(3 * k + i) + 9 * (3 * m + j)] * G0[(3 * qq + m) * NP + col];
-                }
-                s_U[p][rem] = v;
-            }
-            __syncthreads();
-            // rows kk >= 3 cnt of U are zero, so the A rows read there (zero or the next chunk's) add nothing
-            const int ksteps = (3 * cnt + 3) / 4;
-#pragma unroll
-            for (int jj = 0; jj < JPW; ++jj) {
-                const int jb = w + 4 * jj;
-                if (jb < JOBS) {
-                    const int p = jb / (NT * NT), tile = jb % (NT * NT);
-                    const int ti = tile / NT, tj = tile % NT;
-                    for (int ks = 0; ks < ksteps; ++ks) {
-                        const int kk = 4 * ks + (l >> 4);
-                        const double a = G0[kk * NP + 16 * ti + (l & 15)];
-                        const double b = s_U[p][kk * NP + 16 * tj + (l & 15)];
-                        acc[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[jj], 0, 0, 0);
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    double* out = ke + e * (int64_t)M3 * M3;
-#pragma unroll
-    for (int jj = 0; jj < JPW; ++jj) {
-        const int jb = w + 4 * jj;
-        if (jb < JOBS) {
-            const int p = jb / (NT * NT), tile = jb % (NT * NT);
-            const int ti = tile / NT, tj = tile % NT;
-            const int i = p < 3 ? 0 : (p < 5 ? 1 : 2);
-            const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int I = 16 * ti + (4 * v + (l >> 4)), Jn = 16 * tj + (l & 15);
-                if (I < N && Jn < N && (i != j || I >= Jn)) {
-                    out[(3 * I + i) * M3 + 3 * Jn + j] = acc[jj][v];
-                    out[(3 * Jn + j) * M3 + 3 * I + i] = acc[jj][v];
-                }
-            }
-        }
-    }
-}
 
 // One element per 128-lane work-group (two wavefronts; many work-groups per CU, so the SIMDs
 // interleave the phases of different elements). Quadrature points go in chunks of TW_QC:
@@ -377,8 +246,6 @@ __global__ __launch_bounds__(STIFF_THREADS) void k_element_stiffness_mfma(
 //    (i, j) = (w, w..2) for w = 0 and (1, 1), (1, 2), (2, 2) for w = 1, with the B operand
 //    U^{ij}(kk, col) = sum_m C_q(3k + i, 3m + j) G~(3q + m, col) formed in registers at each k-step
 //    (no U staging in LDS).
-// The lane layouts and the write-out (each K_e entry written once) are those of
-// k_element_stiffness_mfma.
 constexpr int TW_QC = 8, TW_THREADS = 128, TW_SUB = TW_THREADS / TW_QC;
 
 template <int ET, int MAT>

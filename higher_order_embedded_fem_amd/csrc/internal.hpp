@@ -265,10 +265,6 @@ struct CgWork {
     std::vector<const void*> batch_key;
     bool use_graphs = true;
     PinnedBuf<CgScalars> h_scalars{3};  // final readback (slot 2)
-    // the solve's segment descriptor (CgSeg, solver.hip): pinned staging and its device copy
-    static constexpr size_t SEG_BYTES = 512;
-    PinnedBuf<char> h_seg{SEG_BYTES};
-    DevBuf<char> d_seg;
     PinnedBuf<int> h_status{1, hipHostMallocCoherent};  // the solve's status, written by the kernel that ends it
     hipEvent_t ev[2] = {nullptr, nullptr};
     void ensure(int64_t n, int nblocks);
@@ -343,7 +339,7 @@ struct CgBatchSystem {
     Model* m = nullptr;
     CgWork* w = nullptr;
     const void* A = nullptr;
-    Schwarz* P = nullptr;
+    Schwarz* P = nullptr;  // nullptr: Jacobi, whose diagonal must already be in w->diag_inv (launch_jacobi_diag)
     const double* b = nullptr;
     double* x = nullptr;
     double tol = 0.0;
@@ -352,13 +348,14 @@ struct CgBatchSystem {
     hoefem_status status = HOEFEM_OK;  // out: as pcg_solve returns it
 };
 
-// State of the batched solver: its stream, the segment descriptors and work-group maps (pinned
-// staging and device copies) and the captured iteration batches, keyed by their launch shapes.
+// State of the batched solver that outlives one pool: its stream, the segment descriptors and
+// work-group maps (pinned staging and device copies) and the captured iteration batches, keyed by
+// their launch shapes.
 struct CgBatch {
     hipStream_t cs = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t done = nullptr;
-    std::vector<hipEvent_t> joins;  // one per distinct caller stream
+    hipEvent_t ev[2] = {nullptr, nullptr};  // the two batches in flight
+    hipEvent_t upload = nullptr;            // after the last descriptor / map upload
+    std::vector<hipEvent_t> spare;          // reusable events
     std::unique_ptr<PinnedBuf<char>> h_seg;
     std::unique_ptr<PinnedBuf<int>> h_map;
     size_t seg_cap = 0, map_cap = 0;
@@ -366,33 +363,54 @@ struct CgBatch {
     DevBuf<int> d_map;
     struct Graph {
         std::vector<int64_t> key;
-        hipGraphExec_t exec = nullptr;
+        hipGraphExec_t exec = nullptr;  // captured on the second sight of the key
+        int seen = 0;
         uint64_t used = 0;
     };
     std::vector<Graph> graphs;
     uint64_t clock = 0;
     bool use_graphs = true;
-    void ensure(size_t seg_bytes, size_t map_ints, size_t streams);
+    int64_t relayouts = 0;  // uploads of a new launch layout (statistics)
+    void ensure(size_t seg_bytes, size_t map_ints);
+    hipEvent_t take_event();
     ~CgBatch();
 };
 
-// Solves every system of `sys` (all of one value type: fp32 selects float) together: the same
-// kernels as pcg_solve, each system's work-groups running exactly as they would alone, so every
-// x, iteration count and status equals what pcg_solve gives that system. One launch per kernel
-// and tile shape covers all systems; each system stops on the device when it converges.
-void pcg_solve_batch(CgBatch& B, std::vector<CgBatchSystem>& sys, bool fp32);
+// Independent PCG solves that share launches (solver.hip): join() adds a system whose inputs are
+// complete on the device (the caller has synchronized with them); progress() enqueues work and
+// reports the systems whose solves ended, without blocking; system(id) then holds the iteration count
+// and the status pcg_solve would return. Every system's x, iterations and status equal pcg_solve's bit
+// for bit; each stops on the device when its solve ends.
+struct CgPool {
+    virtual ~CgPool() = default;
+    virtual int join(const CgBatchSystem& s) = 0;
+    virtual void progress(std::vector<int>& ended) = 0;
+    virtual void wait() = 0;  // blocks until the oldest outstanding work completes
+    virtual const CgBatchSystem& system(int id) const = 0;
+    virtual bool idle() const = 0;
+};
+std::unique_ptr<CgPool> make_cg_pool(CgBatch& B, bool fp32, int capacity);
+// the Jacobi diagonal of A into w.diag_inv on the model stream (zero diagonal entries flagged in m.d_err)
+void launch_jacobi_diag(Model& m, CgWork& w, const void* A, bool fp32);
 
 // ------------------------------------------------------------------ learned coarse correction (learned.hip)
 constexpr int LC_MAX_AGG_NODES = 64;   // nodes per aggregate (LDS staging of the apply kernels)
 constexpr int LC_SNAP_PER_SOLVE = 64;  // iterate snapshots kept per harvested solve
 
-struct LearnedView {  // what the CG apply kernel reads
+struct LearnedView {  // what the CG apply kernels read and write (no padding: its words key graphs)
     int32_t na;
+    float kappa;              // output scale
     const int32_t* agg_ptr;   // [na + 1] offsets into agg_dofs (3 per node)
     const int32_t* agg_dofs;  // DOFs grouped by aggregate
     const float* basis;       // aggregate a: 64 modes x n_a DOFs at 64 * agg_ptr[a]
-    const float* gain;        // [na][64] mode gains kappa * MLP(f_a)
+    const uint16_t* wbf;      // network weights (BF16) and
+    const float* params;      // FP32 parameters (biases)
+    float* x;                 // [na][64] scratch: normalized residual projections, BF16-rounded (network input)
+    float* xf;                // [na][64] scratch: the same in FP32 (the sign guard's direction)
+    float* gn;                // [na] scratch: their norms
+    float* y;                 // [na][64] scratch: network outputs
 };
+static_assert(sizeof(LearnedView) == 8 * 10, "LearnedView has padding");
 
 struct Learned {
     Model* model = nullptr;
@@ -403,7 +421,7 @@ struct Learned {
     DevBuf<float> d_basis;
     bool has_basis = false, trained = false, enabled = false, harvesting = false;
     float kappa = 0.0f;  // mean |P r| / |r| over the first training batch
-    DevBuf<float> d_feat, d_gain;  // [na][64] profile features and mode gains
+    DevBuf<float> d_cg_x, d_cg_xf, d_cg_gn, d_cg_y;  // per-iteration scratch of the CG apply (LearnedView)
     // network
     DevBuf<float> d_params, d_m, d_v, d_grad, d_gpart;
     DevBuf<uint16_t> d_wbf;
@@ -430,8 +448,11 @@ double learned_train(Learned& L, const double* J, Schwarz* P, int steps, double 
 double learned_train(Learned& L, const float* J, Schwarz* P, int steps, double lr, int B, double* initial_loss);
 void learned_mlp_forward(Learned& L, int64_t N, const float* X, float* Y);
 void learned_mlp_grad(Learned& L, int64_t N, const float* X, const float* dY, float* grad, float scale);
-LearnedView learned_view(const Learned& L);
-void learned_refresh_gains(Learned& L);
+LearnedView learned_view(Learned& L);
+// Y = MLP(X) for N samples on stream s (the network of the view), without the training tape
+void launch_mlp_forward(const LearnedView& lv, int64_t N, const float* X, float* Y, hipStream_t s);
+// the sign guard of the correction (learned.hip): y of the samples with xf.y < 0 projected orthogonal to xf
+void launch_lc_guard(int64_t N, const float* xf, float* y, uint8_t* flag, hipStream_t s);
 
 // Snapshot hook of the plain CG: every `every` iterations x_k and r_k are copied into
 // xs / rs (at most max snapshots); *taken receives the count.

@@ -1,28 +1,28 @@
 // Learned coarse correction of the Schwarz preconditioner.
 //
-// The nodes are split into aggregates (greedy breadth-first growth over the node graph, node
-// order seeds). Each aggregate a carries 64 orthonormal modes Phi_a: the restrictions to its
+// The nodes are split into aggregates: the breadth-first order of the node graph cut into the
+// requested number of contiguous, equal pieces. Each aggregate a carries 64 orthonormal modes Phi_a: the restrictions to its
 // DOFs of 64 error snapshots x* - x_k harvested from the plain Schwarz-CG solver's own solves
-// (orthonormalized per aggregate by Gram-Schmidt, Dirichlet DOFs zeroed). The correction is
-// linear, symmetric and positive semidefinite in the residual,
-//     z = P r + sum_a Phi_a diag(gamma_a) Phi_a^T r,     gamma_a = kappa softplus(MLP(f_a)),
-// so the preconditioner stays symmetric positive definite and CG keeps its Krylov optimality
-// (a correction that depends nonlinearly on r, or negative gains, lose that).
-// One network shared by all aggregates maps the aggregate's residual projection profile
-// f_a (64 values: the RMS of Phi_a^T r over harvested residuals, normalized) to its 64 mode
-// gains; kappa = mean |P r| / |r| over the first training batch keeps the output O(1).
-// Training minimizes, over harvested residuals r (unit norm),
+// (orthonormalized per aggregate by Gram-Schmidt, Dirichlet DOFs zeroed). One network shared by
+// all aggregates maps the aggregate's residual projection to a correction in its modes:
+//     g_a = Phi_a^T r,    c_a = kappa |g_a| MLP(g_a / |g_a|),    z = P r + sum_a Phi_a c_a,
+// a 64-dim input and a 64-dim output per aggregate. A sign guard replaces an output y with
+// (g / |g|).y < 0 by its projection orthogonal to g, so every aggregate adds a non-negative amount to
+// r^T z >= r^T P r > 0 and the preconditioned residual keeps a positive inner product. The correction
+// is positively homogeneous in r (CG is blind to the scale of z) but not linear, so the solver that
+// uses it is the flexible (Polak-Ribiere) PCG; kappa = mean |P r| / |r| over the first training batch keeps the network's
+// output O(1). Training minimizes, over harvested residuals r (unit norm),
 //     L(z) = -(r^T z)^2 / (z^T J z)  = min over s of |J^-1 r - s z|_J^2 - |J^-1 r|_J^2,
-// the energy left after the best step along z: the quality of z as a CG search direction,
-// independent of the scale of z (to which CG is blind). The gradient reaches the gains through
-// dL/dz = (2 a / q) ((a / q) J z - r), a = r^T z, q = z^T J z, and Phi_a^T. The last layer
-// starts with zero weights and bias -4: the untrained gains are softplus(-4) kappa = 0.018 kappa,
-// a small perturbation of the plain Schwarz preconditioner with gradients that still flow.
+// the energy left after the best step along z: the quality of z as a search direction,
+// independent of its scale. The gradient reaches the network through
+// dL/dz = (2 a / q) ((a / q) J z - r), a = r^T z, q = z^T J z, then dc_a = Phi_a^T dL/dz,
+// dy_a = kappa |g_a| dc_a and, for a guarded sample, its projection orthogonal to g_a. The last layer starts at zero, so the untrained correction vanishes
+// (the plain Schwarz preconditioner) while the gradients of its weights do not.
 //
-// Kernels: per-aggregate Gram-Schmidt basis, projection and prolongation, profile features,
-// batched network forward / backward (FP32 accumulation over BF16-rounded weights and
-// activations), deterministic gradient reduction, Adam. The per-iteration apply kernel lives
-// with the CG kernels.
+// Kernels: per-aggregate Gram-Schmidt basis, projection and prolongation, batched network
+// forward / backward (FP32 accumulation over BF16-rounded weights and activations),
+// deterministic gradient reduction, Adam. The per-iteration projection and apply kernels live
+// with the CG kernels; they run this network through launch_mlp_forward.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -38,33 +38,37 @@ void learned_setup(Learned& L, Model& m, int target_aggregates, const std::vecto
     L.model = &m;
     const int64_t nv = m.nv;
     if (target_aggregates < 1) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "need at least one aggregate");
-    const int64_t target = std::max<int64_t>(1, (nv + target_aggregates - 1) / target_aggregates);
+    const int64_t na64 = std::min<int64_t>(target_aggregates, std::max<int64_t>(nv, 1));
+    const int64_t target = (nv + na64 - 1) / na64;
     if (target > LC_MAX_AGG_NODES)
         throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "aggregates would exceed " + std::to_string(LC_MAX_AGG_NODES) +
                                                        " nodes; ask for more aggregates");
-    std::vector<int32_t> agg(nv, -1);
-    std::vector<int32_t> queue;
-    int32_t na = 0;
+    // breadth-first order of the node graph (restarting at the lowest unvisited node), cut into
+    // na64 contiguous pieces of equal size: exactly the requested number of aggregates, each a run
+    // of consecutive breadth-first layers
+    std::vector<int32_t> order;
+    order.reserve(nv);
+    std::vector<uint8_t> seen(nv, 0);
     const NodePattern& pat = m.pattern;
-    for (int64_t seed_node = 0; seed_node < nv; ++seed_node) {
-        if (agg[seed_node] >= 0) continue;
-        queue.clear();
-        queue.push_back((int32_t)seed_node);
-        agg[seed_node] = na;
-        int64_t cnt = 1;
-        for (size_t qi = 0; qi < queue.size() && cnt < target; ++qi) {
-            const int32_t a = queue[qi];
-            for (int32_t k = pat.nptr[a]; k < pat.nptr[a + 1] && cnt < target; ++k) {
+    for (int64_t root = 0; root < nv; ++root) {
+        if (seen[root]) continue;
+        seen[root] = 1;
+        size_t head = order.size();
+        order.push_back((int32_t)root);
+        for (; head < order.size(); ++head) {
+            const int32_t a = order[head];
+            for (int32_t k = pat.nptr[a]; k < pat.nptr[a + 1]; ++k) {
                 const int32_t b = pat.ncol[k];
-                if (agg[b] < 0) {
-                    agg[b] = na;
-                    queue.push_back(b);
-                    ++cnt;
+                if (!seen[b]) {
+                    seen[b] = 1;
+                    order.push_back(b);
                 }
             }
         }
-        ++na;
     }
+    std::vector<int32_t> agg(nv, -1);
+    for (int64_t pos = 0; pos < nv; ++pos) agg[order[pos]] = (int32_t)(pos * na64 / std::max<int64_t>(nv, 1));
+    const int32_t na = (int32_t)na64;
     L.na = na;
     L.agg_ptr.assign(na + 1, 0);
     for (int64_t v = 0; v < nv; ++v) L.agg_ptr[agg[v] + 1] += 3;
@@ -96,7 +100,6 @@ void learned_setup(Learned& L, Model& m, int target_aggregates, const std::vecto
     for (int i = 0; i < LC_HID * LC_IN; ++i) p[LC_W1 + i] = (float)((2.0 * unif(LC_W1 + i) - 1.0) / std::sqrt((double)LC_IN));
     for (int i = 0; i < LC_HID * LC_HID; ++i)
         p[LC_W2 + i] = (float)((2.0 * unif(LC_W2 + i) - 1.0) / std::sqrt((double)LC_HID));
-    for (int i = 0; i < LC_OUT; ++i) p[LC_B3 + i] = -4.0f;
     learned_set_params(L, p.data(), false);
     L.d_m.alloc(LC_PARAMS);
     L.d_v.alloc(LC_PARAMS);
@@ -105,8 +108,6 @@ void learned_setup(Learned& L, Model& m, int target_aggregates, const std::vecto
     L.adam_t = 0;
     L.d_grad.alloc(LC_PARAMS);
 }
-
-void learned_refresh_gains(Learned& L);
 
 __global__ void k_params_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -126,7 +127,6 @@ void learned_set_params(Learned& L, const float* host_or_dev, bool on_device) {
     }
     k_params_to_bf16<<<dim3((LC_PARAMS + 255) / 256), dim3(256), 0, m.stream>>>(L.d_params.get(), L.d_wbf.get());
     check_launch("k_params_to_bf16");
-    learned_refresh_gains(L);
 }
 
 // ------------------------------------------------------------------ harvesting
@@ -273,7 +273,8 @@ void learned_build_basis(Learned& L) {
 }
 
 // ------------------------------------------------------------------ projection / prolongation
-// G[b][a][j] = bf16(g_j / |g|) with g = Phi_a^T vec_b restricted to a; gnorm[b][a] = |g|
+// G[b][a][j] = g_j / |g| (normalize 1: rounded to BF16, 2: FP32; 0: g itself) with g = Phi_a^T vec_b
+// restricted to a; gnorm[b][a] = |g| when given
 __global__ __launch_bounds__(256) void k_lc_project(int na, const int32_t* __restrict__ agg_ptr,
                                                     const int32_t* __restrict__ agg_dofs, const float* __restrict__ basis,
                                                     const double* __restrict__ vecs, int64_t n, float* __restrict__ G,
@@ -302,8 +303,9 @@ __global__ __launch_bounds__(256) void k_lc_project(int na, const int32_t* __res
         const double nrm = sqrt(q);
         float* out = G + ((int64_t)b * na + a) * LC_IN;
         if (normalize) {
-            out[t] = nrm > 0.0 ? bf16_round((float)(s_g[t] / nrm)) : 0.0f;
-            if (t == 0) gnorm[(int64_t)b * na + a] = (float)nrm;
+            const float xn = nrm > 0.0 ? (float)(s_g[t] / nrm) : 0.0f;
+            out[t] = normalize == 1 ? bf16_round(xn) : xn;
+            if (t == 0 && gnorm) gnorm[(int64_t)b * na + a] = (float)nrm;
         } else {
             out[t] = (float)s_g[t];
         }
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(256) void k_lc_prolong(int na, const int32_t* __res
 constexpr int MLP_TS = 32;  // samples per work-group
 
 // Z1 = W1 x + b1, H1 = bf16(GELU(Z1)), Z2 = W2 H1 + b2, H2 = bf16(GELU(Z2)), Y = W3 H2 + b3
+// (the tape Z1, H1, Z2, H2 for the backward pass is written when Z1 is given)
 __global__ __launch_bounds__(256) void k_mlp_fwd(int64_t N, const float* __restrict__ X, const uint16_t* __restrict__ wbf,
                                                  const float* __restrict__ params, float* __restrict__ Z1,
                                                  float* __restrict__ H1, float* __restrict__ Z2, float* __restrict__ H2,
@@ -361,7 +364,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(int64_t N, const float* __restr
             acc += bias;
             const float h = bf16_round(gelu(acc));
             sh1[s][o] = h;
-            if (s0 + s < N) {
+            if (Z1 && s0 + s < N) {
                 Z1[(s0 + s) * LC_HID + o] = acc;
                 H1[(s0 + s) * LC_HID + o] = h;
             }
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(int64_t N, const float* __restr
             acc += bias;
             const float h = bf16_round(gelu(acc));
             sh2[s][o] = h;
-            if (s0 + s < N) {
+            if (Z2 && s0 + s < N) {
                 Z2[(s0 + s) * LC_HID + o] = acc;
                 H2[(s0 + s) * LC_HID + o] = h;
             }
@@ -610,82 +613,62 @@ __global__ __launch_bounds__(256) void k_lc_dots(int64_t n, const double* __rest
     }
 }
 
-// f[a][j] = sqrt(mean_b G[b][a][j]^2) / sqrt(sum_j mean_b G[b][a][j]^2): how the harvested
-// residuals' projections spread over the aggregate's modes (zero for an empty aggregate)
-__global__ void k_lc_features(int na, int B, const float* __restrict__ G, float* __restrict__ F) {
-    const int a = blockIdx.x, j = threadIdx.x;  // 64 threads
-    double acc = 0.0;
-    for (int b = 0; b < B; ++b) {
-        const double g = G[((int64_t)b * na + a) * LC_IN + j];
-        acc += g * g;
-    }
-    double tot = acc;
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-    F[(int64_t)a * LC_IN + j] = tot > 0.0 ? (float)sqrt(acc / tot) : 0.0f;
-}
-
-__device__ inline float softplus(float y) { return y > 20.0f ? y : log1pf(expf(y)); }
-__device__ inline float sigmoid(float y) { return 1.0f / (1.0f + expf(-y)); }
-
-// C[b][a][j] = softplus(Y[a][j]) * G[b][a][j]
-__global__ void k_lc_apply_gain(int64_t total, int na, const float* __restrict__ Y, const float* __restrict__ G,
-                                float* __restrict__ C) {
+// dY[i] = kappa gnorm[i / 64] dC[i]: the gradient through c = kappa |g| MLP(g / |g|)
+__global__ void k_lc_out_grad(int64_t total, const float* __restrict__ gnorm, const float* __restrict__ dC, float kappa,
+                              float* __restrict__ dY) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    C[i] = softplus(Y[i % ((int64_t)na * LC_IN)]) * G[i];
+    dY[i] = kappa * gnorm[i / LC_OUT] * dC[i];
 }
 
-// dY[a][j] = kappa sigmoid(Y[a][j]) sum_b G[b][a][j] dC[b][a][j]  (gain = kappa softplus(Y))
-__global__ void k_lc_gain_grad(int na, int B, const float* __restrict__ Y, const float* __restrict__ G,
-                               const float* __restrict__ dC, float kappa, float* __restrict__ dY) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)na * LC_IN) return;
-    double acc = 0.0;
-    for (int b = 0; b < B; ++b) acc += (double)G[(int64_t)b * na * LC_IN + i] * (double)dC[(int64_t)b * na * LC_IN + i];
-    dY[i] = (float)(acc * kappa) * sigmoid(Y[i]);
-}
-
-__global__ void k_lc_scale(int64_t n, const float* __restrict__ y, float kappa, float* __restrict__ gain) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) gain[i] = kappa * softplus(y[i]);
-}
-
-// profile features from up to 64 stored residuals
-static void learned_features(Learned& L) {
-    Model& m = *L.model;
-    const hipStream_t s = m.stream;
-    const int64_t n = 3 * m.nv;
-    const int B = (int)std::min<int64_t>(64, L.count);
-    DevBuf<double> r;
-    r.alloc((size_t)B * n);
-    const int64_t oldest = (L.head - L.count + L.cap) % L.cap;
-    for (int b = 0; b < B; ++b) {
-        const int64_t slot = (oldest + (int64_t)b * (L.count - 1) / std::max(1, B - 1)) % L.cap;
-        HIP_CHECK(hipMemcpyAsync(r.get() + (int64_t)b * n, L.d_res.get() + slot * n, sizeof(double) * n,
-                                 hipMemcpyDeviceToDevice, s));
+// The sign guard of the correction c = kappa |g| y: with xf = g / |g|, r^T Phi_a c = kappa |g|^2 xf.y,
+// so a sample with xf.y < 0 has y replaced by its projection y - (xf.y / xf.xf) xf, orthogonal to xf.
+// Every aggregate's correction then adds a non-negative amount to r^T z = r^T P r + sum_a r^T Phi_a c_a,
+// which keeps the preconditioned residual's inner product positive (no IndefinitePreconditioner
+// breakdown). flag[s] records the samples that were projected (for the backward pass). One 64-lane
+// wavefront per sample.
+__global__ __launch_bounds__(256) void k_lc_guard(int64_t N, const float* __restrict__ xf, float* __restrict__ y,
+                                                  uint8_t* __restrict__ flag) {
+    const int64_t smp = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int j = threadIdx.x % 64;
+    if (smp >= N) return;
+    const float xv = xf[smp * LC_IN + j], yv = y[smp * LC_OUT + j];
+    float sy = xv * yv, sq = xv * xv;
+    for (int off = 32; off > 0; off >>= 1) {
+        sy += __shfl_xor(sy, off, 64);
+        sq += __shfl_xor(sq, off, 64);
     }
-    DevBuf<float> G;
-    G.alloc((size_t)B * L.na * LC_IN);
-    k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(),
-                                                                         L.d_basis.get(), r.get(), n, G.get(), nullptr, 0);
-    L.d_feat.alloc((size_t)L.na * LC_IN);
-    k_lc_features<<<dim3((unsigned)L.na), dim3(64), 0, s>>>(L.na, B, G.get(), L.d_feat.get());
-    check_launch("learned features");
-    HIP_CHECK(hipStreamSynchronize(s));
+    const bool proj = sy < 0.0f && sq > 0.0f;
+    if (proj) y[smp * LC_OUT + j] = yv - (sy / sq) * xv;
+    if (flag && j == 0) flag[smp] = proj ? 1 : 0;
 }
 
-// gains = kappa softplus(MLP(features)) for the apply kernel
-void learned_refresh_gains(Learned& L) {
-    Model& m = *L.model;
-    if (!L.d_feat.p) return;
-    L.ensure_batch(L.na);
-    L.d_gain.ensure((size_t)L.na * LC_IN);
-    const unsigned nwg = (unsigned)((L.na + MLP_TS - 1) / MLP_TS);
-    k_mlp_fwd<<<dim3(nwg), dim3(256), 0, m.stream>>>(L.na, L.d_feat.get(), L.d_wbf.get(), L.d_params.get(),
-                                                     L.d_z1.get(), L.d_h1.get(), L.d_z2.get(), L.d_h2.get(), L.d_y.get());
-    const int64_t tot = (int64_t)L.na * LC_IN;
-    k_lc_scale<<<dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, m.stream>>>(tot, L.d_y.get(), L.kappa, L.d_gain.get());
-    check_launch("learned gains");
+// dY <- (I - xf xf^T / xf.xf) dY for the samples the guard projected
+__global__ __launch_bounds__(256) void k_lc_guard_grad(int64_t N, const float* __restrict__ xf,
+                                                       const uint8_t* __restrict__ flag, float* __restrict__ dy) {
+    const int64_t smp = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int j = threadIdx.x % 64;
+    if (smp >= N || !flag[smp]) return;
+    const float xv = xf[smp * LC_IN + j], dv = dy[smp * LC_OUT + j];
+    float sd = xv * dv, sq = xv * xv;
+    for (int off = 32; off > 0; off >>= 1) {
+        sd += __shfl_xor(sd, off, 64);
+        sq += __shfl_xor(sq, off, 64);
+    }
+    dy[smp * LC_OUT + j] = dv - (sd / sq) * xv;
+}
+
+void launch_lc_guard(int64_t N, const float* xf, float* y, uint8_t* flag, hipStream_t s) {
+    if (N <= 0) return;
+    k_lc_guard<<<dim3((unsigned)((N + 3) / 4)), dim3(256), 0, s>>>(N, xf, y, flag);
+    check_launch("k_lc_guard");
+}
+
+void launch_mlp_forward(const LearnedView& lv, int64_t N, const float* X, float* Y, hipStream_t s) {
+    if (N <= 0) return;
+    const unsigned nwg = (unsigned)((N + MLP_TS - 1) / MLP_TS);
+    k_mlp_fwd<<<dim3(nwg), dim3(256), 0, s>>>(N, X, lv.wbf, lv.params, nullptr, nullptr, nullptr, nullptr, Y);
+    check_launch("k_mlp_fwd");
 }
 
 template <class VT>
@@ -696,10 +679,10 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
     const int64_t n = 3 * m.nv;
     if (!L.has_basis) learned_build_basis(L);
     if (L.count < 1) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "no harvested residuals");
-    if (!L.d_feat.p) learned_features(L);
     B = std::max(1, std::min<int>(B, (int)L.count));
-    const int64_t NG = (int64_t)B * L.na * LC_IN;  // projections of a batch
-    L.ensure_batch(L.na);
+    const int64_t NS = (int64_t)B * L.na;   // network samples of a batch: one per (residual, aggregate)
+    const int64_t NG = NS * LC_IN;          // their projections
+    L.ensure_batch(NS);
     DevBuf<double> r, zs, y, Jz, dz, aq;
     r.alloc((size_t)B * n);
     zs.alloc((size_t)B * n);
@@ -707,10 +690,13 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
     Jz.alloc((size_t)B * n);
     dz.alloc((size_t)B * n);
     aq.alloc(2 * (size_t)B);
-    DevBuf<float> G, C, dC;
+    DevBuf<float> G, Gf, gnorm, dC;
+    DevBuf<uint8_t> gflag;
     G.alloc(NG);
-    C.alloc(NG);
+    Gf.alloc(NG);
+    gnorm.alloc(NS);
     dC.alloc(NG);
+    gflag.alloc(NS);
     DevBuf<int32_t> d_slots;
     d_slots.alloc(B);
     DevBuf<double> d_inv;
@@ -735,14 +721,16 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
     const unsigned gn = (unsigned)((n + 255) / 256);
     const unsigned gBn = (unsigned)((B * n + 255) / 256);
     const unsigned gG = (unsigned)((NG + 255) / 256);
-    const unsigned gNa = (unsigned)(((int64_t)L.na * LC_IN + 255) / 256);
-    const unsigned nwg = (unsigned)((L.na + MLP_TS - 1) / MLP_TS);
+    const unsigned nwg = (unsigned)((NS + MLP_TS - 1) / MLP_TS);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     std::vector<int32_t> slots(B);
     std::vector<double> inv(B), haq(2 * (size_t)B), hl(B);
     double last = 0.0;
     uint64_t cursor = 0;
     for (int step = 0; step <= steps; ++step) {
+        // the last pass only evaluates, on the first batch's residuals: the reported losses before and
+        // after training are on the same samples
+        if (step == steps) cursor = 0;
         for (int b = 0; b < B; ++b) {
             const int64_t slot = usable[(cursor + (uint64_t)b * 7919u) % usable.size()];
             slots[b] = (int32_t)slot;
@@ -767,14 +755,17 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
             L.kappa = (float)(k / B);
             if (!(L.kappa > 0.0f) || !std::isfinite(L.kappa)) throw Error(HOEFEM_ERROR_NOT_FINITE, "bad output scale");
         }
-        // gains = kappa softplus(MLP(features)); z = P r + Phi (gains . Phi^T r)
-        k_mlp_fwd<<<dim3(nwg), dim3(256), 0, s>>>(L.na, L.d_feat.get(), L.d_wbf.get(), L.d_params.get(), L.d_z1.get(),
-                                                  L.d_h1.get(), L.d_z2.get(), L.d_h2.get(), L.d_y.get());
+        // g = Phi^T r per aggregate, c = kappa |g| guard(MLP(g / |g|)), z = P r + Phi c
         k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
-            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), r.get(), n, G.get(), nullptr, 0);
-        k_lc_apply_gain<<<dim3(gG), dim3(256), 0, s>>>(NG, L.na, L.d_y.get(), G.get(), C.get());
+            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), r.get(), n, G.get(), gnorm.get(), 1);
+        k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
+            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), r.get(), n, Gf.get(), nullptr, 2);
+        k_mlp_fwd<<<dim3(nwg), dim3(256), 0, s>>>(NS, G.get(), L.d_wbf.get(), L.d_params.get(), L.d_z1.get(),
+                                                  L.d_h1.get(), L.d_z2.get(), L.d_h2.get(), L.d_y.get());
+        launch_lc_guard(NS, Gf.get(), L.d_y.get(), gflag.get(), s);
         k_lc_prolong<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
-            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), C.get(), nullptr, L.kappa, n, y.get());
+            L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), L.d_y.get(), gnorm.get(), L.kappa, n,
+            y.get());
         k_lc_add<<<dim3(gBn), dim3(256), 0, s>>>(B * n, zs.get(), y.get());  // y <- z = zs + y
         for (int b = 0; b < B; ++b) launch_spmv(m.pattern, Jvals, 0.0, Jz.get() + (int64_t)b * n, 1.0, y.get() + (int64_t)b * n, s);
         check_launch("learned training forward");
@@ -792,9 +783,9 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
         k_lc_graddz<<<dim3(gBn), dim3(256), 0, s>>>(n, B, aq.get(), Jz.get(), r.get(), dz.get());
         k_lc_project<<<dim3((unsigned)L.na, (unsigned)B), dim3(256), 0, s>>>(
             L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), dz.get(), n, dC.get(), nullptr, 0);
-        k_lc_gain_grad<<<dim3(gNa), dim3(256), 0, s>>>(L.na, B, L.d_y.get(), G.get(), dC.get(), L.kappa,
-                                                       L.d_dy.get());
-        k_mlp_bwd<<<dim3(nwg), dim3(256), 0, s>>>(L.na, L.d_feat.get(), L.d_wbf.get(), L.d_z1.get(), L.d_h1.get(),
+        k_lc_out_grad<<<dim3(gG), dim3(256), 0, s>>>(NG, gnorm.get(), dC.get(), L.kappa, L.d_dy.get());
+        k_lc_guard_grad<<<dim3((unsigned)((NS + 3) / 4)), dim3(256), 0, s>>>(NS, Gf.get(), gflag.get(), L.d_dy.get());
+        k_mlp_bwd<<<dim3(nwg), dim3(256), 0, s>>>(NS, G.get(), L.d_wbf.get(), L.d_z1.get(), L.d_h1.get(),
                                                   L.d_z2.get(), L.d_h2.get(), L.d_dy.get(), L.d_gpart.get());
         k_mlp_reduce<<<dim3((LC_PARAMS + 255) / 256), dim3(256), 0, s>>>((int)nwg, L.d_gpart.get(), 1.0f / (float)B,
                                                                         L.d_grad.get());
@@ -805,7 +796,6 @@ static double learned_train_t(Learned& L, const VT* Jvals, Schwarz* P, int steps
                                                                    c1, c2);
         check_launch("learned training step");
     }
-    learned_refresh_gains(L);
     HIP_CHECK(hipStreamSynchronize(s));
     L.trained = true;
     return last;
@@ -821,8 +811,14 @@ double learned_train(Learned& L, const float* J, Schwarz* P, int steps, double l
     return learned_train_t<float>(L, J, P, steps, lr, B, initial_loss);
 }
 
-LearnedView learned_view(const Learned& L) {
-    return LearnedView{L.na, L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(), L.d_gain.get()};
+LearnedView learned_view(Learned& L) {
+    L.d_cg_x.ensure((size_t)L.na * LC_IN);
+    L.d_cg_xf.ensure((size_t)L.na * LC_IN);
+    L.d_cg_gn.ensure((size_t)L.na);
+    L.d_cg_y.ensure((size_t)L.na * LC_OUT);
+    return LearnedView{L.na,           L.kappa,         L.d_agg_ptr.get(), L.d_agg_dofs.get(), L.d_basis.get(),
+                       L.d_wbf.get(),  L.d_params.get(), L.d_cg_x.get(),   L.d_cg_xf.get(),    L.d_cg_gn.get(),
+                       L.d_cg_y.get()};
 }
 
 }  // namespace hoefem

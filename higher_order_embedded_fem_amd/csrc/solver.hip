@@ -480,10 +480,13 @@ struct SegIdx {
     int s;
     int64_t bid, nblk;
 };
-// (s = -1: a padding work-group between two segments of a batched launch, which returns at once)
-template <class VT>
-__device__ inline SegIdx seg_index(const CgSeg<VT>* segs, const int* seg_of, bool a_side) {
-    const int s = seg_of ? seg_of[blockIdx.x] : 0;
+// (s = -1: a padding work-group between two segments of a batched launch, which returns at once).
+// A single solve (SEG false) takes its one segment by value, in the kernel arguments: read by the
+// dispatch with the launch, no dependent load of a descriptor from memory at every kernel's start.
+template <bool SEG, class VT>
+__device__ inline SegIdx seg_index(const CgSeg<VT>& one, const CgSeg<VT>* segs, const int* seg_of, bool a_side) {
+    if (!SEG) return a_side ? SegIdx{0, (int64_t)blockIdx.x, one.nb_a} : SegIdx{0, (int64_t)blockIdx.x, one.nb_b};
+    const int s = seg_of[blockIdx.x];
     if (s < 0) return SegIdx{-1, 0, 0};
     const CgSeg<VT>& S = segs[s];
     return a_side ? SegIdx{s, (int64_t)blockIdx.x - S.base_a, S.nb_a}
@@ -491,11 +494,12 @@ __device__ inline SegIdx seg_index(const CgSeg<VT>* segs, const int* seg_of, boo
 }
 
 // preamble 1: r = b - A x -> ra_new.r ; partial |b|^2
-template <int G, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of) {
-    const SegIdx I = seg_index(segs, seg_of, true);
+template <int G, class VT, bool SEG>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(const CgSeg<VT> one, const CgSeg<VT>* __restrict__ segs,
+                                                        const int* __restrict__ seg_of) {
+    const SegIdx I = seg_index<SEG>(one, segs, seg_of, true);
     if (I.s < 0) return;
-    const CgSeg<VT>& S = segs[I.s];
+    const CgSeg<VT>& S = SEG ? segs[I.s] : one;
     if (S.sc->status != CG_RUNNING) return;  // a segment without a solve in this batch
     const Csr3<VT> A = S.A;
     const CgPtrs P = seg_ptrs(S, 0, 0);
@@ -529,11 +533,12 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(const CgSeg<VT>* __restr
 }
 
 // preamble 2: |b| ; z = P r (r from ra_old) -> zp_new.z ; partial zTr, |r|^2
-template <int G, bool JAC, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of) {
-    const SegIdx I = seg_index(segs, seg_of, false);
+template <int G, bool JAC, bool FLEX, class VT, bool SEG>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(const CgSeg<VT> one, const CgSeg<VT>* __restrict__ segs,
+                                                        const int* __restrict__ seg_of) {
+    const SegIdx I = seg_index<SEG>(one, segs, seg_of, false);
     if (I.s < 0) return;
-    const CgSeg<VT>& S = segs[I.s];
+    const CgSeg<VT>& S = SEG ? segs[I.s] : one;
     if (S.sc->status != CG_RUNNING) return;
     const Csr3<VT> Pm = S.Pm;
     const CgPtrs P = seg_ptrs(S, 1, 0);
@@ -584,12 +589,12 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(const CgSeg<VT>* __restr
 // p_k = z_k + beta p_{k-1} ; Ap_k = A p_k ; partial pAp. The partial sums are reduced before the
 // first entry is issued: issuing the entry early (as the B-kernel does) measured 0.25 us slower
 // per launch with 9 MB more HBM fetch on the Hex20 cylinder.
-template <int G, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of,
-                                                       int cur) {
-    const SegIdx I = seg_index(segs, seg_of, true);
+template <int G, bool FLEX, class VT, bool SEG>
+__global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT> one, const CgSeg<VT>* __restrict__ segs,
+                                                       const int* __restrict__ seg_of, int cur) {
+    const SegIdx I = seg_index<SEG>(one, segs, seg_of, true);
     if (I.s < 0) return;
-    const CgSeg<VT>& S = segs[I.s];
+    const CgSeg<VT>& S = SEG ? segs[I.s] : one;
     CgScalars* sc = S.sc;
     if (sc->status != CG_RUNNING) return;
     const Csr3<VT> A = S.A;
@@ -669,12 +674,12 @@ __global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT>* __restri
 
 // B-kernel of iteration k: indefiniteness tests, alpha = zTr / pAp ; x += alpha p_k ;
 // r_{k+1} = r_k - alpha Ap_k ; z_{k+1} = P r_{k+1} ; partial zTr, |r|^2
-template <int G, bool JAC, bool FLEX, class VT>
-__global__ __launch_bounds__(CG_THREADS) void k_cg_b(const CgSeg<VT>* __restrict__ segs, const int* __restrict__ seg_of,
-                                                     int cur) {
-    const SegIdx I = seg_index(segs, seg_of, false);
+template <int G, bool JAC, bool FLEX, class VT, bool SEG>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_b(const CgSeg<VT> one, const CgSeg<VT>* __restrict__ segs,
+                                                     const int* __restrict__ seg_of, int cur) {
+    const SegIdx I = seg_index<SEG>(one, segs, seg_of, false);
     if (I.s < 0) return;
-    const CgSeg<VT>& S = segs[I.s];
+    const CgSeg<VT>& S = SEG ? segs[I.s] : one;
     const Csr3<VT> Pm = S.Pm;
     const CgPtrs P = seg_ptrs(S, 2, cur);
     CgScalars* sc = P.sc;
@@ -771,15 +776,15 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(const CgSeg<VT>* __restrict
     if (!FLEX) block_partials<2>(part, P.part_b, I.bid);
 }
 
-// Learned coarse correction, one work-group per aggregate: g = Phi_a^T r_new,
-// z += Phi_a (gain_a . g) on the aggregate's DOFs (z holds P r_new), then the partial sums
-// z.r_new, z.r_old and |r_new|^2 of the iteration. Every DOF belongs to exactly one aggregate,
-// so the work-groups' partials add up to the full dot products.
-__global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* __restrict__ ra_new,
-                                                  const double* __restrict__ ra_old, double* __restrict__ zp,
-                                                  double* __restrict__ partials) {
+// Learned coarse correction of an iteration, in four launches: k_lc_project_cg (one work-group
+// per aggregate) forms the network input x_a = bf16(g / |g|), g = Phi_a^T r_new, and |g|; the network
+// maps every x_a to y_a (launch_mlp_forward) and the sign guard (launch_lc_guard) removes a component
+// of y_a against g_a; k_lc_apply adds z += Phi_a (kappa |g| y_a) on the aggregate's DOFs (z holds P r_new)
+// and writes the partial sums z.r_new, z.r_old and |r_new|^2 of the iteration. Every DOF belongs to exactly one aggregate, so the work-groups' partials add up to
+// the full dot products.
+__global__ __launch_bounds__(256) void k_lc_project_cg(LearnedView lv, const double* __restrict__ ra_new) {
     __shared__ double s_r[LC_MAX_AGG_NODES * 3];
-    __shared__ double s_c[LC_IN];
+    __shared__ double s_g[LC_IN];
     const int a = blockIdx.x;
     const int o = lv.agg_ptr[a], nd = lv.agg_ptr[a + 1] - o;
     const int t = threadIdx.x;
@@ -795,9 +800,30 @@ __global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* 
         for (int d = t % 4; d < nd; d += 4) c += (double)B[j * nd + d] * s_r[d];
         c += __shfl_xor(c, 1, 4);
         c += __shfl_xor(c, 2, 4);
-        if (t % 4 == 0) s_c[j] = c * (double)lv.gain[(int64_t)a * LC_IN + j];
+        if (t % 4 == 0) s_g[j] = c;
     }
     __syncthreads();
+    if (t < 64) {
+        double q = s_g[t] * s_g[t];
+        for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+        const double nrm = sqrt(q);
+        const float xn = nrm > 0.0 ? (float)(s_g[t] / nrm) : 0.0f;
+        lv.x[(int64_t)a * LC_IN + t] = bf16_round(xn);
+        lv.xf[(int64_t)a * LC_IN + t] = xn;
+        if (t == 0) lv.gn[a] = (float)nrm;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* __restrict__ ra_new,
+                                                  const double* __restrict__ ra_old, double* __restrict__ zp,
+                                                  double* __restrict__ partials) {
+    __shared__ double s_c[LC_IN];
+    const int a = blockIdx.x;
+    const int o = lv.agg_ptr[a], nd = lv.agg_ptr[a + 1] - o;
+    const int t = threadIdx.x;
+    if (t < LC_IN) s_c[t] = (double)(lv.kappa * lv.gn[a] * lv.y[(int64_t)a * LC_OUT + t]);
+    __syncthreads();
+    const float* B = lv.basis + (int64_t)LC_IN * o;
     double part[3] = {0.0, 0.0, 0.0};
     for (int d = t; d < nd; d += 256) {
         double corr = 0.0;
@@ -806,13 +832,21 @@ __global__ __launch_bounds__(256) void k_lc_apply(LearnedView lv, const double* 
         const int64_t zi = 6 * (int64_t)(dof / 3) + dof % 3;
         const double z = zp[zi] + corr;
         zp[zi] = z;
-        const double rn = s_r[d];
+        const double rn = ra_new[zi];
         const double ro = ra_old ? ra_old[zi] : 0.0;
         part[0] += z * rn;
         part[1] += z * ro;
         part[2] += rn * rn;
     }
     block_partials<3>(part, partials, blockIdx.x);
+}
+
+static void launch_lc_correction(const LearnedView& lv, const double* ra_new, const double* ra_old, double* zp,
+                                 double* partials, hipStream_t s) {
+    k_lc_project_cg<<<dim3((unsigned)lv.na), dim3(256), 0, s>>>(lv, ra_new);
+    launch_mlp_forward(lv, lv.na, lv.x, lv.y, s);
+    launch_lc_guard(lv.na, lv.xf, lv.y, nullptr, s);
+    k_lc_apply<<<dim3((unsigned)lv.na), dim3(256), 0, s>>>(lv, ra_new, ra_old, zp, partials);
 }
 
 __global__ void k_cg_init(CgScalars* sc, double tol, int64_t max_iter) {
@@ -868,7 +902,6 @@ void CgWork::ensure(int64_t nn_dof, int nblocks) {
         scalars.alloc(1);
         HIP_CHECK(hipMemset(scalars.get(), 0, sizeof(CgScalars)));
     }
-    if (!d_seg.p) d_seg.alloc(SEG_BYTES);
     if (!ev[0]) {
         HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -911,37 +944,30 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     HIP_CHECK(hipStreamWaitEvent(s, w.join[0], 0));
     const unsigned blkA = cg_grid(A.n_block, A.n_diag, GA, CG_THREADS_A);
     const unsigned blkB = JAC ? cg_grid(0, Pm.nn, GB) : cg_grid(Pm.n_block, Pm.n_diag, GB);
-    // the solve as a batch of one segment; its descriptor goes to the device through pinned memory
-    // (the previous solve synchronized the stream, so the staging copy is free)
-    CgSeg<VT>* hs_seg = reinterpret_cast<CgSeg<VT>*>(w.h_seg.p);
-    *hs_seg = CgSeg<VT>{};
-    hs_seg->A = A;
-    hs_seg->Pm = Pm;
-    hs_seg->x = x;
-    hs_seg->b = b;
-    hs_seg->dinv = w.diag_inv.get();
+    // the solve's one segment, passed by value in the kernel arguments
+    CgSeg<VT> seg{};
+    seg.A = A;
+    seg.Pm = Pm;
+    seg.x = x;
+    seg.b = b;
+    seg.dinv = w.diag_inv.get();
     for (int k = 0; k < 2; ++k) {
-        hs_seg->zp[k] = w.zp[k].get();
-        hs_seg->ra[k] = w.ra[k].get();
+        seg.zp[k] = w.zp[k].get();
+        seg.ra[k] = w.ra[k].get();
     }
-    hs_seg->part_pre = w.part_pre.get();
-    hs_seg->part_a = w.part_a.get();
-    hs_seg->part_b = w.part_b.get();
-    hs_seg->sc = w.scalars.get();
-    hs_seg->host_status = w.h_status.p;
-    hs_seg->nb_a = (int)blkA;
-    hs_seg->nb_b = (int)blkB;
-    hs_seg->nparts_b = FLEX ? lv->na : (int)blkB;
-    static_assert(sizeof(CgSeg<VT>) <= CgWork::SEG_BYTES, "segment descriptor staging too small");
-    const CgSeg<VT>* segs = reinterpret_cast<const CgSeg<VT>*>(w.d_seg.get());
-    HIP_CHECK(hipMemcpyAsync(w.d_seg.get(), w.h_seg.p, sizeof(CgSeg<VT>), hipMemcpyHostToDevice, s));
+    seg.part_pre = w.part_pre.get();
+    seg.part_a = w.part_a.get();
+    seg.part_b = w.part_b.get();
+    seg.sc = w.scalars.get();
+    seg.host_status = w.h_status.p;
+    seg.nb_a = (int)blkA;
+    seg.nb_b = (int)blkB;
+    seg.nparts_b = FLEX ? lv->na : (int)blkB;
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, w.scalars.get(), tol, max_iter);
     HIP_CHECK(hipMemsetAsync(w.zp[0].get(), 0, sizeof(double) * 2 * w.n, s));  // p_{-1} = 0
-    k_cg_pre1<GA, VT><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
-    k_cg_pre2<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
-    if (FLEX)
-        k_lc_apply<<<dim3((unsigned)lv->na), dim3(256), 0, s>>>(*lv, w.ra[0].get(), nullptr, w.zp[0].get(),
-                                                               w.part_b.get());
+    k_cg_pre1<GA, VT, false><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(seg, nullptr, nullptr);
+    k_cg_pre2<GB, JAC, FLEX, VT, false><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(seg, nullptr, nullptr);
+    if (FLEX) launch_lc_correction(*lv, w.ra[0].get(), nullptr, w.zp[0].get(), w.part_b.get(), s);
     check_launch("cg preamble");
     std::vector<int64_t> snap_k;  // iteration index of every snapshot taken
     int cur = 0;
@@ -951,14 +977,21 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
     // lives on the device and CG_BATCH is even, so the buffer parity repeats); per-iteration
     // hooks (kernel timing, snapshots) need the direct launches
     const bool graph = w.use_graphs && !m.prof && !h;
-    // the batch's kernels take the segment descriptor, so the captured graph does not depend on the
-    // matrices or vectors of a solve, only on the launch shapes and the learned correction's arguments
-    const std::vector<const void*> key = {(const void*)segs, (const void*)(intptr_t)blkA, (const void*)(intptr_t)blkB,
-                                          (const void*)(intptr_t)GA, (const void*)(intptr_t)GB,
-                                          (const void*)(intptr_t)JAC, (const void*)(intptr_t)FLEX,
-                                          (const void*)(intptr_t)(sizeof(VT)), w.zp[0].get(), w.ra[0].get(),
-                                          lv ? lv->basis : nullptr, lv ? lv->gain : nullptr,
-                                          (const void*)(intptr_t)(lv ? lv->na : 0)};
+    // the captured batch holds the kernels' arguments: the descriptor (matrices, vectors, grids),
+    // the tile shapes and the learned correction's arguments
+    std::vector<const void*> key = {(const void*)(intptr_t)GA, (const void*)(intptr_t)GB,
+                                    (const void*)(intptr_t)JAC, (const void*)(intptr_t)FLEX,
+                                    (const void*)(intptr_t)(sizeof(VT))};
+    if (lv) {
+        static_assert(sizeof(LearnedView) % sizeof(void*) == 0, "view words");
+        const void* const* words = reinterpret_cast<const void* const*>(lv);
+        key.insert(key.end(), words, words + sizeof(LearnedView) / sizeof(void*));
+    }
+    {
+        static_assert(sizeof(CgSeg<VT>) % sizeof(void*) == 0, "descriptor words");
+        const void* const* words = reinterpret_cast<const void* const*>(&seg);
+        key.insert(key.end(), words, words + sizeof(CgSeg<VT>) / sizeof(void*));
+    }
     auto launch_batch = [&]() {
         for (int q = 0; q < CG_BATCH; ++q) {
             if (m.prof) {
@@ -969,16 +1002,17 @@ static void cg_launch_all(Model& m, CgWork& w, const Csr3<VT>& A, const Csr3<VT>
                     HIP_CHECK(hipEventCreate(&e));
                     m.prof_ev.push_back(e);
                 }
-                hipExtLaunchKernelGGL(k_cg_a<GA, FLEX, VT>, dim3(blkA), dim3(CG_THREADS_A), 0, s, m.prof_ev[nev],
-                                      m.prof_ev[nev + 1], 0, segs, (const int*)nullptr, cur);
+                hipExtLaunchKernelGGL(k_cg_a<GA, FLEX, VT, false>, dim3(blkA), dim3(CG_THREADS_A), 0, s,
+                                      m.prof_ev[nev], m.prof_ev[nev + 1], 0, seg, (const CgSeg<VT>*)nullptr,
+                                      (const int*)nullptr, cur);
                 nev += 2;
             } else {
-                k_cg_a<GA, FLEX, VT><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(segs, nullptr, cur);
+                k_cg_a<GA, FLEX, VT, false><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(seg, nullptr, nullptr, cur);
             }
-            k_cg_b<GB, JAC, FLEX, VT><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr, cur);
+            k_cg_b<GB, JAC, FLEX, VT, false><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(seg, nullptr, nullptr, cur);
             if (FLEX)
-                k_lc_apply<<<dim3((unsigned)lv->na), dim3(256), 0, s>>>(*lv, w.ra[1 - cur].get(), w.ra[cur].get(),
-                                                                       w.zp[1 - cur].get(), w.part_b.get());
+                launch_lc_correction(*lv, w.ra[1 - cur].get(), w.ra[cur].get(), w.zp[1 - cur].get(), w.part_b.get(),
+                                     s);
             cur ^= 1;
             ++k_next;
             if (h && h->taken + (int)snap_k.size() < h->max && k_next % h->every == 0) {
@@ -1124,15 +1158,14 @@ __global__ void k_fill(int64_t n, double* __restrict__ a, double value);
 // on one GPU race (different models included).
 constexpr int64_t NOALLOC_MAX_DOFS = 262144;
 constexpr int64_t NOALLOC_VEC = 2 * NOALLOC_MAX_DOFS;  // one interleaved [z | p] or [r | Ap] buffer
-// layout in doubles: iteration state, the kernels' status word, the segment descriptor, partial
-// sums, four CG vectors, the Jacobi diagonal (every region starts on a 256-byte boundary)
-constexpr int64_t NA_SCALARS = 0, NA_STATUS = 32, NA_SEG = 64, NA_PART_PRE = 128;
+// layout in doubles: iteration state, the kernels' status word, partial sums, four CG vectors, the
+// Jacobi diagonal (every region starts on a 256-byte boundary)
+constexpr int64_t NA_SCALARS = 0, NA_STATUS = 32, NA_PART_PRE = 64;
 constexpr int64_t NA_PART_A = NA_PART_PRE + CG_MAX_BLOCKS, NA_PART_B = NA_PART_A + CG_MAX_BLOCKS;
 constexpr int64_t NA_VEC0 = NA_PART_B + 3 * CG_MAX_BLOCKS;
 constexpr int64_t NA_DIAG = NA_VEC0 + 4 * NOALLOC_VEC;
 constexpr int64_t NA_TOTAL = NA_DIAG + NOALLOC_MAX_DOFS;  // 9 * 262144 + 5248 doubles = 18.9 MB
 static_assert(sizeof(CgScalars) <= 32 * sizeof(double), "iteration state does not fit its region");
-static_assert(sizeof(CgSeg<double>) <= 64 * sizeof(double), "segment descriptor does not fit its region");
 static_assert(NA_PART_PRE % 32 == 0 && NA_VEC0 % 32 == 0 && NOALLOC_VEC % 32 == 0, "region alignment");
 
 static __device__ __align__(256) double scratch[NA_TOTAL];
@@ -1167,7 +1200,7 @@ static hoefem_status cg_launch_noalloc(Model& m, double* base, const Csr3<double
     double* zp[2] = {base + NA_VEC0, base + NA_VEC0 + NOALLOC_VEC};
     double* ra[2] = {base + NA_VEC0 + 2 * NOALLOC_VEC, base + NA_VEC0 + 3 * NOALLOC_VEC};
     CgScalars* sc = (CgScalars*)(base + NA_SCALARS);
-    CgSeg<double> seg{};  // copied into the scratch array below (pageable source: a synchronous copy)
+    CgSeg<double> seg{};  // passed by value in the kernel arguments
     seg.A = A;
     seg.Pm = Pm;
     seg.x = x;
@@ -1184,20 +1217,18 @@ static hoefem_status cg_launch_noalloc(Model& m, double* base, const Csr3<double
     seg.host_status = (int*)(base + NA_STATUS);  // a device word here: the status is polled from sc
     seg.nb_a = (int)blkA;
     seg.nb_b = seg.nparts_b = (int)blkB;
-    const CgSeg<double>* segs = (const CgSeg<double>*)(base + NA_SEG);
-    HIP_CHECK(hipMemcpyAsync(base + NA_SEG, &seg, sizeof(seg), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, s, sc, tol, max_iter);
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((2 * n + 255) / 256)), dim3(256), 0, s, 2 * n, zp[0], 0.0);  // p_{-1}
-    k_cg_pre1<GA, double><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
-    k_cg_pre2<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr);
+    k_cg_pre1<GA, double, false><<<dim3(blkA), dim3(CG_THREADS), 0, s>>>(seg, nullptr, nullptr);
+    k_cg_pre2<GB, JAC, false, double, false><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(seg, nullptr, nullptr);
     check_launch("cg preamble");
     int cur = 0, status = CG_RUNNING;
     int64_t launched = 0;
     const int64_t max_batches = max_iter >= 0 ? max_iter / CG_BATCH + 2 : INT64_MAX;
     for (int64_t batch = 0; batch < max_batches && status == CG_RUNNING; ++batch) {
         for (int q = 0; q < CG_BATCH; ++q) {
-            k_cg_a<GA, false, double><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(segs, nullptr, cur);
-            k_cg_b<GB, JAC, false, double><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(segs, nullptr, cur);
+            k_cg_a<GA, false, double, false><<<dim3(blkA), dim3(CG_THREADS_A), 0, s>>>(seg, nullptr, nullptr, cur);
+            k_cg_b<GB, JAC, false, double, false><<<dim3(blkB), dim3(CG_THREADS), 0, s>>>(seg, nullptr, nullptr, cur);
             cur ^= 1;
         }
         launched += CG_BATCH;
@@ -1296,45 +1327,26 @@ hoefem_status pcg_solve_learned(Model& m, CgWork& w, const float* A, Schwarz* P,
 // descriptors are uploaded before each solve, so a captured batch of iterations depends only on the
 // launch shapes and is replayed for any systems of the same shapes.
 
-// x = 0, [z | p] = 0 (p_{-1}) and the iteration state of every segment (grid.y = segment)
-template <class VT>
-__global__ __launch_bounds__(256) void k_cg_begin(const CgSeg<VT>* __restrict__ segs) {
-    const CgSeg<VT>& S = segs[blockIdx.y];
-    const int64_t n = 3 * S.A.nn;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += (int64_t)gridDim.x * blockDim.x) {
-        S.zp[0][i] = 0.0;
-        if (i < n) S.x[i] = 0.0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        CgScalars* sc = S.sc;
-        sc->tol = S.tol;
-        sc->max_iter = S.max_iter;
-        sc->status = CG_RUNNING;
-        sc->iters = 0;
-        sc->kcur = 0;
-        sc->beta = 0.0;
-        sc->zrec[0] = sc->zrec[1] = 0.0;
-    }
-}
-
 template <class VT>
 struct SegOps {
-    void (*pre)(const CgSeg<VT>*, const int*, unsigned, const int*, unsigned, hipStream_t);
     void (*a)(const CgSeg<VT>*, const int*, unsigned, int, hipStream_t);
     void (*b)(const CgSeg<VT>*, const int*, unsigned, int, hipStream_t);
+    // the preamble of one system, its descriptor by value (the kernels of a single solve)
+    void (*pre)(const CgSeg<VT>&, hipStream_t);
 };
 
 template <int GA, int GB, bool JAC, class VT>
 struct SegLaunch {
-    static void pre(const CgSeg<VT>* s, const int* ma, unsigned na, const int* mb, unsigned nb, hipStream_t st) {
-        k_cg_pre1<GA, VT><<<dim3(na), dim3(CG_THREADS), 0, st>>>(s, ma);
-        k_cg_pre2<GB, JAC, false, VT><<<dim3(nb), dim3(CG_THREADS), 0, st>>>(s, mb);
-    }
     static void a(const CgSeg<VT>* s, const int* ma, unsigned na, int cur, hipStream_t st) {
-        k_cg_a<GA, false, VT><<<dim3(na), dim3(CG_THREADS_A), 0, st>>>(s, ma, cur);
+        k_cg_a<GA, false, VT, true><<<dim3(na), dim3(CG_THREADS_A), 0, st>>>(CgSeg<VT>{}, s, ma, cur);
     }
     static void b(const CgSeg<VT>* s, const int* mb, unsigned nb, int cur, hipStream_t st) {
-        k_cg_b<GB, JAC, false, VT><<<dim3(nb), dim3(CG_THREADS), 0, st>>>(s, mb, cur);
+        k_cg_b<GB, JAC, false, VT, true><<<dim3(nb), dim3(CG_THREADS), 0, st>>>(CgSeg<VT>{}, s, mb, cur);
+    }
+    static void pre(const CgSeg<VT>& one, hipStream_t st) {
+        k_cg_pre1<GA, VT, false><<<dim3((unsigned)one.nb_a), dim3(CG_THREADS), 0, st>>>(one, nullptr, nullptr);
+        k_cg_pre2<GB, JAC, false, VT, false><<<dim3((unsigned)one.nb_b), dim3(CG_THREADS), 0, st>>>(one, nullptr,
+                                                                                                    nullptr);
     }
 };
 
@@ -1342,7 +1354,7 @@ template <class VT>
 static SegOps<VT> seg_ops(int GA, int GB, bool JAC) {
 #define SEG_CASE(ga, gb, j) \
     if (GA == ga && GB == gb && JAC == j) \
-        return SegOps<VT>{&SegLaunch<ga, gb, j, VT>::pre, &SegLaunch<ga, gb, j, VT>::a, &SegLaunch<ga, gb, j, VT>::b};
+        return SegOps<VT>{&SegLaunch<ga, gb, j, VT>::a, &SegLaunch<ga, gb, j, VT>::b, &SegLaunch<ga, gb, j, VT>::pre};
     SEG_CASE(8, 8, true) SEG_CASE(16, 8, true) SEG_CASE(32, 8, true)
     SEG_CASE(8, 8, false) SEG_CASE(8, 16, false) SEG_CASE(8, 32, false)
     SEG_CASE(16, 8, false) SEG_CASE(16, 16, false) SEG_CASE(16, 32, false)
@@ -1351,47 +1363,75 @@ static SegOps<VT> seg_ops(int GA, int GB, bool JAC) {
     throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "unsupported CG configuration");
 }
 
-void CgBatch::ensure(size_t seg_bytes, size_t map_ints, size_t streams) {
+// work-groups of the two kernels of one system, padded to the next multiple of 8 (upper bound)
+constexpr int64_t CG_SEG_MAX_BLOCKS = CG_MAX_BLOCKS + (int64_t)CG_MAX_BLOCKS * CG_THREADS / CG_THREADS_A + 16;
+
+void CgBatch::ensure(size_t seg_bytes, size_t map_ints) {
     if (!cs) {
         HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&upload, hipEventDisableTiming));
     }
-    while (joins.size() < streams) {
-        hipEvent_t e;
-        HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        joins.push_back(e);
+    if (seg_bytes > seg_cap || map_ints > map_cap) {
+        HIP_CHECK(hipStreamSynchronize(cs));  // no launch reads the buffers being replaced
+        for (Graph& g : graphs)                // and no captured batch refers to them afterwards
+            if (g.exec) HIP_CHECK(hipGraphExecDestroy(g.exec));
+        graphs.clear();
+        if (seg_bytes > seg_cap) {
+            seg_cap = std::max(seg_bytes, 2 * seg_cap);
+            h_seg.reset(new PinnedBuf<char>(seg_cap));
+            d_seg.alloc(seg_cap);
+        }
+        if (map_ints > map_cap) {
+            map_cap = std::max(map_ints, 2 * map_cap);
+            h_map.reset(new PinnedBuf<int>(map_cap));
+            d_map.alloc(map_cap);
+        }
     }
-    if (seg_bytes > seg_cap) {
-        seg_cap = std::max(seg_bytes, 2 * seg_cap);
-        h_seg.reset(new PinnedBuf<char>(seg_cap));
-        d_seg.alloc(seg_cap);
+}
+
+hipEvent_t CgBatch::take_event() {
+    if (!spare.empty()) {
+        hipEvent_t e = spare.back();
+        spare.pop_back();
+        return e;
     }
-    if (map_ints > map_cap) {
-        map_cap = std::max(map_ints, 2 * map_cap);
-        h_map.reset(new PinnedBuf<int>(map_cap));
-        d_map.alloc(map_cap);
-    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
 }
 
 CgBatch::~CgBatch() {
+    if (cs) (void)hipStreamSynchronize(cs);
     for (Graph& g : graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+    for (hipEvent_t e : spare) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev)
         if (e) (void)hipEventDestroy(e);
-    if (done) (void)hipEventDestroy(done);
+    if (upload) (void)hipEventDestroy(upload);
     if (cs) (void)hipStreamDestroy(cs);
 }
 
+// The pool of a batch: systems join when their inputs are ready and leave when their solve ends.
+// Each system's preamble runs as in its own solve (its descriptor by value); its iterations run in
+// the batched launches of the current layout (groups of equal tile shape, segments on multiples of 8
+// work-groups), so its x, iteration count and status equal pcg_solve's bit for bit. The host keeps
+// two batches of CG_BATCH iterations in flight and never blocks: it queries the batches' events, reads
+// the statuses the kernels write into pinned memory, and re-lays the launches out when systems join
+// (at once) or leave (once the running ones hold at most 3/4 of the launched work-groups). The staging
+// buffers are rewritten only after their previous upload has executed; the device copies are ordered
+// after every batch enqueued before them.
 template <class VT>
-static void pcg_batch_t(CgBatch& B, std::vector<CgBatchSystem>& sys) {
+struct CgPoolT final : CgPool {
     struct Item {
-        int sys, ga, gb;
-        bool jac;
-        unsigned blkA, blkB;
+        CgBatchSystem q;
+        int ga = 8, gb = 8;
+        bool jac = false;
+        unsigned blkA = 0, blkB = 0;
         Csr3<VT> A, Pm;
+        SegOps<VT> ops{};
+        int state = 0;  // 0 waiting to join, 1 running, 2 stopped (result copy pending), 3 ended
     };
     struct Group {
         int ga, gb;
@@ -1401,226 +1441,330 @@ static void pcg_batch_t(CgBatch& B, std::vector<CgBatchSystem>& sys) {
         int seg0 = 0;
         int64_t mapA = 0, nA = 0, mapB = 0, nB = 0;
     };
+    struct Layout {
+        std::vector<Group> groups;
+        int nseg = 0;
+        int64_t map_total = 0, blocks = 0;
+        std::vector<int64_t> key;
+    };
+    struct Copy {
+        hipEvent_t ev;
+        std::vector<int> ids;
+    };
+    CgBatch& B;
     std::vector<Item> items;
-    std::vector<Group> groups;
-    std::vector<hipStream_t> streams;
-    for (size_t i = 0; i < sys.size(); ++i) {
-        CgBatchSystem& q = sys[i];
-        q.iterations = 0;
-        q.status = HOEFEM_OK;
-        Model& m = *q.m;
+    std::vector<int> waiting, running;
+    std::vector<Copy> copies;
+    Layout lay;
+    bool layout_dirty = false, uploaded = false;
+    int inflight = 0, oldest = 0, next_slot = 0, cap = 1;
+
+    CgPoolT(CgBatch& b, int capacity) : B(b) {
+        // at most `capacity` systems run at once (one solve per scene at a time)
+        B.ensure(sizeof(CgSeg<VT>) * (size_t)std::max(capacity, 1), (size_t)std::max(capacity, 1) * CG_SEG_MAX_BLOCKS);
+        cap = std::max(capacity, 1);
+    }
+    ~CgPoolT() override {
+        if (B.cs) (void)hipStreamSynchronize(B.cs);
+        for (Copy& c : copies) B.spare.push_back(c.ev);
+    }
+
+    int join(const CgBatchSystem& s) override {
+        Item it;
+        it.q = s;
+        it.q.iterations = 0;
+        it.q.status = HOEFEM_OK;
+        Model& m = *s.m;
         const NodePattern& pat = m.pattern;
-        if (q.P && q.P->fp32 != (sizeof(VT) == sizeof(float)))
+        if (s.P && s.P->fp32 != (sizeof(VT) == sizeof(float)))
             throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "preconditioner precision differs from the operator's");
         const int64_t blocks = (pat.num_nodes * 32 + CG_THREADS - 1) / CG_THREADS + 1;
-        q.w->ensure(pat.ndof(), (int)blocks);
-        Item it{};
-        it.sys = (int)i;
-        it.jac = q.P == nullptr;
+        s.w->ensure(pat.ndof(), (int)blocks);
+        it.jac = s.P == nullptr;
         it.ga = spmv_group_size(pat);
-        it.gb = q.P ? spmv_group_size(q.P->P) : 8;
-        it.A = view(pat, (const VT*)q.A);
-        if (q.P) {
+        it.gb = s.P ? spmv_group_size(s.P->P) : 8;
+        it.A = view(pat, (const VT*)s.A);
+        if (s.P) {
             const VT* pv;
             if constexpr (sizeof(VT) == sizeof(double))
-                pv = (const VT*)q.P->d_vals.get();
+                pv = (const VT*)s.P->d_vals.get();
             else
-                pv = (const VT*)q.P->d_vals32.get();
-            it.Pm = view(q.P->P, pv);
+                pv = (const VT*)s.P->d_vals32.get();
+            it.Pm = view(s.P->P, pv);
         } else {
-            it.Pm = it.A;
-            k_jacobi_diag<VT><<<dim3((unsigned)((pat.ndof() + 255) / 256)), dim3(256), 0, m.stream>>>(
-                pat.ndof(), pat.d_diag_pos.get(), (const VT*)q.A, q.w->diag_inv.get(), m.d_err.get());
-            check_launch("k_jacobi_diag");
-            if (read_and_clear_err(m) & KERR_ZERO_DIAG) {  // what pcg_solve reports by an exception
-                q.status = HOEFEM_ERROR_SINGULAR;
-                continue;
-            }
+            it.Pm = it.A;  // the Jacobi diagonal is already in w->diag_inv
         }
         it.blkA = cg_grid(it.A.n_block, it.A.n_diag, it.ga, CG_THREADS_A);
         it.blkB = it.jac ? cg_grid(0, it.Pm.nn, it.gb) : cg_grid(it.Pm.n_block, it.Pm.n_diag, it.gb);
-        size_t g = 0;
-        while (g < groups.size() && !(groups[g].ga == it.ga && groups[g].gb == it.gb && groups[g].jac == it.jac)) ++g;
-        if (g == groups.size()) {
-            Group ng;
-            ng.ga = it.ga;
-            ng.gb = it.gb;
-            ng.jac = it.jac;
-            ng.ops = seg_ops<VT>(it.ga, it.gb, it.jac);
-            groups.push_back(ng);
-        }
-        groups[g].items.push_back((int)items.size());
+        it.ops = seg_ops<VT>(it.ga, it.gb, it.jac);
+        if ((int)(waiting.size() + running.size()) >= cap)
+            throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "more running systems than the CG pool holds");
         items.push_back(it);
-        if (std::find(streams.begin(), streams.end(), m.stream) == streams.end()) streams.push_back(m.stream);
+        waiting.push_back((int)items.size() - 1);
+        return (int)items.size() - 1;
     }
-    if (items.empty()) return;
-    // layout: segments in group order; per group the A-side map, then the B-side map
-    auto round8 = [](int64_t v) { return (v + 7) / 8 * 8; };
-    int64_t map_total = 0;
-    int nseg = 0;
-    for (Group& g : groups) {
-        g.seg0 = nseg;
-        nseg += (int)g.items.size();
-        int64_t a = 0, b = 0;
-        for (int k : g.items) {
-            a = round8(a) + items[k].blkA;
-            b = round8(b) + items[k].blkB;
+
+    const CgBatchSystem& system(int id) const override { return items[id].q; }
+    bool idle() const override { return waiting.empty() && running.empty() && copies.empty(); }
+
+    CgSeg<VT> descriptor(const Item& it) const {
+        CgWork& w = *it.q.w;
+        CgSeg<VT> S{};
+        S.A = it.A;
+        S.Pm = it.Pm;
+        S.x = it.q.x;
+        S.b = it.q.b;
+        S.dinv = w.diag_inv.get();
+        for (int k = 0; k < 2; ++k) {
+            S.zp[k] = w.zp[k].get();
+            S.ra[k] = w.ra[k].get();
         }
-        g.mapA = map_total;
-        g.nA = a;
-        g.mapB = map_total + a;
-        g.nB = b;
-        map_total += a + b;
+        S.part_pre = w.part_pre.get();
+        S.part_a = w.part_a.get();
+        S.part_b = w.part_b.get();
+        S.sc = w.scalars.get();
+        S.host_status = w.h_status.p;
+        S.nb_a = (int)it.blkA;
+        S.nb_b = S.nparts_b = (int)it.blkB;
+        S.tol = it.q.tol;
+        S.max_iter = it.q.max_iter;
+        return S;
     }
-    if (map_total > INT32_MAX) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "batched grid too large");
-    B.ensure(sizeof(CgSeg<VT>) * nseg, (size_t)map_total, streams.size());
-    CgSeg<VT>* hs = reinterpret_cast<CgSeg<VT>*>(B.h_seg->p);
-    int* hm = B.h_map->p;
-    int64_t max_batches = 0, max_n = 0;
-    for (Group& g : groups) {
-        int64_t a = 0, b = 0;
-        for (size_t j = 0; j < g.items.size(); ++j) {
-            const Item& it = items[g.items[j]];
-            CgBatchSystem& q = sys[it.sys];
-            CgWork& w = *q.w;
-            const int64_t ba = round8(a), bb = round8(b);
-            for (int64_t t = a; t < ba; ++t) hm[g.mapA + t] = -1;
-            for (int64_t t = b; t < bb; ++t) hm[g.mapB + t] = -1;
-            for (int64_t t = 0; t < it.blkA; ++t) hm[g.mapA + ba + t] = (int)j;
-            for (int64_t t = 0; t < it.blkB; ++t) hm[g.mapB + bb + t] = (int)j;
-            a = ba + it.blkA;
-            b = bb + it.blkB;
-            CgSeg<VT>& S = hs[g.seg0 + j];
-            S = CgSeg<VT>{};
-            S.A = it.A;
-            S.Pm = it.Pm;
-            S.x = q.x;
-            S.b = q.b;
-            S.dinv = w.diag_inv.get();
-            for (int k = 0; k < 2; ++k) {
-                S.zp[k] = w.zp[k].get();
-                S.ra[k] = w.ra[k].get();
+
+    // pcg_solve's start of one system on the solver stream: x = 0, iteration state, p_{-1} = 0, preambles
+    void preamble(Item& it) {
+        const hipStream_t cs = B.cs;
+        CgWork& w = *it.q.w;
+        *(volatile int*)w.h_status.p = CG_RUNNING;
+        HIP_CHECK(hipMemsetAsync(it.q.x, 0, sizeof(double) * 3 * it.A.nn, cs));
+        hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, cs, w.scalars.get(), it.q.tol, it.q.max_iter);
+        HIP_CHECK(hipMemsetAsync(w.zp[0].get(), 0, sizeof(double) * 2 * w.n, cs));
+        it.ops.pre(descriptor(it), cs);
+        check_launch("cg preamble");
+    }
+
+    static int64_t round8(int64_t v) { return (v + 7) / 8 * 8; }
+
+    Layout build_layout(const std::vector<int>& live) const {
+        Layout L;
+        for (int k : live) {
+            const Item& it = items[k];
+            size_t g = 0;
+            while (g < L.groups.size() &&
+                   !(L.groups[g].ga == it.ga && L.groups[g].gb == it.gb && L.groups[g].jac == it.jac))
+                ++g;
+            if (g == L.groups.size()) L.groups.push_back(Group{it.ga, it.gb, it.jac, it.ops, {}, 0, 0, 0, 0, 0});
+            L.groups[g].items.push_back(k);
+        }
+        for (Group& g : L.groups) {
+            g.seg0 = L.nseg;
+            L.nseg += (int)g.items.size();
+            int64_t a = 0, b = 0;
+            for (int k : g.items) {
+                a = round8(a) + items[k].blkA;
+                b = round8(b) + items[k].blkB;
             }
-            S.part_pre = w.part_pre.get();
-            S.part_a = w.part_a.get();
-            S.part_b = w.part_b.get();
-            S.sc = w.scalars.get();
-            S.host_status = w.h_status.p;
-            S.nb_a = (int)it.blkA;
-            S.nb_b = S.nparts_b = (int)it.blkB;
-            S.base_a = (int)ba;
-            S.base_b = (int)bb;
-            S.tol = q.tol;
-            S.max_iter = q.max_iter;
-            *(volatile int*)w.h_status.p = CG_RUNNING;
-            max_batches = std::max(max_batches, q.max_iter >= 0 ? q.max_iter / CG_BATCH + 2 : INT64_MAX);
-            max_n = std::max(max_n, it.A.nn);
+            g.mapA = L.map_total;
+            g.nA = a;
+            g.mapB = L.map_total + a;
+            g.nB = b;
+            L.map_total += a + b;
+            L.blocks += a + b;
         }
+        L.key = {(int64_t)sizeof(VT)};
+        for (const Group& g : L.groups)
+            for (int64_t v : {(int64_t)g.ga, (int64_t)g.gb, (int64_t)g.jac, (int64_t)g.seg0, g.mapA, g.nA, g.mapB, g.nB})
+                L.key.push_back(v);
+        return L;
     }
-    const hipStream_t cs = B.cs;
-    for (size_t k = 0; k < streams.size(); ++k) {
-        HIP_CHECK(hipEventRecord(B.joins[k], streams[k]));
-        HIP_CHECK(hipStreamWaitEvent(cs, B.joins[k], 0));
+
+    void upload(const Layout& L) {
+        CgSeg<VT>* hs = reinterpret_cast<CgSeg<VT>*>(B.h_seg->p);
+        int* hm = B.h_map->p;
+        for (const Group& g : L.groups) {
+            int64_t a = 0, b = 0;
+            for (size_t j = 0; j < g.items.size(); ++j) {
+                const Item& it = items[g.items[j]];
+                const int64_t ba = round8(a), bb = round8(b);
+                for (int64_t t = a; t < ba; ++t) hm[g.mapA + t] = -1;
+                for (int64_t t = b; t < bb; ++t) hm[g.mapB + t] = -1;
+                for (int64_t t = 0; t < it.blkA; ++t) hm[g.mapA + ba + t] = (int)j;
+                for (int64_t t = 0; t < it.blkB; ++t) hm[g.mapB + bb + t] = (int)j;
+                a = ba + it.blkA;
+                b = bb + it.blkB;
+                CgSeg<VT> S = descriptor(it);
+                S.base_a = (int)ba;
+                S.base_b = (int)bb;
+                hs[g.seg0 + j] = S;
+            }
+        }
+        const hipStream_t cs = B.cs;
+        HIP_CHECK(hipMemcpyAsync(B.d_seg.get(), hs, sizeof(CgSeg<VT>) * L.nseg, hipMemcpyHostToDevice, cs));
+        HIP_CHECK(hipMemcpyAsync(B.d_map.get(), hm, sizeof(int) * L.map_total, hipMemcpyHostToDevice, cs));
+        HIP_CHECK(hipEventRecord(B.upload, cs));
+        uploaded = true;
+        B.relayouts += 1;
     }
-    const CgSeg<VT>* segs = reinterpret_cast<const CgSeg<VT>*>(B.d_seg.get());
-    const int* dmap = B.d_map.get();
-    HIP_CHECK(hipMemcpyAsync(B.d_seg.get(), hs, sizeof(CgSeg<VT>) * nseg, hipMemcpyHostToDevice, cs));
-    HIP_CHECK(hipMemcpyAsync(B.d_map.get(), hm, sizeof(int) * map_total, hipMemcpyHostToDevice, cs));
-    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((6 * max_n + 255) / 256, 512));
-    k_cg_begin<VT><<<dim3(gx, (unsigned)nseg), dim3(256), 0, cs>>>(segs);
-    for (Group& g : groups)
-        g.ops.pre(segs + g.seg0, dmap + g.mapA, (unsigned)g.nA, dmap + g.mapB, (unsigned)g.nB, cs);
-    check_launch("batched cg preamble");
-    auto launch_batch = [&]() {
+
+    bool upload_free() const { return !uploaded || hipEventQuery(B.upload) == hipSuccess; }
+
+    void launch_batch(const Layout& L) {
+        const hipStream_t cs = B.cs;
+        const CgSeg<VT>* segs = reinterpret_cast<const CgSeg<VT>*>(B.d_seg.get());
+        const int* dmap = B.d_map.get();
         int cur = 0;
         for (int q = 0; q < CG_BATCH; ++q) {
-            for (Group& g : groups) g.ops.a(segs + g.seg0, dmap + g.mapA, (unsigned)g.nA, cur, cs);
-            for (Group& g : groups) g.ops.b(segs + g.seg0, dmap + g.mapB, (unsigned)g.nB, cur, cs);
+            for (const Group& g : L.groups) g.ops.a(segs + g.seg0, dmap + g.mapA, (unsigned)g.nA, cur, cs);
+            for (const Group& g : L.groups) g.ops.b(segs + g.seg0, dmap + g.mapB, (unsigned)g.nB, cur, cs);
             cur ^= 1;  // CG_BATCH is even: every batch starts at parity 0
         }
         check_launch("batched cg iteration");
-    };
-    std::vector<int64_t> key = {(int64_t)sizeof(VT), (int64_t)(intptr_t)segs, (int64_t)(intptr_t)dmap};
-    for (const Group& g : groups)
-        for (int64_t v : {(int64_t)g.ga, (int64_t)g.gb, (int64_t)g.jac, (int64_t)g.seg0, g.mapA, g.nA, g.mapB, g.nB})
-            key.push_back(v);
-    hipGraphExec_t exec = nullptr;
-    if (B.use_graphs) {
-        for (CgBatch::Graph& gr : B.graphs)
-            if (gr.key == key) {
-                exec = gr.exec;
-                gr.used = ++B.clock;
-            }
-        if (!exec) {
-            constexpr size_t MAX_GRAPHS = 16;
-            if (B.graphs.size() >= MAX_GRAPHS) {
-                size_t old = 0;
-                for (size_t k = 1; k < B.graphs.size(); ++k)
-                    if (B.graphs[k].used < B.graphs[old].used) old = k;
-                HIP_CHECK(hipGraphExecDestroy(B.graphs[old].exec));
-                B.graphs.erase(B.graphs.begin() + (std::ptrdiff_t)old);
-            }
-            hipGraph_t gph = nullptr;
-            HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            launch_batch();
-            HIP_CHECK(hipStreamEndCapture(cs, &gph));
-            CgBatch::Graph gr;
-            gr.key = key;
-            HIP_CHECK(hipGraphInstantiate(&gr.exec, gph, nullptr, nullptr, 0));
-            HIP_CHECK(hipGraphDestroy(gph));
-            gr.used = ++B.clock;
-            exec = gr.exec;
-            B.graphs.push_back(std::move(gr));
-        }
     }
-    auto all_stopped = [&]() {
-        for (const Item& it : items)
-            if (*(volatile int*)sys[it.sys].w->h_status.p == CG_RUNNING) return false;
-        return true;
-    };
-    int64_t nbatches = 0;
-    auto submit = [&](int slot) {
+
+    // a layout seen before replays its captured batch; the first sight launches directly
+    void submit() {
+        const hipStream_t cs = B.cs;
+        hipGraphExec_t exec = nullptr;
+        if (B.use_graphs) {
+            CgBatch::Graph* gr = nullptr;
+            for (CgBatch::Graph& g : B.graphs)
+                if (g.key == lay.key) gr = &g;
+            if (!gr) {
+                constexpr size_t MAX_GRAPHS = 256;
+                if (B.graphs.size() >= MAX_GRAPHS) {
+                    size_t old = 0;
+                    for (size_t k = 1; k < B.graphs.size(); ++k)
+                        if (B.graphs[k].used < B.graphs[old].used) old = k;
+                    if (B.graphs[old].exec) HIP_CHECK(hipGraphExecDestroy(B.graphs[old].exec));
+                    B.graphs.erase(B.graphs.begin() + (std::ptrdiff_t)old);
+                }
+                B.graphs.push_back(CgBatch::Graph{});
+                gr = &B.graphs.back();
+                gr->key = lay.key;
+            }
+            gr->used = ++B.clock;
+            if (!gr->exec && ++gr->seen >= 2) {
+                hipGraph_t gph = nullptr;
+                HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+                launch_batch(lay);
+                HIP_CHECK(hipStreamEndCapture(cs, &gph));
+                HIP_CHECK(hipGraphInstantiate(&gr->exec, gph, nullptr, nullptr, 0));
+                HIP_CHECK(hipGraphDestroy(gph));
+            }
+            exec = gr->exec;
+        }
         if (exec)
             HIP_CHECK(hipGraphLaunch(exec, cs));
         else
-            launch_batch();
-        HIP_CHECK(hipEventRecord(B.ev[slot], cs));
-        ++nbatches;
-    };
-    submit(0);
-    if (nbatches < max_batches) submit(1);
-    int poll = 0;
-    while (true) {
-        HIP_CHECK(hipEventSynchronize(B.ev[poll]));
-        if (all_stopped() || nbatches >= max_batches) break;
-        submit(poll);
-        poll ^= 1;
+            launch_batch(lay);
+        HIP_CHECK(hipEventRecord(B.ev[next_slot], cs));
+        next_slot ^= 1;
+        ++inflight;
     }
-    for (const Item& it : items) {
-        CgWork& w = *sys[it.sys].w;
-        HIP_CHECK(hipMemcpyAsync(w.h_scalars.p + 2, w.scalars.get(), sizeof(CgScalars), hipMemcpyDeviceToHost, cs));
-    }
-    HIP_CHECK(hipStreamSynchronize(cs));
-    HIP_CHECK(hipEventRecord(B.done, cs));
-    for (hipStream_t st : streams) HIP_CHECK(hipStreamWaitEvent(st, B.done, 0));
-    for (const Item& it : items) {
-        CgBatchSystem& q = sys[it.sys];
-        const CgScalars& r = q.w->h_scalars.p[2];
-        q.iterations = r.iters;
-        switch (r.status) {
-            case CG_CONVERGED:
-            case CG_ZERO_RHS: q.status = HOEFEM_OK; break;  // x was set to zero by k_cg_begin
-            case CG_RUNNING: q.status = HOEFEM_ERROR_CG_MAX_ITERATIONS; break;
-            default: q.status = (hoefem_status)r.status; break;
+
+    void progress(std::vector<int>& ended) override {
+        const hipStream_t cs = B.cs;
+        // 1. completed batches: the statuses the kernels wrote
+        bool batch_done = false;
+        while (inflight > 0 && hipEventQuery(B.ev[oldest]) == hipSuccess) {
+            --inflight;
+            oldest ^= 1;
+            batch_done = true;
         }
+        if (batch_done) {
+            std::vector<int> stopped;
+            for (size_t k = 0; k < running.size();) {
+                Item& it = items[running[k]];
+                if (*(volatile int*)it.q.w->h_status.p != CG_RUNNING) {
+                    it.state = 2;
+                    stopped.push_back(running[k]);
+                    CgWork& w = *it.q.w;
+                    HIP_CHECK(hipMemcpyAsync(w.h_scalars.p + 2, w.scalars.get(), sizeof(CgScalars),
+                                             hipMemcpyDeviceToHost, cs));
+                    running[k] = running.back();
+                    running.pop_back();
+                } else {
+                    ++k;
+                }
+            }
+            if (!stopped.empty()) {
+                Copy c{B.take_event(), std::move(stopped)};
+                HIP_CHECK(hipEventRecord(c.ev, cs));
+                copies.push_back(std::move(c));
+                layout_dirty = true;
+            }
+        }
+        // 2. finished result copies
+        for (size_t k = 0; k < copies.size();) {
+            if (hipEventQuery(copies[k].ev) != hipSuccess) {
+                ++k;
+                continue;
+            }
+            for (int id : copies[k].ids) {
+                Item& it = items[id];
+                const CgScalars& r = it.q.w->h_scalars.p[2];
+                it.q.iterations = r.iters;
+                switch (r.status) {
+                    case CG_CONVERGED:
+                    case CG_ZERO_RHS: it.q.status = HOEFEM_OK; break;  // x was zeroed by the preamble
+                    case CG_RUNNING: it.q.status = HOEFEM_ERROR_CG_MAX_ITERATIONS; break;
+                    default: it.q.status = (hoefem_status)r.status; break;
+                }
+                it.state = 3;
+                ended.push_back(id);
+            }
+            B.spare.push_back(copies[k].ev);
+            copies.erase(copies.begin() + (std::ptrdiff_t)k);
+        }
+        // 3. joins, 4. a new layout (the staging buffers must be free for either)
+        if ((!waiting.empty() || layout_dirty) && upload_free()) {
+            bool relayout = !waiting.empty();
+            for (int id : waiting) {
+                preamble(items[id]);
+                items[id].state = 1;
+                running.push_back(id);
+            }
+            waiting.clear();
+            if (!relayout && layout_dirty) {
+                int64_t live = 0;
+                for (int id : running) live += round8(items[id].blkA) + round8(items[id].blkB);
+                relayout = running.empty() || live * 4 <= lay.blocks * 3;
+            }
+            if (relayout) {
+                std::sort(running.begin(), running.end());
+                lay = build_layout(running);
+                if (!running.empty()) upload(lay);
+                layout_dirty = false;
+            }
+        }
+        // 5. keep two batches in flight while systems run
+        while (!running.empty() && inflight < 2) submit();
     }
+
+    void wait() override {
+        if (inflight > 0) HIP_CHECK(hipEventSynchronize(B.ev[oldest]));
+        else if (!copies.empty()) HIP_CHECK(hipEventSynchronize(copies.front().ev));
+        else if (uploaded) HIP_CHECK(hipEventSynchronize(B.upload));
+    }
+};
+
+std::unique_ptr<CgPool> make_cg_pool(CgBatch& B, bool fp32, int capacity) {
+    if (fp32) return std::unique_ptr<CgPool>(new CgPoolT<float>(B, capacity));
+    return std::unique_ptr<CgPool>(new CgPoolT<double>(B, capacity));
 }
 
-void pcg_solve_batch(CgBatch& B, std::vector<CgBatchSystem>& sys, bool fp32) {
+void launch_jacobi_diag(Model& m, CgWork& w, const void* A, bool fp32) {
+    const NodePattern& pat = m.pattern;
+    const int64_t blocks = (pat.num_nodes * 32 + CG_THREADS - 1) / CG_THREADS + 1;
+    w.ensure(pat.ndof(), (int)blocks);
+    const unsigned g = (unsigned)((pat.ndof() + 255) / 256);
     if (fp32)
-        pcg_batch_t<float>(B, sys);
+        k_jacobi_diag<float><<<dim3(g), dim3(256), 0, m.stream>>>(pat.ndof(), pat.d_diag_pos.get(), (const float*)A,
+                                                                 w.diag_inv.get(), m.d_err.get());
     else
-        pcg_batch_t<double>(B, sys);
+        k_jacobi_diag<double><<<dim3(g), dim3(256), 0, m.stream>>>(pat.ndof(), pat.d_diag_pos.get(), (const double*)A,
+                                                                  w.diag_inv.get(), m.d_err.get());
+    check_launch("k_jacobi_diag");
 }
 
 // ================================================================== Newton / BE kernels

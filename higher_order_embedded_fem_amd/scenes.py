@@ -671,6 +671,37 @@ def config_scene(index: int, **kw) -> Scene:
     raise ValueError(f"unknown configuration {index}")
 
 
+def pull_scene(scene: Scene, z_top: float = 0.8, z_bottom: float = -0.8) -> Scene:
+    """The scene with its top cap held by a Dirichlet condition too, for prescribed pulls: the nodes at
+    or below z_bottom stay fixed and the nodes at or above z_top follow a prescribed displacement
+    (the hollow cylinder's caps are at z = -0.8 and 0.8). Returns the new scene and the mask of the
+    pulled nodes among its Dirichlet nodes."""
+    z = scene.vertices[:, 2]
+    bottom = np.nonzero(z <= z_bottom + 1e-12)[0]
+    top = np.nonzero(z >= z_top - 1e-12)[0]
+    nodes = np.union1d(bottom, top).astype(np.int32)
+    if nodes.size != bottom.size + top.size:
+        raise ValueError("the caps overlap")
+    out = dataclasses.replace(scene, name=scene.name + "_pull", dirichlet_nodes=nodes)
+    return out, np.isin(nodes, top)
+
+
+def pull_schedule(steps: int, height: float, seed: int = 0, segment: int = 50, lo: float = 0.05, hi: float = 0.2):
+    """Seeded random pull loads: every `segment` steps a new target displacement of the pulled cap (a
+    uniformly random direction, magnitude uniform in [lo, hi] * height), reached by a linear ramp from
+    the previous one over the segment. Returns the cap displacement at the end of each step, (steps, 3)."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((steps, 3))
+    prev = np.zeros(3)
+    for k0 in range(0, steps, segment):
+        d = rng.standard_normal(3)
+        target = d / np.linalg.norm(d) * rng.uniform(lo, hi) * height
+        for j in range(min(segment, steps - k0)):
+            out[k0 + j] = prev + (target - prev) * (j + 1) / segment
+        prev = target
+    return out
+
+
 def lame_from_young_poisson(young: float, poisson: float):
     """From<YoungPoisson> for LameParameters (fenris/src/solid/materials.rs:27-38), with
     the reference's lambda = 0.5*mu*nu/(1-2nu) (errata/tex/errata.tex:24-38)."""
