@@ -513,6 +513,105 @@ __global__ __launch_bounds__(64) void k_block_gather(
     }
 }
 
+// The same gather with the incident blocks' metadata held one incidence per lane (moved by
+// shuffles instead of loaded per block) and each block's row values and slots loaded one block
+// ahead, so a node row's blocks cost one memory latency between them instead of three. For blocks of
+// at most 28 nodes (3 rows x 3m <= 256 values, four per lane) and rows that fit the LDS at once;
+// every entry receives its terms in the same order as in k_block_gather.
+constexpr int GP_VALS = 4;
+template <class TB, class TV>
+__global__ __launch_bounds__(64) void k_block_gather_pf(
+    const int32_t* __restrict__ nptr, const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ diag_only,
+    const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk, const int32_t* __restrict__ inc_loc,
+    const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off, const int64_t* __restrict__ slot_off,
+    const int32_t* __restrict__ slot, const TB* __restrict__ blocks, TV* __restrict__ values, int mode,
+    double c, const double* __restrict__ M, int zero_first, const double* __restrict__ D2, double cd, double cm) {
+    extern __shared__ double s_rb[];
+    const int64_t v = blockIdx.x;
+    const int t = threadIdx.x;
+    if (diag_only[v]) return;
+    const int k = nptr[v + 1] - nptr[v];
+    if (k == 0) return;
+    const int rw = 3 * k, R = 3 * rw;
+    const int64_t r0 = rowptr[3 * v];
+    const int i0 = inc_ptr[v], ninc = inc_ptr[v + 1] - i0;
+    // lane j holds incidence j (the rest, past 64 incident blocks, is read from memory)
+    int a_l = 0, mb_l = 0;
+    int64_t vo_l = 0, so_l = 0;
+    if (t < ninc) {
+        const int b = inc_blk[i0 + t];
+        a_l = inc_loc[i0 + t];
+        mb_l = msz[b];
+        vo_l = val_off[b];
+        so_l = slot_off[b];
+    }
+    for (int idx = t; idx < R; idx += 64) s_rb[idx] = (zero_first || mode == 1) ? 0.0 : (double)values[r0 + idx];
+    auto fetch = [&](int j, double* vv, int* ss, int& bwj) {
+        int a, mb;
+        int64_t vo, so;
+        if (j < 64) {
+            a = __shfl(a_l, j, 64);
+            mb = __shfl(mb_l, j, 64);
+            vo = __shfl(vo_l, j, 64);
+            so = __shfl(so_l, j, 64);
+        } else {
+            const int b = inc_blk[i0 + j];
+            a = inc_loc[i0 + j];
+            mb = msz[b];
+            vo = val_off[b];
+            so = slot_off[b];
+        }
+        const int bw = 3 * mb;
+        bwj = bw;
+        const TB* base = blocks + vo + (int64_t)(3 * a) * bw;
+        const int32_t* sl = slot + so + (int64_t)a * mb;
+#pragma unroll
+        for (int q = 0; q < GP_VALS; ++q) {
+            const int idx = t + 64 * q;
+            if (idx < 3 * bw) {
+                vv[q] = (double)base[idx];
+                ss[q] = sl[(idx % bw) / 3];
+            }
+        }
+    };
+    double cv[GP_VALS] = {0.0, 0.0, 0.0, 0.0}, nv[GP_VALS] = {0.0, 0.0, 0.0, 0.0};
+    int cs[GP_VALS] = {0, 0, 0, 0}, ns[GP_VALS] = {0, 0, 0, 0};
+    int cbw = 0, nbw = 0;
+    if (ninc > 0) fetch(0, cv, cs, cbw);
+    __syncthreads();
+    for (int j = 0; j < ninc; ++j) {
+        if (j + 1 < ninc) fetch(j + 1, nv, ns, nbw);
+#pragma unroll
+        for (int q = 0; q < GP_VALS; ++q) {
+            const int idx = t + 64 * q;
+            if (idx < 3 * cbw) {
+                const int i = idx / cbw, cc = idx % cbw;
+                s_rb[i * rw + 3 * cs[q] + cc % 3] += cv[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < GP_VALS; ++q) {
+            cv[q] = nv[q];
+            cs[q] = ns[q];
+        }
+        cbw = nbw;
+    }
+    for (int idx = t; idx < R; idx += 64) {
+        const int64_t o = r0 + idx;
+        const double sum = s_rb[idx];
+        if (mode == 0)
+            values[o] = (TV)sum;
+        else {
+            double v2 = 0.0;  // stiffness, damping, mass (system_assembly.rs:197-235)
+            v2 += c * sum;
+            if (D2) v2 += cd * D2[o];
+            if (M) v2 += cm * M[o];
+            values[o] = (TV)v2;
+        }
+    }
+}
+
 // f += sum of element force blocks in color order (one thread per node)
 __global__ void k_force_gather(int64_t nn, const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk,
                                const int32_t* __restrict__ inc_loc, int n, const double* __restrict__ fe,
@@ -758,6 +857,18 @@ void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const T
                          double c, const double* M, bool zero_first, hipStream_t s, const double* D2, double cd,
                          double cm, KernelProf* prof) {
     if (pat.num_nodes == 0) return;
+    if (pat.max_nbr <= GATHER_CHUNK && 9 * sc.max_m <= 64 * GP_VALS) {
+        const size_t lds = (size_t)9 * std::max(pat.max_nbr, 1) * sizeof(double);
+        launch_timed(prof, k_block_gather_pf<TB, TV>, dim3((unsigned)pat.num_nodes), dim3(64), (uint32_t)lds, s,
+                     (const int32_t*)pat.d_nptr.get(), (const int32_t*)pat.d_rowptr.get(),
+                     (const uint8_t*)pat.d_diag_only.get(), (const int32_t*)sc.d_inc_ptr.get(),
+                     (const int32_t*)sc.d_inc_blk.get(), (const int32_t*)sc.d_inc_loc.get(),
+                     (const int32_t*)sc.d_m.get(), (const int64_t*)sc.d_val_off.get(),
+                     (const int64_t*)sc.d_slot_off.get(), (const int32_t*)sc.d_slot.get(), blocks, values, mode, c,
+                     M, zero_first ? 1 : 0, D2, cd, cm);
+        check_launch("k_block_gather_pf");
+        return;
+    }
     const int chunk = std::min(std::max(pat.max_nbr, 1), GATHER_CHUNK);
     const size_t lds = (size_t)9 * chunk * sizeof(double);
     launch_timed(prof, k_block_gather<TB, TV>, dim3((unsigned)pat.num_nodes), dim3(64), (uint32_t)lds, s,

@@ -588,7 +588,8 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(const CgSeg<VT> one, con
 // A-kernel of iteration k: stopping tests on r_k, beta_k = zTr_k / zTr_{k-1},
 // p_k = z_k + beta p_{k-1} ; Ap_k = A p_k ; partial pAp. The partial sums are reduced before the
 // first entry is issued: issuing the entry early (as the B-kernel does) measured 0.25 us slower
-// per launch with 9 MB more HBM fetch on the Hex20 cylinder.
+// per launch with 9 MB more HBM fetch on the Hex20 cylinder, and loading the entries two passes and
+// the first column indices one pass ahead 1.0 us slower (19.3 vs 18.3 us, 126 VGPRs).
 template <int G, bool FLEX, class VT, bool SEG>
 __global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT> one, const CgSeg<VT>* __restrict__ segs,
                                                        const int* __restrict__ seg_of, int cur) {
