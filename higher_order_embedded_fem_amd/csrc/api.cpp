@@ -2,6 +2,7 @@
 // Backward-Euler Newton driver (all device work is enqueued on the model's stream).
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstring>
 
@@ -645,9 +646,14 @@ static hoefem_status sim_step(Sim& s, double t, double dt, double* u, double* v,
 // ends on the device. Scenes therefore never wait for each other, and each scene's u and v are the
 // ones sim_step gives it, bit for bit. A scene whose single-step attempt fails continues alone with
 // the substepping of sim_step once the others are done.
+// Scenes go to CG pools by size (DOFs below 2e4, below 8e4, larger), each on its own stream, so the
+// iterations of small systems do not wait for the launches the large ones lengthen.
+constexpr int BATCH_POOLS = 3;
+constexpr int64_t BATCH_POOL_DOFS[BATCH_POOLS - 1] = {20000, 80000};
+
 struct Batch {
     std::vector<Sim*> sims;
-    CgBatch cg;
+    CgBatch cg[BATCH_POOLS];
     std::vector<hipEvent_t> ev;  // one per scene
     ~Batch() {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -689,7 +695,18 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
         B.ev.push_back(e);
     }
     const bool fp32 = B.sims[0]->mixed_mode == 1;
-    std::unique_ptr<CgPool> pool = make_cg_pool(B.cg, fp32, (int)ns);
+    std::vector<int> pool_of(ns);
+    int pool_size[BATCH_POOLS] = {};
+    for (size_t i = 0; i < ns; ++i) {
+        const int64_t ndof = 3 * B.sims[i]->model->nv;
+        int c = 0;
+        while (c < BATCH_POOLS - 1 && ndof >= BATCH_POOL_DOFS[c]) ++c;
+        pool_of[i] = c;
+        pool_size[c] += 1;
+    }
+    std::unique_ptr<CgPool> pools[BATCH_POOLS];
+    for (int c = 0; c < BATCH_POOLS; ++c)
+        if (pool_size[c]) pools[c] = make_cg_pool(B.cg[c], fp32, pool_size[c]);
     std::vector<BatchScene> q(ns);
     const auto t0 = std::chrono::steady_clock::now();
     auto elapsed_ms = [&]() {
@@ -776,7 +793,7 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
         mark(i);
     }
     std::vector<int> ended;
-    std::vector<int> owner;  // scene of every pool system
+    std::vector<int> owner[BATCH_POOLS];  // scene of every pool system
     const double c = 1e-4, alpha_min = 1e-6;
     const double initial[3] = {1.0, 0.75, 0.5};
     while (true) {
@@ -814,9 +831,10 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
                     sys.x = s.d_sol.get();
                     sys.tol = s.d.cg_tolerance;
                     sys.max_iter = s.d.cg_max_iter;
-                    b.cg_id = pool->join(sys);
-                    if ((int)owner.size() <= b.cg_id) owner.resize(b.cg_id + 1, -1);
-                    owner[b.cg_id] = (int)i;
+                    const int pc = pool_of[i];
+                    b.cg_id = pools[pc]->join(sys);
+                    if ((int)owner[pc].size() <= b.cg_id) owner[pc].resize(b.cg_id + 1, -1);
+                    owner[pc][b.cg_id] = (int)i;
                     b.phase = BP_CG;
                 }
             } else if (b.phase == BP_TRIAL) {
@@ -840,24 +858,30 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
                 }
             }
         }
-        ended.clear();
-        pool->progress(ended);
-        for (int id : ended) {
-            BatchScene& b = q[owner[id]];
-            const CgBatchSystem& r = pool->system(id);
-            b.s->cg_total += r.iterations;
-            hoefem_status cs = r.status;
-            if (cs == HOEFEM_ERROR_CG_MAX_ITERATIONS || cs == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) cs = HOEFEM_OK;
-            if (cs != HOEFEM_OK) {
-                fail(b, HOEFEM_ERROR_JACOBIAN);
-                continue;
+        bool any_ended = false;
+        for (int pc = 0; pc < BATCH_POOLS; ++pc) {
+            if (!pools[pc]) continue;
+            ended.clear();
+            pools[pc]->progress(ended);
+            any_ended |= !ended.empty();
+            for (int id : ended) {
+                const size_t i = (size_t)owner[pc][id];
+                BatchScene& b = q[i];
+                const CgBatchSystem& r = pools[pc]->system(id);
+                b.s->cg_total += r.iterations;
+                hoefem_status cs = r.status;
+                if (cs == HOEFEM_ERROR_CG_MAX_ITERATIONS || cs == HOEFEM_ERROR_CG_INDEFINITE_OPERATOR) cs = HOEFEM_OK;
+                if (cs != HOEFEM_OK) {
+                    fail(b, HOEFEM_ERROR_JACOBIAN);
+                    continue;
+                }
+                b.g_initial = 0.5 * b.n2;
+                b.alpha_prev = 0.0;
+                b.alpha = 1.0;
+                b.tail = 0.25;
+                b.idx = 0;
+                trial(i);
             }
-            b.g_initial = 0.5 * b.n2;
-            b.alpha_prev = 0.0;
-            b.alpha = 1.0;
-            b.tail = 0.25;
-            b.idx = 0;
-            trial((size_t)owner[id]);
         }
         // the step goes on while any scene is still in its Newton loop (evaluating, assembling, in the
         // pool or trying a step length)
@@ -867,8 +891,8 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
             host_pending |= b.waiting;
         }
         if (!active) break;
-        // nothing for the host to do until the GPU moves on: wait for the pool's oldest work
-        if (ended.empty() && !host_pending && !pool->idle()) pool->wait();
+        // nothing for the host to do until the GPU moves on
+        if (!any_ended && !host_pending) std::this_thread::yield();
     }
     // scenes whose attempt failed: the rest of sim_step's substepping, one scene at a time
     for (BatchScene& b : q) {

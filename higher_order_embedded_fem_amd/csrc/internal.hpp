@@ -385,7 +385,6 @@ struct CgPool {
     virtual ~CgPool() = default;
     virtual int join(const CgBatchSystem& s) = 0;
     virtual void progress(std::vector<int>& ended) = 0;
-    virtual void wait() = 0;  // blocks until the oldest outstanding work completes
     virtual const CgBatchSystem& system(int id) const = 0;
     virtual bool idle() const = 0;
 };

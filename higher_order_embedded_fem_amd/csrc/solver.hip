@@ -1742,11 +1742,6 @@ struct CgPoolT final : CgPool {
         while (!running.empty() && inflight < 2) submit();
     }
 
-    void wait() override {
-        if (inflight > 0) HIP_CHECK(hipEventSynchronize(B.ev[oldest]));
-        else if (!copies.empty()) HIP_CHECK(hipEventSynchronize(copies.front().ev));
-        else if (uploaded) HIP_CHECK(hipEventSynchronize(B.upload));
-    }
 };
 
 std::unique_ptr<CgPool> make_cg_pool(CgBatch& B, bool fp32, int capacity) {
