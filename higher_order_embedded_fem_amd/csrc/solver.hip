@@ -36,59 +36,67 @@ __device__ inline double wave_sum(double v) {
 }
 
 // tot[k] = sum over b < nb of partials[b * NV + k], the same value in every thread of every
-// block (fixed per-thread strides, fixed wave and cross-wave summation order).
+// block (fixed per-thread strides, fixed wave and cross-wave summation order), in two halves so
+// that the loads can be issued early and the sums formed once they are needed. issue() loads the
+// first CG_MAX_BLOCKS partials (every grid of the solver) by an unrolled loop with no branch
+// (slots past nb re-read partial 0 and are not added): all of a thread's loads are in flight
+// together, where a rolled or branching loop waits for each load before the next. total() adds
+// them (then any beyond CG_MAX_BLOCKS) in the strided order and reduces over the work-group.
+template <int NV, int BT = CG_THREADS>
+struct Partials {
+    static constexpr int PER = (CG_MAX_BLOCKS + BT - 1) / BT;
+    double v[PER][NV];
+    const double* src;
+    int nb;
+
+    __device__ inline void issue(const double* __restrict__ partials, int n) {
+        src = partials;
+        nb = n;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int b = (int)threadIdx.x + i * BT;
+            const int64_t bc = b < n ? b : 0;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[i][k] = partials[bc * NV + k];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads together, ahead of any use
+    }
+
+    __device__ inline void total(double (&tot)[NV]) const {
+        __builtin_amdgcn_sched_barrier(0);  // loads placed before this call are issued before any add
+        double acc[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int i = 0; i < PER; ++i)
+            if ((int)threadIdx.x + i * BT < nb)
+#pragma unroll
+                for (int k = 0; k < NV; ++k) acc[k] += v[i][k];
+        for (int b = (int)threadIdx.x + PER * BT; b < nb; b += BT)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[k] += src[(int64_t)b * NV + k];
+        __shared__ double s_w[BT / 64][NV];
+        const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            const double s = wave_sum(acc[k]);
+            if (lane == 0) s_w[wid][k] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < BT / 64; ++w) s += s_w[w][k];
+            tot[k] = s;
+        }
+    }
+};
+
 template <int NV, int BT = CG_THREADS>
 __device__ inline void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[NV]) {
-    __shared__ double s_w[BT / 64][NV];
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int b = t; b < nb; b += BT)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double s = wave_sum(acc[k]);
-        if (lane == 0) s_w[wid][k] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int w = 0; w < BT / 64; ++w) s += s_w[w][k];
-        tot[k] = s;
-    }
-}
-
-// The two halves of sum_partials, so that the loads can be issued early and the reduction
-// done once they are needed: acc = this thread's strided share of the partials ...
-template <int NV>
-__device__ inline void partials_load(const double* __restrict__ partials, int nb, double (&acc)[NV]) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += CG_THREADS)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += partials[(int64_t)b * NV + k];
-}
-
-// ... and the block-wide total, the same in every thread of every block (sum_partials' order).
-template <int NV>
-__device__ inline void partials_reduce(const double (&acc)[NV], double (&tot)[NV]) {
-    __shared__ double s_w[CG_THREADS / 64][NV];
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        const double s = wave_sum(acc[k]);
-        if (lane == 0) s_w[wid][k] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int w = 0; w < CG_THREADS / 64; ++w) s += s_w[w][k];
-        tot[k] = s;
-    }
+    Partials<NV, BT> pt;
+    pt.issue(partials, nb);
+    pt.total(tot);
 }
 
 // partials[bid * NV + k] = sum of v[k] over the block's threads. BT is the block size the kernel is
@@ -247,31 +255,45 @@ __device__ inline GroupRange group_range(int64_t groups, int64_t b, int64_t n) {
     return {x * groups / 8 + b / 8, (x + 1) * groups / 8, n / 8};
 }
 
-// Loops over the entries of a pattern: block nodes in groups of CG_THREADS / G (G lanes each;
+// Loops over the entries of a pattern: block nodes in groups of BT / G (G lanes each;
 // block(e, lane, valid) is called the same number of times by every thread of a work-group, as
 // the G-lane shuffles need whole groups) over the work-group's group_range, the next group's
-// entry prefetched; then diagonal-only nodes one per thread (diag(e)).
-template <int G, int BT = CG_THREADS, class FB, class FD>
-__device__ inline void for_entries(const NodeEntry* __restrict__ ent, int64_t nb, int64_t nd, FB block, FD diag,
-                                   int64_t bid, int64_t nblk) {
+// entry prefetched; then diagonal-only nodes one per thread (diag(e)). Entries are read at a
+// clamped index with no branch (an entry read for an invalid slot is not used), so no entry load
+// waits where it is issued: the first one is in flight while pro() runs (the kernel's prologue:
+// its own loads, status tests, reductions; false stops the kernel, uniformly over the grid).
+template <int G, int BT = CG_THREADS, class FP, class FB, class FD>
+__device__ inline bool for_entries(const NodeEntry* __restrict__ ent, int64_t nb, int64_t nd, FP pro, FB block,
+                                   FD diag, int64_t bid, int64_t nblk) {
     constexpr int per = BT / G;
     const int lane = (int)(threadIdx.x % G);
     const int64_t groups = (nb + per - 1) / per;
     const int64_t sub = threadIdx.x / G;
     const GroupRange R = group_range(groups, bid, nblk);
+    const int64_t last = nb + nd - 1;
+    if (last < 0) return pro();  // an empty pattern
+    auto load = [&](int64_t g) {
+        const int64_t i = g * per + sub;
+        return ent[i < last ? i : last];
+    };
     int64_t g = R.begin;
-    NodeEntry e{0, 0, 0, 0};
-    if (g < R.end && g * per + sub < nb) e = ent[g * per + sub];
+    NodeEntry e = load(g);
+    if (!pro()) {
+        // the first entry is used on this path too (an empty asm reads it), so the compiler keeps
+        // its load ahead of pro()'s status test instead of sinking it past the test
+        asm volatile("" ::"v"(e.count));
+        return false;
+    }
     for (; g < R.end; g += R.step) {
         const NodeEntry cur = e;
         const bool valid = g * per + sub < nb;
-        const int64_t gn = g + R.step;
-        if (gn < R.end && gn * per + sub < nb) e = ent[gn * per + sub];
+        e = load(g + R.step);
         block(cur, lane, valid);
     }
     // diagonal-only nodes from the last work-group backwards: those have the fewest block groups
     const int64_t rb = nblk - 1 - bid;
     for (int64_t i = rb * BT + threadIdx.x; i < nd; i += nblk * BT) diag(ent[nb + i]);
+    return true;
 }
 
 // The lane's first 3x3 block of a row (j = lane) and its column, loaded a pass ahead.
@@ -352,13 +374,21 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
     const GroupRange R = group_range(groups, bid, nblk);
     const int64_t stride = R.step;
     auto has = [&](int64_t g) { return g < R.end && g * per + sub < nb; };
+    const int64_t last = nb + nd - 1;
+    if (last < 0) {  // an empty pattern
+        pro_issue();
+        return pro_finish();
+    }
+    auto load = [&](int64_t g) {  // clamped, with no branch, as in for_entries
+        const int64_t i = g * per + sub;
+        return A.ent[i < last ? i : last];
+    };
     int64_t g = R.begin;
-    NodeEntry e{0, 0, 0, 0}, e1{0, 0, 0, 0};
-    if (has(g)) e = A.ent[g * per + sub];
+    NodeEntry e = load(g);
     pro_issue();
     RowPf<VT> pf;
     row_pf_issue(A, e, lane, has(g), pf);
-    if (has(g + stride)) e1 = A.ent[(g + stride) * per + sub];
+    NodeEntry e1 = load(g + stride);
     if (!pro_finish()) return false;
     for (; g < R.end; g += stride) {
         const NodeEntry cur = e;
@@ -366,7 +396,7 @@ __device__ inline bool for_entries_pf(const Csr3<VT>& A, PI pro_issue, PF pro_fi
         RowPf<VT> pn;
         row_pf_issue(A, e1, lane, has(g + stride), pn);
         e = e1;
-        if (has(g + 2 * stride)) e1 = A.ent[(g + 2 * stride) * per + sub];
+        e1 = load(g + 2 * stride);
         block(cur, lane, valid, pf);
         pf = pn;
     }
@@ -513,7 +543,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre1(const CgSeg<VT> one, con
         part[0] += b0 * b0 + b1 * b1 + b2 * b2;
     };
     for_entries<G>(
-        A.ent, A.n_block, A.n_diag,
+        A.ent, A.n_block, A.n_diag, [] { return true; },
         [&](const NodeEntry& e, int lane, bool valid) {
             double a0, a1, a2;
             entry_row_dot<G>(A, e, lane, valid, [&](int64_t b, double& x0, double& x1, double& x2) {
@@ -564,7 +594,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_pre2(const CgSeg<VT> one, con
         for (int64_t v = I.bid * CG_THREADS + threadIdx.x; v < S.A.nn; v += I.nblk * CG_THREADS) finish(v, 0.0, 0.0, 0.0);
     } else {
         for_entries<G>(
-            Pm.ent, Pm.n_block, Pm.n_diag,
+            Pm.ent, Pm.n_block, Pm.n_diag, [] { return true; },
             [&](const NodeEntry& e, int lane, bool valid) {
                 double z0, z1, z2;
                 entry_row_dot<G>(Pm, e, lane, valid, [&](int64_t b, double& x0, double& x1, double& x2) {
@@ -597,42 +627,52 @@ __global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT> one, cons
     if (I.s < 0) return;
     const CgSeg<VT>& S = SEG ? segs[I.s] : one;
     CgScalars* sc = S.sc;
-    if (sc->status != CG_RUNNING) return;
     const Csr3<VT> A = S.A;
     const CgPtrs P = seg_ptrs(S, 2, cur);
-    const int64_t k = sc->iters;  // written by the previous B-kernel (0 after the preamble)
     constexpr int NV = FLEX ? 3 : 2;
-    double tot[NV];
-    sum_partials<NV, CG_THREADS_A>(P.part_b, P.nb_b, tot);
-    // z_k.r_k, |r_k|^2 and (flexible variant) z_k.r_{k-1}
-    const double zTr = tot[0], zTr_cross = FLEX ? tot[1] : 0.0, rr = tot[NV - 1];
     const bool lead = I.bid == 0 && threadIdx.x == 0;
-    int decided = CG_RUNNING;
-    if (sc->bnorm == 0.0)
-        decided = CG_ZERO_RHS;
-    else if (sqrt(rr) <= sc->tol * sc->bnorm)
-        decided = CG_CONVERGED;
-    else if (sc->max_iter >= 0 && k >= sc->max_iter)
-        decided = HOEFEM_ERROR_CG_MAX_ITERATIONS;
-    if (decided != CG_RUNNING) {
+    double beta = 0.0;
+    // the status, the iteration index (written by the previous B-kernel, 0 after the preamble), that
+    // kernel's partial sums and (for_entries) the first row entry are loaded together, not one
+    // memory latency after the other
+    auto pro = [&]() -> bool {
+        const int status = sc->status;
+        const int64_t k = sc->iters;
+        Partials<NV, CG_THREADS_A> pt;
+        pt.issue(P.part_b, P.nb_b);
+        if (status != CG_RUNNING) return false;
+        double tot[NV];
+        pt.total(tot);
+        // z_k.r_k, |r_k|^2 and (flexible variant) z_k.r_{k-1}
+        const double zTr = tot[0], zTr_cross = FLEX ? tot[1] : 0.0, rr = tot[NV - 1];
+        int decided = CG_RUNNING;
+        if (sc->bnorm == 0.0)
+            decided = CG_ZERO_RHS;
+        else if (sqrt(rr) <= sc->tol * sc->bnorm)
+            decided = CG_CONVERGED;
+        else if (sc->max_iter >= 0 && k >= sc->max_iter)
+            decided = HOEFEM_ERROR_CG_MAX_ITERATIONS;
+        if (decided != CG_RUNNING) {
+            if (lead) {
+                sc->zTr = zTr;
+                sc->rr = rr;
+                sc->status = decided;
+                *P.host_status = decided;
+            }
+            return false;
+        }
+        // Fletcher-Reeves form for a fixed preconditioner; Polak-Ribiere z_k.(r_k - r_{k-1}) when
+        // the preconditioner depends on its input (Notay, SIAM J. Sci. Comput. 22 (2000) 1444)
+        beta = k == 0 ? 0.0 : (FLEX ? (zTr - zTr_cross) : zTr) / sc->zrec[(k - 1) & 1];
         if (lead) {
+            sc->zrec[k & 1] = zTr;
             sc->zTr = zTr;
             sc->rr = rr;
-            sc->status = decided;
-            *P.host_status = decided;
+            sc->beta = beta;
+            sc->kcur = k;
         }
-        return;
-    }
-    // Fletcher-Reeves form for a fixed preconditioner; Polak-Ribiere z_k.(r_k - r_{k-1}) when
-    // the preconditioner depends on its input (Notay, SIAM J. Sci. Comput. 22 (2000) 1444)
-    const double beta = k == 0 ? 0.0 : (FLEX ? (zTr - zTr_cross) : zTr) / sc->zrec[(k - 1) & 1];
-    if (lead) {
-        sc->zrec[k & 1] = zTr;
-        sc->zTr = zTr;
-        sc->rr = rr;
-        sc->beta = beta;
-        sc->kcur = k;
-    }
+        return true;
+    };
     const double* zp = P.zp_old;
     double part[1] = {0.0};
     auto pvec = [&](int64_t b, double& x0, double& x1, double& x2) {
@@ -651,8 +691,8 @@ __global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT> one, cons
         P.ap_out[6 * v + 5] = a2;
         part[0] += p0 * a0 + p1 * a1 + p2 * a2;
     };
-    for_entries<G, CG_THREADS_A>(
-        A.ent, A.n_block, A.n_diag,
+    const bool ran = for_entries<G, CG_THREADS_A>(
+        A.ent, A.n_block, A.n_diag, pro,
         [&](const NodeEntry& e, int lane, bool valid) {
             double a0, a1, a2;
             entry_row_dot<G>(A, e, lane, valid, pvec, a0, a1, a2);
@@ -670,6 +710,7 @@ __global__ __launch_bounds__(CG_THREADS_A) void k_cg_a(const CgSeg<VT> one, cons
             finish(v, p0, p1, p2, double(d[0] * VT(p0)), double(d[1] * VT(p1)), double(d[2] * VT(p2)));
         },
         I.bid, I.nblk);
+    if (!ran) return;
     block_partials<1, CG_THREADS_A>(part, P.part_a, I.bid);
 }
 
@@ -684,17 +725,27 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(const CgSeg<VT> one, const 
     const Csr3<VT> Pm = S.Pm;
     const CgPtrs P = seg_ptrs(S, 2, cur);
     CgScalars* sc = P.sc;
-    double acc[1];
+    Partials<1> pt;
     double alpha = 0.0;
     const bool lead = I.bid == 0 && threadIdx.x == 0;
-    auto pro_issue = [&] { partials_load<1>(P.part_a, P.nb_a, acc); };
+    int status = CG_RUNNING;
+    int64_t k = 0;
+    double zrec[2];
+    // the partial sums, the status, the iteration index (written by this iteration's A-kernel) and
+    // both zTr records are loaded together; zTr of iteration k is selected afterwards
+    auto pro_issue = [&] {
+        pt.issue(P.part_a, P.nb_a);
+        status = sc->status;
+        k = sc->kcur;
+        zrec[0] = sc->zrec[0];
+        zrec[1] = sc->zrec[1];
+    };
     auto pro_finish = [&]() -> bool {
-        if (sc->status != CG_RUNNING) return false;
-        const int64_t k = sc->kcur;  // written by this iteration's A-kernel
+        if (status != CG_RUNNING) return false;
         double tot[1];
-        partials_reduce<1>(acc, tot);
+        pt.total(tot);
         const double pAp = tot[0];
-        const double zTr = sc->zrec[k & 1];
+        const double zTr = zrec[k & 1];
         int decided = CG_RUNNING;
         if (pAp <= 0.0)
             decided = HOEFEM_ERROR_CG_INDEFINITE_OPERATOR;
