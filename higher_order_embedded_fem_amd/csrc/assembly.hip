@@ -5,6 +5,7 @@
 // :915-966): every CSR entry / force component receives its element contributions in color
 // order, because the incidence lists are built color by color and one work-group owns each
 // node's row block.
+#include <type_traits>
 #include <algorithm>
 
 #include "fem_device.hpp"
@@ -240,12 +241,14 @@ typedef double mf_double4 __attribute__((ext_vector_type(4)));
 // interleave the phases of different elements). Quadrature points go in chunks of TW_QC:
 //  - per-point phase with TW_SUB lanes per point: the Jacobian (each lane), the node gradients split
 //    over the point's lanes (scaled by sqrt(w |det J|) into s_G), u_grad reduced over the lanes by
-//    shuffles, then the 81 entries of dP/dF split over the lanes (snh_dpdf_entry; the material
-//    variants evaluate dP/dF on the point's first lane);
+//    shuffles, then dP/dF as nine 3 x 3 blocks on nine of the point's lanes (snh_dpdf_block; the
+//    material variants evaluate dP/dF on the point's first lane). No memory access waits in this
+//    phase: the first chunk's point data is loaded with the element's nodes, and the reference node
+//    coordinates are decoded from constants (hexn);
 //  - product phase: K^{ij} += G~^T U^{ij} on v_mfma_f64_16x16x4, wavefront w taking the pairs
 //    (i, j) = (w, w..2) for w = 0 and (1, 1), (1, 2), (2, 2) for w = 1, with the B operand
 //    U^{ij}(kk, col) = sum_m C_q(3k + i, 3m + j) G~(3q + m, col) formed in registers at each k-step
-//    (no U staging in LDS).
+//    (no U staging in LDS); a diagonal pair skips its tile above the diagonal.
 constexpr int TW_QC = 8, TW_THREADS = 128, TW_SUB = TW_THREADS / TW_QC;
 
 template <int ET, int MAT>
@@ -264,16 +267,27 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     __shared__ double s_X[3 * GN];
     __shared__ double s_C[TW_QC][81];
     __shared__ double s_G[3 * TW_QC * NP];
+    const int pq = t / TW_SUB, sub = t % TW_SUB;
+    const int q0 = qoff[e], q1 = qoff[e + 1];
     if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
     __syncthreads();
     for (int k = t; k < 3 * N; k += TW_THREADS) s_u[k] = u[3 * (int64_t)s_node[k / 3] + k % 3];
     for (int k = t; k < 3 * GN; k += TW_THREADS) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
+    // the lane's point of the first chunk (rule position and weight), loaded with the element's data
+    // rather than at the start of the per-point phase
+    double pxi[3] = {0.0, 0.0, 0.0}, pw = 0.0;
+    if (q1 > q0) {
+        const int q = min(q0 + pq, q1 - 1);
+        pxi[0] = qxi[3 * q];
+        pxi[1] = qxi[3 * q + 1];
+        pxi[2] = qxi[3 * q + 2];
+        pw = qw[q];
+    }
     mf_double4 acc[JW];
-#pragma unroll
-    for (int j = 0; j < JW; ++j) acc[j] = mf_double4{0.0, 0.0, 0.0, 0.0};
-    const int pq = t / TW_SUB, sub = t % TW_SUB;
-    const int q0 = qoff[e], q1 = qoff[e + 1];
-    for (int qc = q0; qc < q1; qc += TW_QC) {
+    // One chunk of up to TW_QC points. The first chunk is its own instance: its products start from
+    // zero, so the accumulators hold nothing during its per-point phase (fewer live registers there).
+    auto chunk = [&](int qc, auto first_tag) {
+        constexpr bool FIRST = decltype(first_tag)::value;
         const int cnt = min(TW_QC, q1 - qc);
         for (int k = t; k < 3 * TW_QC * NP; k += TW_THREADS) s_G[k] = 0.0;
         __syncthreads();
@@ -283,14 +297,25 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         double ug[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // u_grad(a, b) = sum_I G(a, I) u(b, I)
         if (act) {
             const int q = qc + pq;
-            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
+            double xi[3], wq;
+            if constexpr (FIRST) {
+                xi[0] = pxi[0];
+                xi[1] = pxi[1];
+                xi[2] = pxi[2];
+                wq = pw;
+            } else {
+                xi[0] = qxi[3 * q];
+                xi[1] = qxi[3 * q + 1];
+                xi[2] = qxi[3 * q + 2];
+                wq = qw[q];
+            }
             double J[9], Jinv[9], psi[4];
             ref_jacobian<ET>(s_X, xi, J);
             const double detJ = det3(J);
             ok = inv3(J, Jinv);
             tet4_psi(xi, psi);
             if (ok) {
-                const double sq = sqrt(qw[q] * fabs(detJ));
+                const double sq = sqrt(wq * fabs(detJ));
                 for (int I = sub; I < N; I += TW_SUB) {
                     double Gr[3], G[3];
                     grad_node<ET>(I, xi, psi, Gr);
@@ -326,8 +351,8 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
                     const double beta = I_C + 1.0;
                     const double diag = mu * (1.0 - 1.0 / beta), cf = mu * 2.0 / (beta * beta);
                     const double scale = lambda * (Jf - alpha);
-                    for (int idx = sub; idx < 81; idx += TW_SUB)
-                        C[idx] = snh_dpdf_entry(idx % 9, idx / 9, F, g, diag, cf, lambda, scale);
+                    // the 81 entries of dP/dF as nine 3 x 3 blocks, one per lane of the point
+                    if (sub < 9) snh_dpdf_block(sub % 3, sub / 3, F, g, diag, cf, lambda, scale, C);
                 }
             } else {
                 int okl = 1;
@@ -356,7 +381,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
             const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                if (ks < ksteps) {
+                if (ks == 0 || ks < ksteps) {  // a chunk has at least one point: k-step 0 always runs
                     const int kk = 4 * ks + (l >> 4), qq = kk / 3, k = kk % 3;
                     const bool live = qq < cnt;
                     const double* Cq = s_C[live ? qq : 0];
@@ -370,14 +395,24 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
                             b = c0 * s_G[(3 * qq) * NP + col] + c1 * s_G[(3 * qq + 1) * NP + col] +
                                 c2 * s_G[(3 * qq + 2) * NP + col];
 #pragma unroll
-                        for (int ti = 0; ti < NT; ++ti)
-                            acc[(pp * NT + ti) * NT + tj] =
-                                __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti][ks], b, acc[(pp * NT + ti) * NT + tj], 0, 0, 0);
+                        for (int ti = 0; ti < NT; ++ti) {
+                            if (i == j && ti < tj) continue;  // a diagonal pair's tile above its diagonal is never stored
+                            mf_double4& A = acc[(pp * NT + ti) * NT + tj];
+                            A = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                a[ti][ks], b, (FIRST && ks == 0) ? mf_double4{0.0, 0.0, 0.0, 0.0} : A, 0, 0, 0);
+                        }
                     }
                 }
             }
         }
         __syncthreads();
+    };
+    if (q1 > q0) {
+        chunk(q0, std::true_type{});
+        for (int qc = q0 + TW_QC; qc < q1; qc += TW_QC) chunk(qc, std::false_type{});
+    } else {
+#pragma unroll
+        for (int j = 0; j < JW; ++j) acc[j] = mf_double4{0.0, 0.0, 0.0, 0.0};
     }
     double* out = ke + e * (int64_t)M3 * M3;
 #pragma unroll
@@ -387,6 +422,7 @@ __global__ __launch_bounds__(TW_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
         const int p = 3 * w + pp;
         const int i = p < 3 ? 0 : (p < 5 ? 1 : 2);
         const int j = p < 3 ? p : (p < 5 ? p - 2 : 2);
+        if (i == j && ti < tj) continue;  // not computed: every entry of it lies above the diagonal
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int I = 16 * ti + (4 * v + (l >> 4)), Jn = 16 * tj + (l & 15);

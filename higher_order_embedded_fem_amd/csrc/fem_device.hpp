@@ -20,11 +20,23 @@ template <> struct ElemTraits<5> { static constexpr int N = 27, GN = 8; static c
 
 // Hex27 reference nodes (the first 20 are the Hex20 nodes, the first 8 the Hex8 nodes),
 // fenris/src/element.rs:1250-1298.
-__constant__ static const signed char HEXN[27][3] = {
+constexpr signed char HEX_NODES[27][3] = {
     {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1},
     {-1, 1, 1},   {0, -1, -1}, {-1, 0, -1}, {-1, -1, 0}, {1, 0, -1}, {1, -1, 0}, {0, 1, -1},
     {1, 1, 0},    {-1, 1, 0},  {0, -1, 1},  {-1, 0, 1},  {1, 0, 1},  {0, 1, 1},  {0, 0, -1},
     {0, -1, 0},   {-1, 0, 0},  {1, 0, 0},   {0, 1, 0},   {0, 0, 1},  {0, 0, 0}};
+// Per axis, coordinate + 1 of node I in bits 2I and 2I + 1: a lane decodes its node's coordinates
+// with a shift (a lane-dependent index into a table in memory is a load on the critical path).
+constexpr unsigned long long hex_axis_bits(int d) {
+    unsigned long long m = 0;
+    for (int I = 0; I < 27; ++I) m |= (unsigned long long)(HEX_NODES[I][d] + 1) << (2 * I);
+    return m;
+}
+__device__ inline int hexn(int I, int d) {
+    constexpr unsigned long long B0 = hex_axis_bits(0), B1 = hex_axis_bits(1), B2 = hex_axis_bits(2);
+    const unsigned long long b = d == 0 ? B0 : (d == 1 ? B1 : B2);
+    return (int)((b >> (2 * I)) & 3ull) - 1;
+}
 // Tet4 reference gradients times 2 (exact small integers; the 0.5 factor is applied in T)
 __constant__ static const signed char TETG2[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
 __constant__ static const signed char TET10E[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {2, 3}, {1, 3}};
@@ -88,17 +100,17 @@ __device__ inline void grad_node(int I, const T* xi, const T* p, T* g) {
                 g[k] = (tetg<T>(a, k) * p[b] * p[c] + tetg<T>(b, k) * p[a] * p[c] + tetg<T>(c, k) * p[a] * p[b]) * T(27);
         }
     } else if constexpr (ET == 3) {
-        const T a = HEXN[I][0], b = HEXN[I][1], c = HEXN[I][2];
+        const T a = hexn(I, 0), b = hexn(I, 1), c = hexn(I, 2);
         g[0] = lin1g(a) * lin1(b, xi[1]) * lin1(c, xi[2]);
         g[1] = lin1(a, xi[0]) * lin1g(b) * lin1(c, xi[2]);
         g[2] = lin1(a, xi[0]) * lin1(b, xi[1]) * lin1g(c);
     } else if constexpr (ET == 5) {
-        const T a = HEXN[I][0], b = HEXN[I][1], c = HEXN[I][2];
+        const T a = hexn(I, 0), b = hexn(I, 1), c = hexn(I, 2);
         g[0] = quad1g(a, xi[0]) * quad1(b, xi[1]) * quad1(c, xi[2]);
         g[1] = quad1(a, xi[0]) * quad1g(b, xi[1]) * quad1(c, xi[2]);
         g[2] = quad1(a, xi[0]) * quad1(b, xi[1]) * quad1g(c, xi[2]);
     } else {  // Hex20, fenris/src/element.rs:1556-1625
-        const T a = HEXN[I][0], b = HEXN[I][1], c = HEXN[I][2];
+        const T a = hexn(I, 0), b = hexn(I, 1), c = hexn(I, 2);
         if (I < 8) {
             const T f = a * xi[0] + b * xi[1] + c * xi[2] - T(2);
             const T gg = (T(1) + a * xi[0]) * (T(1) + b * xi[1]) * (T(1) + c * xi[2]);
@@ -141,11 +153,11 @@ __device__ inline T basis_node(int I, const T* xi, const T* p) {
         }
         return T(27) * p[TET20F[I - 16][0]] * p[TET20F[I - 16][1]] * p[TET20F[I - 16][2]];
     } else if constexpr (ET == 3) {
-        return lin1(T(HEXN[I][0]), xi[0]) * lin1(T(HEXN[I][1]), xi[1]) * lin1(T(HEXN[I][2]), xi[2]);
+        return lin1(T(hexn(I, 0)), xi[0]) * lin1(T(hexn(I, 1)), xi[1]) * lin1(T(hexn(I, 2)), xi[2]);
     } else if constexpr (ET == 5) {
-        return quad1(T(HEXN[I][0]), xi[0]) * quad1(T(HEXN[I][1]), xi[1]) * quad1(T(HEXN[I][2]), xi[2]);
+        return quad1(T(hexn(I, 0)), xi[0]) * quad1(T(hexn(I, 1)), xi[1]) * quad1(T(hexn(I, 2)), xi[2]);
     } else {
-        const T a = HEXN[I][0], b = HEXN[I][1], c = HEXN[I][2];
+        const T a = hexn(I, 0), b = hexn(I, 1), c = hexn(I, 2);
         if (I < 8)
             return T(0.125) * (T(1) + a * xi[0]) * (T(1) + b * xi[1]) * (T(1) + c * xi[2]) *
                    (a * xi[0] + b * xi[1] + c * xi[2] - T(2));
@@ -165,7 +177,7 @@ __device__ inline void ref_jacobian(const T* X, const T* xi, T* J) {
             for (int c = 0; c < 3; ++c) G[k][c] = tetg<T>(k, c);
     } else {
         for (int k = 0; k < 8; ++k) {
-            const T a = HEXN[k][0], b = HEXN[k][1], c = HEXN[k][2];
+            const T a = hexn(k, 0), b = hexn(k, 1), c = hexn(k, 2);
             G[k][0] = lin1g(a) * lin1(b, xi[1]) * lin1(c, xi[2]);
             G[k][1] = lin1(a, xi[0]) * lin1g(b) * lin1(c, xi[2]);
             G[k][2] = lin1(a, xi[0]) * lin1(b, xi[1]) * lin1g(c);
@@ -313,6 +325,44 @@ __device__ inline T snh_dpdf_entry(int R, int C, const T* F, const T* g, T diag,
         v += (plus_block != cyclic) ? x : -x;
     }
     return v;
+}
+
+// The 3x3 block (br, bc) of snh_dpdf_entry's matrix, written into dp (column-major 9 x 9): the same
+// arithmetic per entry, with the indices inside the block known at compile time, so a lane selects
+// only the block's rows and columns of F and g instead of indexing by a lane-dependent entry.
+template <class T>
+__device__ inline T sel3(const T* v, int b, int i) {
+    return b == 0 ? v[i] : (b == 1 ? v[3 + i] : v[6 + i]);
+}
+template <class T>
+__device__ inline void snh_dpdf_block(int br, int bc, const T* F, const T* g, T diag, T cf, T lambda, T scale,
+                                      T* dp) {
+    const int k = br != bc ? 3 - br - bc : 0;  // the third block index (used when br != bc)
+    const bool plus_block = (br == 1 && bc == 0) || (br == 0 && bc == 2) || (br == 2 && bc == 1);
+    T Fr[3], Fc[3], gr[3], gc[3], Fk[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        Fr[i] = sel3(F, br, i);
+        Fc[i] = sel3(F, bc, i);
+        gr[i] = sel3(g, br, i);
+        gc[i] = sel3(g, bc, i);
+        Fk[i] = sel3(F, k, i);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            T v = (br == bc && r == c) ? diag : T(0);
+            v += cf * Fr[r] * Fc[c];
+            v += lambda * gr[r] * gc[c];
+            if (r != c && br != bc) {
+                const int m = 3 - r - c;
+                const bool cyclic = (r + 1) % 3 == c;
+                const T x = Fk[m] * scale;
+                v += (plus_block != cyclic) ? x : -x;
+            }
+            dp[(3 * br + r) + 9 * (3 * bc + c)] = v;
+        }
 }
 
 // ------------------------------------------------------------------ material variants
