@@ -306,10 +306,10 @@ void hoefem_sim_destroy(hoefem_sim* sim);
 double hoefem_sim_representative_force(const hoefem_sim* sim);
 /* mass matrix with Dirichlet conditions applied (device, nnz) */
 hoefem_status hoefem_sim_mass_values(hoefem_sim* sim, double* values);
-/* J = M + dt^2 K with Dirichlet conditions of the last Newton iteration (device, nnz) */
 /* The damping matrix D of the current (sub)step on the model pattern (device output, nnz);
  * zeros when the simulation has no damping. */
 hoefem_status hoefem_sim_damping_values(hoefem_sim* sim, double* values);
+/* J = M + dt^2 K with Dirichlet conditions of the last Newton iteration (device, nnz) */
 hoefem_status hoefem_sim_jacobian_values(hoefem_sim* sim, double* values);
 /* F(w) of BackwardEulerImplicitFunction::eval_into at (u, v) (device vectors) */
 hoefem_status hoefem_sim_residual(hoefem_sim* sim, double t, double dt, const double* u, const double* v,
