@@ -571,7 +571,7 @@ __global__ __launch_bounds__(64) void k_block_gather_pf(
     const int rw = 3 * k, R = 3 * rw;
     const int64_t r0 = rowptr[3 * v];
     const int i0 = inc_ptr[v], ninc = inc_ptr[v + 1] - i0;
-    // lane j holds incidence j (the rest, past 64 incident blocks, is read from memory)
+    // lane j holds incidence j (at most 64, checked by the launch)
     int a_l = 0, mb_l = 0;
     int64_t vo_l = 0, so_l = 0;
     if (t < ninc) {
@@ -582,21 +582,12 @@ __global__ __launch_bounds__(64) void k_block_gather_pf(
         so_l = slot_off[b];
     }
     for (int idx = t; idx < R; idx += 64) s_rb[idx] = (zero_first || mode == 1) ? 0.0 : (double)values[r0 + idx];
+    // incidence j's row values and slots: every lane loads (a lane past the block's 3 x 3m values
+    // re-reads its first one and does not use it), so the loads in flight are counted statically and
+    // adding one incidence waits only for its own loads
     auto fetch = [&](int j, double* vv, int* ss, int& bwj) {
-        int a, mb;
-        int64_t vo, so;
-        if (j < 64) {
-            a = __shfl(a_l, j, 64);
-            mb = __shfl(mb_l, j, 64);
-            vo = __shfl(vo_l, j, 64);
-            so = __shfl(so_l, j, 64);
-        } else {
-            const int b = inc_blk[i0 + j];
-            a = inc_loc[i0 + j];
-            mb = msz[b];
-            vo = val_off[b];
-            so = slot_off[b];
-        }
+        const int a = __shfl(a_l, j, 64), mb = __shfl(mb_l, j, 64);
+        const int64_t vo = __shfl(vo_l, j, 64), so = __shfl(so_l, j, 64);
         const int bw = 3 * mb;
         bwj = bw;
         const TB* base = blocks + vo + (int64_t)(3 * a) * bw;
@@ -604,34 +595,42 @@ __global__ __launch_bounds__(64) void k_block_gather_pf(
 #pragma unroll
         for (int q = 0; q < GP_VALS; ++q) {
             const int idx = t + 64 * q;
-            if (idx < 3 * bw) {
-                vv[q] = (double)base[idx];
-                ss[q] = sl[(idx % bw) / 3];
-            }
+            const int id = idx < 3 * bw ? idx : 0;
+            vv[q] = (double)base[id];
+            ss[q] = sl[(id % bw) / 3];
         }
     };
-    double cv[GP_VALS] = {0.0, 0.0, 0.0, 0.0}, nv[GP_VALS] = {0.0, 0.0, 0.0, 0.0};
-    int cs[GP_VALS] = {0, 0, 0, 0}, ns[GP_VALS] = {0, 0, 0, 0};
-    int cbw = 0, nbw = 0;
-    if (ninc > 0) fetch(0, cv, cs, cbw);
-    __syncthreads();
-    for (int j = 0; j < ninc; ++j) {
-        if (j + 1 < ninc) fetch(j + 1, nv, ns, nbw);
+    // two incidence buffers used in turn (no register copy between them, which would wait for the
+    // loads): the next incidence's loads are in flight while this one's values are added
+    double av[GP_VALS] = {0.0, 0.0, 0.0, 0.0}, bv[GP_VALS] = {0.0, 0.0, 0.0, 0.0};
+    int as[GP_VALS] = {0, 0, 0, 0}, bs[GP_VALS] = {0, 0, 0, 0};
+    int abw = 0, bbw = 0;
+    auto add = [&](const double* vv, const int* ss, int bw) {
 #pragma unroll
         for (int q = 0; q < GP_VALS; ++q) {
             const int idx = t + 64 * q;
-            if (idx < 3 * cbw) {
-                const int i = idx / cbw, cc = idx % cbw;
-                s_rb[i * rw + 3 * cs[q] + cc % 3] += cv[q];
+            if (idx < 3 * bw) {
+                const int i = idx / bw, cc = idx % bw;
+                s_rb[i * rw + 3 * ss[q] + cc % 3] += vv[q];
             }
         }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < GP_VALS; ++q) {
-            cv[q] = nv[q];
-            cs[q] = ns[q];
+    };
+    // The work-group is one wavefront, whose LDS accesses take effect in program order: the row's sums
+    // need no s_barrier, only a compiler barrier that keeps the LDS accesses of consecutive incidences
+    // in order (an s_barrier's fence would also wait for the loads of the incidence ahead). The fetches
+    // are unconditional inside the block (the last incidence again past the end), so every path reaches
+    // the loop with the same loads in flight and each addition waits only for its own incidence.
+    if (ninc > 0) {
+        fetch(0, av, as, abw);
+        __builtin_amdgcn_wave_barrier();
+        for (int j = 0; j < ninc; j += 2) {
+            fetch(min(j + 1, ninc - 1), bv, bs, bbw);
+            add(av, as, abw);
+            __builtin_amdgcn_wave_barrier();
+            fetch(min(j + 2, ninc - 1), av, as, abw);
+            if (j + 1 < ninc) add(bv, bs, bbw);
+            __builtin_amdgcn_wave_barrier();
         }
-        cbw = nbw;
     }
     for (int idx = t; idx < R; idx += 64) {
         const int64_t o = r0 + idx;
@@ -893,7 +892,7 @@ void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const T
                          double c, const double* M, bool zero_first, hipStream_t s, const double* D2, double cd,
                          double cm, KernelProf* prof) {
     if (pat.num_nodes == 0) return;
-    if (pat.max_nbr <= GATHER_CHUNK && 9 * sc.max_m <= 64 * GP_VALS) {
+    if (pat.max_nbr <= GATHER_CHUNK && 9 * sc.max_m <= 64 * GP_VALS && sc.max_inc <= 64) {
         const size_t lds = (size_t)9 * std::max(pat.max_nbr, 1) * sizeof(double);
         launch_timed(prof, k_block_gather_pf<TB, TV>, dim3((unsigned)pat.num_nodes), dim3(64), (uint32_t)lds, s,
                      (const int32_t*)pat.d_nptr.get(), (const int32_t*)pat.d_rowptr.get(),

@@ -139,6 +139,7 @@ struct BlockScatter {
     std::vector<int64_t> slot_off;  // offset of block b's m*m slot table
     std::vector<int32_t> slot;      // slot[slot_off[b] + a*m + c] = local index of node c in nbr(node a)
     std::vector<int32_t> inc_ptr;   // per pattern node
+    int max_inc = 0;                // most incident blocks of one node
     std::vector<int32_t> inc_blk, inc_loc;  // incident (block, local node) in color order
     DevBuf<int32_t> d_m, d_slot, d_inc_ptr, d_inc_blk, d_inc_loc;
     DevBuf<int64_t> d_val_off, d_slot_off;

@@ -295,6 +295,8 @@ void BlockScatter::build(const NodePattern& pat, int64_t nb, const int64_t* off,
         for (const auto& color : color_lists[l])
             for (int32_t b : color)
                 for (int a = 0; a < m[b]; ++a) inc_ptr[nodes[off[b] + a] + 1]++;
+    max_inc = 0;
+    for (int64_t v = 0; v < pat.num_nodes; ++v) max_inc = std::max(max_inc, inc_ptr[v + 1]);
     for (int64_t v = 0; v < pat.num_nodes; ++v) inc_ptr[v + 1] += inc_ptr[v];
     inc_blk.resize(inc_ptr[pat.num_nodes]);
     inc_loc.resize(inc_ptr[pat.num_nodes]);
