@@ -127,7 +127,8 @@ def tangent_flops(sc):
     """(algorithmic, issued) flops of one set of element tangents. Algorithmic: per point
     n(n+1)/2 * 162 for the contraction plus ~1000 for dP/dF and the gradients. Issued: the
     v_mfma_f64_16x16x4 products of the tangent kernel (six component pairs, nodes padded to 16 or
-    32, ceil(3 c / 4) k-steps per chunk of c <= 8 points, 2048 flops each)."""
+    32 so a pair is 1 or 2 x 2 tiles, the three diagonal pairs without their tile above the diagonal;
+    ceil(3 c / 4) k-steps per chunk of c <= 8 points, 2048 flops each)."""
     import numpy as np
 
     n = sc.nodes_per_element
@@ -136,7 +137,8 @@ def tangent_flops(sc):
     nt = 2 if n > 16 else 1
     full, rest = q // 8, q % 8
     ksteps = full * 6 + (3 * rest + 3) // 4
-    issued = float(ksteps.sum()) * 6 * nt * nt * 2048
+    tiles = 3 * nt * nt + 3 * nt * (nt + 1) // 2
+    issued = float(ksteps.sum()) * tiles * 2048
     return alg, issued
 
 
