@@ -18,6 +18,7 @@ EXPORTED = [
     "hoefem_nodes_per_element",
     "hoefem_lame_from_young_poisson",
     "hoefem_material_eval",
+    "hoefem_material_energy",
     "hoefem_material_eval_kind",
     "hoefem_rotation_svd",
     "hoefem_mesh_reorder",
@@ -214,6 +215,7 @@ def load(path: str = LIB_PATH):
     L.hoefem_lame_from_young_poisson.restype = None
     L.hoefem_material_eval.argtypes = [f64, f64, i64, vp, vp, vp]
     L.hoefem_material_eval_kind.argtypes = [i32, f64, f64, f64, i64, vp, vp, vp]
+    L.hoefem_material_energy.argtypes = [f64, f64, i64, vp, vp]
     L.hoefem_rotation_svd.argtypes = [i64, vp, vp, vp, vp]
     L.hoefem_mesh_reorder.argtypes = [i32, i64, i64, vp, vp, vp]
     L.hoefem_model_create.argtypes = [C.POINTER(ModelDesc), C.POINTER(vp)]

@@ -1050,6 +1050,13 @@ hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const d
     });
 }
 
+hoefem_status hoefem_material_energy(double mu, double lambda, int64_t nq, const double* F, double* psi) {
+    return guarded([&] {
+        if (nq < 0 || (nq > 0 && (!F || !psi))) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "bad arguments");
+        material_energy(mu, lambda, nq, F, psi);
+    });
+}
+
 hoefem_status hoefem_material_eval_kind(int32_t kind, double inversion_threshold, double mu, double lambda, int64_t nq,
                                        const double* F, double* P, double* dPdF) {
     return guarded([&] {

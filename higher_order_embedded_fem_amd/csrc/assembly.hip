@@ -803,6 +803,16 @@ __global__ void k_material(int mat, double thr, double mu, double lambda, int64_
     }
 }
 
+// Stable Neo-Hookean energy density of nq deformation gradients (column-major F[9 q .. 9 q + 8])
+__global__ void k_material_energy(double mu, double lambda, double mu_r, double lambda_r, int64_t nq,
+                                  const double* __restrict__ F, double* __restrict__ psi) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    double f[9];
+    for (int k = 0; k < 9; ++k) f[k] = F[9 * q + k];
+    psi[q] = snh_energy(mu, lambda, mu_r, lambda_r, f);
+}
+
 // rotation_svd (fenris/src/util.rs:68-94) on nq matrices: U, Vt column-major, S[3] signed.
 __global__ void k_rotation_svd(int64_t nq, const double* __restrict__ F, double* __restrict__ U,
                                double* __restrict__ S, double* __restrict__ Vt) {
@@ -1025,6 +1035,16 @@ int autotune_spmv(NodePattern& pat, hipStream_t s, double* times_us) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return best;
+}
+
+void material_energy(double mu, double lambda, int64_t nq, const double* F, double* psi) {
+    if (nq <= 0) return;
+    double mu_r, lambda_r;
+    snh_reparam(mu, lambda, &mu_r, &lambda_r);
+    hipLaunchKernelGGL(k_material_energy, dim3((unsigned)((nq + 127) / 128)), dim3(128), 0, nullptr, mu, lambda, mu_r,
+                       lambda_r, nq, F, psi);
+    check_launch("k_material_energy");
+    HIP_CHECK(hipDeviceSynchronize());
 }
 
 void material_eval(int mat, double thr, double mu_r, double lambda_r, int64_t nq, const double* F, double* P,

@@ -250,6 +250,21 @@ __host__ __device__ inline void snh_reparam(double mu, double lambda, double* mu
     *lambda_r = lambda + (1.0 - 2.0 / (d * (d + 1.0))) * mu;
 }
 
+// compute_strain_energy_density (fenris/src/solid/materials.rs:269-297): the energy density at F with
+// the Lame parameters as given (mu, lambda) and alpha = 1 + mu_r / lambda_r * d / (d + 1) from the
+// reparametrized ones (d = 3). A diagnostic: the Newton-CG step never evaluates it.
+template <class T>
+__device__ inline T snh_energy(T mu, T lambda, T mu_r, T lambda_r, const T* F) {
+    const T d = T(3);
+    const T a = mu_r / lambda_r * (d / (d + T(1)));
+    const T alpha = T(1) + a;
+    T I_C = T(0);
+    for (int k = 0; k < 9; ++k) I_C += F[k] * F[k];
+    const T Jma = det3(F) - alpha;
+    return T(0.5) * mu * (I_C - d) + T(0.5) * lambda * (Jma * Jma - a * a) -
+           T(0.5) * mu * log((I_C + T(1)) / (d + T(1)));
+}
+
 // compute_stress_tensor (fenris/src/solid/materials.rs:299-322); reparametrized parameters
 template <class T>
 __device__ inline bool snh_stress(T mu, T lambda, const T* F, T* P) {

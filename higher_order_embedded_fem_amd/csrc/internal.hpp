@@ -311,6 +311,8 @@ int autotune_spmv(NodePattern& pat, hipStream_t s, double* times_us);
 void launch_element_psd_projection(Model& m);  // in place on m.d_ke
 void launch_lincomb(hipStream_t s, int64_t n, double* out, double a, const double* A);  // out = 0 + a A
 void rotation_svd_eval(int64_t nq, const double* F, double* U, double* S, double* Vt);
+// psi[q] = Stable Neo-Hookean energy density of F[9 q ..] (Lame parameters as given, not reparametrized)
+void material_energy(double mu, double lambda, int64_t nq, const double* F, double* psi);
 void material_eval(int mat, double thr, double mu_r, double lambda_r, int64_t nq, const double* F, double* P,
                    double* dPdF);
 int read_and_clear_err(Model& m);  // synchronizes the model stream

@@ -75,6 +75,18 @@ class StableNeoHookeanMaterial:
         Dout = D.transpose(1, 2) if D is not None else None  # [q, row, col]
         return Pout, Dout
 
+    def energy(self, F: torch.Tensor) -> torch.Tensor:
+        """Strain energy density of the plain Stable Neo-Hookean material at each F (nq, 3, 3), device
+        float64 (compute_strain_energy_density, fenris/src/solid/materials.rs:269-297)."""
+        if self.kind != 0:
+            raise ValueError("the energy density is provided for the plain Stable Neo-Hookean material")
+        L = _lib.load()
+        nq = F.shape[0]
+        Fc = F.transpose(1, 2).contiguous()
+        psi = torch.empty(nq, dtype=torch.float64, device=F.device)
+        check(L.hoefem_material_energy(self.mu, self.lam, nq, Fc.data_ptr(), psi.data_ptr()))
+        return psi
+
 
 class InvertibleMaterial(StableNeoHookeanMaterial):
     """InvertibleMaterial::new(StableNeoHookeanMaterial, threshold) (fenris/src/solid/materials.rs:862-945):

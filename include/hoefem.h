@@ -107,6 +107,11 @@ void hoefem_lame_from_young_poisson(double young, double poisson, double* mu, do
  * F, P: [nq * 9] column-major; dPdF: [nq * 81]. Either output may be NULL. Device pointers. */
 hoefem_status hoefem_material_eval(double mu, double lambda, int64_t nq, const double* F, double* P, double* dPdF);
 
+/* StableNeoHookeanMaterial::compute_strain_energy_density (fenris/src/solid/materials.rs:269-297) on nq
+ * deformation gradients: psi[q] for F[9 q .. 9 q + 8] (column-major). Not on the Newton-CG path (a
+ * diagnostic). Device pointers. */
+hoefem_status hoefem_material_energy(double mu, double lambda, int64_t nq, const double* F, double* psi);
+
 /* The elastic models of simulation_toolbox/src/fem/fe_model.rs:35-45 built on Stable Neo-Hookean. */
 typedef enum hoefem_material_kind {
     /* StableNeoHookeanMaterial (fenris/src/solid/materials.rs:229-377); the default */

@@ -100,6 +100,11 @@ def test_material_eval_matches_oracle(fem):
     P, D = mat.evaluate(dev(F))
     assert rel(host(P), O.snh_stress(mu, lam, F)) < 1e-13
     assert rel(host(D), O.snh_dpdf(mu, lam, F)) < 1e-13
+    # the energy density (a diagnostic off the Newton path), also at F = I where it vanishes
+    G = np.concatenate([F, np.eye(3)[None]])
+    psi = host(mat.energy(dev(G)))
+    assert rel(psi, O.snh_energy(mu, lam, G)) < 1e-13
+    assert abs(psi[-1]) <= 1e-9 * mu
 
 
 @pytest.mark.parametrize("name", list(SMALL_SCENES))
