@@ -17,8 +17,15 @@ using namespace dev;
 
 // ------------------------------------------------------------------ element forces
 // assemble_generalized_element_elliptic_term (fenris/src/assembly.rs:674-711) with
-// g = -P(F)^T (fenris/src/solid/assembly.rs:33-38). One 64-lane work-group per element:
-// lanes = quadrature points (phase 1), then lanes = element nodes (phase 2).
+// g = -P(F)^T (fenris/src/solid/assembly.rs:33-38). One 64-lane work-group per element; quadrature
+// points in chunks of FC_PTS:
+//  - phase 1, FC_SUB lanes per point: the Jacobian (each lane), the node sum of F = I + u_grad split over
+//    the point's lanes and reduced by shuffles, then the stress; the point's first lane stores
+//    A = -P J^-T, w and |det J| (and the point's position for phase 2) in LDS;
+//  - phase 2, lanes = element nodes: f_I += A G_ref(I) w |det J| over the chunk's points in order.
+// The first chunk's point data is loaded with the element's nodes, not at the start of phase 1.
+constexpr int FC_PTS = 8, FC_SUB = 64 / FC_PTS;
+
 template <int ET, int MAT>
 __global__ __launch_bounds__(64) void k_element_forces(int64_t ne, const int32_t* __restrict__ conn,
                                                        const double* __restrict__ verts,
@@ -29,67 +36,95 @@ __global__ __launch_bounds__(64) void k_element_forces(int64_t ne, const int32_t
                                                        double* __restrict__ fe, int* err) {
     constexpr int N = ElemTraits<ET>::N, GN = ElemTraits<ET>::GN;
     const int64_t e = blockIdx.x;
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, pq = t / FC_SUB, sub = t % FC_SUB;
     __shared__ int s_node[N];
     __shared__ double s_u[3 * N];
     __shared__ double s_X[3 * GN];
-    __shared__ double s_A[64][12];
+    __shared__ double s_A[FC_PTS][12];
+    __shared__ double s_xi[FC_PTS][3];
+    const int q0 = qoff[e], q1 = qoff[e + 1];
     if (t < N) s_node[t] = conn[(int64_t)t * ne + e];
     __syncthreads();
     for (int k = t; k < 3 * N; k += 64) s_u[k] = u[3 * (int64_t)s_node[k / 3] + k % 3];
     for (int k = t; k < 3 * GN; k += 64) s_X[k] = verts[3 * (int64_t)s_node[k / 3] + k % 3];
+    double pxi[3] = {0.0, 0.0, 0.0}, pw = 0.0;  // the lane's point of the first chunk
+    if (q1 > q0) {
+        const int q = min(q0 + pq, q1 - 1);
+        pxi[0] = qxi[3 * q];
+        pxi[1] = qxi[3 * q + 1];
+        pxi[2] = qxi[3 * q + 2];
+        pw = qw[q];
+    }
     __syncthreads();
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    const int q0 = qoff[e], q1 = qoff[e + 1];
-    for (int qb = q0; qb < q1; qb += 64) {
-        const int q = qb + t;
-        if (q < q1) {
-            const double xi[3] = {qxi[3 * q], qxi[3 * q + 1], qxi[3 * q + 2]};
-            double J[9], Jinv[9];
-            ref_jacobian<ET>(s_X, xi, J);
-            const double detJ = det3(J);
-            bool ok = inv3(J, Jinv);
-            double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            double psi[4];
-            tet4_psi(xi, psi);
+    for (int qb = q0; qb < q1; qb += FC_PTS) {
+        const int q = qb + pq;
+        const bool act = q < q1;
+        double xi[3], wq = 0.0;
+        if (qb == q0) {
+            xi[0] = pxi[0];
+            xi[1] = pxi[1];
+            xi[2] = pxi[2];
+            wq = pw;
+        } else {
+            const int qc = act ? q : q1 - 1;
+            xi[0] = qxi[3 * qc];
+            xi[1] = qxi[3 * qc + 1];
+            xi[2] = qxi[3 * qc + 2];
+            wq = qw[qc];
+        }
+        double J[9], Jinv[9], psi[4];
+        ref_jacobian<ET>(s_X, xi, J);
+        const double detJ = det3(J);
+        bool ok = inv3(J, Jinv);
+        tet4_psi(xi, psi);
+        double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (act && ok) {
+            // mm(b, a) = sum_I u_I[b] * (G_ref^T J^-1)(I, a), nodes split over the point's lanes
+            for (int I = sub; I < N; I += FC_SUB) {
+                double G[3];
+                grad_node<ET>(I, xi, psi, G);
+                double tmp[3];
+                for (int a = 0; a < 3; ++a) tmp[a] = G[0] * Jinv[0 + 3 * a] + G[1] * Jinv[1 + 3 * a] + G[2] * Jinv[2 + 3 * a];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) F[b + 3 * a] += s_u[3 * I + b] * tmp[a];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+#pragma unroll
+            for (int off = 1; off < FC_SUB; off <<= 1) F[k] += __shfl_xor(F[k], off, 64);
+        if (act && sub == 0) {
+            s_xi[pq][0] = xi[0];
+            s_xi[pq][1] = xi[1];
+            s_xi[pq][2] = xi[2];
+            double P[9];
             if (ok) {
-                // mm(b, a) = sum_I u_I[b] * (G_ref^T J^-1)(I, a)
-                for (int I = 0; I < N; ++I) {
-                    double G[3];
-                    grad_node<ET>(I, xi, psi, G);
-                    double tmp[3];
-                    for (int a = 0; a < 3; ++a) tmp[a] = G[0] * Jinv[0 + 3 * a] + G[1] * Jinv[1 + 3 * a] + G[2] * Jinv[2 + 3 * a];
-                    for (int a = 0; a < 3; ++a)
-                        for (int b = 0; b < 3; ++b) F[b + 3 * a] += s_u[3 * I + b] * tmp[a];
-                }
                 F[0] += 1.0;
                 F[4] += 1.0;
                 F[8] += 1.0;
-                double P[9];
                 ok = material_stress(MAT, thr, mu, lambda, F, P);
-                if (ok) {
-                    // A = (-P) * J^-T  : A(i, a) = sum_c -P(i, c) * Jinv(a, c)
-                    for (int a = 0; a < 3; ++a)
-                        for (int i = 0; i < 3; ++i)
-                            s_A[t][i + 3 * a] = (-P[i]) * Jinv[a] + (-P[i + 3]) * Jinv[a + 3] + (-P[i + 6]) * Jinv[a + 6];
-                    s_A[t][9] = qw[q];
-                    s_A[t][10] = fabs(detJ);
-                }
             }
-            if (!ok) {
+            if (ok) {
+                // A = (-P) * J^-T  : A(i, a) = sum_c -P(i, c) * Jinv(a, c)
+                for (int a = 0; a < 3; ++a)
+                    for (int i = 0; i < 3; ++i)
+                        s_A[pq][i + 3 * a] = (-P[i]) * Jinv[a] + (-P[i + 3]) * Jinv[a + 3] + (-P[i + 6]) * Jinv[a + 6];
+                s_A[pq][9] = wq;
+                s_A[pq][10] = fabs(detJ);
+            } else {
                 atomicOr(err, KERR_SINGULAR);
-                for (int k = 0; k < 11; ++k) s_A[t][k] = 0.0;
+                for (int k = 0; k < 11; ++k) s_A[pq][k] = 0.0;
             }
         }
         __syncthreads();
         if (t < N) {
-            const int cnt = min(64, q1 - qb);
+            const int cnt = min(FC_PTS, q1 - qb);
             for (int k = 0; k < cnt; ++k) {
-                const int q2 = qb + k;
-                const double xi[3] = {qxi[3 * q2], qxi[3 * q2 + 1], qxi[3 * q2 + 2]};
-                double psi[4], G[3];
-                tet4_psi(xi, psi);
-                grad_node<ET>(t, xi, psi, G);
+                const double xk[3] = {s_xi[k][0], s_xi[k][1], s_xi[k][2]};
+                double pk[4], G[3];
+                tet4_psi(xk, pk);
+                grad_node<ET>(t, xk, pk, G);
                 const double* A = s_A[k];
                 const double w = A[9], dj = A[10];
                 f0 += (A[0] * G[0] + A[3] * G[1] + A[6] * G[2]) * w * dj;
