@@ -146,8 +146,8 @@ __device__ inline double block_sum(double v, double* s_red) {
 //   2  pivots positive, the cheap bound too loose: the inverse replaces the block and
 //      k_spd_power_certify decides with a sharper bound;
 //   1  a pivot not positive: B is kept for the eigen path.
-// RR x RR register tiles per thread: RR = 6 covers blocks up to 96 x 96, RR = 4 up to 64 x 64 (with a
-// third of the instructions and registers).
+// RR x RR register tiles per thread: RR = 6 covers blocks up to 96 x 96 (used above 64 x 64; the
+// smaller blocks go to k_spd_inverse_mf on the matrix cores, with the same results per block).
 
 // Pivot row k = 16 R + kt and column k of the register-held block into LDS (owners only).
 template <int R, int RR>
@@ -285,6 +285,138 @@ __global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse(const int32_t* __re
             if (i < dim && j < dim) blk[(int64_t)i * dim + j] = 0.5 * (a[r][c] + s_x[c][partner]);
         }
         __syncthreads();
+    }
+    if (t == 0) fallback[b] = cert < SPD_COND_LIMIT ? 0 : 2;
+}
+
+typedef double sgj_double4 __attribute__((ext_vector_type(4)));
+
+// The same inversion on the matrix cores: in-place blocked Gauss-Jordan over NT x NT tiles of 16 x 16
+// (dim <= 16 NT; the padding is the identity), one work-group of four wavefronts per block, the block
+// in LDS (row stride LD = 16 NT + 1). For each diagonal tile k:
+//   D = A_kk^-1        wavefront 0, pivot by pivot (the pivots are those of the unblocked elimination,
+//                      so a non-positive one marks the block exactly as k_spd_inverse does)
+//   A_kj = D A_kj      (j != k)
+//   A_ij -= A_ik A_kj  (i, j != k; the new A_kj, the old A_ik)
+//   A_ik = -A_ik D     (i != k);  A_kk = D
+// The tile products run on v_mfma_f64_16x16x4 (lane l: A operand X(l % 16, 4 s + l / 16), B operand
+// Y(4 s + l / 16, l % 16), result item v = (4 v + l / 16, l % 16)), the tiles of each step dealt to
+// the wavefronts in turn. The certificate and the symmetrized write-back are k_spd_inverse's.
+template <int NT>
+__global__ __launch_bounds__(EIG_THREADS) void k_spd_inverse_mf(const int32_t* __restrict__ msz,
+                                                                const int64_t* __restrict__ val_off,
+                                                                double* __restrict__ blocks, int* __restrict__ fallback) {
+    constexpr int PD = 16 * NT, LD = PD + 1;
+    extern __shared__ double s_a[];  // [PD][LD]
+    __shared__ double s_red[EIG_THREADS];
+    __shared__ int s_bad;
+    const int64_t b = blockIdx.x;
+    const int dim = 3 * msz[b];
+    if (dim > SCHWARZ_SMALL_DIM || dim > PD) {  // handled by the blocked path in global memory
+        if (threadIdx.x == 0) fallback[b] = 4;
+        return;
+    }
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    double* blk = blocks + val_off[b];
+    for (int idx = t; idx < PD * PD; idx += EIG_THREADS) {
+        const int i = idx / PD, j = idx % PD;
+        s_a[i * LD + j] = (i < dim && j < dim) ? blk[(int64_t)i * dim + j] : (i == j ? 1.0 : 0.0);
+    }
+    if (t == 0) s_bad = 0;
+    __syncthreads();
+    // |.|_F^2 and |.|_inf of the block's dim x dim part
+    auto norms = [&](double& f2, double& finf) {
+        double loc = 0.0, row = 0.0;
+        for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+            const double x = s_a[(idx / dim) * LD + idx % dim];
+            loc += x * x;
+        }
+        if (t < dim)
+            for (int j = 0; j < dim; ++j) row += fabs(s_a[t * LD + j]);
+        f2 = block_sum(loc, s_red);
+        finf = block_max(row, s_red);
+    };
+    double nb2, nbinf;
+    norms(nb2, nbinf);
+    // C = (neg ? -1 : 1) X Y (+ Cin), 16 x 16 tiles at (xr, xc), (yr, yc) of s_a
+    auto tile_mul = [&](int xr, int xc, int yr, int yc, bool neg, sgj_double4 acc) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+            const double x = s_a[(xr + (l & 15)) * LD + xc + 4 * s4 + (l >> 4)];
+            const double y = s_a[(yr + 4 * s4 + (l >> 4)) * LD + yc + (l & 15)];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(neg ? -x : x, y, acc, 0, 0, 0);
+        }
+        return acc;
+    };
+    auto tile_store = [&](int r0, int c0, const sgj_double4& acc) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s_a[(r0 + 4 * v + (l >> 4)) * LD + c0 + (l & 15)] = acc[v];
+    };
+    auto tile_load = [&](int r0, int c0) {
+        sgj_double4 acc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[v] = s_a[(r0 + 4 * v + (l >> 4)) * LD + c0 + (l & 15)];
+        return acc;
+    };
+    const sgj_double4 zero = {0.0, 0.0, 0.0, 0.0};
+    for (int kt = 0; kt < NT && 16 * kt < dim; ++kt) {
+        const int K = 16 * kt;
+        if (w == 0) {  // D = A_kk^-1 in place, one wavefront: its LDS accesses take effect in order
+            for (int p = 0; p < 16; ++p) {
+                const double piv = s_a[(K + p) * LD + K + p];
+                if (!(piv > 0.0) || !isfinite(piv)) {
+                    if (l == 0) s_bad = 1;
+                    break;
+                }
+                const double ip = 1.0 / piv;
+                double nv[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int e = l + 64 * m, i = e >> 4, j = e & 15;
+                    const double aij = s_a[(K + i) * LD + K + j];
+                    const double aip = s_a[(K + i) * LD + K + p], apj = s_a[(K + p) * LD + K + j];
+                    nv[m] = (i == p && j == p) ? ip : (i == p ? apj * ip : (j == p ? -aip * ip : aij - aip * apj * ip));
+                }
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int e = l + 64 * m;
+                    s_a[(K + (e >> 4)) * LD + K + (e & 15)] = nv[m];
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+        if (s_bad) {  // the same in every thread: uniform exit
+            if (t == 0) fallback[b] = 1;
+            return;
+        }
+        // row panel A_kj = D A_kj: the tile's old values are read (by its wavefront) before it is written
+        for (int j = 0, n = 0; j < NT; ++j) {
+            if (j == kt) continue;
+            if ((n++ & 3) == w) tile_store(K, 16 * j, tile_mul(K, K, K, 16 * j, false, zero));
+        }
+        __syncthreads();
+        for (int i = 0, n = 0; i < NT; ++i) {
+            if (i == kt) continue;
+            for (int j = 0; j < NT; ++j) {
+                if (j == kt) continue;
+                if ((n++ & 3) == w) tile_store(16 * i, 16 * j, tile_mul(16 * i, K, K, 16 * j, true, tile_load(16 * i, 16 * j)));
+            }
+        }
+        __syncthreads();
+        for (int i = 0, n = 0; i < NT; ++i) {
+            if (i == kt) continue;
+            if ((n++ & 3) == w) tile_store(16 * i, K, tile_mul(16 * i, K, K, K, true, zero));
+        }
+        __syncthreads();
+    }
+    double ni2, niinf;
+    norms(ni2, niinf);
+    const double cert = fmin(sqrt(nb2) * sqrt(ni2), nbinf * niinf);
+    for (int idx = t; idx < dim * dim; idx += EIG_THREADS) {
+        const int i = idx / dim, j = idx % dim;
+        blk[(int64_t)i * dim + j] = 0.5 * (s_a[i * LD + j] + s_a[j * LD + i]);
     }
     if (t == 0) fallback[b] = cert < SPD_COND_LIMIT ? 0 : 2;
 }
@@ -974,13 +1106,20 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
         // Gauss-Jordan fast path (blocks up to 96 x 96; it marks the larger ones 4), the sharper
         // certificate for the blocks it could not accept, then the eigen path for the rest; blocks
         // above 96 x 96 go through the blocked path
-        if (dim <= 64)
-            hipLaunchKernelGGL(k_spd_inverse<4>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
-                               s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
-        else
+        // blocks up to 64 x 64 on the matrix cores (Hex20 cylinder: 93.3 vs 100.3 us per update); larger
+        // ones by the register Gauss-Jordan (Hex27 cylinder's 81 x 81: 869 vs 938 us on the matrix cores,
+        // whose LDS round trips and serial diagonal tiles dominate at that size)
+        if (dim <= 64) {
+            const size_t lds = (size_t)64 * 65 * sizeof(double);
+            raise_lds_limit((const void*)k_spd_inverse_mf<4>, lds);
+            hipLaunchKernelGGL(k_spd_inverse_mf<4>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), lds, st,
+                               s.scatter.d_m.get(), s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
+            check_launch("k_spd_inverse_mf");
+        } else {
             hipLaunchKernelGGL(k_spd_inverse<6>, dim3((unsigned)s.nsub), dim3(EIG_THREADS), 0, st, s.scatter.d_m.get(),
                                s.scatter.d_val_off.get(), s.d_blocks.get(), s.d_fallback.get());
-        check_launch("k_spd_inverse");
+            check_launch("k_spd_inverse");
+        }
         const int dp = (dim + 3) & ~3;
         const int ld_pc = dp + 1;
         const size_t lds_pc = (size_t)2 * dp * ld_pc * sizeof(double);  // 149 KB at dim 96
