@@ -6,7 +6,8 @@ sharper bound (|B^8|_F^(1/8) |B^-8|_F^(1/8)) decides; the rest go to the Jacobi 
 reference's clamp max(|lambda|, 1e-12 max|lambda|) (simulation_toolbox/src/fem/schwarz_precond.rs:
 329-342). Scaling the three DOFs of one node by sigma drives the condition numbers of the blocks
 that hold it through all three cases (checked below with numpy, so the sweep covers each), and
-the GPU operator must stay within what the conditioning allows of numpy's eigh-based one.
+the GPU operator must stay within what the conditioning allows of numpy's eigh-based one. Blocks up
+to 64 x 64 and larger ones take different inversion kernels; both sizes are driven through.
 """
 
 import numpy as np
@@ -43,10 +44,28 @@ def _case(B):
     return ("power" if _bound(B) * _bound(Bi) < 5e11 else "eigen"), kappa
 
 
-def test_block_inverse_paths():
+def _tet10_cube():
+    sc = scenes.cube_scene(3, scenes.TET10)
+    return sc, int(sc.subdomain_nodes[0]), (1.0, 6e-6, 4e-6, 2e-6, 1e-7)
+
+
+def _hex27_cylinder():
+    # a node of a full 81 x 81 block; the cut cells already make these blocks ill-conditioned
+    sc = scenes.cylinder_scene(6, scenes.HEX27, 27)
+    off, nodes = np.asarray(sc.subdomain_offsets), np.asarray(sc.subdomain_nodes)
+    dset = set(sc.dirichlet_nodes.tolist())
+    v = next(int(n) for b in range(len(off) - 1) if off[b + 1] - off[b] == 27 for n in nodes[off[b]:off[b + 1]]
+             if int(n) not in dset)
+    return sc, v, (1.0, 0.04, 0.03, 0.02)
+
+
+# 30 x 30 blocks (inverted on the matrix cores, k_spd_inverse_mf) and 81 x 81 blocks (in registers,
+# k_spd_inverse): each kernel's accepted, re-certified and rejected blocks
+@pytest.mark.parametrize("make", [_tet10_cube, _hex27_cylinder], ids=["tet10_30x30", "hex27_81x81"])
+def test_block_inverse_paths(make):
     from higher_order_embedded_fem_amd import fem
 
-    sc = scenes.cube_scene(3, scenes.TET10)
+    sc, v, sigmas = make()
     om = O.Model(sc)
     mu, lam = scenes.lame_from_young_poisson(sc.young, sc.poisson)
     rp, col = om.pattern()
@@ -56,12 +75,11 @@ def test_block_inverse_paths():
     gsw = fem.SchwarzPreconditioner.from_scene(gm, sc)
     prp, pcol = gsw.pattern()
     off, nodes = sc.subdomain_offsets, sc.subdomain_nodes
-    v = int(nodes[0])
     assert v not in set(sc.dirichlet_nodes.tolist())
     held = [b for b in range(len(off) - 1) if v in nodes[off[b]:off[b + 1]]]
     rows = np.repeat(np.arange(sc.ndof), np.diff(rp))
     seen = set()
-    for sigma in (1.0, 6e-6, 4e-6, 2e-6, 1e-7):
+    for sigma in sigmas:
         s = np.ones(sc.ndof)
         s[3 * v:3 * v + 3] = sigma
         J = J0 * s[rows] * s[col]
