@@ -67,6 +67,8 @@ def parse():
                    help="FP32 element tangents / Jacobian SpMV / Schwarz operator inside the FP64 Newton refinement")
     p.add_argument("--mixed-refined", action="store_true",
                    help="FP64 tangents and Jacobian, CG on FP32 copies inside FP64 iterative refinement")
+    p.add_argument("--mixed-f64-tangents", action="store_true",
+                   help="as --mixed, but FP64 element tangents, each Jacobian entry rounded to FP32 once")
     p.add_argument("--no-autotune", action="store_true", help="keep the heuristic SpMV lanes per node")
     p.add_argument("--stiffness-damping", type=float, default=0.0,
                    help="Material stiffness_damping_coefficient (the reference's cylinder shell scene uses 0.05)")
@@ -146,7 +148,8 @@ def tangent_flops(sc):
 def _probe_command(args, counter, outdir):
     cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir, "-o", "run", "--",
            sys.executable, os.path.abspath(__file__), "--traffic-probe", "--config", str(args.config)]
-    for flag, on in (("--mixed", args.mode == 1), ("--mixed-refined", args.mode == 2), ("--reorder", args.reorder),
+    for flag, on in (("--mixed", args.mode == 1), ("--mixed-refined", args.mode == 2),
+                     ("--mixed-f64-tangents", args.mode == 3), ("--reorder", args.reorder),
                      ("--no-autotune", args.no_autotune)):
         if on:
             cmd.append(flag)
@@ -443,7 +446,7 @@ def run_probe(args):
 
 def main():
     args = parse()
-    args.mode = 2 if args.mixed_refined else (1 if args.mixed else 0)
+    args.mode = 3 if args.mixed_f64_tangents else (2 if args.mixed_refined else (1 if args.mixed else 0))
     args.mixed = args.mode != 0
     if args.traffic_probe:
         run_probe(args)
@@ -488,7 +491,7 @@ def main():
     if lanes is not None:
         print(f"[bench] autotune us per CG iteration, rows J lanes 8/16/32, columns P lanes 8/16/32:\n"
               f"{np.array2string(sim.autotune_us, precision=2)}", file=sys.stderr, flush=True)
-    precision = {0: "fp64", 1: "mixed", 2: "mixed-refined"}[args.mode]
+    precision = {0: "fp64", 1: "mixed", 2: "mixed-refined", 3: "mixed-f64-tangents"}[args.mode]
 
     def run(steps, record=None):
         u = torch.zeros(sc.ndof, dtype=torch.float64, device="cuda")

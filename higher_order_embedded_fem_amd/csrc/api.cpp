@@ -155,7 +155,10 @@ struct Sim {
     DevBuf<double> d_M, d_J, d_gforce, d_up, d_vn, d_w, d_F, d_sol, d_uu, d_us, d_vs, d_scale, d_norm2;
     bool mixed = false;   // FP32 CG operator (J and Schwarz values)
     int mixed_mode = 0;   // 1: FP32 element tangents, the Newton loop refines; 2: FP64 tangents and J, FP32
-                          //    copy for the CG, FP64 iterative refinement of every linear solve
+                          //    copy for the CG, FP64 iterative refinement of every linear solve; 3: FP64
+                          //    tangents rounded into the FP32 J at the gather, the Newton loop refines
+    // J is assembled straight into FP32 values (modes 1 and 3): the CG runs on d_J32 and d_J is unused
+    bool fp32_jacobian() const { return mixed_mode == 1 || mixed_mode == 3; }
     DevBuf<float> d_J32;
     DevBuf<double> d_res, d_dx;  // iterative refinement (mode 2)
     int64_t refinements = 0;
@@ -217,7 +220,8 @@ static void sim_create(Sim& sref, Model& m, const hoefem_sim_desc* d) {
     for (DevBuf<double>* b : {&s->d_gforce, &s->d_up, &s->d_vn, &s->d_w, &s->d_F, &s->d_sol, &s->d_uu, &s->d_us,
                               &s->d_vs, &s->d_f, &s->d_su, &s->d_sv})
         b->ensure(n);
-    if (d->mixed_precision < 0 || d->mixed_precision > 2) throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "mixed_precision must be 0, 1 or 2");
+    if (d->mixed_precision < 0 || d->mixed_precision > 3)
+        throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "mixed_precision must be 0, 1, 2 or 3");
     if (!(d->mass_damping_coefficient >= 0.0 && std::isfinite(d->mass_damping_coefficient)) ||
         !(d->stiffness_damping_coefficient >= 0.0 && std::isfinite(d->stiffness_damping_coefficient)))
         throw Error(HOEFEM_ERROR_INVALID_ARGUMENT, "damping coefficients must be finite and >= 0");
@@ -331,9 +335,12 @@ static void sim_assemble_jacobian_launch(Sim& s, bool has_a, double a, bool has_
         launch_element_stiffness(m, s.mu_r, s.lambda_r, uu, s.mixed_mode == 1);
         m.prof_tag = -1;
     }
-    if (s.mixed_mode == 1) {
-        if (has_a)
+    if (s.fp32_jacobian()) {
+        if (has_a && s.mixed_mode == 1)
             launch_block_gather(m.pattern, m.scatter, m.d_ke32.get(), s.d_J32.get(), 1, -a, s.d_M.get(), true,
+                                m.stream, D, cd, 1.0, gprof);
+        else if (has_a)  // mode 3: each entry is summed in FP64 and rounded once
+            launch_block_gather(m.pattern, m.scatter, m.d_ke.get(), s.d_J32.get(), 1, -a, s.d_M.get(), true,
                                 m.stream, D, cd, 1.0, gprof);
         else
             launch_comb(m, nnz, s.d_J32.get(), cd, D, 1.0, s.d_M.get());
@@ -446,7 +453,7 @@ static hoefem_status sim_cg(Sim& s, const double* rhs, double* sol, int64_t* ite
                 break;
             }
         }
-    } else if (s.mixed_mode == 1) {
+    } else if (s.fp32_jacobian()) {
         st = pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter, &it);
     } else {
         st = pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), rhs, sol, s.d.cg_tolerance, s.d.cg_max_iter, &it);
@@ -694,7 +701,7 @@ static void batch_step(Batch& B, const double* dt, double* const* u, double* con
         HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         B.ev.push_back(e);
     }
-    const bool fp32 = B.sims[0]->mixed_mode == 1;
+    const bool fp32 = B.sims[0]->fp32_jacobian();
     std::vector<int> pool_of(ns);
     int pool_size[BATCH_POOLS] = {};
     for (size_t i = 0; i < ns; ++i) {
@@ -1519,7 +1526,7 @@ hoefem_status hoefem_sim_jacobian_values(hoefem_sim* sim, double* values) {
         Model& m = *sim->s.model;
         CallIo io(m);
         double* dv = io.out(values, (size_t)m.pattern.nnz());
-        if (sim->s.mixed_mode == 1)
+        if (sim->s.fp32_jacobian())
             launch_f32_to_f64(m, sim->s.d_J32.get(), dv, m.pattern.nnz());
         else
             HIP_CHECK(hipMemcpyAsync(dv, sim->s.d_J.get(), sizeof(double) * m.pattern.nnz(), hipMemcpyDeviceToDevice,

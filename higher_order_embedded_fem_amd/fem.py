@@ -310,7 +310,7 @@ class NewtonCgIntegrator:
         d.max_newton_iter = max_newton_iter
         d.cg_tolerance = cg_tolerance
         d.cg_max_iter = cg_max_iter
-        d.mixed_precision = int(mixed_precision)  # False/0, True/1 or 2 (see hoefem_sim_desc)
+        d.mixed_precision = int(mixed_precision)  # False/0, True/1, 2 or 3 (see hoefem_sim_desc)
         # Material::mass_damping_coefficient / stiffness_damping_coefficient (fe_model.rs:131-136); 0 = None
         d.mass_damping_coefficient = float(mass_damping)
         d.stiffness_damping_coefficient = float(stiffness_damping)

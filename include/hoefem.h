@@ -286,7 +286,9 @@ typedef struct hoefem_sim_desc {
                                               tangents, Jacobian and Schwarz factorizations; the CG runs on
                                               FP32 copies of the Jacobian and Schwarz values inside an FP64
                                               iterative refinement of each linear solve (same relative
-                                              tolerance and iteration limit as the plain CG). */
+                                              tolerance and iteration limit as the plain CG). 3: as 1, but
+                                              the element tangents are computed in FP64 and each Jacobian
+                                              entry is summed in FP64 and rounded to FP32 once. */
     /* Damping D = mass_damping_coefficient M + stiffness_damping_coefficient K+(u), K+ the stiffness
        at the (sub)step's initial displacement with every element matrix projected onto the positive
        semi-definite cone (unless the material is ProjectedStableNeoHookean), rebuilt at the start of
