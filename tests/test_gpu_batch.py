@@ -9,6 +9,8 @@
   substeps, simulation_toolbox/src/fem/newton_cg.rs:146-228) and one whose substepping gives up
   (Newton limit 1 at dt 0.2: status SUBSTEPPING): statuses, statistics and states equal the
   single-scene results, and the substepped scene matches the oracle in parity mode.
+* The FP32-operator precision modes (1 and 3) on a 12-scene batch: equal to each scene alone, bit
+  for bit.
 """
 
 import numpy as np
@@ -62,6 +64,33 @@ def test_batched_64_scenes_equal_each_scene_alone():
               f"{sum(s['newton_iterations'] for s in r['stats'])}, cg {sum(s['cg_iterations'] for s in r['stats'])}",
               flush=True)
     torch.cuda.synchronize()
+    alone = run_alone(stepper, steps)
+    for k, (u, v, stats, status) in enumerate(alone):
+        assert status == 0, (k, status)
+        assert torch.equal(u, stepper.u[k]) and torch.equal(v, stepper.v[k]), k
+        for i in range(steps):
+            assert stats[i] == bstats[i][k], (k, i, stats[i], bstats[i][k])
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("mode", [1, 3])
+def test_batched_fp32_operator_equals_each_scene_alone(mode):
+    """The FP32-operator modes (1: FP32 tangents; 3: FP64 tangents rounded into the FP32 Jacobian) run
+    their batched solves on the FP32 CG kernels: 12 scenes of 2e3..3e4 DOFs, two batched steps, equal
+    to each scene stepped alone bit for bit."""
+    from higher_order_embedded_fem_amd import batch
+
+    scs = batch.batched_scenes(12, seed=11, dof_range=(2e3, 3e4))
+    stepper = batch.BatchedStepper(scs, newton_tolerance_factor=1e-6, mixed_precision=mode, autotune=True)
+    steps = 2
+    bstats = []
+    for i in range(steps):
+        r = stepper.step()
+        assert all(s == 0 for s in r["status"]), r["status"]
+        bstats.append(r["stats"])
+    torch.cuda.synchronize()
+    print(f"mode {mode}: newton {[s['newton_iterations'] for s in bstats[-1]]}, "
+          f"cg {[s['cg_iterations'] for s in bstats[-1]]}", flush=True)
     alone = run_alone(stepper, steps)
     for k, (u, v, stats, status) in enumerate(alone):
         assert status == 0, (k, status)
