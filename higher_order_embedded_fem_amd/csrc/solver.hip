@@ -807,12 +807,24 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_b(const CgSeg<VT> one, const 
         const bool ran = for_entries_pf<G>(
             Pm, pro_issue, pro_finish,
             [&](const NodeEntry& e, int lane, bool valid, const RowPf<VT>& pf) {
+                // the row's own r_k, Ap_k, p_k and x, component c on lane c < 3 (lanes 3.. read
+                // component 0 and do not use it), loaded ahead of the product: x and p_k are not
+                // among the product's loads, so reading them after the reduction cost one more
+                // memory latency per pass (12.7 -> 11.9 us per launch on the Hex20 cylinder)
+                const int64_t v = e.node;
+                const int c = lane < 3 ? lane : 0;
+                const double orr = ra[6 * v + c], oap = ra[6 * v + 3 + c];
+                const double opk = P.zp_new[6 * v + 3 + c], ox = P.x[3 * v + c];
                 double z0, z1, z2;
                 entry_row_dot_pf<G>(Pm, e, lane, valid, pf, rvec, z0, z1, z2);
-                if (valid && lane == 0) {
-                    double r0, r1, r2;
-                    rvec(e.node, r0, r1, r2);
-                    finish(e.node, r0, r1, r2, z0, z1, z2);
+                if (valid && lane < 3) {
+                    const double r = orr - alpha * oap;
+                    const double z = lane == 0 ? z0 : (lane == 1 ? z1 : z2);
+                    P.x[3 * v + lane] = ox + alpha * opk;
+                    P.ra_new[6 * v + lane] = r;
+                    P.zp_new[6 * v + lane] = z;
+                    part[0] += z * r;
+                    part[1] += r * r;
                 }
             },
             [&](const NodeEntry& e) {
