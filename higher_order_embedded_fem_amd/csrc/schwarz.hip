@@ -15,6 +15,9 @@
 
 namespace hoefem {
 
+// the partial sums per norm of a block above 96 x 96 (k_bgj_norm)
+constexpr int BGJ_NORM_SPLIT = 32;
+
 void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const int32_t* nodes, int64_t nunt,
                    const int32_t* unt) {
     s.model = &m;
@@ -77,11 +80,14 @@ void schwarz_setup(Schwarz& s, Model& m, int64_t nsub, const int64_t* off, const
         s.d_colp_off.upload(colp_off, m.stream);
         s.d_piv.ensure(s.large.size() * 32 * 32);
         s.d_colp.ensure((size_t)colp_total);
-        s.d_lnorm.ensure(2 * s.large.size());
+        s.d_lnorm.ensure(2 * s.large.size() * BGJ_NORM_SPLIT);
     }
 }
 
-// B_s(r, c) = J[3 g_a + i, 3 g_c + j], row-major (3m)^2, always FP64
+// B_s(r, c) = J[3 g_a + i, 3 g_c + j], row-major (3m)^2, always FP64. Work-group (s, y) gathers
+// entries [y SG_CHUNK, (y + 1) SG_CHUNK) of block s, so a block of hundreds of DOFs (the 64-element
+// patches) is spread over many work-groups instead of walking its entries on one.
+constexpr int SG_CHUNK = 2048;
 template <class VJ>
 __global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off,
                                  const int64_t* __restrict__ slot_off, const int64_t* __restrict__ noff,
@@ -90,9 +96,12 @@ __global__ void k_schwarz_gather(const int32_t* __restrict__ msz, const int64_t*
                                  double* __restrict__ blocks) {
     const int64_t b = blockIdx.x;
     const int mb = msz[b], dim = 3 * mb;
+    const int c0 = (int)blockIdx.y * SG_CHUNK;
+    if (c0 >= dim * dim) return;
+    const int c1 = min(dim * dim, c0 + SG_CHUNK);
     const int32_t* g = nodes + noff[b];
     double* out = blocks + val_off[b];
-    for (int idx = threadIdx.x; idx < dim * dim; idx += blockDim.x) {
+    for (int idx = c0 + (int)threadIdx.x; idx < c1; idx += blockDim.x) {
         const int r = idx / dim, c = idx % dim;
         const int a = r / 3, i = r % 3, cb = c / 3, j = c % 3;
         const int k = jslot[slot_off[b] + (int64_t)a * mb + cb];
@@ -871,6 +880,9 @@ __device__ inline void bgj_mfma(bgj_double4& acc, const double* A, const double*
     }
 }
 
+// |B|_F^2 (which = 0) or |B^-1|_F^2 (which = 1) of every large block in BGJ_NORM_SPLIT partial sums:
+// work-group (l, y) sums the y-th contiguous chunk of block l's entries into
+// out[(2 l + which) BGJ_NORM_SPLIT + y]; k_bgj_finish adds the partials in order.
 __global__ __launch_bounds__(EIG_THREADS) void k_bgj_norm(const int32_t* __restrict__ large,
                                                           const int32_t* __restrict__ msz,
                                                           const int64_t* __restrict__ val_off,
@@ -879,11 +891,14 @@ __global__ __launch_bounds__(EIG_THREADS) void k_bgj_norm(const int32_t* __restr
     __shared__ double s_red[EIG_THREADS];
     const int b = large[blockIdx.x];
     const int dim = 3 * msz[b];
+    const int64_t n = (int64_t)dim * dim;
+    const int64_t chunk = (n + BGJ_NORM_SPLIT - 1) / BGJ_NORM_SPLIT;
+    const int64_t c0 = blockIdx.y * chunk, c1 = min(n, c0 + chunk);
     const double* blk = blocks + val_off[b];
     double loc = 0.0;
-    for (int64_t idx = threadIdx.x; idx < (int64_t)dim * dim; idx += EIG_THREADS) loc += blk[idx] * blk[idx];
+    for (int64_t idx = c0 + threadIdx.x; idx < c1; idx += EIG_THREADS) loc += blk[idx] * blk[idx];
     const double tot = block_sum(loc, s_red);
-    if (threadIdx.x == 0) out[2 * blockIdx.x + which] = tot;
+    if (threadIdx.x == 0) out[(2 * (int64_t)blockIdx.x + which) * BGJ_NORM_SPLIT + blockIdx.y] = tot;
 }
 
 // (1) P = A_kk^-1 for every large block with a panel k
@@ -1033,7 +1048,10 @@ __global__ __launch_bounds__(EIG_THREADS) void k_bgj_finish(const int32_t* __res
     const int dim = 3 * msz[b];
     const int nt = (dim + BT - 1) / BT;
     if (ti >= nt || tj >= nt || ti > tj || fallback[b] != 4) return;
-    const double cert = sqrt(lnorm[2 * lb]) * sqrt(lnorm[2 * lb + 1]);
+    double n2[2] = {0.0, 0.0};  // the partial sums of k_bgj_norm, in order
+    for (int w = 0; w < 2; ++w)
+        for (int y = 0; y < BGJ_NORM_SPLIT; ++y) n2[w] += lnorm[(2 * (int64_t)lb + w) * BGJ_NORM_SPLIT + y];
+    const double cert = sqrt(n2[0]) * sqrt(n2[1]);
     if (!(cert < SPD_COND_LIMIT)) {  // uniform over the block's work-groups
         if (threadIdx.x == 0 && ti == 0 && tj == 0) atomicOr(err, KERR_SCHWARZ);
         return;
@@ -1058,8 +1076,8 @@ static void schwarz_update_large(Schwarz& s) {
     const int32_t* lg = s.d_large.get();
     const int32_t* msz = s.scatter.d_m.get();
     const int64_t* vo = s.scatter.d_val_off.get();
-    hipLaunchKernelGGL(k_bgj_norm, dim3(nl), dim3(EIG_THREADS), 0, st, lg, msz, vo, (const double*)s.d_blocks.get(),
-                       s.d_lnorm.get(), 0);
+    hipLaunchKernelGGL(k_bgj_norm, dim3(nl, BGJ_NORM_SPLIT), dim3(EIG_THREADS), 0, st, lg, msz, vo,
+                       (const double*)s.d_blocks.get(), s.d_lnorm.get(), 0);
     for (int k = 0; k < ntm; ++k) {
         hipLaunchKernelGGL(k_bgj_pivot, dim3(nl), dim3(EIG_THREADS), 0, st, lg, msz, vo,
                            (const double*)s.d_blocks.get(), s.d_piv.get(), s.d_fallback.get(), m.d_err.get(), k);
@@ -1070,8 +1088,8 @@ static void schwarz_update_large(Schwarz& s) {
                            s.d_blocks.get(), (const double*)s.d_piv.get(), (const double*)s.d_colp.get(),
                            (const int64_t*)s.d_colp_off.get(), (const int*)s.d_fallback.get(), k, ntm);
     }
-    hipLaunchKernelGGL(k_bgj_norm, dim3(nl), dim3(EIG_THREADS), 0, st, lg, msz, vo, (const double*)s.d_blocks.get(),
-                       s.d_lnorm.get(), 1);
+    hipLaunchKernelGGL(k_bgj_norm, dim3(nl, BGJ_NORM_SPLIT), dim3(EIG_THREADS), 0, st, lg, msz, vo,
+                       (const double*)s.d_blocks.get(), s.d_lnorm.get(), 1);
     hipLaunchKernelGGL(k_bgj_finish, dim3(nl, (unsigned)(ntm * ntm)), dim3(EIG_THREADS), 0, st, lg, msz, vo,
                        s.d_blocks.get(), (const double*)s.d_lnorm.get(), s.d_fallback.get(), m.d_err.get(), ntm);
     check_launch("blocked Gauss-Jordan");
@@ -1098,7 +1116,9 @@ static void schwarz_update_t(Schwarz& s, const VJ* J) {
     Model& m = *s.model;
     const hipStream_t st = m.stream;
     if (s.nsub > 0) {
-        k_schwarz_gather<VJ><<<dim3((unsigned)s.nsub), dim3(256), 0, st>>>(
+        const int64_t gmax = std::max(s.small_max_dim, s.large_max_dim);
+        const unsigned chunks = (unsigned)std::max<int64_t>(1, (gmax * gmax + SG_CHUNK - 1) / SG_CHUNK);
+        k_schwarz_gather<VJ><<<dim3((unsigned)s.nsub, chunks), dim3(256), 0, st>>>(
             s.scatter.d_m.get(), s.scatter.d_val_off.get(), s.scatter.d_slot_off.get(), s.d_off.get(), s.d_nodes.get(),
             s.d_jslot.get(), m.pattern.d_rowptr.get(), J, s.d_blocks.get());
         check_launch("k_schwarz_gather");
