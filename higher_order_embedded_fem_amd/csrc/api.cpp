@@ -1388,32 +1388,40 @@ static void sim_autotune_cg(Sim& s, int* ga_out, int* gb_out, double* candidate_
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     const int cand[3] = {8, 16, 32};
-    int best_a = 8, best_b = 8;
-    double best_t = 1e300;
-    for (int ia = 0; ia < 3; ++ia)
-        for (int ib = 0; ib < (s.schwarz ? 3 : 1); ++ib) {
-            m.pattern.tuned_group = cand[ia];
-            if (s.schwarz) s.schwarz->P.tuned_group = cand[ib];
-            int64_t it = 0;
-            auto run = [&](int64_t iters) {
-                launch_fill(m, s.d_sol.get(), n, 0.0);
-                if (s.mixed)
-                    (void)pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), s.d_F.get(), s.d_sol.get(), 0.0, iters, &it);
-                else
-                    (void)pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), s.d_F.get(), s.d_sol.get(), 0.0, iters, &it);
-            };
-            run(4);  // also captures this configuration's batch graph
-            float ms = 1e30f;
-            for (int rep = 0; rep < 3; ++rep) {  // best of three 96-iteration solves
+    const int nb = s.schwarz ? 3 : 1;
+    int64_t it = 0;
+    auto run = [&](int64_t iters) {
+        launch_fill(m, s.d_sol.get(), n, 0.0);
+        if (s.mixed)
+            (void)pcg_solve(m, s.cg, s.d_J32.get(), s.schwarz.get(), s.d_F.get(), s.d_sol.get(), 0.0, iters, &it);
+        else
+            (void)pcg_solve(m, s.cg, s.d_J.get(), s.schwarz.get(), s.d_F.get(), s.d_sol.get(), 0.0, iters, &it);
+    };
+    // AUTOTUNE_ROUNDS rounds over all pairs, each pair's best 96-iteration solve kept: a drift of the
+    // GPU's speed during tuning then reaches every pair instead of favouring the ones measured last
+    constexpr int AUTOTUNE_ROUNDS = 5;
+    float best_ms[9];
+    std::fill(best_ms, best_ms + 9, 1e30f);
+    for (int rep = 0; rep < AUTOTUNE_ROUNDS; ++rep)
+        for (int ia = 0; ia < 3; ++ia)
+            for (int ib = 0; ib < nb; ++ib) {
+                m.pattern.tuned_group = cand[ia];
+                if (s.schwarz) s.schwarz->P.tuned_group = cand[ib];
+                run(4);  // also captures this pair's batch graph
                 HIP_CHECK(hipEventRecord(e0, m.stream));
                 run(96);
                 HIP_CHECK(hipEventRecord(e1, m.stream));
                 HIP_CHECK(hipEventSynchronize(e1));
                 float t = 0.0f;
                 HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
-                ms = std::min(ms, t);
+                best_ms[3 * ia + ib] = std::min(best_ms[3 * ia + ib], t);
             }
-            if (candidate_us) candidate_us[3 * ia + ib] = 1e3 * (double)ms / 96.0;
+    int best_a = 8, best_b = 8;
+    double best_t = 1e300;
+    for (int ia = 0; ia < 3; ++ia)
+        for (int ib = 0; ib < nb; ++ib) {
+            const double ms = best_ms[3 * ia + ib];
+            if (candidate_us) candidate_us[3 * ia + ib] = 1e3 * ms / 96.0;
             if (ms < best_t) {
                 best_t = ms;
                 best_a = cand[ia];

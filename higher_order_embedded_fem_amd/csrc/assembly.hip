@@ -524,8 +524,11 @@ __global__ __launch_bounds__(256) void k_element_mass(int64_t ne, const int32_t*
 // chunks of `chunk` nodes, each a pass over the incident blocks; every entry still receives its
 // terms in color order, so the sums do not depend on the chunking.
 constexpr int GATHER_CHUNK = 896;  // 9 * 896 doubles = 63 KiB of LDS
+// threads of k_block_gather: four waves share a node row, so a row of hundreds of blocks (the
+// Schwarz operator of large subdomains) is not walked by one wave
+constexpr int GB_THREADS = 256;
 template <class TB, class TV>
-__global__ __launch_bounds__(64) void k_block_gather(
+__global__ __launch_bounds__(GB_THREADS) void k_block_gather(
     const int32_t* __restrict__ nptr, const int32_t* __restrict__ rowptr, const uint8_t* __restrict__ diag_only,
     const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc_blk, const int32_t* __restrict__ inc_loc,
     const int32_t* __restrict__ msz, const int64_t* __restrict__ val_off, const int64_t* __restrict__ slot_off,
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(64) void k_block_gather(
         const int cw = min(chunk, k - c0);
         const int cr = 3 * cw;  // a row's width within the chunk
         const int R = 3 * cr;
-        for (int idx = t; idx < R; idx += 64) {
+        for (int idx = t; idx < R; idx += GB_THREADS) {
             const int i = idx / cr, cc = idx % cr;
             s_rb[idx] = (zero_first || mode == 1) ? 0.0 : (double)values[r0 + i * rw + 3 * c0 + cc];
         }
@@ -556,7 +559,7 @@ __global__ __launch_bounds__(64) void k_block_gather(
             const int bw = 3 * mb;
             const TB* base = blocks + val_off[b] + (int64_t)(3 * a) * bw;
             const int32_t* sl = slot + slot_off[b] + (int64_t)a * mb;
-            for (int idx = t; idx < 3 * bw; idx += 64) {
+            for (int idx = t; idx < 3 * bw; idx += GB_THREADS) {
                 const int i = idx / bw, cc = idx % bw;
                 const int bb = cc / 3, jj = cc % 3;
                 const int col = sl[bb] - c0;
@@ -564,7 +567,7 @@ __global__ __launch_bounds__(64) void k_block_gather(
             }
             __syncthreads();
         }
-        for (int idx = t; idx < R; idx += 64) {
+        for (int idx = t; idx < R; idx += GB_THREADS) {
             const int i = idx / cr, cc = idx % cr;
             const int64_t o = r0 + i * rw + 3 * c0 + cc;
             const double s = s_rb[idx];
@@ -951,7 +954,7 @@ void launch_block_gather(const NodePattern& pat, const BlockScatter& sc, const T
     }
     const int chunk = std::min(std::max(pat.max_nbr, 1), GATHER_CHUNK);
     const size_t lds = (size_t)9 * chunk * sizeof(double);
-    launch_timed(prof, k_block_gather<TB, TV>, dim3((unsigned)pat.num_nodes), dim3(64), (uint32_t)lds, s,
+    launch_timed(prof, k_block_gather<TB, TV>, dim3((unsigned)pat.num_nodes), dim3(GB_THREADS), (uint32_t)lds, s,
                  (const int32_t*)pat.d_nptr.get(), (const int32_t*)pat.d_rowptr.get(),
                  (const uint8_t*)pat.d_diag_only.get(), (const int32_t*)sc.d_inc_ptr.get(),
                  (const int32_t*)sc.d_inc_blk.get(), (const int32_t*)sc.d_inc_loc.get(), (const int32_t*)sc.d_m.get(),
